@@ -1,0 +1,364 @@
+"""CPU FP64 oracle for RAVE-SIM's 2D Fresnel multislice path.
+
+TEST INFRASTRUCTURE ONLY.  This module restates, in numpy/scipy complex128,
+the arithmetic of big-wave's in-memory (``NumpyVector``) 2D path so that the
+HIP engine can be checked against it.  It is imported only by ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py``; the
+product package never imports it and has no CPU fallback.
+
+Every function cites the reference file:line whose semantics it follows
+(paths relative to the reference checkout).  The restatement is written from
+the reference's documented behaviour, not copied: operation order is kept
+where it changes rounding (phase constants, interpolation order), because the
+parity tests compare at 1e-12..2e-14 relative.
+
+Pinning (what checks each function against something other than itself):
+
+* ``fft2`` / ``ifft2``          -- numpy-derived vectors from
+  fast-wave/test/fwtest.cu:89-334 (tests/golden/fwtest_fft_vectors.json) and
+  brute-force DFT on tiny sizes.
+* ``frequency_squared`` / ``apply_frequency_cutoff_2d`` -- fftfreq N=10 vector
+  (fwtest.cu:59-69) and the axis/diagonal boundary cells of
+  tests/big_wave_2d/test_p5.py:126-140.
+* ``propagate_2d``              -- plane and tilted plane wave closed form
+  (test_p5.py:84-100), Parseval power conservation, rectangular aperture vs
+  Fresnel integrals (test_p5.py:142-172).
+* ``propagate_analytically_2d`` -- 1/r amplitude, circular phase, peak position
+  (test_p5.py:64-82).
+* ``material_factor`` / ``sample_deltabeta_slice`` / ``sample_apply_2d`` --
+  uniform slab phase + Beer-Lambert (test_p5.py:102-124), offset slab vs
+  explicit transmission (test_p5.py:174-213), per-pixel brute-force bilinear
+  map (test_p1.py:283-317), hollow-capsule x/y symmetry (test_p5.py:215-256).
+* ``area_v1``                   -- brute-force overlap loops
+  (tests/big_wave_2d/test_p3.py:85-123) and uniform-field area preservation
+  (test_p3.py:222-243).
+* ``run_point_source_sample``   -- the reference's own end-to-end golden
+  fingerprints tests/big_wave_2d/p5_results/{vacuum,sample}_convergence_4096.json
+  (inputs in tests/big_wave_2d/p5_benchmark.py:69-185).
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+from scipy import fft as spfft
+
+# big-wave/propagation.py:199-201
+PLANCK_H = 6.62607004 * 10 ** (-34)
+SPEED_OF_LIGHT = 299792458
+EV_TO_JOULE = 1.602176634 * 10 ** (-19)
+
+
+def convert_energy_wavelength(energy_or_wavelength: float) -> float:
+    """big-wave/propagation.py:260-262."""
+    return PLANCK_H * SPEED_OF_LIGHT / (energy_or_wavelength * EV_TO_JOULE)
+
+
+# ---------------------------------------------------------------------------
+# FFT conventions: big-wave/vector.py:341-347 (scipy.fft.fft2/ifft2 on the
+# (ny, nx) view; forward unnormalised, inverse 1/(nx*ny)).
+# ---------------------------------------------------------------------------
+
+def fft2(field2d: np.ndarray, workers: int = 1) -> np.ndarray:
+    return spfft.fft2(field2d, workers=workers)
+
+
+def ifft2(field2d: np.ndarray, workers: int = 1) -> np.ndarray:
+    return spfft.ifft2(field2d, workers=workers)
+
+
+def fftfreq(n: int) -> np.ndarray:
+    """numpy.fft.fftfreq(n) with d=1: integer bins times 1/n."""
+    val = 1.0 / n
+    bins = np.empty(n, dtype=np.int64)
+    half = (n - 1) // 2 + 1
+    bins[:half] = np.arange(0, half)
+    bins[half:] = np.arange(-(n // 2), 0)
+    return bins * val
+
+
+def frequency_squared(n: int, spacing: float) -> np.ndarray:
+    """big-wave/propagation.py:293-295: (fftfreq(n)/spacing)**2."""
+    f = fftfreq(n) / spacing
+    return f * f
+
+
+def apply_frequency_cutoff_2d(U2d: np.ndarray, freq: float, dx: float, dy: float) -> np.ndarray:
+    """big-wave/propagation.py:298-326: zero where kx^2+ky^2 > 2 f^2 (boundary kept)."""
+    ny, nx = U2d.shape
+    freq_sq = 2.0 * freq ** 2
+    kx2 = frequency_squared(nx, dx)
+    ky2 = frequency_squared(ny, dy)
+    mask = ky2[:, None] + kx2[None, :] > freq_sq
+    out = U2d.copy()
+    out[mask] = 0.0
+    return out
+
+
+def chirp_2d(nx: int, ny: int, dx: float, dy: float, wl: float, dz: float) -> np.ndarray:
+    """Transfer function of big-wave/propagation.py:427-435 (no cutoff).
+
+    H = exp(2*pi*i*dz/wl) * exp(-i*pi*wl*dz*(kx^2+ky^2)), with the same
+    expression order as the reference so the FP64 phase rounds identically.
+    """
+    kx2 = frequency_squared(nx, dx)
+    ky2 = frequency_squared(ny, dy)
+    global_phase = np.exp(1j * 2 * np.pi / wl * dz)
+    phase_scale = -1j * np.pi * wl * dz
+    k_sq = ky2[:, None] + kx2[None, :]
+    return global_phase * np.exp(phase_scale * k_sq)
+
+
+def propagate_2d(u2d: np.ndarray, dx: float, dy: float, wl: float, dz: float,
+                 cutoff_frequency: float, U2d: np.ndarray | None = None,
+                 workers: int = 1) -> tuple[np.ndarray, np.ndarray]:
+    """big-wave/propagation.py:396-442.
+
+    Returns (u_new, U_new) where U_new = FFT2(u_new).  When ``U2d`` is given it
+    is taken as FFT2(u2d) and the forward transform is skipped (``skip_fft``).
+    The result keeps the input dtype, as NumpyVector does (vector.py:341-347):
+    complex64 inputs are transformed in single precision by scipy.
+    """
+    ny, nx = u2d.shape
+    U = fft2(u2d, workers) if U2d is None else U2d.copy()
+    H = chirp_2d(nx, ny, dx, dy, wl, dz)
+    # In-place `tile *= global_phase * exp(...)` on a complex64 tile computes in
+    # complex128 and casts back (propagation.py:433-435).
+    U = (U * H).astype(U.dtype, copy=False)
+    U = apply_frequency_cutoff_2d(U, cutoff_frequency, dx, dy)
+    return ifft2(U, workers), U
+
+
+# ---------------------------------------------------------------------------
+# Point source: big-wave/propagation.py:458-484 and source.py:50-113
+# ---------------------------------------------------------------------------
+
+def grid_density_check(dz: float, x_source: float, n: int, dx: float, wl: float) -> bool:
+    """big-wave/propagation.py:204-225 (returns the predicate instead of raising)."""
+    def dist_x(idx):
+        return np.sqrt(((n / 2 - idx) * dx + np.abs(x_source)) ** 2 + dz ** 2)
+    return bool(np.abs(dist_x(0) - dist_x(1)) <= wl / 2)
+
+
+def propagate_analytically_2d(nx: int, ny: int, dx: float, dy: float, z: float,
+                              x_source: float, y_source: float, wl: float) -> np.ndarray:
+    """u = exp(2*pi*i*r/wl)/r, r = sqrt(y^2 + x^2 + z^2) (propagation.py:474-482)."""
+    x = (np.arange(nx, dtype=np.float64) - nx / 2.0) * dx - x_source
+    y = (np.arange(ny, dtype=np.float64) - ny / 2.0) * dy - y_source
+    x2 = np.square(x)
+    y2 = np.square(y)
+    wave_number = 2j * np.pi / wl
+    r = np.sqrt(y2[:, None] + x2[None, :] + z ** 2)
+    return np.exp(r * wave_number) / r
+
+
+def point_source_propagate_to(nx, ny, dx, dy, dz, xs, ys, wl, cutoff_freq,
+                              dtype=np.complex128, check_density=True, workers=1):
+    """PointSource.propagate_to, 2D non-Fresnel branch (source.py:97-107).
+
+    Analytic init, FFT2, circular cutoff, IFFT2.  The spectrum is NOT returned:
+    the reference does not guarantee U = FFT(u) after a source (source.py:38-40).
+    """
+    if check_density:
+        if not (grid_density_check(dz, xs, nx, dx, wl) and grid_density_check(dz, ys, ny, dy, wl)):
+            raise ValueError("Nyquist not satisfied, dx too large")
+    u = propagate_analytically_2d(nx, ny, dx, dy, dz, xs, ys, wl).astype(dtype)
+    U = apply_frequency_cutoff_2d(fft2(u, workers), cutoff_freq, dx, dy)
+    return ifft2(U, workers)
+
+
+# ---------------------------------------------------------------------------
+# Sample transmission: big-wave/optical_element.py:39-42, 383-596
+# ---------------------------------------------------------------------------
+
+def material_factor(deltabeta, thickness: float, wl: float):
+    """exp(-2*pi*i*t/wl*conj(db)) (optical_element.py:39-42)."""
+    return np.exp(-2j * np.pi * thickness / wl * np.conj(deltabeta))
+
+
+def axis_map(n: int, spacing: float, scale: float, pos: float, grid_len: int, pixel: float):
+    """Field axis -> voxel axis mapping (optical_element.py:538-546, 555-563).
+
+    Returns (floor_clipped, frac, inside) for each field index.
+    """
+    sim = (np.arange(n, dtype=np.float64) - n / 2.0) * spacing
+    idx = (sim * scale + pos + grid_len * pixel * 0.5) / pixel
+    floor_raw = np.floor(idx).astype(np.int64)
+    frac = idx - floor_raw
+    inside = (idx >= 0.0) & (idx < float(grid_len) - 1.0)
+    floor_c = np.clip(floor_raw, 0, grid_len - 2)
+    return floor_c, frac, inside
+
+
+def sample_deltabeta_slice(grid: np.ndarray, db_list: np.ndarray, rowidx: int,
+                           nx: int, ny: int, dx: float, dy: float,
+                           px: float, py: float, x_pos: float, y_pos: float,
+                           scale: float = 1.0) -> np.ndarray:
+    """Bilinear delta+i*beta of one z-slice on the field grid, 0 outside.
+
+    Follows optical_element.py:566-579: gather the four corner ids, interpolate
+    in delta-beta space along x then y, zero outside the voxel box.
+    """
+    _, y_len, x_len = grid.shape
+    x0, xf, inside_x = axis_map(nx, dx, scale, x_pos, x_len, px)
+    y0, yf, inside_y = axis_map(ny, dy, scale, y_pos, y_len, py)
+    g = grid[rowidx]
+    X0 = x0[None, :]
+    Y0 = y0[:, None]
+    db00 = db_list[g[Y0, X0]]
+    db01 = db_list[g[Y0, X0 + 1]]
+    db10 = db_list[g[Y0 + 1, X0]]
+    db11 = db_list[g[Y0 + 1, X0 + 1]]
+    db_y0 = db00 * (1.0 - xf[None, :]) + db01 * xf[None, :]
+    db_y1 = db10 * (1.0 - xf[None, :]) + db11 * xf[None, :]
+    interp = db_y0 * (1.0 - yf[:, None]) + db_y1 * yf[:, None]
+    interp[~(inside_y[:, None] & inside_x[None, :])] = 0.0
+    return interp
+
+
+@dataclass
+class SampleSpec:
+    """The fields of big-wave's Sample (optical_element.py:383-425) the 2D path reads."""
+    grid: np.ndarray            # (nz, ny_g, nx_g) integer material ids, 0 = vacuum
+    db_list: np.ndarray         # complex128, db_list[0] = 0 (store_deltabetas, :433-440)
+    pixel_size_x: float
+    pixel_size_z: float
+    pixel_size_y: float = 0.0
+    x_pos: float = 0.0
+    y_pos: float = 0.0
+    z_start: float = 0.0
+
+    @property
+    def py(self) -> float:
+        return self.pixel_size_y if self.pixel_size_y != 0.0 else self.pixel_size_x
+
+
+def sample_apply_2d(u2d: np.ndarray, sample: SampleSpec, dx: float, dy: float, wl: float,
+                    cutoff_freq: float, first_slice: int = 0, n_slices: int | None = None,
+                    workers: int = 1) -> tuple[np.ndarray, np.ndarray]:
+    """Sample._apply_2d (optical_element.py:502-596) without Fresnel scaling.
+
+    Per slice: u *= material_factor(interpolated db, pz, wl); then
+    propagate_2d over pz.  Returns (u, U) with U = FFT2(u) (the apply contract,
+    optical_element.py:97).
+    """
+    ny, nx = u2d.shape
+    nz = sample.grid.shape[0]
+    last = nz if n_slices is None else min(nz, first_slice + n_slices)
+    u = u2d.copy()
+    U = None
+    for rowidx in range(first_slice, last):
+        db = sample_deltabeta_slice(sample.grid, sample.db_list, rowidx, nx, ny, dx, dy,
+                                    sample.pixel_size_x, sample.py, sample.x_pos, sample.y_pos)
+        t = material_factor(db, sample.pixel_size_z, wl)
+        u = (u * t).astype(u.dtype, copy=False)
+        u, U = propagate_2d(u, dx, dy, wl, sample.pixel_size_z, cutoff_freq, workers=workers)
+    return u, U
+
+
+# ---------------------------------------------------------------------------
+# Detector: big-wave/propagation.py:764-816 (_detector_geometry),
+# 949-977 (_area_axis_plan), 980-992 (_prefix_integral_at), 995-1110 (_area_v1_stream)
+# ---------------------------------------------------------------------------
+
+def detector_shape(size_x: float, size_y: float, ds_x: float, ds_y: float) -> tuple[int, int]:
+    """(out_y, out_x) = (int(size_y // ds_y), int(size_x // ds_x)) (propagation.py:793-794)."""
+    return int(size_y // ds_y), int(size_x // ds_x)
+
+
+def area_axis_plan(positions: np.ndarray, ds: float, n: int, spacing: float):
+    """Clipped overlap plan of one detector axis (propagation.py:949-977)."""
+    grid_left = (-(n // 2) - 0.5) * spacing
+    grid_right = grid_left + n * spacing
+    lo = positions - ds * 0.5
+    hi = positions + ds * 0.5
+    clo = np.clip(lo, grid_left, grid_right)
+    chi = np.maximum(np.clip(hi, grid_left, grid_right), clo)
+    slo = (clo - grid_left) / spacing
+    shi = (chi - grid_left) / spacing
+    ilo = np.floor(slo).astype(np.int64)
+    ihi = np.floor(shi).astype(np.int64)
+    flo = slo - ilo
+    fhi = shi - ihi
+    rlo = ilo >= n
+    rhi = ihi >= n
+    ilo[rlo] = n
+    ihi[rhi] = n
+    flo[rlo] = 0.0
+    fhi[rhi] = 0.0
+    return ilo, flo, ihi, fhi, lo, hi
+
+
+def area_v1(u2d: np.ndarray, dx: float, dy: float, size_x: float, size_y: float,
+            ds_x: float, ds_y: float, z: float) -> np.ndarray:
+    """Area-weighted |u|^2 detector, FP64 (propagation.py:995-1110).
+
+    I[q,p] = sum_cells |u|^2 * overlap_x * overlap_y * z/sqrt(x_p^2+y_q^2+z^2).
+    The x overlap is integrated with a row prefix sum; the y overlap with the
+    explicit min/max interval formula, as the reference does.
+    """
+    ny, nx = u2d.shape
+    out_y, out_x = detector_shape(size_x, size_y, ds_x, ds_y)
+    if out_x <= 0 or out_y <= 0:
+        raise ValueError("2D detector output must be non-empty")
+    x_det = (np.arange(out_x, dtype=np.float64) - out_x // 2) * ds_x
+    y_det = (np.arange(out_y, dtype=np.float64) - out_y // 2) * ds_y
+    xi0, xf0, xi1, xf1, _, _ = area_axis_plan(x_det, ds_x, nx, dx)
+    _, _, _, _, y_lo, y_hi = area_axis_plan(y_det, ds_y, ny, dy)
+    grid_y_left = (-(ny // 2) - 0.5) * dy
+    if u2d.dtype == np.complex64:
+        inten = (u2d.real * u2d.real + u2d.imag * u2d.imag).astype(np.float64)
+    else:
+        inten = u2d.real * u2d.real + u2d.imag * u2d.imag
+    prefix = np.zeros((ny, nx + 1), dtype=np.float64)
+    np.cumsum(inten, axis=1, out=prefix[:, 1:])
+
+    def at(idx, frac):
+        res = prefix[:, idx] * dx
+        interior = idx < nx
+        res[:, interior] += frac[interior] * dx * inten[:, idx[interior]]
+        return res
+
+    x_int = at(xi1, xf1) - at(xi0, xf0)          # (ny, out_x)
+    cell_lo = grid_y_left + np.arange(ny) * dy
+    cell_hi = cell_lo + dy
+    overlap = np.minimum(cell_hi[None, :], y_hi[:, None]) - np.maximum(cell_lo[None, :], y_lo[:, None])
+    overlap = np.where(overlap > 0, overlap, 0.0)  # (out_y, ny)
+    cosine = z / np.sqrt(np.square(x_det)[None, :] + np.square(y_det)[:, None] + z * z)
+    return (overlap @ x_int) * cosine
+
+
+# ---------------------------------------------------------------------------
+# Driver ordering: big-wave/wavesim.py:89-317 (single source, no phase stepping)
+# ---------------------------------------------------------------------------
+
+def run_point_source_sample(nx, ny, dx, dy, wl, source_xyz, cutoff_freq, z_detector,
+                            det_size_x, det_size_y, ds_x, ds_y, sample: SampleSpec | None = None,
+                            dtype=np.complex128, workers=1, return_field=False):
+    """Point source -> [Sample] -> final leg (skip_fft) -> area_v1 detector.
+
+    Mirrors wavesim.run_simulation for one 2D Sample and one cutoff frequency
+    (wavesim.py:175-186, 257-260, 279-303, 314-315).
+    """
+    xs, ys, zs = source_xyz
+    current_z = sample.z_start if sample is not None else z_detector
+    u = point_source_propagate_to(nx, ny, dx, dy, current_z - zs, xs, ys, wl, cutoff_freq,
+                                  dtype=dtype, workers=workers)
+    U = None
+    if sample is not None:
+        u, U = sample_apply_2d(u, sample, dx, dy, wl, cutoff_freq, workers=workers)
+        current_z = sample.z_start + sample.pixel_size_z * sample.grid.shape[0]
+    if current_z < z_detector - 1e-8:
+        u, U = propagate_2d(u, dx, dy, wl, z_detector - current_z, cutoff_freq, U2d=U,
+                            workers=workers)
+    det = area_v1(u, dx, dy, det_size_x, det_size_y, ds_x, ds_y, z_detector)
+    return (det, u) if return_field else det
+
+
+def relative_l2(actual, expected) -> float:
+    """tests/big_wave_2d/test_p5.py:44-50."""
+    a = np.asarray(actual)
+    e = np.asarray(expected)
+    return float(np.linalg.norm((a - e).reshape(-1)) / max(np.linalg.norm(e.reshape(-1)), np.finfo(float).eps))
