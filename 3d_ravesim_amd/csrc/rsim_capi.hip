@@ -1,0 +1,527 @@
+// extern "C" entry points of libravesim_hip.so (declared in include/ravesim.h).
+#include <atomic>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "rsim_fft_cfg.hpp"
+#include "rsim_internal.hpp"
+#include "rsim_tables.hpp"
+
+namespace rsim {
+
+int point_source(void* u, int nx, int ny, int dtype, double dx, double dy, double z, double xs,
+                 double ys, double wl, hipStream_t s);
+int transmission(void* u, int nx, int ny, int dtype, double dx, double dy, double wl,
+                 const rsim_volume* vol, int slice, double px, double py, double pz, double xpos,
+                 double ypos, hipStream_t s);
+int detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double dy, int ox,
+                     int oy, double dsx, double dsy, double z, double weight, bool accumulate,
+                     double* out_dev, hipStream_t s);
+int paint_spheres(rsim_volume* vol, const double* sph, const int* mats, int n, hipStream_t s);
+int propagate(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
+              double dz, double freq, bool skip_fft, hipStream_t s);
+int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
+                 double freq, const rsim_volume* vol, double px, double py, double pz,
+                 double xpos, double ypos, int first, int count, int engine, int flags,
+                 hipStream_t s);
+int rsim_release_tables();
+int release_tables();
+int release_multislice_tables();
+
+namespace {
+thread_local std::string t_err;
+std::mutex g_mu;
+std::map<int, hipStream_t> g_streams;
+std::atomic<int> g_cancel{0};
+rsim_progress_fn g_progress = nullptr;
+void* g_progress_user = nullptr;
+}  // namespace
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  t_err = buf;
+  return code;
+}
+
+void clear_error() { t_err.clear(); }
+const char* last_error_cstr() { return t_err.c_str(); }
+void set_cancel(int v) { g_cancel.store(v); }
+void set_progress(rsim_progress_fn fn, void* user) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_progress = fn;
+  g_progress_user = user;
+}
+void release_streams() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto& kv : g_streams) hipStreamDestroy(kv.second);
+  g_streams.clear();
+}
+
+int current_device() {
+  int d = 0;
+  if (hipGetDevice(&d) != hipSuccess) return 0;
+  return d;
+}
+
+hipStream_t resolve_stream(void* s) {
+  if (s) return (hipStream_t)s;
+  std::lock_guard<std::mutex> lk(g_mu);
+  const int dev = current_device();
+  auto it = g_streams.find(dev);
+  if (it != g_streams.end()) return it->second;
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  g_streams[dev] = st;
+  return st;
+}
+
+size_t max_lds_per_block() {
+  static std::map<int, size_t> cache;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
+  const int dev = current_device();
+  auto it = cache.find(dev);
+  if (it != cache.end()) return it->second;
+  int v = 65536;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    v = 65536;
+  cache[dev] = (size_t)v;
+  return (size_t)v;
+}
+
+bool cancel_requested() { return g_cancel.load() != 0; }
+
+// ---- kernel timing ----
+namespace {
+struct KEv {
+  int kind;
+  hipEvent_t a, b;
+};
+std::mutex g_kt_mu;
+bool g_kt_on = false;
+std::vector<KEv> g_kt_done;
+std::vector<hipEvent_t> g_kt_pool;
+KEv g_kt_open[3];
+bool g_kt_has_open[3] = {false, false, false};
+
+hipEvent_t kt_event() {
+  if (!g_kt_pool.empty()) {
+    hipEvent_t e = g_kt_pool.back();
+    g_kt_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+}  // namespace
+
+bool ktime_on() { return g_kt_on; }
+
+void ktime_begin(int kind, hipStream_t s) {
+  if (!g_kt_on) return;
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  KEv ev{kind, kt_event(), kt_event()};
+  hipEventRecord(ev.a, s);
+  g_kt_open[kind] = ev;
+  g_kt_has_open[kind] = true;
+}
+
+void ktime_end(int kind, hipStream_t s) {
+  if (!g_kt_on) return;
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  if (!g_kt_has_open[kind]) return;
+  hipEventRecord(g_kt_open[kind].b, s);
+  g_kt_done.push_back(g_kt_open[kind]);
+  g_kt_has_open[kind] = false;
+}
+
+int ktime_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  g_kt_on = on != 0;
+  return RSIM_OK;
+}
+
+int ktime_read(double* ms3, int* n3) {
+  std::lock_guard<std::mutex> lk(g_kt_mu);
+  double ms[3] = {0, 0, 0};
+  int n[3] = {0, 0, 0};
+  for (auto& e : g_kt_done) {
+    RSIM_HIP(hipEventSynchronize(e.b));
+    float t = 0.f;
+    RSIM_HIP(hipEventElapsedTime(&t, e.a, e.b));
+    ms[e.kind] += t;
+    n[e.kind] += 1;
+    g_kt_pool.push_back(e.a);
+    g_kt_pool.push_back(e.b);
+  }
+  g_kt_done.clear();
+  for (int k = 0; k < 3; ++k) {
+    if (ms3) ms3[k] = ms[k];
+    if (n3) n3[k] = n[k];
+  }
+  return RSIM_OK;
+}
+
+bool g_progress_sync() { return g_progress != nullptr; }
+
+void report_progress(const char* stage, long long completed, long long total) {
+  if (g_progress) g_progress(stage, completed, total, g_progress_user);
+}
+
+}  // namespace rsim
+
+using namespace rsim;
+
+namespace {
+
+int check_field(const void* p, int nx, int ny, int dtype, const char* name) {
+  if (!p) return fail(RSIM_E_INVALID_ARG, "%s is NULL", name);
+  if (nx <= 0 || ny <= 0) return fail(RSIM_E_INVALID_ARG, "nx and ny must be positive, got nx=%d, ny=%d", nx, ny);
+  if (!valid_dtype(dtype)) return fail(RSIM_E_INVALID_ARG, "dtype must be RSIM_C64 or RSIM_C128, got %d", dtype);
+  return RSIM_OK;
+}
+
+int check_fft_sizes(int nx, int ny) {
+  if (!fft_size_supported(nx) || !fft_size_supported(ny))
+    return fail(RSIM_E_INVALID_ARG, "FFT sizes must factor into 2, 3, 5: nx=%d, ny=%d", nx, ny);
+  return RSIM_OK;
+}
+
+}  // namespace
+
+
+extern "C" {
+
+const char* rsim_last_error(void) { return last_error_cstr(); }
+
+const char* rsim_version(void) { return "ravesim-mi355x 0.1 (gfx950)"; }
+
+int rsim_device_count(int* count) {
+  if (!count) return fail(RSIM_E_INVALID_ARG, "count is NULL");
+  RSIM_HIP(hipGetDeviceCount(count));
+  return RSIM_OK;
+}
+
+int rsim_set_device(int device) {
+  RSIM_HIP(hipSetDevice(device));
+  return RSIM_OK;
+}
+
+int rsim_malloc(void** dptr, size_t bytes) {
+  if (!dptr) return fail(RSIM_E_INVALID_ARG, "dptr is NULL");
+  *dptr = nullptr;
+  if (bytes == 0) bytes = 1;
+  hipError_t e = hipMalloc(dptr, bytes);
+  if (e == hipErrorOutOfMemory) return fail(RSIM_E_NOMEM, "hipMalloc(%zu) out of device memory", bytes);
+  if (e != hipSuccess) return fail(RSIM_E_HIP, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  return RSIM_OK;
+}
+
+int rsim_free(void* dptr) {
+  if (dptr) RSIM_HIP(hipFree(dptr));
+  return RSIM_OK;
+}
+
+int rsim_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return RSIM_OK;
+  if (!dst || !src) return fail(RSIM_E_INVALID_ARG, "NULL pointer in memcpy");
+  hipStream_t s = resolve_stream(stream);
+  RSIM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+  RSIM_HIP(hipStreamSynchronize(s));
+  return RSIM_OK;
+}
+
+int rsim_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return RSIM_OK;
+  if (!dst || !src) return fail(RSIM_E_INVALID_ARG, "NULL pointer in memcpy");
+  hipStream_t s = resolve_stream(stream);
+  RSIM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+  RSIM_HIP(hipStreamSynchronize(s));
+  return RSIM_OK;
+}
+
+int rsim_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
+  if (bytes == 0) return RSIM_OK;
+  if (!dst || !src) return fail(RSIM_E_INVALID_ARG, "NULL pointer in memcpy");
+  RSIM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, resolve_stream(stream)));
+  return RSIM_OK;
+}
+
+int rsim_memset(void* dptr, int value, size_t bytes, void* stream) {
+  if (bytes == 0) return RSIM_OK;
+  if (!dptr) return fail(RSIM_E_INVALID_ARG, "dptr is NULL");
+  RSIM_HIP(hipMemsetAsync(dptr, value, bytes, resolve_stream(stream)));
+  return RSIM_OK;
+}
+
+int rsim_synchronize(void* stream) {
+  RSIM_HIP(hipStreamSynchronize(resolve_stream(stream)));
+  return RSIM_OK;
+}
+
+int rsim_stream_create(void** stream) {
+  if (!stream) return fail(RSIM_E_INVALID_ARG, "stream is NULL");
+  hipStream_t s;
+  RSIM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = (void*)s;
+  return RSIM_OK;
+}
+
+int rsim_stream_destroy(void* stream) {
+  if (stream) RSIM_HIP(hipStreamDestroy((hipStream_t)stream));
+  return RSIM_OK;
+}
+
+int rsim_release_caches(void) {
+  RSIM_HIP(hipDeviceSynchronize());
+  rsim_release_tables();
+  release_tables();
+  release_multislice_tables();
+  return RSIM_OK;
+}
+
+int rsim_kernel_timing(int enable) { return ktime_enable(enable); }
+
+int rsim_kernel_timing_read(double* ms3, int* launches3) { return ktime_read(ms3, launches3); }
+
+int rsim_set_progress(rsim_progress_fn fn, void* user) {
+  set_progress(fn, user);
+  return RSIM_OK;
+}
+
+int rsim_request_cancel(void) {
+  set_cancel(1);
+  return RSIM_OK;
+}
+
+int rsim_clear_cancel(void) {
+  set_cancel(0);
+  return RSIM_OK;
+}
+
+int rsim_fft2(void* dst, const void* src, int nx, int ny, int dtype, int direction, void* stream) {
+  RSIM_CHECK(check_field(src, nx, ny, dtype, "src"));
+  RSIM_CHECK(check_field(dst, nx, ny, dtype, "dst"));
+  RSIM_CHECK(check_fft_sizes(nx, ny));
+  if (direction != RSIM_FORWARD && direction != RSIM_INVERSE)
+    return fail(RSIM_E_INVALID_ARG, "direction must be RSIM_FORWARD or RSIM_INVERSE");
+  return fft2(dtype, direction, src, dst, nx, ny, resolve_stream(stream));
+}
+
+int rsim_fft1d_rows(void* dst, const void* src, int n, int batch, int dtype, int direction,
+                    void* stream) {
+  RSIM_CHECK(check_field(src, n, batch, dtype, "src"));
+  RSIM_CHECK(check_field(dst, n, batch, dtype, "dst"));
+  RSIM_CHECK(check_fft_sizes(n, 1));
+  if (direction != RSIM_FORWARD && direction != RSIM_INVERSE)
+    return fail(RSIM_E_INVALID_ARG, "direction must be RSIM_FORWARD or RSIM_INVERSE");
+  const double scale = direction > 0 ? 1.0 / n : 1.0;
+  return fft_axis(dtype, direction, src, dst, n, 1, n, batch, 1, 0, scale, resolve_stream(stream));
+}
+
+int rsim_apply_frequency_cutoff_2d(void* U, int nx, int ny, int dtype, double freq, double dx,
+                                   double dy, void* stream) {
+  RSIM_CHECK(check_field(U, nx, ny, dtype, "U"));
+  if (!(dx > 0) || !(dy > 0)) return fail(RSIM_E_INVALID_ARG, "dx and dy must be positive");
+  return chirp_cutoff(U, nx, ny, dtype, dx, dy, 1.0, 0.0, freq, false, resolve_stream(stream));
+}
+
+int rsim_propagate_2d(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
+                      double dz, double cutoff_freq, int skip_fft, void* stream) {
+  RSIM_CHECK(check_field(u, nx, ny, dtype, "u"));
+  RSIM_CHECK(check_field(U, nx, ny, dtype, "U"));
+  RSIM_CHECK(check_fft_sizes(nx, ny));
+  if (!(dx > 0) || !(dy > 0) || !(wl > 0))
+    return fail(RSIM_E_INVALID_ARG, "dx, dy and wl must be positive");
+  return propagate(u, U, nx, ny, dtype, dx, dy, wl, dz, cutoff_freq, skip_fft != 0,
+                   resolve_stream(stream));
+}
+
+int rsim_propagate_analytically_2d(void* u, int nx, int ny, int dtype, double dx, double dy,
+                                   double z, double x_source, double y_source, double wl,
+                                   void* stream) {
+  RSIM_CHECK(check_field(u, nx, ny, dtype, "u"));
+  if (!(wl > 0)) return fail(RSIM_E_INVALID_ARG, "wl must be positive");
+  return point_source(u, nx, ny, dtype, dx, dy, z, x_source, y_source, wl, resolve_stream(stream));
+}
+
+static int volume_alloc(rsim_volume** vol, int nz, int ny_g, int nx_g, const double* db, int n_db) {
+  if (!vol) return fail(RSIM_E_INVALID_ARG, "vol is NULL");
+  *vol = nullptr;
+  if (nz <= 0 || ny_g < 2 || nx_g < 2)
+    return fail(RSIM_E_INVALID_ARG, "volume must have nz >= 1, ny_g >= 2, nx_g >= 2 (got %d, %d, %d)", nz, ny_g, nx_g);
+  if (!db || n_db < 1 || n_db > 256)
+    return fail(RSIM_E_INVALID_ARG, "deltabeta table must hold 1..256 entries (got %d)", n_db);
+  if (db[0] != 0.0 || db[1] != 0.0)
+    return fail(RSIM_E_INVALID_ARG, "deltabeta entry 0 (vacuum) must be 0");
+  rsim_volume* v = new rsim_volume();
+  v->device = current_device();
+  v->nz = nz;
+  v->ny_g = ny_g;
+  v->nx_g = nx_g;
+  v->n_db = n_db;
+  v->db_host = new double[2 * (size_t)n_db];
+  std::memcpy(v->db_host, db, sizeof(double) * 2 * n_db);
+  const size_t nvox = (size_t)nz * ny_g * nx_g;
+  hipError_t e = hipMalloc((void**)&v->ids, nvox);
+  if (e != hipSuccess) {
+    delete[] v->db_host;
+    delete v;
+    return fail(e == hipErrorOutOfMemory ? RSIM_E_NOMEM : RSIM_E_HIP, "volume allocation of %zu bytes failed: %s", nvox, hipGetErrorString(e));
+  }
+  e = hipMalloc((void**)&v->db_dev, sizeof(double) * 2 * n_db);
+  if (e == hipSuccess) e = hipMemcpy(v->db_dev, db, sizeof(double) * 2 * n_db, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(v->ids);
+    delete[] v->db_host;
+    delete v;
+    return fail(RSIM_E_HIP, "volume table upload failed: %s", hipGetErrorString(e));
+  }
+  *vol = v;
+  return RSIM_OK;
+}
+
+int rsim_volume_create(rsim_volume** vol, const void* ids, int id_bytes, int nz, int ny_g, int nx_g,
+                       const double* deltabeta, int n_db) {
+  if (!ids) return fail(RSIM_E_INVALID_ARG, "ids is NULL");
+  if (id_bytes != 1 && id_bytes != 2 && id_bytes != 4)
+    return fail(RSIM_E_INVALID_ARG, "id_bytes must be 1, 2 or 4 (got %d)", id_bytes);
+  RSIM_CHECK(volume_alloc(vol, nz, ny_g, nx_g, deltabeta, n_db));
+  rsim_volume* v = *vol;
+  const size_t plane = (size_t)ny_g * nx_g;
+  std::vector<uint8_t> buf(plane);
+  for (int z = 0; z < nz; ++z) {
+    const size_t off = (size_t)z * plane;
+    for (size_t i = 0; i < plane; ++i) {
+      uint32_t id;
+      if (id_bytes == 1) id = ((const uint8_t*)ids)[off + i];
+      else if (id_bytes == 2) id = ((const uint16_t*)ids)[off + i];
+      else id = ((const uint32_t*)ids)[off + i];
+      if (id >= (uint32_t)n_db) {
+        rsim_volume_destroy(v);
+        *vol = nullptr;
+        return fail(RSIM_E_INVALID_ARG, "material id %u at voxel %zu is outside the %d-entry deltabeta table", id, off + i, n_db);
+      }
+      buf[i] = (uint8_t)id;
+    }
+    hipError_t e = hipMemcpy(v->ids + off, buf.data(), plane, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      rsim_volume_destroy(v);
+      *vol = nullptr;
+      return fail(RSIM_E_HIP, "volume upload failed: %s", hipGetErrorString(e));
+    }
+  }
+  return RSIM_OK;
+}
+
+int rsim_volume_create_empty(rsim_volume** vol, int nz, int ny_g, int nx_g, const double* deltabeta,
+                             int n_db) {
+  RSIM_CHECK(volume_alloc(vol, nz, ny_g, nx_g, deltabeta, n_db));
+  RSIM_HIP(hipMemset((*vol)->ids, 0, (size_t)nz * ny_g * nx_g));
+  return RSIM_OK;
+}
+
+int rsim_volume_destroy(rsim_volume* vol) {
+  if (!vol) return RSIM_OK;
+  hipFree(vol->ids);
+  hipFree(vol->db_dev);
+  delete[] vol->db_host;
+  delete vol;
+  return RSIM_OK;
+}
+
+int rsim_volume_info(const rsim_volume* vol, void** dev_ids, int* nz, int* ny_g, int* nx_g) {
+  if (!vol) return fail(RSIM_E_INVALID_ARG, "vol is NULL");
+  if (dev_ids) *dev_ids = vol->ids;
+  if (nz) *nz = vol->nz;
+  if (ny_g) *ny_g = vol->ny_g;
+  if (nx_g) *nx_g = vol->nx_g;
+  return RSIM_OK;
+}
+
+int rsim_volume_paint_spheres(rsim_volume* vol, const double* spheres, const int* mats, int n,
+                              void* stream) {
+  if (!vol) return fail(RSIM_E_INVALID_ARG, "vol is NULL");
+  if (n < 0 || (n > 0 && (!spheres || !mats))) return fail(RSIM_E_INVALID_ARG, "bad sphere list");
+  return paint_spheres(vol, spheres, mats, n, resolve_stream(stream));
+}
+
+static int check_sample(const rsim_volume* vol, int slice0, int count, double px, double py,
+                        double pz) {
+  if (!vol) return fail(RSIM_E_INVALID_ARG, "vol is NULL");
+  if (!(px > 0) || !(py > 0) || !(pz > 0))
+    return fail(RSIM_E_INVALID_ARG, "pixel sizes must be positive");
+  if (slice0 < 0 || count < 0 || slice0 + count > vol->nz)
+    return fail(RSIM_E_INVALID_ARG, "invalid Sample slice range %d..%d of %d", slice0, slice0 + count, vol->nz);
+  if (vol->device != current_device())
+    return fail(RSIM_E_INVALID_ARG, "volume lives on device %d, current device is %d", vol->device, current_device());
+  return RSIM_OK;
+}
+
+int rsim_transmission_2d(void* u, int nx, int ny, int dtype, double dx, double dy, double wl,
+                         const rsim_volume* vol, int slice, double pixel_size_x,
+                         double pixel_size_y, double pixel_size_z, double x_pos, double y_pos,
+                         void* stream) {
+  RSIM_CHECK(check_field(u, nx, ny, dtype, "u"));
+  RSIM_CHECK(check_sample(vol, slice, 1, pixel_size_x, pixel_size_y, pixel_size_z));
+  if (!(wl > 0)) return fail(RSIM_E_INVALID_ARG, "wl must be positive");
+  return transmission(u, nx, ny, dtype, dx, dy, wl, vol, slice, pixel_size_x, pixel_size_y,
+                      pixel_size_z, x_pos, y_pos, resolve_stream(stream));
+}
+
+int rsim_sample_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
+                         double wl, double cutoff_freq, const rsim_volume* vol,
+                         double pixel_size_x, double pixel_size_y, double pixel_size_z,
+                         double x_pos, double y_pos, int first_slice, int n_slices, int engine,
+                         int flags, void* stream) {
+  RSIM_CHECK(check_field(u, nx, ny, dtype, "u"));
+  RSIM_CHECK(check_field(U, nx, ny, dtype, "U"));
+  RSIM_CHECK(check_fft_sizes(nx, ny));
+  RSIM_CHECK(check_sample(vol, first_slice, n_slices, pixel_size_x, pixel_size_y, pixel_size_z));
+  if (engine != RSIM_PLAIN && engine != RSIM_FUSED)
+    return fail(RSIM_E_INVALID_ARG, "engine must be RSIM_PLAIN or RSIM_FUSED");
+  if (!(wl > 0) || !(dx > 0) || !(dy > 0)) return fail(RSIM_E_INVALID_ARG, "dx, dy and wl must be positive");
+  return sample_apply(u, U, nx, ny, dtype, dx, dy, wl, cutoff_freq, vol, pixel_size_x,
+                      pixel_size_y, pixel_size_z, x_pos, y_pos, first_slice, n_slices, engine,
+                      flags, resolve_stream(stream));
+}
+
+int rsim_detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double dy,
+                          int out_x, int out_y, double ds_x, double ds_y, double z,
+                          double* out_host, void* stream) {
+  RSIM_CHECK(check_field(u, nx, ny, dtype, "u"));
+  if (!out_host) return fail(RSIM_E_INVALID_ARG, "out_host is NULL");
+  if (!(dx > 0) || !(dy > 0) || !(ds_x > 0) || !(ds_y > 0))
+    return fail(RSIM_E_INVALID_ARG, "2D detector grid and pixel spacings must be positive");
+  hipStream_t s = resolve_stream(stream);
+  const size_t bytes = sizeof(double) * (size_t)out_x * (size_t)(out_y > 0 ? out_y : 0);
+  double* d = nullptr;
+  if (out_x > 0 && out_y > 0) RSIM_HIP(hipMallocAsync((void**)&d, bytes, s));
+  int rc = detector_area_v1(u, nx, ny, dtype, dx, dy, out_x, out_y, ds_x, ds_y, z, 1.0, false, d, s);
+  if (rc == RSIM_OK) {
+    RSIM_HIP(hipMemcpyAsync(out_host, d, bytes, hipMemcpyDeviceToHost, s));
+    RSIM_HIP(hipStreamSynchronize(s));
+  }
+  if (d) hipFreeAsync(d, s);
+  return rc;
+}
+
+int rsim_detector_area_v1_accumulate(const void* u, int nx, int ny, int dtype, double dx, double dy,
+                                     int out_x, int out_y, double ds_x, double ds_y, double z,
+                                     double weight, double* out_dev, void* stream) {
+  RSIM_CHECK(check_field(u, nx, ny, dtype, "u"));
+  if (!out_dev) return fail(RSIM_E_INVALID_ARG, "out_dev is NULL");
+  if (!(dx > 0) || !(dy > 0) || !(ds_x > 0) || !(ds_y > 0))
+    return fail(RSIM_E_INVALID_ARG, "2D detector grid and pixel spacings must be positive");
+  return detector_area_v1(u, nx, ny, dtype, dx, dy, out_x, out_y, ds_x, ds_y, z, weight, true,
+                          out_dev, resolve_stream(stream));
+}
+
+}  // extern "C"

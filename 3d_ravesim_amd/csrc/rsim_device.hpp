@@ -1,0 +1,195 @@
+// Device-side building blocks: complex arithmetic and small DFT butterflies.
+//
+// Sign convention: DIR = -1 is the forward (unnormalised) transform
+// X[k] = sum_n x[n] exp(-2*pi*i*n*k/N); DIR = +1 the inverse kernel
+// exp(+2*pi*i*n*k/N).  This is scipy.fft's convention, which big-wave's
+// NumpyVector uses (big-wave/vector.py:341-347).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rsim {
+
+template <typename R>
+struct alignas(2 * sizeof(R)) cpx {
+  R x, y;
+};
+
+template <typename R>
+__host__ __device__ __forceinline__ cpx<R> mk(R a, R b) {
+  cpx<R> r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> operator+(cpx<R> a, cpx<R> b) { return mk(a.x + b.x, a.y + b.y); }
+template <typename R>
+__device__ __forceinline__ cpx<R> operator-(cpx<R> a, cpx<R> b) { return mk(a.x - b.x, a.y - b.y); }
+template <typename R>
+__device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+template <typename R>
+__device__ __forceinline__ cpx<R> cmulc(cpx<R> a, cpx<R> b) {
+  return mk(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> cscale(cpx<R> a, R s) { return mk(a.x * s, a.y * s); }
+// a * (c + i s)
+template <typename R>
+__device__ __forceinline__ cpx<R> cmulcs(cpx<R> a, R c, R s) {
+  return mk(a.x * c - a.y * s, a.x * s + a.y * c);
+}
+// a * (DIR * i)
+template <int DIR, typename R>
+__device__ __forceinline__ cpx<R> rot(cpx<R> a) {
+  return DIR < 0 ? mk(a.y, -a.x) : mk(-a.y, a.x);
+}
+// twiddle application: forward uses the table value w, inverse conj(w)
+template <int DIR, typename R>
+__device__ __forceinline__ cpx<R> twmul(cpx<R> a, cpx<R> w) {
+  return DIR < 0 ? cmul(a, w) : cmulc(a, w);
+}
+
+// ---------------------------------------------------------------------------
+// Butterflies: in-place DFT of size R on v[0..R-1] (natural order in and out).
+// ---------------------------------------------------------------------------
+
+template <int DIR, typename R>
+__device__ __forceinline__ void dft2(cpx<R>* v) {
+  cpx<R> a = v[0], b = v[1];
+  v[0] = a + b;
+  v[1] = a - b;
+}
+
+template <int DIR, typename R>
+__device__ __forceinline__ void dft3(cpx<R>* v) {
+  const R c = R(-0.5);
+  const R s = R(DIR) * R(0.86602540378443864676372317075294);
+  cpx<R> a = v[1] + v[2];
+  cpx<R> b = v[1] - v[2];
+  cpx<R> m = mk(v[0].x + c * a.x, v[0].y + c * a.y);
+  cpx<R> t = mk(-s * b.y, s * b.x);  // i*s*b
+  v[0] = v[0] + a;
+  v[1] = m + t;
+  v[2] = m - t;
+}
+
+template <int DIR, typename R>
+__device__ __forceinline__ void dft4(cpx<R>* v) {
+  cpx<R> t0 = v[0] + v[2], t1 = v[0] - v[2];
+  cpx<R> t2 = v[1] + v[3], t3 = rot<DIR>(v[1] - v[3]);
+  v[0] = t0 + t2;
+  v[2] = t0 - t2;
+  v[1] = t1 + t3;
+  v[3] = t1 - t3;
+}
+
+template <int DIR, typename R>
+__device__ __forceinline__ void dft5(cpx<R>* v) {
+  const R c1 = R(0.30901699437494742410229341718282);   // cos(2pi/5)
+  const R c2 = R(-0.80901699437494742410229341718282);  // cos(4pi/5)
+  const R s1 = R(DIR) * R(0.95105651629515357211643933337938);
+  const R s2 = R(DIR) * R(0.58778525229247312916870595463907);
+  cpx<R> a1 = v[1] + v[4], b1 = v[1] - v[4];
+  cpx<R> a2 = v[2] + v[3], b2 = v[2] - v[3];
+  cpx<R> x0 = v[0];
+  cpx<R> m1 = mk(x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y);
+  cpx<R> m2 = mk(x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y);
+  cpx<R> n1 = mk(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y);
+  cpx<R> n2 = mk(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y);
+  // i * n
+  cpx<R> in1 = mk(-n1.y, n1.x), in2 = mk(-n2.y, n2.x);
+  v[0] = x0 + a1 + a2;
+  v[1] = m1 + in1;
+  v[4] = m1 - in1;
+  v[2] = m2 + in2;
+  v[3] = m2 - in2;
+}
+
+template <int DIR, typename R>
+__device__ __forceinline__ void dft8(cpx<R>* v) {
+  const R h = R(0.70710678118654752440084436210485);
+  cpx<R> e[4] = {v[0], v[2], v[4], v[6]};
+  cpx<R> o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<DIR>(e);
+  dft4<DIR>(o);
+  // W8^1 = (h, DIR*h), W8^2 = DIR*i, W8^3 = W8^2 * W8^1
+  cpx<R> o1 = cmulcs(o[1], h, R(DIR) * h);
+  cpx<R> o2 = rot<DIR>(o[2]);
+  cpx<R> o3 = rot<DIR>(cmulcs(o[3], h, R(DIR) * h));
+  v[0] = e[0] + o[0];
+  v[4] = e[0] - o[0];
+  v[1] = e[1] + o1;
+  v[5] = e[1] - o1;
+  v[2] = e[2] + o2;
+  v[6] = e[2] - o2;
+  v[3] = e[3] + o3;
+  v[7] = e[3] - o3;
+}
+
+template <int DIR, typename R>
+__device__ __forceinline__ void dft16(cpx<R>* v) {
+  // 16 = 4 x 4: Y[n2][k1] = DFT4 over n1 of x[4 n1 + n2], twiddle W16^(n2 k1),
+  // X[k1 + 4 k2] = DFT4 over n2.
+  const R c1 = R(0.92387953251128675612818318939679);  // cos(pi/8)
+  const R s1 = R(0.38268343236508977172845998403040);  // sin(pi/8)
+  const R h = R(0.70710678118654752440084436210485);
+  cpx<R> y[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    y[n2][0] = v[n2];
+    y[n2][1] = v[4 + n2];
+    y[n2][2] = v[8 + n2];
+    y[n2][3] = v[12 + n2];
+    dft4<DIR>(y[n2]);
+  }
+  const R d = R(DIR);
+  // n2 = 1: W^1, W^2, W^3
+  y[1][1] = cmulcs(y[1][1], c1, d * s1);
+  y[1][2] = cmulcs(y[1][2], h, d * h);
+  y[1][3] = cmulcs(y[1][3], s1, d * c1);
+  // n2 = 2: W^2, W^4, W^6
+  y[2][1] = cmulcs(y[2][1], h, d * h);
+  y[2][2] = rot<DIR>(y[2][2]);
+  y[2][3] = rot<DIR>(cmulcs(y[2][3], h, d * h));
+  // n2 = 3: W^3, W^6, W^9 = -W^1
+  y[3][1] = cmulcs(y[3][1], s1, d * c1);
+  y[3][2] = rot<DIR>(cmulcs(y[3][2], h, d * h));
+  y[3][3] = cmulcs(y[3][3], -c1, -d * s1);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    cpx<R> z[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+    dft4<DIR>(z);
+    v[k1] = z[0];
+    v[k1 + 4] = z[1];
+    v[k1 + 8] = z[2];
+    v[k1 + 12] = z[3];
+  }
+}
+
+template <int RADIX, int DIR, typename R>
+__device__ __forceinline__ void dft(cpx<R>* v) {
+  if constexpr (RADIX == 2) dft2<DIR>(v);
+  else if constexpr (RADIX == 3) dft3<DIR>(v);
+  else if constexpr (RADIX == 4) dft4<DIR>(v);
+  else if constexpr (RADIX == 5) dft5<DIR>(v);
+  else if constexpr (RADIX == 8) dft8<DIR>(v);
+  else if constexpr (RADIX == 16) dft16<DIR>(v);
+}
+
+// sincos of a phase already reduced to [-pi, pi] (single precision output).
+__device__ __forceinline__ void sincos_reduced(double ph, float* s, float* c) {
+  sincosf((float)ph, s, c);
+}
+
+// Reduce an FP64 phase to [-pi, pi] before a single-precision sincos.
+__device__ __forceinline__ double reduce_2pi(double ph) {
+  const double two_pi = 6.283185307179586476925286766559;
+  return ph - two_pi * rint(ph / two_pi);
+}
+
+}  // namespace rsim

@@ -1,0 +1,302 @@
+// Stockham autosort FFT kernels for gfx950.
+//
+// Two kernel families share one stage formula.  For a length-N transform
+// factored N = R_0 * R_1 * ... and stage s with radix R and stride
+// L = R_0*...*R_{s-1}, butterfly b in [0, N/R) with k = b mod L reads
+//   x_j = in[b + j*N/R] * W_{L*R}^{j*k}        (j in [0, R))
+// applies a length-R DFT and writes
+//   out[(b - k)*R + k + m*L] = y_m              (m in [0, R)).
+// After the last stage the data are in natural order.
+//
+// * fft_lds_kernel: runtime factorization (radix 2,3,4,5,8,16), the whole
+//   tile of C sequences staged in LDS with ping-pong buffers.  Handles every
+//   supported size up to the LDS capacity; used for small and uncommon sizes.
+// * reg_fft (fft_reg_kernel and the fused multislice kernels): compile-time
+//   factorization, every thread keeps E = N/TC elements in VGPRs, LDS is only
+//   the exchange buffer between stages (optionally re/im split to halve it).
+//   The first stage loads from HBM and the last stage stores to HBM directly,
+//   and pointwise work (transmission, chirp, scaling) is applied in registers.
+#pragma once
+
+#include "rsim_device.hpp"
+
+namespace rsim {
+
+constexpr int kMaxStages = 12;
+
+struct AxisDesc {
+  int N;
+  int nst;
+  int radix[kMaxStages];
+  int ls[kMaxStages];
+  long long es_in, es_out;   // element stride along the transform
+  long long ss_in, ss_out;   // stride between neighbouring sequences
+  long long bs_in, bs_out;   // stride between batches (grid.y)
+  int nseq;                  // sequences per batch
+  int C;                     // sequences per workgroup tile
+  double scale;              // output multiplier (1 or 1/N)
+  const void* tw;            // W_N^m, m in [0, N), forward sign, in the transform's precision
+};
+
+template <int RR, int DIR, typename R>
+__device__ __forceinline__ void lds_bfly(const cpx<R>* __restrict__ src, cpx<R>* __restrict__ dst,
+                                         const cpx<R>* __restrict__ tw, int N, int LS, int C, int c,
+                                         int b) {
+  const int nb = N / RR;
+  const int k = b % LS;
+  cpx<R> x[RR];
+#pragma unroll
+  for (int j = 0; j < RR; ++j) x[j] = src[(b + j * nb) * C + c];
+  if (LS > 1) {
+    const int step = N / (LS * RR);
+#pragma unroll
+    for (int j = 1; j < RR; ++j) x[j] = twmul<DIR>(x[j], tw[j * k * step]);
+  }
+  dft<RR, DIR>(x);
+  const int base = (b - k) * RR + k;
+#pragma unroll
+  for (int m = 0; m < RR; ++m) dst[(base + m * LS) * C + c] = x[m];
+}
+
+template <typename R, int DIR>
+__global__ void __launch_bounds__(256) fft_lds_kernel(const cpx<R>* __restrict__ in,
+                                                      cpx<R>* __restrict__ out, AxisDesc d) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<R>* buf0 = reinterpret_cast<cpx<R>*>(smem_raw);
+  cpx<R>* buf1 = buf0 + (size_t)d.C * d.N;
+  const cpx<R>* tw = reinterpret_cast<const cpx<R>*>(d.tw);
+  const int T = blockDim.x;
+  const int tid = threadIdx.x;
+  const int N = d.N, C = d.C;
+  const int seq0 = blockIdx.x * C;
+  const long long boff_in = (long long)blockIdx.y * d.bs_in;
+  const long long boff_out = (long long)blockIdx.y * d.bs_out;
+  const bool pos_fast_in = d.es_in <= d.ss_in;
+  const bool pos_fast_out = d.es_out <= d.ss_out;
+  const int total = C * N;
+
+  for (int idx = tid; idx < total; idx += T) {
+    int pos, c;
+    if (pos_fast_in) { pos = idx % N; c = idx / N; }
+    else { c = idx % C; pos = idx / C; }
+    const int seq = seq0 + c;
+    cpx<R> v = mk(R(0), R(0));
+    if (seq < d.nseq) v = in[boff_in + (long long)seq * d.ss_in + (long long)pos * d.es_in];
+    buf0[pos * C + c] = v;
+  }
+  __syncthreads();
+
+  cpx<R>* src = buf0;
+  cpx<R>* dst = buf1;
+  for (int s = 0; s < d.nst; ++s) {
+    const int RR = d.radix[s];
+    const int LS = d.ls[s];
+    const int work = C * (N / RR);
+    for (int w = tid; w < work; w += T) {
+      const int c = w % C;
+      const int b = w / C;
+      switch (RR) {
+        case 2: lds_bfly<2, DIR>(src, dst, tw, N, LS, C, c, b); break;
+        case 3: lds_bfly<3, DIR>(src, dst, tw, N, LS, C, c, b); break;
+        case 4: lds_bfly<4, DIR>(src, dst, tw, N, LS, C, c, b); break;
+        case 5: lds_bfly<5, DIR>(src, dst, tw, N, LS, C, c, b); break;
+        case 8: lds_bfly<8, DIR>(src, dst, tw, N, LS, C, c, b); break;
+        default: lds_bfly<16, DIR>(src, dst, tw, N, LS, C, c, b); break;
+      }
+    }
+    __syncthreads();
+    cpx<R>* t = src;
+    src = dst;
+    dst = t;
+  }
+
+  const R sc = R(d.scale);
+  for (int idx = tid; idx < total; idx += T) {
+    int pos, c;
+    if (pos_fast_out) { pos = idx % N; c = idx / N; }
+    else { c = idx % C; pos = idx / C; }
+    const int seq = seq0 + c;
+    if (seq < d.nseq) {
+      cpx<R> v = src[pos * C + c];
+      if (d.scale != 1.0) v = cscale(v, sc);
+      out[boff_out + (long long)seq * d.ss_out + (long long)pos * d.es_out] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Register-resident FFT (compile-time factorization)
+// ---------------------------------------------------------------------------
+
+// Factorization descriptor: N = R0*R1*R2*R3 (trailing 1s unused), E elements
+// per thread, TC = N / E threads per sequence.  E must be a multiple of
+// every radix.
+template <int N_, int E_, int R0_, int R1_ = 1, int R2_ = 1, int R3_ = 1>
+struct RegFac {
+  static constexpr int N = N_;
+  static constexpr int E = E_;
+  static constexpr int TC = N_ / E_;
+  static constexpr int R0 = R0_, R1 = R1_, R2 = R2_, R3 = R3_;
+  static constexpr int NS = 1 + (R1_ > 1) + (R2_ > 1) + (R3_ > 1);
+  static_assert(R0_ * R1_ * R2_ * R3_ == N_, "radices must multiply to N");
+  static_assert(E_ % R0_ == 0 && E_ % R1_ == 0 && E_ % R2_ == 0 && E_ % R3_ == 0,
+                "E must be a multiple of every radix");
+  static constexpr int radix(int s) { return s == 0 ? R0 : s == 1 ? R1 : s == 2 ? R2 : R3; }
+  static constexpr int stride(int s) {
+    return s == 0 ? 1 : s == 1 ? R0 : s == 2 ? R0 * R1 : R0 * R1 * R2;
+  }
+  static constexpr int last_radix() { return radix(NS - 1); }
+};
+
+// Twiddle tables for a RegFac: table s (s >= 1) holds W_{L_s R_s}^m for
+// m in [0, L_s R_s), forward sign.
+template <typename R>
+struct StageTw {
+  const cpx<R>* t[4];
+};
+
+// LDS index with one pad slot every 2^PS slots (PS = 0: no padding).
+template <int PS>
+__device__ __forceinline__ int lpad(int e) {
+  if constexpr (PS == 0) return e;
+  else return e + (e >> PS);
+}
+
+template <class F, int C, int PS>
+struct LdsSize {
+  static constexpr int slots = F::N * C + ((PS == 0) ? 0 : ((F::N * C) >> PS)) + 1;
+};
+
+// Output position of element (i, m) of stage s (radix RR, stride LS).
+template <class F, int RR, int LS>
+__device__ __forceinline__ int out_pos(int t, int i, int m) {
+  const int b = t + i * F::TC;
+  const int k = b % LS;
+  return (b - k) * RR + k + m * LS;
+}
+template <class F, int RR>
+__device__ __forceinline__ int in_pos(int t, int i, int j) {
+  return t + i * F::TC + j * (F::N / RR);
+}
+
+// Exchange between stage S-1 (outputs in v) and stage S (inputs into v).
+template <class F, int S, int C, int PS, bool SPLIT, typename R>
+__device__ __forceinline__ void reg_exchange(cpx<R> (&v)[F::E], void* lds_raw, int t, int c) {
+  constexpr int RP = F::radix(S - 1);
+  constexpr int LP = F::stride(S - 1);
+  constexpr int RN = F::radix(S);
+  if constexpr (!SPLIT) {
+    cpx<R>* lds = reinterpret_cast<cpx<R>*>(lds_raw);
+#pragma unroll
+    for (int i = 0; i < F::E / RP; ++i)
+#pragma unroll
+      for (int m = 0; m < RP; ++m) lds[lpad<PS>(out_pos<F, RP, LP>(t, i, m) * C + c)] = v[i * RP + m];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < F::E / RN; ++i)
+#pragma unroll
+      for (int j = 0; j < RN; ++j) v[i * RN + j] = lds[lpad<PS>(in_pos<F, RN>(t, i, j) * C + c)];
+    __syncthreads();
+  } else {
+    R* lds = reinterpret_cast<R*>(lds_raw);
+#pragma unroll
+    for (int i = 0; i < F::E / RP; ++i)
+#pragma unroll
+      for (int m = 0; m < RP; ++m) lds[lpad<PS>(out_pos<F, RP, LP>(t, i, m) * C + c)] = v[i * RP + m].x;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < F::E / RN; ++i)
+#pragma unroll
+      for (int j = 0; j < RN; ++j) v[i * RN + j].x = lds[lpad<PS>(in_pos<F, RN>(t, i, j) * C + c)];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < F::E / RP; ++i)
+#pragma unroll
+      for (int m = 0; m < RP; ++m) lds[lpad<PS>(out_pos<F, RP, LP>(t, i, m) * C + c)] = v[i * RP + m].y;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < F::E / RN; ++i)
+#pragma unroll
+      for (int j = 0; j < RN; ++j) v[i * RN + j].y = lds[lpad<PS>(in_pos<F, RN>(t, i, j) * C + c)];
+    __syncthreads();
+  }
+}
+
+template <class F, int S, int DIR, typename R>
+__device__ __forceinline__ void reg_stage(cpx<R> (&v)[F::E], const StageTw<R>& tw, int t) {
+  constexpr int RR = F::radix(S);
+  constexpr int LS = F::stride(S);
+  if constexpr (LS > 1) {
+    const cpx<R>* T = tw.t[S];
+#pragma unroll
+    for (int i = 0; i < F::E / RR; ++i) {
+      const int b = t + i * F::TC;
+      const int k = b % LS;
+#pragma unroll
+      for (int j = 1; j < RR; ++j) v[i * RR + j] = twmul<DIR>(v[i * RR + j], T[j * k]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < F::E / RR; ++i) dft<RR, DIR>(&v[i * RR]);
+}
+
+// Full transform on register-resident data.  On entry v holds the stage-0
+// inputs (v[i*R0 + j] = x[t + i*TC + j*N/R0]); on exit v holds the outputs
+// of the last stage (v[i*RL + m] = X[t + i*TC + m*N/RL]).
+template <class F, int DIR, int C, int PS, bool SPLIT, typename R>
+__device__ __forceinline__ void reg_fft(cpx<R> (&v)[F::E], const StageTw<R>& tw, void* lds, int t,
+                                        int c) {
+  reg_stage<F, 0, DIR>(v, tw, t);
+  if constexpr (F::NS > 1) {
+    reg_exchange<F, 1, C, PS, SPLIT>(v, lds, t, c);
+    reg_stage<F, 1, DIR>(v, tw, t);
+  }
+  if constexpr (F::NS > 2) {
+    reg_exchange<F, 2, C, PS, SPLIT>(v, lds, t, c);
+    reg_stage<F, 2, DIR>(v, tw, t);
+  }
+  if constexpr (F::NS > 3) {
+    reg_exchange<F, 3, C, PS, SPLIT>(v, lds, t, c);
+    reg_stage<F, 3, DIR>(v, tw, t);
+  }
+}
+
+// Position of register e on entry (stage-0 input layout).
+template <class F>
+__device__ __forceinline__ int first_pos(int t, int e) {
+  constexpr int R0 = F::R0;
+  return t + (e / R0) * F::TC + (e % R0) * (F::N / R0);
+}
+// Position of register e on exit (last-stage output layout).
+template <class F>
+__device__ __forceinline__ int last_pos(int t, int e) {
+  constexpr int RL = F::last_radix();
+  return t + (e / RL) * F::TC + (e % RL) * (F::N / RL);
+}
+
+// Plain single-direction transform along one axis, C sequences per WG.
+// Sequence q = blockIdx.x*C + c, element p at base + q*ss + p*es.
+template <typename R, class F, int DIR, int C, int PS, bool SPLIT>
+__global__ void __launch_bounds__(F::TC * C) fft_reg_kernel(const cpx<R>* __restrict__ in,
+                                                            cpx<R>* __restrict__ out, long long es,
+                                                            long long ss, long long bs, int nseq,
+                                                            StageTw<R> tw, R scale) {
+  __shared__ unsigned char lds[LdsSize<F, C, PS>::slots * (SPLIT ? sizeof(R) : sizeof(cpx<R>))];
+  const int tid = threadIdx.x;
+  const int c = tid % C;
+  const int t = tid / C;
+  const int q = blockIdx.x * C + c;
+  const bool live = q < nseq;
+  const long long base = (long long)blockIdx.y * bs + (long long)(live ? q : 0) * ss;
+  cpx<R> v[F::E];
+#pragma unroll
+  for (int e = 0; e < F::E; ++e)
+    v[e] = live ? in[base + (long long)first_pos<F>(t, e) * es] : mk(R(0), R(0));
+  reg_fft<F, DIR, C, PS, SPLIT>(v, tw, lds, t, c);
+  if (live) {
+#pragma unroll
+    for (int e = 0; e < F::E; ++e) out[base + (long long)last_pos<F>(t, e) * es] = cscale(v[e], scale);
+  }
+}
+
+}  // namespace rsim

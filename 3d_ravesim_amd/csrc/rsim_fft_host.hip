@@ -1,0 +1,264 @@
+// FFT planning and dispatch: twiddle tables (FP64-computed, rounded once to
+// the transform precision), schedule selection and 2D row/column passes.
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "rsim_fft_cfg.hpp"
+#include "rsim_internal.hpp"
+
+namespace rsim {
+
+// Generic LDS kernel instantiations.
+template __global__ void fft_lds_kernel<float, -1>(const cpx<float>*, cpx<float>*, AxisDesc);
+template __global__ void fft_lds_kernel<float, 1>(const cpx<float>*, cpx<float>*, AxisDesc);
+template __global__ void fft_lds_kernel<double, -1>(const cpx<double>*, cpx<double>*, AxisDesc);
+template __global__ void fft_lds_kernel<double, 1>(const cpx<double>*, cpx<double>*, AxisDesc);
+
+namespace {
+
+struct Tables {
+  void* full = nullptr;          // W_N^m, m in [0, N)
+  void* stage[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool reg = false;
+};
+
+std::mutex g_mu;
+std::map<std::tuple<int, int, int>, Tables> g_tables;  // (device, dtype, N)
+
+// exp(-2*pi*i*m/M) with the angle reduced to the first octant so the FP64
+// value is correct to a few ulp for every m.
+void twiddle(long long m, long long M, double* re, double* im) {
+  m %= M;
+  if (m < 0) m += M;
+  // use symmetry: angle = 2*pi*m/M; reduce via (8m)/M octant index
+  const double pi = 3.14159265358979323846264338327950288;
+  long long m8 = 8 * m;
+  long long oct = m8 / M;            // 0..7
+  long long rem = m8 - oct * M;      // 8m mod M
+  double a = (pi / 4.0) * ((double)rem / (double)M);   // in [0, pi/4)
+  double c, s;
+  // angle = oct*pi/4 + a; for odd octants use the complement for accuracy
+  if (oct & 1) {
+    double b = (pi / 4.0) * ((double)(M - rem) / (double)M);  // pi/4 - a
+    double cb = std::cos(b), sb = std::sin(b);
+    // angle = (oct+1)*pi/4 - b
+    long long q = (oct + 1);
+    // cos/sin of q*pi/4 - b for q even (2,4,6,8)
+    switch (q % 8) {
+      case 2: c = sb; s = cb; break;           // pi/2 - b
+      case 4: c = -cb; s = sb; break;          // pi - b
+      case 6: c = -sb; s = -cb; break;         // 3pi/2 - b
+      default: c = cb; s = -sb; break;         // 2pi - b
+    }
+  } else {
+    double ca = std::cos(a), sa = std::sin(a);
+    switch (oct) {
+      case 0: c = ca; s = sa; break;
+      case 2: c = -sa; s = ca; break;          // pi/2 + a
+      case 4: c = -ca; s = -sa; break;         // pi + a
+      default: c = sa; s = -ca; break;         // 3pi/2 + a
+    }
+  }
+  *re = c;
+  *im = -s;
+}
+
+int upload_table(int dtype, long long M, void** out) {
+  std::vector<double> d(2 * M);
+  for (long long m = 0; m < M; ++m) twiddle(m, M, &d[2 * m], &d[2 * m + 1]);
+  size_t bytes;
+  if (dtype == RSIM_C128) {
+    bytes = 16 * M;
+    RSIM_HIP(hipMalloc(out, bytes));
+    RSIM_HIP(hipMemcpy(*out, d.data(), bytes, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> f(2 * M);
+    for (long long i = 0; i < 2 * M; ++i) f[i] = (float)d[i];
+    bytes = 8 * M;
+    RSIM_HIP(hipMalloc(out, bytes));
+    RSIM_HIP(hipMemcpy(*out, f.data(), bytes, hipMemcpyHostToDevice));
+  }
+  return RSIM_OK;
+}
+
+int get_tables(int dtype, int N, Tables** out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto key = std::make_tuple(current_device(), dtype, N);
+  auto it = g_tables.find(key);
+  if (it != g_tables.end()) {
+    *out = &it->second;
+    return RSIM_OK;
+  }
+  Tables t;
+  RSIM_CHECK(upload_table(dtype, N, &t.full));
+  SchedInfo si;
+  if (sched_info(N, &si)) {
+    t.reg = true;
+    int ls = 1;
+    for (int s = 0; s < si.nst; ++s) {
+      if (s > 0) RSIM_CHECK(upload_table(dtype, (long long)ls * si.radix[s], &t.stage[s]));
+      ls *= si.radix[s];
+    }
+  }
+  auto res = g_tables.emplace(key, t);
+  *out = &res.first->second;
+  return RSIM_OK;
+}
+
+// Greedy mixed-radix factorization for the generic kernel.
+bool factorize(int n, std::vector<int>* r) {
+  r->clear();
+  int m = n;
+  while (m % 16 == 0 && m > 16) { r->push_back(16); m /= 16; }
+  const int cand[] = {16, 8, 4, 2, 5, 3};
+  while (m > 1) {
+    bool ok = false;
+    for (int c : cand) {
+      if (m % c == 0) {
+        r->push_back(c);
+        m /= c;
+        ok = true;
+        break;
+      }
+    }
+    if (!ok) return false;
+  }
+  if (r->empty()) r->push_back(1);
+  return true;
+}
+
+}  // namespace
+
+bool fft_size_supported(int n) {
+  if (n < 1) return false;
+  std::vector<int> r;
+  return factorize(n, &r);
+}
+
+bool reg_stage_tables(int dtype, int N, const void* out[4]) {
+  Tables* t = nullptr;
+  if (get_tables(dtype, N, &t) != RSIM_OK || !t->reg) return false;
+  for (int i = 0; i < 4; ++i) out[i] = t->stage[i];
+  return true;
+}
+
+int rsim_release_tables() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int dev = current_device();
+  for (auto it = g_tables.begin(); it != g_tables.end();) {
+    if (std::get<0>(it->first) == dev) {
+      hipFree(it->second.full);
+      for (int i = 0; i < 4; ++i)
+        if (it->second.stage[i]) hipFree(it->second.stage[i]);
+      it = g_tables.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  return RSIM_OK;
+}
+
+int fft_axis(int dtype, int dir, const void* in, void* out, int N, long long es, long long ss,
+             int nseq, int batch, long long bs, double scale, hipStream_t s) {
+  if (N == 1) {
+    // length-1 transform is the identity (times scale)
+    if (in != out || scale != 1.0) {
+      // handled by the generic kernel below (cheap and rare)
+    } else {
+      return RSIM_OK;
+    }
+  }
+  Tables* t = nullptr;
+  RSIM_CHECK(get_tables(dtype, N, &t));
+  const bool col = es != 1;
+  const int kind = col ? KT_COL : KT_ROW;
+  if (t->reg) {
+    ktime_begin(kind, s);
+    const void* tw[4] = {t->stage[0], t->stage[1], t->stage[2], t->stage[3]};
+    bool ok = dtype == RSIM_C64
+                  ? launch_reg_fft_c64(dir, N, col, in, out, es, ss, bs, nseq, batch, tw, scale, s)
+                  : launch_reg_fft_c128(dir, N, col, in, out, es, ss, bs, nseq, batch, tw, scale, s);
+    if (ok) {
+      ktime_end(kind, s);
+      RSIM_LAUNCHED();
+      return RSIM_OK;
+    }
+  }
+  // Generic LDS kernel.
+  std::vector<int> r;
+  if (!factorize(N, &r)) return fail(RSIM_E_INVALID_ARG, "FFT length %d has a prime factor > 5", N);
+  if ((int)r.size() > kMaxStages) return fail(RSIM_E_UNSUPPORTED, "FFT length %d needs too many stages", N);
+  AxisDesc d{};
+  d.N = N;
+  d.nst = (r.size() == 1 && r[0] == 1) ? 0 : (int)r.size();
+  int ls = 1;
+  for (int i = 0; i < d.nst; ++i) {
+    d.radix[i] = r[i];
+    d.ls[i] = ls;
+    ls *= r[i];
+  }
+  d.es_in = d.es_out = es;
+  d.ss_in = d.ss_out = ss;
+  d.bs_in = d.bs_out = bs;
+  d.nseq = nseq;
+  d.scale = scale;
+  d.tw = t->full;
+  const size_t eb = elem_bytes(dtype);
+  const size_t lds_cap = max_lds_per_block();
+  // tile: enough sequences for >= 64 B contiguous runs on strided axes,
+  // bounded by LDS (two ping-pong buffers)
+  int C = col ? 8 : 1;
+  while (C > 1 && (size_t)2 * C * N * eb > lds_cap) C /= 2;
+  if ((size_t)2 * N * eb > lds_cap)
+    return fail(RSIM_E_UNSUPPORTED, "FFT length %d exceeds the single-pass LDS capacity", N);
+  if (!col) {
+    // several short rows per workgroup
+    while (C * N < 1024 && C < 64 && (size_t)4 * C * N * eb <= lds_cap) C *= 2;
+  }
+  d.C = C;
+  const size_t shmem = (size_t)2 * C * N * eb;
+  dim3 grid((nseq + C - 1) / C, batch);
+  dim3 block(256);
+  ktime_begin(kind, s);
+  if (dtype == RSIM_C64) {
+    if (dir < 0) {
+      hipFuncSetAttribute((const void*)fft_lds_kernel<float, -1>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+      hipLaunchKernelGGL((fft_lds_kernel<float, -1>), grid, block, shmem, s,
+                         (const cpx<float>*)in, (cpx<float>*)out, d);
+    } else {
+      hipFuncSetAttribute((const void*)fft_lds_kernel<float, 1>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+      hipLaunchKernelGGL((fft_lds_kernel<float, 1>), grid, block, shmem, s,
+                         (const cpx<float>*)in, (cpx<float>*)out, d);
+    }
+  } else {
+    if (dir < 0) {
+      hipFuncSetAttribute((const void*)fft_lds_kernel<double, -1>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+      hipLaunchKernelGGL((fft_lds_kernel<double, -1>), grid, block, shmem, s,
+                         (const cpx<double>*)in, (cpx<double>*)out, d);
+    } else {
+      hipFuncSetAttribute((const void*)fft_lds_kernel<double, 1>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+      hipLaunchKernelGGL((fft_lds_kernel<double, 1>), grid, block, shmem, s,
+                         (const cpx<double>*)in, (cpx<double>*)out, d);
+    }
+  }
+  ktime_end(kind, s);
+  RSIM_LAUNCHED();
+  return RSIM_OK;
+}
+
+int fft2(int dtype, int dir, const void* in, void* out, int nx, int ny, hipStream_t s) {
+  const double scale = dir > 0 ? 1.0 / ((double)nx * (double)ny) : 1.0;
+  // rows (contiguous) then columns (stride nx); normalisation on the last pass
+  RSIM_CHECK(fft_axis(dtype, dir, in, out, nx, 1, nx, ny, 1, 0, 1.0, s));
+  RSIM_CHECK(fft_axis(dtype, dir, out, out, ny, nx, 1, nx, 1, 0, scale, s));
+  return RSIM_OK;
+}
+
+}  // namespace rsim

@@ -1,0 +1,91 @@
+// Host-side internals shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/ravesim.h"
+
+namespace rsim {
+
+int fail(int code, const char* fmt, ...);
+void clear_error();
+
+#define RSIM_HIP(call)                                                                     \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return ::rsim::fail(RSIM_E_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                          __FILE__, __LINE__);                                             \
+  } while (0)
+
+#define RSIM_CHECK(expr)          \
+  do {                            \
+    int rc_ = (expr);             \
+    if (rc_ != RSIM_OK) return rc_; \
+  } while (0)
+
+#define RSIM_LAUNCHED()                                                                        \
+  do {                                                                                         \
+    hipError_t e_ = hipGetLastError();                                                         \
+    if (e_ != hipSuccess)                                                                      \
+      return ::rsim::fail(RSIM_E_HIP, "kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), \
+                          __FILE__, __LINE__);                                                 \
+  } while (0)
+
+hipStream_t resolve_stream(void* s);
+int current_device();
+size_t max_lds_per_block();
+
+inline size_t elem_bytes(int dtype) { return dtype == RSIM_C128 ? 16 : 8; }
+inline bool valid_dtype(int dtype) { return dtype == RSIM_C64 || dtype == RSIM_C128; }
+
+// ---- FFT (rsim_fft_host.hip) ----
+bool fft_size_supported(int n);
+// One batch of 1D transforms of length N along an axis.  Sequence q in
+// [0, nseq) of batch b starts at b*bs + q*ss; its elements are es apart.
+int fft_axis(int dtype, int dir, const void* in, void* out, int N, long long es, long long ss,
+             int nseq, int batch, long long bs, double scale, hipStream_t s);
+int fft2(int dtype, int dir, const void* in, void* out, int nx, int ny, hipStream_t s);
+// Per-stage twiddle tables of the register schedule for N (nullptr entries
+// for stages without twiddles); false when N has no register schedule.
+bool reg_stage_tables(int dtype, int N, const void* out[4]);
+
+// ---- launchers implemented in the per-precision register FFT units ----
+bool launch_reg_fft_c64(int dir, int N, bool colmode, const void* in, void* out, long long es,
+                        long long ss, long long bs, int nseq, int batch, const void* const* tw,
+                        double scale, hipStream_t s);
+bool launch_reg_fft_c128(int dir, int N, bool colmode, const void* in, void* out, long long es,
+                         long long ss, long long bs, int nseq, int batch, const void* const* tw,
+                         double scale, hipStream_t s);
+
+// ---- pointwise (rsim_pointwise.hip) ----
+int chirp_cutoff(void* U, int nx, int ny, int dtype, double dx, double dy, double wl, double dz,
+                 double freq, bool chirp, hipStream_t s);
+
+// ---- kernel timing (kinds: 0 row pass, 1 column pass, 2 pointwise) ----
+enum { KT_ROW = 0, KT_COL = 1, KT_PW = 2 };
+bool ktime_on();
+void ktime_begin(int kind, hipStream_t s);
+void ktime_end(int kind, hipStream_t s);
+
+// ---- cancel / progress ----
+bool cancel_requested();
+// true when a progress callback is installed (the slice loop then waits for
+// each slice so the reported progress is real)
+bool g_progress_sync();
+void report_progress(const char* stage, long long completed, long long total);
+
+}  // namespace rsim
+
+struct rsim_volume {
+  int device;
+  int nz, ny_g, nx_g;
+  uint8_t* ids;      // device, nz*ny_g*nx_g
+  int n_db;
+  double* db_host;   // 2*n_db (re, im), entry 0 = vacuum
+  double* db_dev;    // device copy of db_host
+};

@@ -1,0 +1,365 @@
+// Pointwise and reduction kernels of the plain engine, evaluated in FP64 in
+// the reference's operation order (contraction disabled) so the complex128
+// path tracks big-wave's NumpyVector arithmetic to round-off.
+#include <cmath>
+#include <vector>
+
+#include "rsim_device.hpp"
+#include "rsim_internal.hpp"
+#include "rsim_tables.hpp"
+
+#pragma clang fp contract(off)
+
+namespace rsim {
+
+// ---------------------------------------------------------------------------
+// Chirp + circular cutoff (big-wave/propagation.py:396-442, 298-326)
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256) chirp_cutoff_kernel(cpx<R>* U, int nx, int ny,
+                                                           const double* __restrict__ kx2,
+                                                           const double* __restrict__ ky2,
+                                                           double fsq, int chirp, double ctheta,
+                                                           double gr, double gi) {
+  const long long n = (long long)nx * ny;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int y = (int)(idx / nx);
+    const int x = (int)(idx - (long long)y * nx);
+    const double k2 = ky2[y] + kx2[x];
+    cpx<R> v = U[idx];
+    if (k2 > fsq) {
+      v = mk(R(0), R(0));
+    } else if (chirp) {
+      double s, c;
+      sincos(ctheta * k2, &s, &c);
+      // h = global_phase * exp(i*theta)
+      const double hr = gr * c - gi * s;
+      const double hi = gr * s + gi * c;
+      const double vr = (double)v.x, vi = (double)v.y;
+      v = mk((R)(vr * hr - vi * hi), (R)(vr * hi + vi * hr));
+    }
+    U[idx] = v;
+  }
+}
+
+int chirp_cutoff(void* U, int nx, int ny, int dtype, double dx, double dy, double wl, double dz,
+                 double freq, bool chirp, hipStream_t s) {
+  const AxisK2* kx = nullptr;
+  const AxisK2* ky = nullptr;
+  RSIM_CHECK(axis_k2(nx, dx, &kx));
+  RSIM_CHECK(axis_k2(ny, dy, &ky));
+  const double fsq = 2.0 * (freq * freq);
+  // -1j*pi*wl*dz (propagation.py:430) and exp(1j*2*pi/wl*dz) (:429)
+  const double ctheta = ((-M_PI) * wl) * dz;
+  const double gph = ((2.0 * M_PI) / wl) * dz;
+  const double gr = std::cos(gph), gi = std::sin(gph);
+  const long long n = (long long)nx * ny;
+  int blocks = (int)std::min<long long>((n + 255) / 256, 65536);
+  ktime_begin(KT_PW, s);
+  if (dtype == RSIM_C64)
+    hipLaunchKernelGGL(chirp_cutoff_kernel<float>, dim3(blocks), dim3(256), 0, s, (cpx<float>*)U,
+                       nx, ny, kx->dev, ky->dev, fsq, chirp ? 1 : 0, ctheta, gr, gi);
+  else
+    hipLaunchKernelGGL(chirp_cutoff_kernel<double>, dim3(blocks), dim3(256), 0, s,
+                       (cpx<double>*)U, nx, ny, kx->dev, ky->dev, fsq, chirp ? 1 : 0, ctheta, gr,
+                       gi);
+  ktime_end(KT_PW, s);
+  RSIM_LAUNCHED();
+  return RSIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Point source (big-wave/propagation.py:458-484)
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256) point_source_kernel(cpx<R>* u, int nx, int ny,
+                                                           const double* __restrict__ x2,
+                                                           const double* __restrict__ y2,
+                                                           double z2, double k) {
+  const long long n = (long long)nx * ny;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int y = (int)(idx / nx);
+    const int x = (int)(idx - (long long)y * nx);
+    const double r = sqrt(y2[y] + x2[x] + z2);
+    double s, c;
+    sincos(r * k, &s, &c);
+    u[idx] = mk((R)(c / r), (R)(s / r));
+  }
+}
+
+int point_source(void* u, int nx, int ny, int dtype, double dx, double dy, double z, double xs,
+                 double ys, double wl, hipStream_t s) {
+  std::vector<double> hx(nx), hy(ny);
+  for (int i = 0; i < nx; ++i) {
+    const double v = ((double)i - nx / 2.0) * dx - xs;
+    hx[i] = v * v;
+  }
+  for (int j = 0; j < ny; ++j) {
+    const double v = ((double)j - ny / 2.0) * dy - ys;
+    hy[j] = v * v;
+  }
+  double* d = nullptr;
+  RSIM_HIP(hipMallocAsync((void**)&d, sizeof(double) * (nx + ny), s));
+  RSIM_HIP(hipMemcpyAsync(d, hx.data(), sizeof(double) * nx, hipMemcpyHostToDevice, s));
+  RSIM_HIP(hipMemcpyAsync(d + nx, hy.data(), sizeof(double) * ny, hipMemcpyHostToDevice, s));
+  const double k = (2.0 * M_PI) / wl;
+  const long long n = (long long)nx * ny;
+  int blocks = (int)std::min<long long>((n + 255) / 256, 65536);
+  if (dtype == RSIM_C64)
+    hipLaunchKernelGGL(point_source_kernel<float>, dim3(blocks), dim3(256), 0, s,
+                       (cpx<float>*)u, nx, ny, d, d + nx, z * z, k);
+  else
+    hipLaunchKernelGGL(point_source_kernel<double>, dim3(blocks), dim3(256), 0, s,
+                       (cpx<double>*)u, nx, ny, d, d + nx, z * z, k);
+  RSIM_LAUNCHED();
+  RSIM_HIP(hipFreeAsync(d, s));
+  // host vectors must outlive the async copies
+  RSIM_HIP(hipStreamSynchronize(s));
+  return RSIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Sample transmission, plain engine (big-wave/optical_element.py:538-584)
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256) transmission_kernel(cpx<R>* u, int nx, int ny,
+                                                           const uint8_t* __restrict__ ids,
+                                                           int nxg, const AxisMapDev xm,
+                                                           const AxisMapDev ym,
+                                                           const double* __restrict__ db,
+                                                           double K) {
+  const long long n = (long long)nx * ny;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int y = (int)(idx / nx);
+    const int x = (int)(idx - (long long)y * nx);
+    if (!(xm.inside[x] && ym.inside[y])) continue;   // vacuum: factor exactly 1
+    const int x0 = xm.i0[x], y0 = ym.i0[y];
+    const double fx = xm.frac[x], fy = ym.frac[y];
+    const uint8_t* r0 = ids + (long long)y0 * nxg + x0;
+    const uint8_t* r1 = r0 + nxg;
+    const int a = r0[0], b = r0[1], c = r1[0], d = r1[1];
+    const double gx = 1.0 - fx, gy = 1.0 - fy;
+    const double dy0r = db[2 * a] * gx + db[2 * b] * fx;
+    const double dy0i = db[2 * a + 1] * gx + db[2 * b + 1] * fx;
+    const double dy1r = db[2 * c] * gx + db[2 * d] * fx;
+    const double dy1i = db[2 * c + 1] * gx + db[2 * d + 1] * fx;
+    const double delta = dy0r * gy + dy1r * fy;
+    const double beta = dy0i * gy + dy1i * fy;
+    const double er = -(K * beta);
+    const double ei = -(K * delta);
+    const double e = exp(er);
+    double s, co;
+    sincos(ei, &s, &co);
+    const double tr = e * co, ti = e * s;
+    const cpx<R> v = u[idx];
+    const double vr = (double)v.x, vi = (double)v.y;
+    u[idx] = mk((R)(vr * tr - vi * ti), (R)(vr * ti + vi * tr));
+  }
+}
+
+int transmission(void* u, int nx, int ny, int dtype, double dx, double dy, double wl,
+                 const rsim_volume* vol, int slice, double px, double py, double pz, double xpos,
+                 double ypos, hipStream_t s) {
+  const AxisMap* xm = nullptr;
+  const AxisMap* ym = nullptr;
+  RSIM_CHECK(axis_map(nx, dx, 1.0, xpos, vol->nx_g, px, &xm));
+  RSIM_CHECK(axis_map(ny, dy, 1.0, ypos, vol->ny_g, py, &ym));
+  const double* db = nullptr;
+  RSIM_CHECK(volume_db_dev(vol, &db));
+  // -2j*pi*thickness/wl (optical_element.py:42)
+  const double K = ((2.0 * M_PI) * pz) / wl;
+  const uint8_t* ids = vol->ids + (long long)slice * vol->ny_g * vol->nx_g;
+  const long long n = (long long)nx * ny;
+  int blocks = (int)std::min<long long>((n + 255) / 256, 65536);
+  ktime_begin(KT_PW, s);
+  if (dtype == RSIM_C64)
+    hipLaunchKernelGGL(transmission_kernel<float>, dim3(blocks), dim3(256), 0, s,
+                       (cpx<float>*)u, nx, ny, ids, vol->nx_g, xm->dev, ym->dev, db, K);
+  else
+    hipLaunchKernelGGL(transmission_kernel<double>, dim3(blocks), dim3(256), 0, s,
+                       (cpx<double>*)u, nx, ny, ids, vol->nx_g, xm->dev, ym->dev, db, K);
+  ktime_end(KT_PW, s);
+  RSIM_LAUNCHED();
+  return RSIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// area_v1 detector (big-wave/propagation.py:764-816, 949-1110)
+// ---------------------------------------------------------------------------
+struct DetPlanDev {
+  const int* xi0;      // per output column
+  const int* xi1;
+  const double* xf0;
+  const double* xf1;
+  const double* xdet;  // detector x centres
+  const int* jlo;      // per output row: first / one-past-last field row
+  const int* jhi;
+  const double* ylo;   // detector row edges
+  const double* yhi;
+  const double* ydet;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(256) area_v1_kernel(const cpx<R>* __restrict__ u, int nx,
+                                                      int ny, int ox, int oy, DetPlanDev pl,
+                                                      double dx, double dy, double gyl,
+                                                      double z, double weight, int accumulate,
+                                                      double* out) {
+  const int p = blockIdx.x * 16 + threadIdx.x % 16;
+  const int q = blockIdx.y * 16 + threadIdx.x / 16;
+  if (p >= ox || q >= oy) return;
+  const int i0 = pl.xi0[p], i1 = pl.xi1[p];
+  const double f0 = pl.xf0[p], f1 = pl.xf1[p];
+  const double ylo = pl.ylo[q], yhi = pl.yhi[q];
+  const double xd = pl.xdet[p], yd = pl.ydet[q];
+  const double cosine = z / sqrt(xd * xd + yd * yd + z * z);
+  double acc = 0.0;
+  for (int j = pl.jlo[q]; j < pl.jhi[q]; ++j) {
+    const double cell_lo = gyl + j * dy;
+    const double cell_hi = cell_lo + dy;
+    const double ov = fmin(cell_hi, yhi) - fmax(cell_lo, ylo);
+    if (!(ov > 0.0)) continue;
+    const cpx<R>* row = u + (long long)j * nx;
+    double s = 0.0;
+    for (int i = i0; i < i1; ++i) {
+      const cpx<R> v = row[i];
+      s += (double)(v.x * v.x + v.y * v.y);
+    }
+    double xi = s * dx;
+    if (i1 < nx) {
+      const cpx<R> v = row[i1];
+      xi += f1 * dx * (double)(v.x * v.x + v.y * v.y);
+    }
+    if (i0 < nx) {
+      const cpx<R> v = row[i0];
+      xi -= f0 * dx * (double)(v.x * v.x + v.y * v.y);
+    }
+    acc += xi * ov * cosine;
+  }
+  const long long o = (long long)q * ox + p;
+  if (accumulate) out[o] += weight * acc;
+  else out[o] = acc;
+}
+
+int detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double dy, int ox,
+                     int oy, double dsx, double dsy, double z, double weight, bool accumulate,
+                     double* out_dev, hipStream_t s) {
+  if (ox <= 0 || oy <= 0) return fail(RSIM_E_INVALID_ARG, "detector output must be non-empty");
+  if (!std::isfinite(z) || z == 0.0)
+    return fail(RSIM_E_INVALID_ARG, "detector current_z must be finite and non-zero");
+  // per-axis overlap plans (propagation.py:949-977)
+  std::vector<double> xdet(ox), ydet(oy);
+  for (int p = 0; p < ox; ++p) xdet[p] = ((double)p - (double)(ox / 2)) * dsx;
+  for (int q = 0; q < oy; ++q) ydet[q] = ((double)q - (double)(oy / 2)) * dsy;
+  std::vector<int> xi0(ox), xi1(ox), jlo(oy), jhi(oy);
+  std::vector<double> xf0(ox), xf1(ox), ylo(oy), yhi(oy);
+  {
+    const double gl = (-(double)(nx / 2) - 0.5) * dx;
+    const double gr = gl + nx * dx;
+    for (int p = 0; p < ox; ++p) {
+      const double lo = xdet[p] - dsx * 0.5, hi = xdet[p] + dsx * 0.5;
+      const double clo = std::min(std::max(lo, gl), gr);
+      double chi = std::min(std::max(hi, gl), gr);
+      chi = std::max(chi, clo);
+      const double slo = (clo - gl) / dx, shi = (chi - gl) / dx;
+      long long a = (long long)std::floor(slo), b = (long long)std::floor(shi);
+      double fa = slo - (double)a, fb = shi - (double)b;
+      if (a >= nx) { a = nx; fa = 0.0; }
+      if (b >= nx) { b = nx; fb = 0.0; }
+      xi0[p] = (int)a; xi1[p] = (int)b; xf0[p] = fa; xf1[p] = fb;
+    }
+  }
+  const double gyl = (-(double)(ny / 2) - 0.5) * dy;
+  for (int q = 0; q < oy; ++q) {
+    ylo[q] = ydet[q] - dsy * 0.5;
+    yhi[q] = ydet[q] + dsy * 0.5;
+    // rows whose cell [gyl + j dy, gyl + j dy + dy] can overlap (checked exactly in the kernel)
+    long long a = (long long)std::floor((ylo[q] - gyl) / dy) - 1;
+    long long b = (long long)std::floor((yhi[q] - gyl) / dy) + 2;
+    a = std::max<long long>(0, std::min<long long>(a, ny));
+    b = std::max<long long>(0, std::min<long long>(b, ny));
+    jlo[q] = (int)a;
+    jhi[q] = (int)std::max(a, b);
+  }
+  // one device blob for the plan
+  const size_t nI = 2 * (size_t)ox + 2 * (size_t)oy;
+  const size_t nD = 3 * (size_t)ox + 3 * (size_t)oy;
+  std::vector<unsigned char> blob(nI * 4 + 8 + nD * 8);
+  int* bi = (int*)blob.data();
+  std::copy(xi0.begin(), xi0.end(), bi);
+  std::copy(xi1.begin(), xi1.end(), bi + ox);
+  std::copy(jlo.begin(), jlo.end(), bi + 2 * ox);
+  std::copy(jhi.begin(), jhi.end(), bi + 2 * ox + oy);
+  const size_t doff = ((nI * 4 + 7) / 8) * 8;
+  double* bd = (double*)(blob.data() + doff);
+  std::copy(xf0.begin(), xf0.end(), bd);
+  std::copy(xf1.begin(), xf1.end(), bd + ox);
+  std::copy(xdet.begin(), xdet.end(), bd + 2 * ox);
+  std::copy(ylo.begin(), ylo.end(), bd + 3 * ox);
+  std::copy(yhi.begin(), yhi.end(), bd + 3 * ox + oy);
+  std::copy(ydet.begin(), ydet.end(), bd + 3 * ox + 2 * oy);
+  unsigned char* d = nullptr;
+  RSIM_HIP(hipMallocAsync((void**)&d, blob.size(), s));
+  RSIM_HIP(hipMemcpyAsync(d, blob.data(), blob.size(), hipMemcpyHostToDevice, s));
+  DetPlanDev pl;
+  const int* di = (const int*)d;
+  const double* dd = (const double*)(d + doff);
+  pl.xi0 = di; pl.xi1 = di + ox; pl.jlo = di + 2 * ox; pl.jhi = di + 2 * ox + oy;
+  pl.xf0 = dd; pl.xf1 = dd + ox; pl.xdet = dd + 2 * ox;
+  pl.ylo = dd + 3 * ox; pl.yhi = dd + 3 * ox + oy; pl.ydet = dd + 3 * ox + 2 * oy;
+  dim3 grid((ox + 15) / 16, (oy + 15) / 16);
+  if (dtype == RSIM_C64)
+    hipLaunchKernelGGL(area_v1_kernel<float>, grid, dim3(256), 0, s, (const cpx<float>*)u, nx, ny,
+                       ox, oy, pl, dx, dy, gyl, z, weight, accumulate ? 1 : 0, out_dev);
+  else
+    hipLaunchKernelGGL(area_v1_kernel<double>, grid, dim3(256), 0, s, (const cpx<double>*)u, nx,
+                       ny, ox, oy, pl, dx, dy, gyl, z, weight, accumulate ? 1 : 0, out_dev);
+  RSIM_LAUNCHED();
+  RSIM_HIP(hipFreeAsync(d, s));
+  RSIM_HIP(hipStreamSynchronize(s));   // host plan vectors must outlive the copy
+  return RSIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Volume painting: seeded synthetic samples (spheres, later ones overwrite)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) paint_sphere_kernel(uint8_t* ids, int nz, int ny, int nx,
+                                                           int z0, int y0, int x0, int bz, int by,
+                                                           int bx, double cz, double cy,
+                                                           double cx, double r2, uint8_t mat) {
+  const long long n = (long long)bz * by * bx;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int lx = (int)(idx % bx);
+    const int ly = (int)((idx / bx) % by);
+    const int lz = (int)(idx / ((long long)bx * by));
+    const int z = z0 + lz, y = y0 + ly, x = x0 + lx;
+    const double dz = z + 0.5 - cz, dy = y + 0.5 - cy, dx = x + 0.5 - cx;
+    if (dz * dz + dy * dy + dx * dx <= r2) ids[((long long)z * ny + y) * nx + x] = mat;
+  }
+}
+
+int paint_spheres(rsim_volume* vol, const double* sph, const int* mats, int n, hipStream_t s) {
+  for (int k = 0; k < n; ++k) {
+    const double cz = sph[4 * k], cy = sph[4 * k + 1], cx = sph[4 * k + 2], r = sph[4 * k + 3];
+    if (mats[k] < 0 || mats[k] >= vol->n_db || mats[k] > 255)
+      return fail(RSIM_E_INVALID_ARG, "sphere %d: material id %d out of range", k, mats[k]);
+    if (!(r > 0)) continue;
+    const int z0 = std::max(0, (int)std::floor(cz - r - 1)), z1 = std::min(vol->nz, (int)std::ceil(cz + r + 1));
+    const int y0 = std::max(0, (int)std::floor(cy - r - 1)), y1 = std::min(vol->ny_g, (int)std::ceil(cy + r + 1));
+    const int x0 = std::max(0, (int)std::floor(cx - r - 1)), x1 = std::min(vol->nx_g, (int)std::ceil(cx + r + 1));
+    if (z1 <= z0 || y1 <= y0 || x1 <= x0) continue;
+    const long long cnt = (long long)(z1 - z0) * (y1 - y0) * (x1 - x0);
+    int blocks = (int)std::min<long long>((cnt + 255) / 256, 65536);
+    hipLaunchKernelGGL(paint_sphere_kernel, dim3(blocks), dim3(256), 0, s, vol->ids, vol->nz,
+                       vol->ny_g, vol->nx_g, z0, y0, x0, z1 - z0, y1 - y0, x1 - x0, cz, cy, cx,
+                       r * r, (uint8_t)mats[k]);
+    RSIM_LAUNCHED();
+  }
+  return RSIM_OK;
+}
+
+}  // namespace rsim

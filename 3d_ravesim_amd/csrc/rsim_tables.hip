@@ -1,0 +1,124 @@
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "rsim_tables.hpp"
+
+// Host FP64 arithmetic follows the reference expression by expression.
+#pragma clang fp contract(off)
+
+namespace rsim {
+namespace {
+
+uint64_t bits(double v) {
+  uint64_t b;
+  std::memcpy(&b, &v, 8);
+  return b;
+}
+
+std::mutex g_mu;
+std::map<std::tuple<int, int, uint64_t>, AxisK2> g_k2;
+std::map<std::tuple<int, int, uint64_t, uint64_t, uint64_t, int, uint64_t>, AxisMap> g_maps;
+std::vector<void*> g_allocs;
+
+}  // namespace
+
+int axis_k2(int n, double spacing, const AxisK2** out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto key = std::make_tuple(current_device(), n, bits(spacing));
+  auto it = g_k2.find(key);
+  if (it != g_k2.end()) {
+    *out = &it->second;
+    return RSIM_OK;
+  }
+  std::vector<double> h(n);
+  const double val = 1.0 / (double)n;   // numpy fftfreq: results * (1/(n*d))
+  const int half = (n - 1) / 2 + 1;
+  for (int i = 0; i < n; ++i) {
+    const long long b = i < half ? i : (long long)i - n;
+    const double f = ((double)b * val) / spacing;
+    h[i] = f * f;
+  }
+  AxisK2 t;
+  RSIM_HIP(hipMalloc((void**)&t.dev, sizeof(double) * n));
+  RSIM_HIP(hipMemcpy(t.dev, h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  g_allocs.push_back(t.dev);
+  auto r = g_k2.emplace(key, t);
+  *out = &r.first->second;
+  return RSIM_OK;
+}
+
+int axis_map(int n, double spacing, double scale, double pos, int grid_len, double pixel,
+             const AxisMap** out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto key = std::make_tuple(current_device(), n, bits(spacing), bits(scale), bits(pos), grid_len,
+                             bits(pixel));
+  auto it = g_maps.find(key);
+  if (it != g_maps.end()) {
+    *out = &it->second;
+    return RSIM_OK;
+  }
+  std::vector<int> i0(n);
+  std::vector<double> fr(n);
+  std::vector<float> frf(n);
+  std::vector<uint8_t> ins(n);
+  bool integral = true, any = false;
+  for (int i = 0; i < n; ++i) {
+    // sim = (i - n/2.0)*spacing; idx = (sim*scale + pos + len*pixel*0.5)/pixel
+    const double sim = ((double)i - n / 2.0) * spacing;
+    const double idx = (sim * scale + pos + (double)grid_len * pixel * 0.5) / pixel;
+    const double fl = std::floor(idx);
+    const long long fraw = (long long)fl;
+    fr[i] = idx - fl;
+    frf[i] = (float)fr[i];
+    ins[i] = (idx >= 0.0 && idx < (double)grid_len - 1.0) ? 1 : 0;
+    long long c = fraw < 0 ? 0 : fraw;
+    if (c > grid_len - 2) c = grid_len - 2;
+    i0[i] = (int)c;
+    if (ins[i]) {
+      any = true;
+      if (fr[i] != 0.0) integral = false;
+    }
+  }
+  AxisMap m;
+  m.all_integral = integral;
+  m.any_inside = any;
+  const size_t bytes = n * (4 + 8 + 4 + 1) + 64;
+  unsigned char* d = nullptr;
+  RSIM_HIP(hipMalloc((void**)&d, bytes));
+  g_allocs.push_back(d);
+  size_t o = 0;
+  RSIM_HIP(hipMemcpy(d + o, fr.data(), 8 * n, hipMemcpyHostToDevice));
+  m.dev.frac = (const double*)(d + o);
+  o += 8 * (size_t)n;
+  RSIM_HIP(hipMemcpy(d + o, i0.data(), 4 * n, hipMemcpyHostToDevice));
+  m.dev.i0 = (const int*)(d + o);
+  o += 4 * (size_t)n;
+  RSIM_HIP(hipMemcpy(d + o, frf.data(), 4 * n, hipMemcpyHostToDevice));
+  m.dev.fracf = (const float*)(d + o);
+  o += 4 * (size_t)n;
+  RSIM_HIP(hipMemcpy(d + o, ins.data(), n, hipMemcpyHostToDevice));
+  m.dev.inside = (const uint8_t*)(d + o);
+  auto r = g_maps.emplace(key, m);
+  *out = &r.first->second;
+  return RSIM_OK;
+}
+
+int volume_db_dev(const rsim_volume* vol, const double** out) {
+  *out = vol->db_dev;
+  return RSIM_OK;
+}
+
+int release_tables() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (void* p : g_allocs) hipFree(p);
+  g_allocs.clear();
+  g_k2.clear();
+  g_maps.clear();
+  return RSIM_OK;
+}
+
+}  // namespace rsim

@@ -1,0 +1,38 @@
+"""Point source of the 2D path (big-wave/source.py:44-113)."""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+from .propagation import (
+    SimParams,
+    apply_frequency_cutoff_2d,
+    grid_density_check_2d,
+    propagate_analytically_2d,
+)
+from .vector import DeviceVector
+
+
+@dataclass
+class PointSource:
+    x: float
+    z: float
+    y: float = 0.0
+
+    def propagate_to(self, z_out: float, sim_params: SimParams, cutoff_freq: float,
+                     u: DeviceVector, U: DeviceVector, history) -> None:
+        """Analytic spherical wave to z_out, then the circular cutoff.
+
+        big-wave/source.py:97-107.  U is scratch afterwards (not FFT2(u)),
+        as in the reference (source.py:38-40).
+        """
+        if history is not None:
+            raise NotImplementedError("history frames are not produced by the HIP engine")
+        dz = z_out - self.z
+        grid_density_check_2d(dz, self.x, sim_params.nx, sim_params.dx, self.y, sim_params.ny,
+                              sim_params.get_dy(), sim_params.wl)
+        propagate_analytically_2d(u, dz, self.x, self.y, sim_params)
+        u.fft2(U, sim_params.nx, sim_params.ny)
+        apply_frequency_cutoff_2d(U, cutoff_freq, sim_params.dx, sim_params.get_dy(),
+                                  sim_params.nx, sim_params.ny, sim_params.chunk_size)
+        U.ifft2(u, sim_params.nx, sim_params.ny)

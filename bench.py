@@ -1,0 +1,252 @@
+"""Headline benchmark: fused complex64 multislice slice-steps/s at 4096^2.
+
+Workload (voxelized 3D sample multislice): field 4096x4096, 256 slices of
+0.5 um, uint8 material ids painted with 200 seeded spheres (seed 2,
+r ~ U(5, 40) um, materials (delta, beta) = (3.5e-7, 2e-10), (1.1e-6, 3e-9),
+(2.6e-7, 1.5e-10)), lambda = 0.5 A, cutoff 0.4/dx, then 0.5 m free space to an
+area_v1 detector binned 4x4.  One "step" is one 256-slice pass over the
+volume (transmission + FFT2 + chirp/cutoff + IFFT2 per slice) with the field
+and volume resident in HBM.
+
+Prints one JSON line.  Besides the fused complex64 rate it times the plain
+(unfused) complex128 GPU path, reports relative L2 errors of the fused field
+and detector image against that FP64 GPU path, and times the FP64 CPU oracle
+on a bounded sample of the same workload (cpu_baseline).
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import importlib
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "slice-steps/s at 4096² (c64 fused vs c128 plain, rel err≤1e-5); % HBM peak"
+HBM_PEAK_GBS = 8000.0
+DB = np.array([0, 3.5e-7 + 2e-10j, 1.1e-6 + 3e-9j, 2.6e-7 + 1.5e-10j])
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_volume(rs, nz, n, seed=2, nsph=200, voxel=0.5e-6):
+    rng = np.random.default_rng(seed)
+    cz = rng.uniform(0, nz, nsph)
+    cy = rng.uniform(0, n, nsph)
+    cx = rng.uniform(0, n, nsph)
+    r = rng.uniform(5e-6, 40e-6, nsph) / voxel
+    mats = rng.integers(1, 4, nsph)
+    vol = rs.DeviceVolume.empty((nz, n, n), DB)
+    vol.paint_spheres(np.stack([cz, cy, cx, r], axis=1), mats)
+    return vol
+
+
+def run_multislice(lib, u, U, n, vol, engine, dx, wl, pz, nz, first=0, flags=0, stream=None):
+    lib.call("rsim_sample_apply_2d", u.ptr, U.ptr, n, n, u.code, dx, dx, wl, 0.4 / dx, vol.handle,
+             dx, dx, pz, 0.0, 0.0, first, nz, engine, flags, stream)
+
+
+def cpu_info():
+    model = platform.processor() or ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--slices", type=int, default=256)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--cpu-slices", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU; the multislice engine has no "
+                                   "multi-GPU path (independent instances shard across GPUs by "
+                                   "process)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    rs = importlib.import_module("3d_ravesim_amd")
+    lib = rs._lib
+    lib.load()
+    if lib.device_count() < 1:
+        raise RuntimeError("no HIP device visible")
+    n, nz = args.n, args.slices
+    dx = 0.5e-6
+    wl = 0.5e-10
+    pz = 0.5e-6
+    N = n * n
+
+    t0 = time.perf_counter()
+    vol = make_volume(rs, nz, n)
+    log(f"volume {nz}x{n}x{n} painted in {time.perf_counter() - t0:.2f}s")
+    stream = C.c_void_p()
+    lib.call("rsim_stream_create", C.byref(stream))
+    s = stream.value
+
+    ones64 = np.ones(N, dtype=np.complex64)
+    u = rs.DeviceVector.from_numpy(ones64)
+    U = rs.DeviceVector(N, np.complex64)
+
+    # ---- fused complex64: warmup + timed region -------------------------
+    for _ in range(args.warmup):
+        run_multislice(lib, u, U, n, vol, lib.RSIM_FUSED, dx, wl, pz, nz, stream=s)
+    lib.call("rsim_synchronize", s)
+    lib.call("rsim_kernel_timing_read", None, None)
+    lib.call("rsim_kernel_timing", 1)
+    lib.call("rsim_synchronize", s)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_multislice(lib, u, U, n, vol, lib.RSIM_FUSED, dx, wl, pz, nz, stream=s)
+    lib.call("rsim_synchronize", s)
+    t_fused = time.perf_counter() - t0
+    lib.call("rsim_kernel_timing", 0)
+    ms3 = (C.c_double * 3)()
+    cnt3 = (C.c_int * 3)()
+    lib.call("rsim_kernel_timing_read", ms3, cnt3)
+    slice_steps = args.steps * nz
+    value = slice_steps / t_fused
+    ms_per_step = 1e3 * t_fused / args.steps
+    log(f"fused c64: {value:.1f} slice-steps/s, {ms_per_step:.2f} ms per {nz}-slice step")
+
+    # algorithmic bytes (65 B/point/slice-step: 4 field passes x 16 B + 1 B id),
+    # split over the fused kernels: row pass 33 B/pt, column pass 32 B/pt
+    row_ms, col_ms = ms3[0], ms3[1]
+    row_n, col_n = cnt3[0], cnt3[1]
+    row_bytes = 33.0 * N * slice_steps
+    col_bytes = 32.0 * N * slice_steps
+    dom = "column" if col_ms >= row_ms else "row"
+    d_ms, d_n, d_bytes = (col_ms, col_n, col_bytes) if dom == "column" else (row_ms, row_n, row_bytes)
+    achieved = d_bytes / (d_ms * 1e-3) / 1e9 if d_ms > 0 else 0.0
+    step_gbs = 65.0 * N * slice_steps / t_fused / 1e9
+    log(f"kernels: row {row_ms / max(row_n, 1) * 1e3:.1f} us x{row_n}, col {col_ms / max(col_n, 1) * 1e3:.1f} us x{col_n}")
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "slice-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"voxelized multislice {n}x{n}, {nz} slices x 0.5um, 200 seeded spheres (seed 2), "
+                        "3 materials, lambda 0.5A, fused complex64 engine",
+            "nx": n, "ny": n, "slices": nz, "dx_m": dx, "wl_m": wl, "dz_m": pz,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": f"ms_{'col' if dom == 'column' else 'row'}_kernel<{n}>",
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "avg_launch_us": round(d_ms / max(d_n, 1) * 1e3, 2),
+            "alg_bytes_per_launch": (32.0 if dom == "column" else 33.0) * N,
+        },
+        "step_hbm_GBs_65B_model": round(step_gbs, 1),
+        "step_hbm_frac_65B_model": round(step_gbs / HBM_PEAK_GBS, 4),
+        "kernel_avg_us": {"row": round(row_ms / max(row_n, 1) * 1e3, 2),
+                          "col": round(col_ms / max(col_n, 1) * 1e3, 2)},
+        "reference_published_gpu_slice_steps_per_s": 23.1,
+    }
+
+    # ---- plain complex128 GPU path ---------------------------------------
+    u2 = rs.DeviceVector.from_numpy(np.ones(N, dtype=np.complex128))
+    U2 = rs.DeviceVector(N, np.complex128)
+    if args.plain_steps > 0:
+        run_multislice(lib, u2, U2, n, vol, lib.RSIM_PLAIN, dx, wl, pz, min(nz, 8), stream=s)
+        lib.call("rsim_synchronize", s)
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            run_multislice(lib, u2, U2, n, vol, lib.RSIM_PLAIN, dx, wl, pz, nz, stream=s)
+        lib.call("rsim_synchronize", s)
+        t_plain = time.perf_counter() - t0
+        plain_rate = args.plain_steps * nz / t_plain
+        result["plain_c128_slice_steps_per_s"] = round(plain_rate, 2)
+        result["fused_over_plain"] = round(value / plain_rate, 3)
+        log(f"plain c128: {plain_rate:.1f} slice-steps/s; fused/plain = {value / plain_rate:.2f}x")
+
+    # ---- parity: fused c64 vs plain c128 from the same plane wave ----------
+    if not args.no_parity:
+        u.upload(ones64)
+        u2.upload(np.ones(N, dtype=np.complex128))
+        run_multislice(lib, u, U, n, vol, lib.RSIM_FUSED, dx, wl, pz, nz, stream=s)
+        run_multislice(lib, u2, U2, n, vol, lib.RSIM_PLAIN, dx, wl, pz, nz, stream=s)
+        lib.call("rsim_synchronize", s)
+        a, b = u.to_numpy(), u2.to_numpy()
+        err_field = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        # 0.5 m free space (skip_fft: U holds FFT2(u)) and the 4x4 area_v1 detector
+        z_det = 1.0 + nz * pz + 0.5
+        p = rs.SimParams(N=N, nx=n, ny=n, dx=dx, z_detector=z_det, detector_size=n * dx,
+                         detector_pixel_size_x=4 * dx, detector_pixel_size_y=4 * dx, wl=wl,
+                         chunk_size=n)
+        rs.propagate_2d(u, U, dx, dx, wl, 0.5, n, 0.4 / dx, n, n, skip_fft=True)
+        rs.propagate_2d(u2, U2, dx, dx, wl, 0.5, n, 0.4 / dx, n, n, skip_fft=True)
+        d1 = rs.square_and_downsample_2d(u, p, z_det)
+        d2 = rs.square_and_downsample_2d(u2, p, z_det)
+        err_det = float(np.linalg.norm(d1 - d2) / np.linalg.norm(d2))
+        result["rel_l2_field_fused_vs_c128"] = err_field
+        result["rel_l2_detector_fused_vs_c128"] = err_det
+        result["detector_shape"] = list(d1.shape)
+        log(f"parity vs FP64 GPU path: field {err_field:.3e}, detector {err_det:.3e}")
+        del a, b
+
+    # ---- CPU baseline: the FP64 oracle on a bounded sample ----------------
+    if not args.no_cpu:
+        import oracle as O
+        k = args.cpu_slices
+        ids_p = C.c_void_p()
+        lib.call("rsim_volume_info", vol.handle, C.byref(ids_p), None, None, None)
+        slab = np.empty((k, n, n), dtype=np.uint8)
+        lib.call("rsim_memcpy_d2h", slab.ctypes.data, ids_p.value, slab.nbytes, None)
+        spec = O.SampleSpec(grid=slab, db_list=DB, pixel_size_x=dx, pixel_size_y=dx, pixel_size_z=pz)
+        u0 = np.ones((n, n), dtype=np.complex128)
+        t0 = time.perf_counter()
+        ue, _ = O.sample_apply_2d(u0, spec, dx, dx, wl, 0.4 / dx, workers=1)
+        t_cpu = time.perf_counter() - t0
+        model, ncpu = cpu_info()
+        result["cpu_baseline"] = {
+            "value": round(k / t_cpu, 4), "unit": "slice-steps/s", "cores": 1, "kind": "port",
+            "sample": f"FP64 numpy/scipy oracle (scipy.fft workers=1), first {k} slice-steps of the "
+                      f"same {n}x{n} workload from a unit plane wave; host {model}, {ncpu} logical CPUs",
+        }
+        # the same slices on the GPU (fused) against the oracle
+        u.upload(ones64)
+        run_multislice(lib, u, U, n, vol, lib.RSIM_FUSED, dx, wl, pz, k, stream=s)
+        lib.call("rsim_synchronize", s)
+        got = u.to_numpy().reshape(n, n)
+        result["rel_l2_field_fused_vs_oracle_first_slices"] = O.relative_l2(got, ue)
+        log(f"cpu oracle: {k / t_cpu:.3f} slice-steps/s; fused vs oracle ({k} slices): "
+            f"{result['rel_l2_field_fused_vs_oracle_first_slices']:.3e}")
+
+    lib.call("rsim_stream_destroy", s)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,170 @@
+/*
+ * ravesim.h -- C ABI of the MI355X-native RAVE-SIM 2D Fresnel multislice engine
+ * (libravesim_hip.so, built from 3d_ravesim_amd/csrc for gfx950).
+ *
+ * Conventions
+ *  - Every entry point returns RSIM_OK (0) or a negative RSIM_E_* code; the
+ *    message of the last failure on the calling thread is rsim_last_error().
+ *    Nothing in the library calls exit() (the reference's fast-wave does,
+ *    fast-wave/fwcuda/types.hpp:226-239).
+ *  - Fields are device buffers of nx*ny complex values, row-major (ny rows of
+ *    nx), interleaved re/im: complex64 (dtype RSIM_C64) or complex128
+ *    (RSIM_C128) -- the numpy layout big-wave uses (big-wave/vector.py:319-385)
+ *    and fast-wave's DevComplex (fast-wave/fwcuda/types.hpp:14-39).
+ *  - `stream` is a hipStream_t; NULL selects the library's own stream for the
+ *    current device.  Calls are asynchronous on that stream unless stated.
+ *  - Physical quantities are SI (metres, 1/metres).
+ */
+#ifndef RAVESIM_H
+#define RAVESIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  RSIM_OK = 0,
+  RSIM_E_INVALID_ARG = -1, /* -> ValueError */
+  RSIM_E_HIP = -2,         /* -> RuntimeError */
+  RSIM_E_CANCELLED = -3,   /* -> InterruptedError */
+  RSIM_E_NOMEM = -4,       /* -> MemoryError */
+  RSIM_E_UNSUPPORTED = -5  /* -> NotImplementedError */
+};
+
+enum { RSIM_C64 = 0, RSIM_C128 = 1 };
+enum { RSIM_FORWARD = -1, RSIM_INVERSE = 1 };
+/* multislice engines: PLAIN = unfused kernels in the field precision;
+ * FUSED = complex64 fused row/column kernels with FP64 phase tables. */
+enum { RSIM_PLAIN = 0, RSIM_FUSED = 1 };
+/* rsim_sample_apply_2d flags */
+enum { RSIM_FLAG_NO_SPECTRUM = 1 };
+
+typedef struct rsim_volume rsim_volume;
+
+/* ---- library / memory plumbing (no reference counterpart: device buffers) ---- */
+const char* rsim_last_error(void);
+const char* rsim_version(void);
+int rsim_device_count(int* count);
+int rsim_set_device(int device);
+int rsim_malloc(void** dptr, size_t bytes);
+int rsim_free(void* dptr);
+int rsim_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream); /* synchronous */
+int rsim_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream); /* synchronous */
+int rsim_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+int rsim_memset(void* dptr, int value, size_t bytes, void* stream);
+int rsim_synchronize(void* stream);
+int rsim_stream_create(void** stream);
+int rsim_stream_destroy(void* stream);
+/* Release cached FFT plans, tables and workspaces of the current device. */
+int rsim_release_caches(void);
+
+/* Cooperative cancel + progress (big-wave/propagation.py:669-688,
+ * big-fourier/src/lib.rs:132-187): the multislice loop polls the flag between
+ * slices and returns RSIM_E_CANCELLED; progress is reported as
+ * (stage, completed, total) after each slice. */
+typedef void (*rsim_progress_fn)(const char* stage, int64_t completed, int64_t total, void* user);
+int rsim_set_progress(rsim_progress_fn fn, void* user);
+int rsim_request_cancel(void);
+int rsim_clear_cancel(void);
+
+/* Kernel timing (the reference's CudaTimerSection, fast-wave/src/simulation.cpp:14-33):
+ * while enabled, every row pass, column pass and pointwise launch of the FFT
+ * and multislice engines is bracketed by hipEvents on its own stream.
+ * rsim_kernel_timing_read waits for them and returns, per kind
+ * (0 = row pass, 1 = column pass, 2 = pointwise), the summed kernel time in
+ * ms and the launch count since the last read, then resets. */
+int rsim_kernel_timing(int enable);
+int rsim_kernel_timing_read(double* ms3, int* launches3);
+
+/* ---- FFT ----
+ * Replaces NumpyVector.fft2 / ifft2 (big-wave/vector.py:341-347) and fast-wave
+ * FFT<S>::forward / inverse + scale_kernel (fast-wave/src/fft.cpp:35-57,
+ * fast-wave/fwcuda/kernels.cu:861-872).  direction RSIM_FORWARD is
+ * unnormalised, RSIM_INVERSE multiplies by 1/(nx*ny).  src == dst allowed.
+ * Sizes: any nx, ny whose prime factors are 2, 3, 5. */
+int rsim_fft2(void* dst, const void* src, int nx, int ny, int dtype, int direction, void* stream);
+/* Batched 1D transforms along the contiguous axis: `batch` rows of length n. */
+int rsim_fft1d_rows(void* dst, const void* src, int n, int batch, int dtype, int direction,
+                    void* stream);
+
+/* ---- Fresnel propagation ---- */
+/* Replaces apply_frequency_cutoff_2d (big-wave/propagation.py:298-326):
+ * zero where kx^2 + ky^2 > 2 f^2, in place on a spectrum. */
+int rsim_apply_frequency_cutoff_2d(void* U, int nx, int ny, int dtype, double freq, double dx,
+                                   double dy, void* stream);
+/* Replaces propagate_2d (big-wave/propagation.py:396-442; fast-wave
+ * simulation.cpp:472-480 + kernels.cu:166-212): U = FFT2(u) unless skip_fft,
+ * U *= exp(2 pi i dz/wl) exp(-i pi wl dz (kx^2+ky^2)), circular cutoff,
+ * u = IFFT2(U).  On return U holds FFT2(u). */
+int rsim_propagate_2d(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
+                      double dz, double cutoff_freq, int skip_fft, void* stream);
+/* Replaces propagate_analytically_2d (big-wave/propagation.py:458-484;
+ * fast-wave kernels.cu:107-153): u = exp(2 pi i r/wl)/r. */
+int rsim_propagate_analytically_2d(void* u, int nx, int ny, int dtype, double dx, double dy,
+                                   double z, double x_source, double y_source, double wl,
+                                   void* stream);
+
+/* ---- Sample volumes ----
+ * Replaces the device upload of fast-wave apply_sample_2d
+ * (fast-wave/src/simulation.cpp:651-665) and Sample.store_deltabetas
+ * (big-wave/optical_element.py:433-440).  ids: host array (nz, ny_g, nx_g),
+ * C order, id_bytes = 1, 2 or 4 (uint8/uint16/uint32 or int32); every id must
+ * be < n_db.  deltabeta: n_db complex values as (re, im) pairs, entry 0 is
+ * vacuum and must be 0.  The device copy is uint8. */
+int rsim_volume_create(rsim_volume** vol, const void* ids, int id_bytes, int nz, int ny_g,
+                       int nx_g, const double* deltabeta, int n_db);
+/* Same, allocating an all-vacuum volume to be filled on the device. */
+int rsim_volume_create_empty(rsim_volume** vol, int nz, int ny_g, int nx_g,
+                             const double* deltabeta, int n_db);
+int rsim_volume_destroy(rsim_volume* vol);
+/* Device pointer of the uint8 ids and the volume shape. */
+int rsim_volume_info(const rsim_volume* vol, void** dev_ids, int* nz, int* ny_g, int* nx_g);
+/* Paint spheres into the volume in order (later spheres overwrite earlier
+ * ones).  spheres: n x (cz, cy, cx, radius) in voxel units; mats: n ids. */
+int rsim_volume_paint_spheres(rsim_volume* vol, const double* spheres, const int* mats, int n,
+                              void* stream);
+
+/* Replaces the transmission step of Sample._apply_2d
+ * (big-wave/optical_element.py:533-584; fast-wave kernels.cu:215-407):
+ * u *= exp(-2 pi i pz/wl conj(db)) with db bilinear in delta-beta space from
+ * the four voxel corners of slice `slice`, vacuum outside the voxel box. */
+int rsim_transmission_2d(void* u, int nx, int ny, int dtype, double dx, double dy, double wl,
+                         const rsim_volume* vol, int slice, double pixel_size_x,
+                         double pixel_size_y, double pixel_size_z, double x_pos, double y_pos,
+                         void* stream);
+
+/* Replaces Sample._apply_2d (big-wave/optical_element.py:502-596) and fast-wave
+ * apply_sample_2d (fast-wave/src/simulation.cpp:598-733): for each slice in
+ * [first_slice, first_slice + n_slices): transmission, then propagate_2d over
+ * pixel_size_z.  engine RSIM_PLAIN or RSIM_FUSED (FUSED needs RSIM_C64; other
+ * sizes fall back to PLAIN kernels).  Unless flags has RSIM_FLAG_NO_SPECTRUM,
+ * U holds FFT2(u) on return (OpticalElement.apply contract,
+ * big-wave/optical_element.py:97).  Polls the cancel flag between slices. */
+int rsim_sample_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
+                         double wl, double cutoff_freq, const rsim_volume* vol,
+                         double pixel_size_x, double pixel_size_y, double pixel_size_z,
+                         double x_pos, double y_pos, int first_slice, int n_slices, int engine,
+                         int flags, void* stream);
+
+/* ---- Detector ----
+ * Replaces square_and_downsample_2d with the area_v1 integrator
+ * (big-wave/propagation.py:995-1130; fast-wave kernels.cu:525-623):
+ * out[q,p] = sum |u|^2 overlap_x overlap_y z/sqrt(x_p^2+y_q^2+z^2), FP64.
+ * out_host: out_y*out_x float64, written before return (synchronous). */
+int rsim_detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double dy,
+                          int out_x, int out_y, double ds_x, double ds_y, double z,
+                          double* out_host, void* stream);
+/* Same with a device output buffer (float64) and accumulation weight:
+ * out_dev += weight * detector (for spectral sums). */
+int rsim_detector_area_v1_accumulate(const void* u, int nx, int ny, int dtype, double dx, double dy,
+                                     int out_x, int out_y, double ds_x, double ds_y, double z,
+                                     double weight, double* out_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RAVESIM_H */
