@@ -181,6 +181,50 @@ __device__ __forceinline__ void dft(cpx<R>* v) {
   else if constexpr (RADIX == 16) dft16<DIR>(v);
 }
 
+// ---------------------------------------------------------------------------
+// Buffer-resource global I/O: the base address lives in SGPRs (a V#), each
+// lane supplies a 32-bit byte offset and each element a uniform SGPR offset,
+// so a thread's E loads/stores cost one address VGPR instead of E 64-bit
+// pointers.  num_records bounds every access (out-of-range loads return 0,
+// stores are dropped).  Word 3 0x00020000 is the gfx9-family raw-buffer
+// format.
+// ---------------------------------------------------------------------------
+template <typename R>
+struct Gio {
+  __amdgpu_buffer_rsrc_t rs;
+  __device__ __forceinline__ Gio(const void* base, uint32_t nbytes)
+      : rs(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)nbytes,
+                                             0x00020000)) {}
+  __device__ __forceinline__ cpx<R> ld(uint32_t voff, uint32_t soff) const {
+    if constexpr (sizeof(R) == 4) {
+      return __builtin_bit_cast(cpx<R>, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0));
+    } else {
+      return __builtin_bit_cast(cpx<R>, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0));
+    }
+  }
+  __device__ __forceinline__ void st(cpx<R> v, uint32_t voff, uint32_t soff) const {
+    if constexpr (sizeof(R) == 4) {
+      typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), rs, voff, soff, 0);
+    } else {
+      typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), rs, voff, soff, 0);
+    }
+  }
+};
+
+// Compiler-only memory fence: keeps the twiddle loads of one transform from
+// being merged with (and held live across) those of the next one.
+__device__ __forceinline__ void compiler_fence() { __asm__ __volatile__("" ::: "memory"); }
+
+// XCD-aware tile order: consecutive workgroup ids are dispatched round-robin
+// over the 8 XCDs, so map them to tiles such that each XCD walks a contiguous
+// run of tiles (neighbouring column tiles share 128 B lines in its L2).
+__device__ __forceinline__ int xcd_tile(int b, int ntiles) {
+  if (ntiles % 8 != 0) return b;
+  return (b % 8) * (ntiles / 8) + b / 8;
+}
+
 // sincos of a phase already reduced to [-pi, pi] (single precision output).
 __device__ __forceinline__ void sincos_reduced(double ph, float* s, float* c) {
   sincosf((float)ph, s, c);

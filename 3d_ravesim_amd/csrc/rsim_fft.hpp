@@ -274,28 +274,42 @@ __device__ __forceinline__ int last_pos(int t, int e) {
   return t + (e / RL) * F::TC + (e % RL) * (F::N / RL);
 }
 
+// Byte offset of register e relative to the thread's first element, on entry
+// (stage-0 layout) and on exit (last-stage layout); es_b = element stride in bytes.
+template <class F>
+__device__ __forceinline__ uint32_t first_off(int e, uint32_t es_b) {
+  return (uint32_t)((e / F::R0) * F::TC + (e % F::R0) * (F::N / F::R0)) * es_b;
+}
+template <class F>
+__device__ __forceinline__ uint32_t last_off(int e, uint32_t es_b) {
+  constexpr int RL = F::last_radix();
+  return (uint32_t)((e / RL) * F::TC + (e % RL) * (F::N / RL)) * es_b;
+}
+
 // Plain single-direction transform along one axis, C sequences per WG.
-// Sequence q = blockIdx.x*C + c, element p at base + q*ss + p*es.
+// Batch b starts at in + b*bs (elements); sequence q of it at q*ss_b bytes,
+// its element p at p*es_b bytes; nbytes bounds one batch.
 template <typename R, class F, int DIR, int C, int PS, bool SPLIT>
-__global__ void __launch_bounds__(F::TC * C) fft_reg_kernel(const cpx<R>* __restrict__ in,
-                                                            cpx<R>* __restrict__ out, long long es,
-                                                            long long ss, long long bs, int nseq,
-                                                            StageTw<R> tw, R scale) {
+__global__ void __launch_bounds__(F::TC * C) fft_reg_kernel(const cpx<R>* in, cpx<R>* out,
+                                                            uint32_t es_b, uint32_t ss_b,
+                                                            long long bs, uint32_t nbytes,
+                                                            int nseq, StageTw<R> tw, R scale) {
   __shared__ unsigned char lds[LdsSize<F, C, PS>::slots * (SPLIT ? sizeof(R) : sizeof(cpx<R>))];
   const int tid = threadIdx.x;
   const int c = tid % C;
   const int t = tid / C;
-  const int q = blockIdx.x * C + c;
+  const int q = xcd_tile(blockIdx.x, gridDim.x) * C + c;
   const bool live = q < nseq;
-  const long long base = (long long)blockIdx.y * bs + (long long)(live ? q : 0) * ss;
+  const Gio<R> gi(in + (long long)blockIdx.y * bs, nbytes);
+  const Gio<R> go(out + (long long)blockIdx.y * bs, nbytes);
+  const uint32_t vo = (uint32_t)(live ? q : 0) * ss_b + (uint32_t)t * es_b;
   cpx<R> v[F::E];
 #pragma unroll
-  for (int e = 0; e < F::E; ++e)
-    v[e] = live ? in[base + (long long)first_pos<F>(t, e) * es] : mk(R(0), R(0));
+  for (int e = 0; e < F::E; ++e) v[e] = live ? gi.ld(vo, first_off<F>(e, es_b)) : mk(R(0), R(0));
   reg_fft<F, DIR, C, PS, SPLIT>(v, tw, lds, t, c);
   if (live) {
 #pragma unroll
-    for (int e = 0; e < F::E; ++e) out[base + (long long)last_pos<F>(t, e) * es] = cscale(v[e], scale);
+    for (int e = 0; e < F::E; ++e) go.st(cscale(v[e], scale), vo, last_off<F>(e, es_b));
   }
 }
 

@@ -5,6 +5,8 @@
 #include "rsim_fft_cfg.hpp"
 #include "rsim_internal.hpp"
 
+#include <type_traits>
+
 namespace rsim {
 
 template <typename R, int N, int DIR, bool COL>
@@ -13,25 +15,29 @@ static void launch_one(const void* in, void* out, long long es, long long ss, lo
   using F = typename Sched<N>::F;
   StageTw<R> t;
   for (int i = 0; i < 4; ++i) t.t[i] = reinterpret_cast<const cpx<R>*>(tw[i]);
-  if constexpr (COL) {
-    using S = ColShape<R, N>;
-    dim3 grid((nseq + S::C - 1) / S::C, batch);
-    hipLaunchKernelGGL((fft_reg_kernel<R, F, DIR, S::C, S::PS, S::SPLIT>), grid, dim3(F::TC * S::C),
-                       0, s, reinterpret_cast<const cpx<R>*>(in), reinterpret_cast<cpx<R>*>(out),
-                       es, ss, bs, nseq, t, (R)scale);
-  } else {
-    using S = RowShape<R, N>;
-    dim3 grid((nseq + S::C - 1) / S::C, batch);
-    hipLaunchKernelGGL((fft_reg_kernel<R, F, DIR, S::C, S::PS, S::SPLIT>), grid, dim3(F::TC * S::C),
-                       0, s, reinterpret_cast<const cpx<R>*>(in), reinterpret_cast<cpx<R>*>(out),
-                       es, ss, bs, nseq, t, (R)scale);
-  }
+  const long long esz = sizeof(cpx<R>);
+  const uint32_t es_b = (uint32_t)(es * esz), ss_b = (uint32_t)(ss * esz);
+  const uint32_t nbytes = (uint32_t)(((long long)(nseq - 1) * ss + (long long)(N - 1) * es + 1) * esz);
+  using S = typename std::conditional<COL, ColShape<R, N>, RowShape<R, N>>::type;
+  dim3 grid((nseq + S::C - 1) / S::C, batch);
+  hipLaunchKernelGGL((fft_reg_kernel<R, F, DIR, S::C, S::PS, S::SPLIT>), grid, dim3(F::TC * S::C), 0,
+                     s, reinterpret_cast<const cpx<R>*>(in), reinterpret_cast<cpx<R>*>(out), es_b,
+                     ss_b, bs, nbytes, nseq, t, (R)scale);
+}
+
+// Byte extents of one batch must fit the 32-bit buffer offsets.
+template <typename R, int N>
+static bool fits32(long long es, long long ss, int nseq) {
+  const long long ext = ((long long)(nseq - 1) * ss + (long long)(N - 1) * es + 1) * (long long)sizeof(cpx<R>);
+  return ext < (1LL << 32) && es * (long long)sizeof(cpx<R>) < (1LL << 32) &&
+         ss * (long long)sizeof(cpx<R>) < (1LL << 32);
 }
 
 template <typename R, int N>
-static void launch_n(int dir, bool col, const void* in, void* out, long long es, long long ss,
+static bool launch_n(int dir, bool col, const void* in, void* out, long long es, long long ss,
                      long long bs, int nseq, int batch, const void* const* tw, double scale,
                      hipStream_t s) {
+  if (!fits32<R, N>(es, ss, nseq)) return false;
   if (dir < 0) {
     if (col) launch_one<R, N, -1, true>(in, out, es, ss, bs, nseq, batch, tw, scale, s);
     else launch_one<R, N, -1, false>(in, out, es, ss, bs, nseq, batch, tw, scale, s);
@@ -39,6 +45,7 @@ static void launch_n(int dir, bool col, const void* in, void* out, long long es,
     if (col) launch_one<R, N, 1, true>(in, out, es, ss, bs, nseq, batch, tw, scale, s);
     else launch_one<R, N, 1, false>(in, out, es, ss, bs, nseq, batch, tw, scale, s);
   }
+  return true;
 }
 
 template <typename R>
@@ -46,12 +53,12 @@ static bool launch_reg(int dir, int N, bool col, const void* in, void* out, long
                        long long ss, long long bs, int nseq, int batch, const void* const* tw,
                        double scale, hipStream_t s) {
   switch (N) {
-    case 256: launch_n<R, 256>(dir, col, in, out, es, ss, bs, nseq, batch, tw, scale, s); return true;
-    case 512: launch_n<R, 512>(dir, col, in, out, es, ss, bs, nseq, batch, tw, scale, s); return true;
-    case 1024: launch_n<R, 1024>(dir, col, in, out, es, ss, bs, nseq, batch, tw, scale, s); return true;
-    case 2048: launch_n<R, 2048>(dir, col, in, out, es, ss, bs, nseq, batch, tw, scale, s); return true;
-    case 4096: launch_n<R, 4096>(dir, col, in, out, es, ss, bs, nseq, batch, tw, scale, s); return true;
-    case 8192: launch_n<R, 8192>(dir, col, in, out, es, ss, bs, nseq, batch, tw, scale, s); return true;
+    case 256: return launch_n<R, 256>(dir, col, in, out, es, ss, bs, nseq, batch, tw, scale, s);
+    case 512: return launch_n<R, 512>(dir, col, in, out, es, ss, bs, nseq, batch, tw, scale, s);
+    case 1024: return launch_n<R, 1024>(dir, col, in, out, es, ss, bs, nseq, batch, tw, scale, s);
+    case 2048: return launch_n<R, 2048>(dir, col, in, out, es, ss, bs, nseq, batch, tw, scale, s);
+    case 4096: return launch_n<R, 4096>(dir, col, in, out, es, ss, bs, nseq, batch, tw, scale, s);
+    case 8192: return launch_n<R, 8192>(dir, col, in, out, es, ss, bs, nseq, batch, tw, scale, s);
     default: return false;
   }
 }

@@ -71,8 +71,8 @@ enum { ROW_FIRST = 0, ROW_MID = 1, ROW_LAST = 2 };
 
 template <class F, int C, int PS, bool SPLIT, int MODE>
 __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(const cpx<float>* in, cpx<float>* out,
-                                                           int ny, StageTw<float> tw,
-                                                           TransArgsF ta) {
+                                                           int ny, uint32_t nbytes,
+                                                           StageTw<float> tw, TransArgsF ta) {
   constexpr int N = F::N;
   __shared__ unsigned char lds[LdsSize<F, C, PS>::slots * (SPLIT ? 4 : 8)];
   const int tid = threadIdx.x;
@@ -81,10 +81,11 @@ __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(const cpx<float>* in,
   const int y = blockIdx.x * C + c;
   const bool live = y < ny;
   const int yy = live ? y : 0;
-  const long long base = (long long)yy * N;
+  const Gio<float> gi(in, nbytes), go(out, nbytes);
+  const uint32_t vo = ((uint32_t)yy * N + t) * 8u;
   cpx<float> v[F::E];
 #pragma unroll
-  for (int e = 0; e < F::E; ++e) v[e] = in[base + first_pos<F>(t, e)];
+  for (int e = 0; e < F::E; ++e) v[e] = gi.ld(vo, first_off<F>(e, 8u));
   if constexpr (MODE == ROW_FIRST) {
     const int y0 = ta.ym.i0[yy];
     const float fy = ta.ym.fracf[yy];
@@ -94,25 +95,27 @@ __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(const cpx<float>* in,
     reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
   } else if constexpr (MODE == ROW_MID) {
     reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);
+    compiler_fence();
     const int y0 = ta.ym.i0[yy];
     const float fy = ta.ym.fracf[yy];
     const bool iny = ta.ym.inside[yy];
 #pragma unroll
     for (int e = 0; e < F::E; ++e) v[e] = transmit(v[e], ta, last_pos<F>(t, e), y0, fy, iny);
+    compiler_fence();
     reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
   } else {
     reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);
   }
   if (live) {
 #pragma unroll
-    for (int e = 0; e < F::E; ++e) out[base + last_pos<F>(t, e)] = v[e];
+    for (int e = 0; e < F::E; ++e) go.st(v[e], vo, last_off<F>(e, 8u));
   }
 }
 
 template <class F, int C, int PS, bool SPLIT, int SPEC>
 __global__ void __launch_bounds__(F::TC * C) ms_col_kernel(const cpx<float>* in, cpx<float>* out,
                                                            cpx<float>* spec, int nx,
-                                                           StageTw<float> tw,
+                                                           uint32_t nbytes, StageTw<float> tw,
                                                            const double2* __restrict__ Bx,
                                                            const double2* __restrict__ Ay,
                                                            const int* __restrict__ Rmax,
@@ -122,38 +125,42 @@ __global__ void __launch_bounds__(F::TC * C) ms_col_kernel(const cpx<float>* in,
   const int tid = threadIdx.x;
   const int c = tid % C;
   const int t = tid / C;
-  const int x = blockIdx.x * C + c;
+  const int x = xcd_tile(blockIdx.x, gridDim.x) * C + c;
   const bool live = x < nx;
   const int xx = live ? x : 0;
+  const Gio<float> gi(in, nbytes), go(out, nbytes), gs(spec, nbytes);
+  const uint32_t row_b = (uint32_t)nx * 8u;
+  const uint32_t vo = (uint32_t)t * row_b + (uint32_t)xx * 8u;
   cpx<float> v[F::E];
 #pragma unroll
-  for (int e = 0; e < F::E; ++e) v[e] = in[(long long)first_pos<F>(t, e) * nx + xx];
+  for (int e = 0; e < F::E; ++e) v[e] = gi.ld(vo, first_off<F>(e, row_b));
   reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
+  compiler_fence();
   const double2 bx = Bx[xx];
   const int rmax = Rmax[xx];
   constexpr int half = (N - 1) / 2 + 1;
+  // groups of 4 with a compiler fence in between bound the number of FP64
+  // table values in flight (register pressure)
 #pragma unroll
   for (int e = 0; e < F::E; ++e) {
     const int y = last_pos<F>(t, e);
     const int sy = y < half ? y : y - N;
     const int ay = sy < 0 ? -sy : sy;
-    if (ay > rmax) {
-      v[e] = mk(0.f, 0.f);
-    } else {
-      const double2 a = Ay[y];
-      const double mr = bx.x * a.x - bx.y * a.y;
-      const double mi = bx.x * a.y + bx.y * a.x;
-      const double vr = (double)v[e].x, vi = (double)v[e].y;
-      v[e] = mk((float)(vr * mr - vi * mi), (float)(vr * mi + vi * mr));
-    }
+    const double2 a = Ay[y];
+    const double mr = bx.x * a.x - bx.y * a.y;
+    const double mi = bx.x * a.y + bx.y * a.x;
+    const double vr = (double)v[e].x, vi = (double)v[e].y;
+    const cpx<float> w = mk((float)(vr * mr - vi * mi), (float)(vr * mi + vi * mr));
+    v[e] = ay > rmax ? mk(0.f, 0.f) : w;
     if constexpr (SPEC) {
-      if (live) spec[(long long)y * nx + x] = mk(v[e].x * spec_scale, v[e].y * spec_scale);
+      if (live) gs.st(mk(v[e].x * spec_scale, v[e].y * spec_scale), vo, last_off<F>(e, row_b));
     }
+    if ((e & 3) == 3) compiler_fence();
   }
   reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);
   if (live) {
 #pragma unroll
-    for (int e = 0; e < F::E; ++e) out[(long long)last_pos<F>(t, e) * nx + x] = v[e];
+    for (int e = 0; e < F::E; ++e) go.st(v[e], vo, last_off<F>(e, row_b));
   }
 }
 
@@ -165,15 +172,16 @@ struct FusedRow {
   using F = typename S::F;
   static void launch(int mode, const void* in, void* out, int ny, const StageTw<float>& tw,
                      const TransArgsF& ta, hipStream_t s) {
+    const uint32_t nb = (uint32_t)((long long)N * ny * 8);
     dim3 grid((ny + S::C - 1) / S::C), block(F::TC * S::C);
     const cpx<float>* i = (const cpx<float>*)in;
     cpx<float>* o = (cpx<float>*)out;
     if (mode == ROW_FIRST)
-      hipLaunchKernelGGL((ms_row_kernel<F, S::C, S::PS, S::SPLIT, ROW_FIRST>), grid, block, 0, s, i, o, ny, tw, ta);
+      hipLaunchKernelGGL((ms_row_kernel<F, S::C, S::PS, S::SPLIT, ROW_FIRST>), grid, block, 0, s, i, o, ny, nb, tw, ta);
     else if (mode == ROW_MID)
-      hipLaunchKernelGGL((ms_row_kernel<F, S::C, S::PS, S::SPLIT, ROW_MID>), grid, block, 0, s, i, o, ny, tw, ta);
+      hipLaunchKernelGGL((ms_row_kernel<F, S::C, S::PS, S::SPLIT, ROW_MID>), grid, block, 0, s, i, o, ny, nb, tw, ta);
     else
-      hipLaunchKernelGGL((ms_row_kernel<F, S::C, S::PS, S::SPLIT, ROW_LAST>), grid, block, 0, s, i, o, ny, tw, ta);
+      hipLaunchKernelGGL((ms_row_kernel<F, S::C, S::PS, S::SPLIT, ROW_LAST>), grid, block, 0, s, i, o, ny, nb, tw, ta);
   }
 };
 
@@ -185,13 +193,14 @@ struct FusedCol {
                      const StageTw<float>& tw, const double2* Bx, const double2* Ay,
                      const int* Rmax, float spec_scale, hipStream_t s) {
     dim3 grid((nx + S::C - 1) / S::C), block(F::TC * S::C);
+    const uint32_t nb = (uint32_t)((long long)N * nx * 8);
     const cpx<float>* i = (const cpx<float>*)in;
     cpx<float>* o = (cpx<float>*)out;
     cpx<float>* sp = (cpx<float>*)spec;
     if (spec_out)
-      hipLaunchKernelGGL((ms_col_kernel<F, S::C, S::PS, S::SPLIT, 1>), grid, block, 0, s, i, o, sp, nx, tw, Bx, Ay, Rmax, spec_scale);
+      hipLaunchKernelGGL((ms_col_kernel<F, S::C, S::PS, S::SPLIT, 1>), grid, block, 0, s, i, o, sp, nx, nb, tw, Bx, Ay, Rmax, spec_scale);
     else
-      hipLaunchKernelGGL((ms_col_kernel<F, S::C, S::PS, S::SPLIT, 0>), grid, block, 0, s, i, o, sp, nx, tw, Bx, Ay, Rmax, spec_scale);
+      hipLaunchKernelGGL((ms_col_kernel<F, S::C, S::PS, S::SPLIT, 0>), grid, block, 0, s, i, o, sp, nx, nb, tw, Bx, Ay, Rmax, spec_scale);
   }
 };
 
