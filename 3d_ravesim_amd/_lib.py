@@ -29,6 +29,7 @@ RSIM_INVERSE = 1
 RSIM_PLAIN = 0
 RSIM_FUSED = 1
 RSIM_FLAG_NO_SPECTRUM = 1
+RSIM_FLAG_MATH_FP32 = 2
 
 _vp = C.c_void_p
 _i = C.c_int
@@ -73,6 +74,7 @@ SIGNATURES = {
     "rsim_transmission_2d": [_vp, _i, _i, _i, _d, _d, _d, _vp, _i, _d, _d, _d, _d, _d, _vp],
     "rsim_sample_apply_2d": [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _vp, _d, _d, _d, _d, _d, _i,
                              _i, _i, _i, _vp],
+    "rsim_fused_schedule": [_i, _i, _ip, _ip],
     "rsim_detector_area_v1": [_vp, _i, _i, _i, _d, _d, _i, _i, _d, _d, _d, _dp, _vp],
     "rsim_detector_area_v1_accumulate": [_vp, _i, _i, _i, _d, _d, _i, _i, _d, _d, _d, _d, _vp,
                                          _vp],

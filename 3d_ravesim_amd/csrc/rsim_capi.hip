@@ -29,6 +29,8 @@ int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double 
 int rsim_release_tables();
 int release_tables();
 int release_multislice_tables();
+int release_tune();
+int fused_schedule_lookup(int nx, int ny, int* rv, int* cv);
 
 namespace {
 thread_local std::string t_err;
@@ -284,6 +286,7 @@ int rsim_release_caches(void) {
   rsim_release_tables();
   release_tables();
   release_multislice_tables();
+  release_tune();
   return RSIM_OK;
 }
 
@@ -491,6 +494,11 @@ int rsim_sample_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx,
   return sample_apply(u, U, nx, ny, dtype, dx, dy, wl, cutoff_freq, vol, pixel_size_x,
                       pixel_size_y, pixel_size_z, x_pos, y_pos, first_slice, n_slices, engine,
                       flags, resolve_stream(stream));
+}
+
+int rsim_fused_schedule(int nx, int ny, int* row_variant, int* col_variant) {
+  if (!row_variant || !col_variant) return fail(RSIM_E_INVALID_ARG, "NULL output pointer");
+  return fused_schedule_lookup(nx, ny, row_variant, col_variant);
 }
 
 int rsim_detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double dy,

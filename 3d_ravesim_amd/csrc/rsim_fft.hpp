@@ -155,16 +155,20 @@ struct StageTw {
   const cpx<R>* t[4];
 };
 
-// LDS index with one pad slot every 2^PS slots (PS = 0: no padding).
+// LDS padding code: PS = (shift << 8) | k inserts k pad slots after every
+// 2^shift slots (PS = 0: no padding).  Chosen per exchange pattern to spread
+// the strided stage writes over the banks.
+constexpr int pad_code(int shift, int k) { return (shift << 8) | k; }
 template <int PS>
 __device__ __forceinline__ int lpad(int e) {
   if constexpr (PS == 0) return e;
-  else return e + (e >> PS);
+  else return e + (PS & 255) * (e >> (PS >> 8));
 }
 
 template <class F, int C, int PS>
 struct LdsSize {
-  static constexpr int slots = F::N * C + ((PS == 0) ? 0 : ((F::N * C) >> PS)) + 1;
+  static constexpr int slots =
+      F::N * C + ((PS == 0) ? 0 : (PS & 255) * ((F::N * C) >> (PS >> 8))) + 1;
 };
 
 // Output position of element (i, m) of stage s (radix RR, stride LS).
@@ -222,29 +226,54 @@ __device__ __forceinline__ void reg_exchange(cpx<R> (&v)[F::E], void* lds_raw, i
   }
 }
 
-template <class F, int S, int DIR, typename R>
-__device__ __forceinline__ void reg_stage(cpx<R> (&v)[F::E], const StageTw<R>& tw, int t) {
+// One Stockham stage on register-resident data.  With FP64 twiddle tables
+// (TW = double) and FP32 data, each butterfly is evaluated in FP64 and
+// rounded back to FP32 once: single-precision butterflies lose norm
+// systematically (about 1e-7 per transform pair), which compounds linearly
+// over hundreds of slices, while rounding only the stage outputs grows as a
+// random walk.
+template <class F, int S, int DIR, typename R, typename TW>
+__device__ __forceinline__ void reg_stage(cpx<R> (&v)[F::E], const StageTw<TW>& tw, int t) {
   constexpr int RR = F::radix(S);
   constexpr int LS = F::stride(S);
-  if constexpr (LS > 1) {
-    const cpx<R>* T = tw.t[S];
+  if constexpr (sizeof(TW) > sizeof(R)) {
 #pragma unroll
     for (int i = 0; i < F::E / RR; ++i) {
-      const int b = t + i * F::TC;
-      const int k = b % LS;
+      cpx<TW> x[RR];
 #pragma unroll
-      for (int j = 1; j < RR; ++j) v[i * RR + j] = twmul<DIR>(v[i * RR + j], T[j * k]);
+      for (int j = 0; j < RR; ++j) x[j] = mk((TW)v[i * RR + j].x, (TW)v[i * RR + j].y);
+      if constexpr (LS > 1) {
+        const cpx<TW>* T = tw.t[S];
+        const int b = t + i * F::TC;
+        const int k = b % LS;
+#pragma unroll
+        for (int j = 1; j < RR; ++j) x[j] = twmul<DIR>(x[j], T[j * k]);
+      }
+      dft<RR, DIR>(x);
+#pragma unroll
+      for (int j = 0; j < RR; ++j) v[i * RR + j] = mk((R)x[j].x, (R)x[j].y);
     }
-  }
+  } else {
+    if constexpr (LS > 1) {
+      const cpx<TW>* T = tw.t[S];
 #pragma unroll
-  for (int i = 0; i < F::E / RR; ++i) dft<RR, DIR>(&v[i * RR]);
+      for (int i = 0; i < F::E / RR; ++i) {
+        const int b = t + i * F::TC;
+        const int k = b % LS;
+#pragma unroll
+        for (int j = 1; j < RR; ++j) v[i * RR + j] = twmul<DIR>(v[i * RR + j], T[j * k]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < F::E / RR; ++i) dft<RR, DIR>(&v[i * RR]);
+  }
 }
 
 // Full transform on register-resident data.  On entry v holds the stage-0
 // inputs (v[i*R0 + j] = x[t + i*TC + j*N/R0]); on exit v holds the outputs
 // of the last stage (v[i*RL + m] = X[t + i*TC + m*N/RL]).
-template <class F, int DIR, int C, int PS, bool SPLIT, typename R>
-__device__ __forceinline__ void reg_fft(cpx<R> (&v)[F::E], const StageTw<R>& tw, void* lds, int t,
+template <class F, int DIR, int C, int PS, bool SPLIT, typename R, typename TW>
+__device__ __forceinline__ void reg_fft(cpx<R> (&v)[F::E], const StageTw<TW>& tw, void* lds, int t,
                                         int c) {
   reg_stage<F, 0, DIR>(v, tw, t);
   if constexpr (F::NS > 1) {

@@ -1,0 +1,296 @@
+// Fused complex64 multislice kernels and their launch-shape variants.
+//
+// Row pass (one or more rows y per workgroup, positions x):
+//   FIRST  V = FFT_x(T_s * u)
+//   MID    V = FFT_x(T_s * IFFT_x(W))      (IFFT unnormalised; 1/N is in M)
+//   LAST   u = IFFT_x(W)
+// Column pass (C columns x per workgroup, positions y):
+//   W = IFFT_y(M * FFT_y(V)),  M = exp(i(theta_g+theta_x+theta_y)) * cutoff / (nx*ny)
+// optionally storing the spectrum M*FFT2 (= FFT2 of the new field) on the
+// last slice.  T_s is the voxel transmission of slice s (optical_element.py:
+// 552-584); in the aligned case (every voxel coordinate an exact integer,
+// e.g. voxel pitch == field pitch) it is a per-material lookup.
+#pragma once
+
+#include "rsim_fft_cfg.hpp"
+#include "rsim_tables.hpp"
+
+namespace rsim {
+
+struct TransArgsF {
+  const uint8_t* ids;     // slice base (ny_g x nx_g)
+  int nxg;
+  AxisMapDev xm, ym;
+  const float4* lut;      // per material (hi.re, hi.im, lo.re, lo.im) of the FP64 factor
+  const float2* dbf;      // per material (delta, beta), bilinear mode
+  float K;                // 2*pi*pz/wl
+};
+
+// Transmission of one element; LUT: nearest-voxel lookup of the hi+lo split
+// factor (exact when the voxel map is integral), else bilinear delta-beta
+// interpolation and an FP32 exponential.
+template <bool LUT>
+__device__ __forceinline__ cpx<float> transmit(cpx<float> v, const TransArgsF& a, int x, int y0,
+                                               float fy, bool iny) {
+  if (!(iny && a.xm.inside[x])) return v;
+  const int x0 = a.xm.i0[x];
+  const uint8_t* r0 = a.ids + (long long)y0 * a.nxg + x0;
+  if constexpr (LUT) {
+    const float4 t = a.lut[r0[0]];
+    const float rr = v.x * t.x - v.y * t.y + (v.x * t.z - v.y * t.w);
+    const float ii = v.x * t.y + v.y * t.x + (v.x * t.w + v.y * t.z);
+    return mk(rr, ii);
+  } else {
+    const uint8_t* r1 = r0 + a.nxg;
+    const float fx = a.xm.fracf[x];
+    const float2 d00 = a.dbf[r0[0]], d01 = a.dbf[r0[1]], d10 = a.dbf[r1[0]], d11 = a.dbf[r1[1]];
+    const float gx = 1.f - fx, gy = 1.f - fy;
+    const float de = (d00.x * gx + d01.x * fx) * gy + (d10.x * gx + d11.x * fx) * fy;
+    const float be = (d00.y * gx + d01.y * fx) * gy + (d10.y * gx + d11.y * fx) * fy;
+    const float e = expf(-a.K * be);
+    float s, c;
+    sincosf(a.K * de, &s, &c);
+    const float tr = e * c, ti = -e * s;
+    return mk(v.x * tr - v.y * ti, v.x * ti + v.y * tr);
+  }
+}
+
+enum { ROW_FIRST = 0, ROW_MID = 1, ROW_LAST = 2 };
+
+template <class F, int C, int PS, bool SPLIT, int MODE, bool LUT, typename TW>
+__global__ void __launch_bounds__(F::TC * C) ms_row_kernel(const cpx<float>* in, cpx<float>* out,
+                                                           int ny, uint32_t nbytes,
+                                                           StageTw<TW> tw, TransArgsF ta) {
+  constexpr int N = F::N;
+  __shared__ unsigned char lds[LdsSize<F, C, PS>::slots * (SPLIT ? 4 : 8)];
+  const int tid = threadIdx.x;
+  const int c = tid % C;
+  const int t = tid / C;
+  const int y = blockIdx.x * C + c;
+  const bool live = y < ny;
+  const int yy = live ? y : 0;
+  const Gio<float> gi(in, nbytes), go(out, nbytes);
+  const uint32_t vo = ((uint32_t)yy * N + t) * 8u;
+  cpx<float> v[F::E];
+#pragma unroll
+  for (int e = 0; e < F::E; ++e) v[e] = gi.ld(vo, first_off<F>(e, 8u));
+  if constexpr (MODE == ROW_FIRST) {
+    const int y0 = ta.ym.i0[yy];
+    const float fy = ta.ym.fracf[yy];
+    const bool iny = ta.ym.inside[yy];
+#pragma unroll
+    for (int e = 0; e < F::E; ++e) {
+      v[e] = transmit<LUT>(v[e], ta, first_pos<F>(t, e), y0, fy, iny);
+      if ((e & 3) == 3) compiler_fence();
+    }
+    reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
+  } else if constexpr (MODE == ROW_MID) {
+    reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);
+    compiler_fence();
+    const int y0 = ta.ym.i0[yy];
+    const float fy = ta.ym.fracf[yy];
+    const bool iny = ta.ym.inside[yy];
+#pragma unroll
+    for (int e = 0; e < F::E; ++e) {
+      v[e] = transmit<LUT>(v[e], ta, last_pos<F>(t, e), y0, fy, iny);
+      if ((e & 3) == 3) compiler_fence();
+    }
+    reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
+  } else {
+    reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);
+  }
+  if (live) {
+#pragma unroll
+    for (int e = 0; e < F::E; ++e) go.st(v[e], vo, last_off<F>(e, 8u));
+  }
+}
+
+template <class F, int C, int PS, bool SPLIT, int SPEC, typename TW>
+__global__ void __launch_bounds__(F::TC * C) ms_col_kernel(const cpx<float>* in, cpx<float>* out,
+                                                           cpx<float>* spec, int nx,
+                                                           uint32_t nbytes, StageTw<TW> tw,
+                                                           const double2* __restrict__ Bx,
+                                                           const double2* __restrict__ Ay,
+                                                           const int* __restrict__ Rmax,
+                                                           float spec_scale) {
+  constexpr int N = F::N;
+  __shared__ unsigned char lds[LdsSize<F, C, PS>::slots * (SPLIT ? 4 : 8)];
+  const int tid = threadIdx.x;
+  const int c = tid % C;
+  const int t = tid / C;
+  const int x = xcd_tile(blockIdx.x, gridDim.x) * C + c;
+  const bool live = x < nx;
+  const int xx = live ? x : 0;
+  const Gio<float> gi(in, nbytes), go(out, nbytes), gs(spec, nbytes);
+  const uint32_t row_b = (uint32_t)nx * 8u;
+  const uint32_t vo = (uint32_t)t * row_b + (uint32_t)xx * 8u;
+  cpx<float> v[F::E];
+#pragma unroll
+  for (int e = 0; e < F::E; ++e) v[e] = gi.ld(vo, first_off<F>(e, row_b));
+  reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
+  compiler_fence();
+  const double2 bx = Bx[xx];
+  const int rmax = Rmax[xx];
+  constexpr int half = (N - 1) / 2 + 1;
+  // groups of 4 with a compiler fence in between bound the number of FP64
+  // table values in flight (register pressure)
+#pragma unroll
+  for (int e = 0; e < F::E; ++e) {
+    const int y = last_pos<F>(t, e);
+    const int sy = y < half ? y : y - N;
+    const int ay = sy < 0 ? -sy : sy;
+    const double2 a = Ay[y];
+    const double mr = bx.x * a.x - bx.y * a.y;
+    const double mi = bx.x * a.y + bx.y * a.x;
+    const double vr = (double)v[e].x, vi = (double)v[e].y;
+    const cpx<float> w = mk((float)(vr * mr - vi * mi), (float)(vr * mi + vi * mr));
+    v[e] = ay > rmax ? mk(0.f, 0.f) : w;
+    if constexpr (SPEC) {
+      if (live) gs.st(mk(v[e].x * spec_scale, v[e].y * spec_scale), vo, last_off<F>(e, row_b));
+    }
+    if ((e & 3) == 3) compiler_fence();
+  }
+  reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);
+  if (live) {
+#pragma unroll
+    for (int e = 0; e < F::E; ++e) go.st(v[e], vo, last_off<F>(e, row_b));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Launch-shape variants.  A variant fixes E (elements per thread) and C
+// (sequences per workgroup) for one size; the radix schedule (and so the
+// twiddle tables) is Sched<N>'s.
+// ---------------------------------------------------------------------------
+template <int N, int E>
+struct FacE {
+  using B = typename Sched<N>::F;
+  using F = RegFac<N, E, B::R0, B::R1, B::R2, B::R3>;
+};
+
+template <int N, int E, int C, bool SPLIT_ = ((size_t)N * C * 8 > 65536)>
+struct Shape {
+  using F = typename FacE<N, E>::F;
+  static constexpr int CC = C;
+  static constexpr bool SPLIT = SPLIT_;
+  // rows: 1 pad slot per 32; columns: C pad slots per 16*C (bank model)
+  static constexpr int PS = C == 1 ? pad_code(5, 1)
+                                   : pad_code(4 + (C >= 2) + (C >= 4) + (C >= 8) + (C >= 16), C);
+  static constexpr int threads = F::TC * C;
+};
+
+template <class S, typename TW>
+struct RowLaunch {
+  static void go(int mode, bool lut, const void* in, void* out, int ny, uint32_t nb,
+                 const StageTw<TW>& tw, const TransArgsF& ta, hipStream_t s) {
+    using F = typename S::F;
+    constexpr int C = S::CC;
+    dim3 grid((ny + C - 1) / C), block(S::threads);
+    const cpx<float>* i = (const cpx<float>*)in;
+    cpx<float>* o = (cpx<float>*)out;
+    if (mode == ROW_FIRST) {
+      if (lut) hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_FIRST, true, TW>), grid, block, 0, s, i, o, ny, nb, tw, ta);
+      else hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_FIRST, false, TW>), grid, block, 0, s, i, o, ny, nb, tw, ta);
+    } else if (mode == ROW_MID) {
+      if (lut) hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_MID, true, TW>), grid, block, 0, s, i, o, ny, nb, tw, ta);
+      else hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_MID, false, TW>), grid, block, 0, s, i, o, ny, nb, tw, ta);
+    } else {
+      hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_LAST, true, TW>), grid, block, 0, s, i, o, ny, nb, tw, ta);
+    }
+  }
+};
+
+template <class S, typename TW>
+struct ColLaunch {
+  static void go(bool spec_out, const void* in, void* out, void* spec, int nx, uint32_t nb,
+                 const StageTw<TW>& tw, const double2* Bx, const double2* Ay, const int* Rmax,
+                 float spec_scale, hipStream_t s) {
+    using F = typename S::F;
+    constexpr int C = S::CC;
+    dim3 grid((nx + C - 1) / C), block(S::threads);
+    const cpx<float>* i = (const cpx<float>*)in;
+    cpx<float>* o = (cpx<float>*)out;
+    cpx<float>* sp = (cpx<float>*)spec;
+    if (spec_out)
+      hipLaunchKernelGGL((ms_col_kernel<F, C, S::PS, S::SPLIT, 1, TW>), grid, block, 0, s, i, o, sp, nx, nb, tw, Bx, Ay, Rmax, spec_scale);
+    else
+      hipLaunchKernelGGL((ms_col_kernel<F, C, S::PS, S::SPLIT, 0, TW>), grid, block, 0, s, i, o, sp, nx, nb, tw, Bx, Ay, Rmax, spec_scale);
+  }
+};
+
+// Variant lists: X(N, id, E, C, SPLIT).  Variant 0 is the default.
+#define RSIM_ROW_VARIANTS(N, X)                                                           \
+  X(N, 0, 16, (256 / (N / 16) > 0 ? 256 / (N / 16) : 1), ((size_t)N * (256 / (N / 16) > 0 ? 256 / (N / 16) : 1) * 8 > 65536)) \
+  X(N, 1, 16, (512 / (N / 16) > 0 ? 512 / (N / 16) : 1), ((size_t)N * (512 / (N / 16) > 0 ? 512 / (N / 16) : 1) * 8 > 65536)) \
+  X(N, 2, 16, (256 / (N / 16) > 0 ? 256 / (N / 16) : 1), true)
+#define RSIM_COL_C16(N) (1024 / (N / 16) < 16 ? 1024 / (N / 16) : 16)
+#define RSIM_COL_VARIANTS(N, X)                                                           \
+  X(N, 0, 16, RSIM_COL_C16(N), true)                                                      \
+  X(N, 1, 16, RSIM_COL_C16(N), ((size_t)N * RSIM_COL_C16(N) * 8 > 163840 - 8192))        \
+  X(N, 2, 32, (512 / (N / 32) < 16 ? 512 / (N / 32) : 16), true)                         \
+  X(N, 3, 16, (RSIM_COL_C16(N) / 2 > 0 ? RSIM_COL_C16(N) / 2 : 1), true)
+constexpr int kRowVariants = 3;
+constexpr int kColVariants = 4;
+
+// Per-size launchers (explicitly instantiated in rsim_fused_<N>.hip).
+template <int N, typename TW>
+bool row_go(int v, int mode, bool lut, const void* in, void* out, int ny, const StageTw<TW>& tw,
+            const TransArgsF& ta, hipStream_t s);
+template <int N, typename TW>
+bool col_go(int v, bool spec_out, const void* in, void* out, void* spec, int nx,
+            const StageTw<TW>& tw, const double2* Bx, const double2* Ay, const int* Rmax,
+            float ss, hipStream_t s);
+
+#define RSIM_ROW_CASE(N, V, E, C, SP)                                               \
+  case V:                                                                          \
+    RowLaunch<Shape<N, E, C, SP>, TW>::go(mode, lut, in, out, ny, nb, tw, ta, s);  \
+    return true;
+#define RSIM_COL_CASE(N, V, E, C, SP)                                                                 \
+  case V:                                                                                             \
+    ColLaunch<Shape<N, E, C, SP>, TW>::go(spec_out, in, out, spec, nx, nb, tw, Bx, Ay, Rmax, ss, s);  \
+    return true;
+
+#define RSIM_FUSED_INSTANTIATE(N)                                                                 \
+  template <int NN, typename TW>                                                                  \
+  bool row_go_impl(int v, int mode, bool lut, const void* in, void* out, int ny,                  \
+                   const StageTw<TW>& tw, const TransArgsF& ta, hipStream_t s) {                  \
+    const uint32_t nb = (uint32_t)((long long)NN * ny * 8);                                      \
+    switch (v) {                                                                                  \
+      RSIM_ROW_VARIANTS(NN, RSIM_ROW_CASE)                                                        \
+      default: return false;                                                                      \
+    }                                                                                             \
+  }                                                                                               \
+  template <int NN, typename TW>                                                                  \
+  bool col_go_impl(int v, bool spec_out, const void* in, void* out, void* spec, int nx,           \
+                   const StageTw<TW>& tw, const double2* Bx, const double2* Ay, const int* Rmax,  \
+                   float ss, hipStream_t s) {                                                     \
+    const uint32_t nb = (uint32_t)((long long)NN * nx * 8);                                      \
+    switch (v) {                                                                                  \
+      RSIM_COL_VARIANTS(NN, RSIM_COL_CASE)                                                        \
+      default: return false;                                                                      \
+    }                                                                                             \
+  }                                                                                               \
+  template <>                                                                                     \
+  bool row_go<N, float>(int v, int mode, bool lut, const void* in, void* out, int ny,             \
+                        const StageTw<float>& tw, const TransArgsF& ta, hipStream_t s) {          \
+    return row_go_impl<N, float>(v, mode, lut, in, out, ny, tw, ta, s);                           \
+  }                                                                                               \
+  template <>                                                                                     \
+  bool row_go<N, double>(int v, int mode, bool lut, const void* in, void* out, int ny,            \
+                         const StageTw<double>& tw, const TransArgsF& ta, hipStream_t s) {        \
+    return row_go_impl<N, double>(v, mode, lut, in, out, ny, tw, ta, s);                          \
+  }                                                                                               \
+  template <>                                                                                     \
+  bool col_go<N, float>(int v, bool spec_out, const void* in, void* out, void* spec, int nx,      \
+                        const StageTw<float>& tw, const double2* Bx, const double2* Ay,           \
+                        const int* Rmax, float ss, hipStream_t s) {                               \
+    return col_go_impl<N, float>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);        \
+  }                                                                                               \
+  template <>                                                                                     \
+  bool col_go<N, double>(int v, bool spec_out, const void* in, void* out, void* spec, int nx,     \
+                         const StageTw<double>& tw, const double2* Bx, const double2* Ay,         \
+                         const int* Rmax, float ss, hipStream_t s) {                              \
+    return col_go_impl<N, double>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);       \
+  }
+
+}  // namespace rsim

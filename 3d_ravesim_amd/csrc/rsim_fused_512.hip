@@ -1,0 +1,7 @@
+// Fused multislice kernels for length-512 axes (one translation unit per size
+// so the kernel variants compile in parallel).
+#include "rsim_fused.hpp"
+
+namespace rsim {
+RSIM_FUSED_INSTANTIATE(512)
+}  // namespace rsim

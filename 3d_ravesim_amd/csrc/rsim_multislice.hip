@@ -13,7 +13,10 @@
 // slice to slice; voxel transmission factors are FP64 values split into FP32
 // hi+lo pairs for the same reason.
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -22,6 +25,7 @@
 #include "rsim_fft_cfg.hpp"
 #include "rsim_internal.hpp"
 #include "rsim_tables.hpp"
+#include "rsim_fused.hpp"
 
 namespace rsim {
 
@@ -29,204 +33,33 @@ int transmission(void* u, int nx, int ny, int dtype, double dx, double dy, doubl
                  const rsim_volume* vol, int slice, double px, double py, double pz, double xpos,
                  double ypos, hipStream_t s);
 
-// ---------------------------------------------------------------------------
-// Fused kernels
-// ---------------------------------------------------------------------------
-struct TransArgsF {
-  const uint8_t* ids;     // slice base (ny_g x nx_g)
-  int nxg;
-  AxisMapDev xm, ym;
-  const float4* lut;      // per material (hi.re, hi.im, lo.re, lo.im)
-  const float2* dbf;      // per material (delta, beta), bilinear mode
-  float K;                // 2*pi*pz/wl
-  int lut_mode;
-};
-
-__device__ __forceinline__ cpx<float> transmit(cpx<float> v, const TransArgsF& a, int x, int y0,
-                                               float fy, bool iny) {
-  if (!(iny && a.xm.inside[x])) return v;
-  const int x0 = a.xm.i0[x];
-  const uint8_t* r0 = a.ids + (long long)y0 * a.nxg + x0;
-  if (a.lut_mode) {
-    const float4 t = a.lut[r0[0]];
-    // v * (hi + lo)
-    const float rr = v.x * t.x - v.y * t.y + (v.x * t.z - v.y * t.w);
-    const float ii = v.x * t.y + v.y * t.x + (v.x * t.w + v.y * t.z);
-    return mk(rr, ii);
-  }
-  const uint8_t* r1 = r0 + a.nxg;
-  const float fx = a.xm.fracf[x];
-  const float2 d00 = a.dbf[r0[0]], d01 = a.dbf[r0[1]], d10 = a.dbf[r1[0]], d11 = a.dbf[r1[1]];
-  const float gx = 1.f - fx, gy = 1.f - fy;
-  const float de = (d00.x * gx + d01.x * fx) * gy + (d10.x * gx + d11.x * fx) * fy;
-  const float be = (d00.y * gx + d01.y * fx) * gy + (d10.y * gx + d11.y * fx) * fy;
-  const float e = expf(-a.K * be);
-  float s, c;
-  sincosf(a.K * de, &s, &c);
-  const float tr = e * c, ti = -e * s;
-  return mk(v.x * tr - v.y * ti, v.x * ti + v.y * tr);
-}
-
-enum { ROW_FIRST = 0, ROW_MID = 1, ROW_LAST = 2 };
-
-template <class F, int C, int PS, bool SPLIT, int MODE>
-__global__ void __launch_bounds__(F::TC * C) ms_row_kernel(const cpx<float>* in, cpx<float>* out,
-                                                           int ny, uint32_t nbytes,
-                                                           StageTw<float> tw, TransArgsF ta) {
-  constexpr int N = F::N;
-  __shared__ unsigned char lds[LdsSize<F, C, PS>::slots * (SPLIT ? 4 : 8)];
-  const int tid = threadIdx.x;
-  const int c = tid % C;
-  const int t = tid / C;
-  const int y = blockIdx.x * C + c;
-  const bool live = y < ny;
-  const int yy = live ? y : 0;
-  const Gio<float> gi(in, nbytes), go(out, nbytes);
-  const uint32_t vo = ((uint32_t)yy * N + t) * 8u;
-  cpx<float> v[F::E];
-#pragma unroll
-  for (int e = 0; e < F::E; ++e) v[e] = gi.ld(vo, first_off<F>(e, 8u));
-  if constexpr (MODE == ROW_FIRST) {
-    const int y0 = ta.ym.i0[yy];
-    const float fy = ta.ym.fracf[yy];
-    const bool iny = ta.ym.inside[yy];
-#pragma unroll
-    for (int e = 0; e < F::E; ++e) v[e] = transmit(v[e], ta, first_pos<F>(t, e), y0, fy, iny);
-    reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
-  } else if constexpr (MODE == ROW_MID) {
-    reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);
-    compiler_fence();
-    const int y0 = ta.ym.i0[yy];
-    const float fy = ta.ym.fracf[yy];
-    const bool iny = ta.ym.inside[yy];
-#pragma unroll
-    for (int e = 0; e < F::E; ++e) v[e] = transmit(v[e], ta, last_pos<F>(t, e), y0, fy, iny);
-    compiler_fence();
-    reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
-  } else {
-    reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);
-  }
-  if (live) {
-#pragma unroll
-    for (int e = 0; e < F::E; ++e) go.st(v[e], vo, last_off<F>(e, 8u));
-  }
-}
-
-template <class F, int C, int PS, bool SPLIT, int SPEC>
-__global__ void __launch_bounds__(F::TC * C) ms_col_kernel(const cpx<float>* in, cpx<float>* out,
-                                                           cpx<float>* spec, int nx,
-                                                           uint32_t nbytes, StageTw<float> tw,
-                                                           const double2* __restrict__ Bx,
-                                                           const double2* __restrict__ Ay,
-                                                           const int* __restrict__ Rmax,
-                                                           float spec_scale) {
-  constexpr int N = F::N;
-  __shared__ unsigned char lds[LdsSize<F, C, PS>::slots * (SPLIT ? 4 : 8)];
-  const int tid = threadIdx.x;
-  const int c = tid % C;
-  const int t = tid / C;
-  const int x = xcd_tile(blockIdx.x, gridDim.x) * C + c;
-  const bool live = x < nx;
-  const int xx = live ? x : 0;
-  const Gio<float> gi(in, nbytes), go(out, nbytes), gs(spec, nbytes);
-  const uint32_t row_b = (uint32_t)nx * 8u;
-  const uint32_t vo = (uint32_t)t * row_b + (uint32_t)xx * 8u;
-  cpx<float> v[F::E];
-#pragma unroll
-  for (int e = 0; e < F::E; ++e) v[e] = gi.ld(vo, first_off<F>(e, row_b));
-  reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
-  compiler_fence();
-  const double2 bx = Bx[xx];
-  const int rmax = Rmax[xx];
-  constexpr int half = (N - 1) / 2 + 1;
-  // groups of 4 with a compiler fence in between bound the number of FP64
-  // table values in flight (register pressure)
-#pragma unroll
-  for (int e = 0; e < F::E; ++e) {
-    const int y = last_pos<F>(t, e);
-    const int sy = y < half ? y : y - N;
-    const int ay = sy < 0 ? -sy : sy;
-    const double2 a = Ay[y];
-    const double mr = bx.x * a.x - bx.y * a.y;
-    const double mi = bx.x * a.y + bx.y * a.x;
-    const double vr = (double)v[e].x, vi = (double)v[e].y;
-    const cpx<float> w = mk((float)(vr * mr - vi * mi), (float)(vr * mi + vi * mr));
-    v[e] = ay > rmax ? mk(0.f, 0.f) : w;
-    if constexpr (SPEC) {
-      if (live) gs.st(mk(v[e].x * spec_scale, v[e].y * spec_scale), vo, last_off<F>(e, row_b));
-    }
-    if ((e & 3) == 3) compiler_fence();
-  }
-  reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);
-  if (live) {
-#pragma unroll
-    for (int e = 0; e < F::E; ++e) go.st(v[e], vo, last_off<F>(e, row_b));
-  }
-}
-
 namespace {
 
-template <int N>
-struct FusedRow {
-  using S = RowShape<float, N>;
-  using F = typename S::F;
-  static void launch(int mode, const void* in, void* out, int ny, const StageTw<float>& tw,
-                     const TransArgsF& ta, hipStream_t s) {
-    const uint32_t nb = (uint32_t)((long long)N * ny * 8);
-    dim3 grid((ny + S::C - 1) / S::C), block(F::TC * S::C);
-    const cpx<float>* i = (const cpx<float>*)in;
-    cpx<float>* o = (cpx<float>*)out;
-    if (mode == ROW_FIRST)
-      hipLaunchKernelGGL((ms_row_kernel<F, S::C, S::PS, S::SPLIT, ROW_FIRST>), grid, block, 0, s, i, o, ny, nb, tw, ta);
-    else if (mode == ROW_MID)
-      hipLaunchKernelGGL((ms_row_kernel<F, S::C, S::PS, S::SPLIT, ROW_MID>), grid, block, 0, s, i, o, ny, nb, tw, ta);
-    else
-      hipLaunchKernelGGL((ms_row_kernel<F, S::C, S::PS, S::SPLIT, ROW_LAST>), grid, block, 0, s, i, o, ny, nb, tw, ta);
-  }
-};
-
-template <int N>
-struct FusedCol {
-  using S = ColShape<float, N>;
-  using F = typename S::F;
-  static void launch(bool spec_out, const void* in, void* out, void* spec, int nx,
-                     const StageTw<float>& tw, const double2* Bx, const double2* Ay,
-                     const int* Rmax, float spec_scale, hipStream_t s) {
-    dim3 grid((nx + S::C - 1) / S::C), block(F::TC * S::C);
-    const uint32_t nb = (uint32_t)((long long)N * nx * 8);
-    const cpx<float>* i = (const cpx<float>*)in;
-    cpx<float>* o = (cpx<float>*)out;
-    cpx<float>* sp = (cpx<float>*)spec;
-    if (spec_out)
-      hipLaunchKernelGGL((ms_col_kernel<F, S::C, S::PS, S::SPLIT, 1>), grid, block, 0, s, i, o, sp, nx, nb, tw, Bx, Ay, Rmax, spec_scale);
-    else
-      hipLaunchKernelGGL((ms_col_kernel<F, S::C, S::PS, S::SPLIT, 0>), grid, block, 0, s, i, o, sp, nx, nb, tw, Bx, Ay, Rmax, spec_scale);
-  }
-};
-
-bool fused_row(int N, int mode, const void* in, void* out, int ny, const StageTw<float>& tw,
-               const TransArgsF& ta, hipStream_t s) {
+template <typename TW>
+bool fused_row(int N, int v, int mode, bool lut, const void* in, void* out, int ny,
+               const StageTw<TW>& tw, const TransArgsF& ta, hipStream_t s) {
   switch (N) {
-    case 256: FusedRow<256>::launch(mode, in, out, ny, tw, ta, s); return true;
-    case 512: FusedRow<512>::launch(mode, in, out, ny, tw, ta, s); return true;
-    case 1024: FusedRow<1024>::launch(mode, in, out, ny, tw, ta, s); return true;
-    case 2048: FusedRow<2048>::launch(mode, in, out, ny, tw, ta, s); return true;
-    case 4096: FusedRow<4096>::launch(mode, in, out, ny, tw, ta, s); return true;
-    case 8192: FusedRow<8192>::launch(mode, in, out, ny, tw, ta, s); return true;
+    case 256: return row_go<256, TW>(v, mode, lut, in, out, ny, tw, ta, s);
+    case 512: return row_go<512, TW>(v, mode, lut, in, out, ny, tw, ta, s);
+    case 1024: return row_go<1024, TW>(v, mode, lut, in, out, ny, tw, ta, s);
+    case 2048: return row_go<2048, TW>(v, mode, lut, in, out, ny, tw, ta, s);
+    case 4096: return row_go<4096, TW>(v, mode, lut, in, out, ny, tw, ta, s);
+    case 8192: return row_go<8192, TW>(v, mode, lut, in, out, ny, tw, ta, s);
     default: return false;
   }
 }
 
-bool fused_col(int N, bool spec_out, const void* in, void* out, void* spec, int nx,
-               const StageTw<float>& tw, const double2* Bx, const double2* Ay, const int* Rmax,
-               float spec_scale, hipStream_t s) {
+template <typename TW>
+bool fused_col(int N, int v, bool spec_out, const void* in, void* out, void* spec, int nx,
+               const StageTw<TW>& tw, const double2* Bx, const double2* Ay, const int* Rmax,
+               float ss, hipStream_t s) {
   switch (N) {
-    case 256: FusedCol<256>::launch(spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, spec_scale, s); return true;
-    case 512: FusedCol<512>::launch(spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, spec_scale, s); return true;
-    case 1024: FusedCol<1024>::launch(spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, spec_scale, s); return true;
-    case 2048: FusedCol<2048>::launch(spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, spec_scale, s); return true;
-    case 4096: FusedCol<4096>::launch(spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, spec_scale, s); return true;
-    case 8192: FusedCol<8192>::launch(spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, spec_scale, s); return true;
+    case 256: return col_go<256, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
+    case 512: return col_go<512, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
+    case 1024: return col_go<1024, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
+    case 2048: return col_go<2048, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
+    case 4096: return col_go<4096, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
+    case 8192: return col_go<8192, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
     default: return false;
   }
 }
@@ -349,6 +182,115 @@ int material_tables(const rsim_volume* vol, double K, const float4** lut, const 
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Launch-shape autotuner: the first fused call for a (device, nx, ny,
+// transmission mode) times every row-pass and column-pass variant on a
+// scratch field (hipEvents, 1 warm-up + 3 timed launches each) and keeps the
+// fastest.  RAVESIM_FUSED_VARIANT="row,col" overrides the choice.
+// ---------------------------------------------------------------------------
+namespace {
+std::map<std::tuple<int, int, int, int>, std::pair<int, int>> g_tune;
+std::mutex g_tune_mu;
+
+int time_launches(hipStream_t s, const std::function<bool()>& launch, float* ms) {
+  hipEvent_t a, b;
+  RSIM_HIP(hipEventCreate(&a));
+  RSIM_HIP(hipEventCreate(&b));
+  bool ok = launch();
+  hipError_t e = hipGetLastError();
+  if (!ok || e != hipSuccess) {
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    *ms = -1.f;
+    return RSIM_OK;
+  }
+  RSIM_HIP(hipEventRecord(a, s));
+  for (int i = 0; i < 3; ++i) launch();
+  RSIM_HIP(hipEventRecord(b, s));
+  RSIM_HIP(hipEventSynchronize(b));
+  RSIM_HIP(hipEventElapsedTime(ms, a, b));
+  e = hipGetLastError();
+  if (e != hipSuccess) *ms = -1.f;
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  return RSIM_OK;
+}
+}  // namespace
+
+template <typename TW>
+int fused_schedule(int nx, int ny, bool lut, const StageTw<TW>& tx, const StageTw<TW>& ty,
+                   const TransArgsF& ta, const ChirpTables* ct, hipStream_t s, int* rv, int* cv) {
+  if (const char* env = std::getenv("RAVESIM_FUSED_VARIANT")) {
+    int r = 0, c = 0;
+    if (std::sscanf(env, "%d,%d", &r, &c) == 2 && r >= 0 && r < kRowVariants && c >= 0 &&
+        c < kColVariants) {
+      *rv = r;
+      *cv = c;
+      return RSIM_OK;
+    }
+  }
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  const int math = sizeof(TW) == 8 ? 2 : 0;
+  auto key = std::make_tuple(current_device(), nx, ny, (lut ? 1 : 0) | math);
+  auto it = g_tune.find(key);
+  if (it != g_tune.end()) {
+    *rv = it->second.first;
+    *cv = it->second.second;
+    return RSIM_OK;
+  }
+  void* tmp = nullptr;
+  RSIM_HIP(hipMalloc(&tmp, (size_t)nx * ny * 8));
+  RSIM_HIP(hipMemsetAsync(tmp, 0, (size_t)nx * ny * 8, s));
+  int best_r = 0, best_c = 0;
+  float tr_best = 1e30f, tc_best = 1e30f;
+  const bool verbose = std::getenv("RAVESIM_TUNE_VERBOSE") != nullptr;
+  for (int v = 0; v < kRowVariants; ++v) {
+    float ms = -1.f;
+    int rc = time_launches(s, [&] { return fused_row(nx, v, ROW_MID, lut, tmp, tmp, ny, tx, ta, s); }, &ms);
+    if (rc != RSIM_OK) { hipFree(tmp); return rc; }
+    if (ms > 0.f && ms < tr_best) { tr_best = ms; best_r = v; }
+    if (verbose)
+      std::fprintf(stderr, "[rsim tune] %dx%d math=%s row variant %d: %.1f us\n", nx, ny,
+                   math ? "fp64" : "fp32", v, ms * 1e3f / 3);
+  }
+  for (int v = 0; v < kColVariants; ++v) {
+    float ms = -1.f;
+    int rc = time_launches(s, [&] {
+      return fused_col(ny, v, false, tmp, tmp, tmp, nx, ty, ct->Bx, ct->Ay, ct->Rmax, 1.f, s);
+    }, &ms);
+    if (rc != RSIM_OK) { hipFree(tmp); return rc; }
+    if (ms > 0.f && ms < tc_best) { tc_best = ms; best_c = v; }
+    if (verbose)
+      std::fprintf(stderr, "[rsim tune] %dx%d math=%s col variant %d: %.1f us\n", nx, ny,
+                   math ? "fp64" : "fp32", v, ms * 1e3f / 3);
+  }
+  RSIM_HIP(hipStreamSynchronize(s));
+  RSIM_HIP(hipFree(tmp));
+  g_tune[key] = {best_r, best_c};
+  *rv = best_r;
+  *cv = best_c;
+  return RSIM_OK;
+}
+
+int fused_schedule_lookup(int nx, int ny, int* rv, int* cv) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  *rv = *cv = -1;
+  for (auto& kv : g_tune) {
+    if (std::get<0>(kv.first) == current_device() && std::get<1>(kv.first) == nx &&
+        std::get<2>(kv.first) == ny) {
+      *rv = kv.second.first;
+      *cv = kv.second.second;
+    }
+  }
+  return RSIM_OK;
+}
+
+int release_tune() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.clear();
+  return RSIM_OK;
+}
+
 int release_multislice_tables() {
   std::lock_guard<std::mutex> lk(g_mu);
   for (auto& kv : g_chirp) {
@@ -380,6 +322,73 @@ int propagate(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
   return propagate_plain(u, U, nx, ny, dtype, dx, dy, wl, dz, freq, skip_fft, s);
 }
 
+template <typename TW>
+static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, double wl,
+                     double freq, const rsim_volume* vol, double px, double py, double pz,
+                     double xpos, double ypos, int first, int count, bool want_spec,
+                     hipStream_t s) {
+  const int total = count;
+  const int tdtype = sizeof(TW) == 8 ? RSIM_C128 : RSIM_C64;
+  const void* twx[4];
+  const void* twy[4];
+  if (!reg_stage_tables(tdtype, nx, twx) || !reg_stage_tables(tdtype, ny, twy))
+    return fail(RSIM_E_HIP, "missing FFT schedule tables");
+  StageTw<TW> tx, ty;
+  for (int i = 0; i < 4; ++i) {
+    tx.t[i] = (const cpx<TW>*)twx[i];
+    ty.t[i] = (const cpx<TW>*)twy[i];
+  }
+  const ChirpTables* ct = nullptr;
+  RSIM_CHECK(chirp_tables(nx, ny, dx, dy, wl, pz, freq, &ct));
+  const AxisMap* xm = nullptr;
+  const AxisMap* ym = nullptr;
+  RSIM_CHECK(axis_map(nx, dx, 1.0, xpos, vol->nx_g, px, &xm));
+  RSIM_CHECK(axis_map(ny, dy, 1.0, ypos, vol->ny_g, py, &ym));
+  const double K = ((2.0 * M_PI) * pz) / wl;
+  TransArgsF ta;
+  RSIM_CHECK(material_tables(vol, K, &ta.lut, &ta.dbf));
+  ta.nxg = vol->nx_g;
+  ta.xm = xm->dev;
+  ta.ym = ym->dev;
+  ta.K = (float)K;
+  const bool lut = xm->all_integral && ym->all_integral;
+  const float spec_scale = (float)((double)nx * (double)ny);
+  const size_t slice_bytes = (size_t)vol->ny_g * vol->nx_g;
+  ta.ids = vol->ids + (size_t)first * slice_bytes;
+  int rv = 0, cv = 0;
+  RSIM_CHECK(fused_schedule<TW>(nx, ny, lut, tx, ty, ta, ct, s, &rv, &cv));
+  for (int k = 0; k < count; ++k) {
+    const int sl = first + k;
+    ta.ids = vol->ids + (size_t)sl * slice_bytes;
+    const bool last = k == count - 1;
+    ktime_begin(KT_ROW, s);
+    if (!fused_row(nx, rv, k == 0 ? ROW_FIRST : ROW_MID, lut, k == 0 ? u : U, U, ny, tx, ta, s))
+      return fail(RSIM_E_HIP, "no fused row kernel for nx=%d", nx);
+    ktime_end(KT_ROW, s);
+    RSIM_LAUNCHED();
+    // last slice: W -> u (and the spectrum -> U); otherwise in place in U
+    ktime_begin(KT_COL, s);
+    if (!fused_col(ny, cv, last && want_spec, U, last ? u : U, U, nx, ty, ct->Bx, ct->Ay,
+                   ct->Rmax, spec_scale, s))
+      return fail(RSIM_E_HIP, "no fused column kernel for ny=%d", ny);
+    ktime_end(KT_COL, s);
+    RSIM_LAUNCHED();
+    const bool cancel = !last && cancel_requested();
+    if (cancel || last) {
+      // row IFFT of W; after a cancel this leaves u consistent at a slice
+      // boundary (U then does not hold FFT2(u))
+      ktime_begin(KT_ROW, s);
+      fused_row(nx, rv, ROW_LAST, lut, last ? u : U, u, ny, tx, ta, s);
+      ktime_end(KT_ROW, s);
+      RSIM_LAUNCHED();
+      if (cancel) return fail(RSIM_E_CANCELLED, "multislice cancelled after slice %d", sl);
+    }
+    if (g_progress_sync()) RSIM_HIP(hipStreamSynchronize(s));
+    report_progress("sample_slice", k + 1, total);
+  }
+  return RSIM_OK;
+}
+
 int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
                  double freq, const rsim_volume* vol, double px, double py, double pz,
                  double xpos, double ypos, int first, int count, int engine, int flags,
@@ -398,60 +407,12 @@ int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double 
     return RSIM_OK;
   }
   // ---- fused complex64 engine ----
-  const void* twx[4];
-  const void* twy[4];
-  if (!reg_stage_tables(dtype, nx, twx) || !reg_stage_tables(dtype, ny, twy))
-    return fail(RSIM_E_HIP, "missing FFT schedule tables");
-  StageTw<float> tx, ty;
-  for (int i = 0; i < 4; ++i) {
-    tx.t[i] = (const cpx<float>*)twx[i];
-    ty.t[i] = (const cpx<float>*)twy[i];
-  }
-  const ChirpTables* ct = nullptr;
-  RSIM_CHECK(chirp_tables(nx, ny, dx, dy, wl, pz, freq, &ct));
-  const AxisMap* xm = nullptr;
-  const AxisMap* ym = nullptr;
-  RSIM_CHECK(axis_map(nx, dx, 1.0, xpos, vol->nx_g, px, &xm));
-  RSIM_CHECK(axis_map(ny, dy, 1.0, ypos, vol->ny_g, py, &ym));
-  const double K = ((2.0 * M_PI) * pz) / wl;
-  TransArgsF ta;
-  RSIM_CHECK(material_tables(vol, K, &ta.lut, &ta.dbf));
-  ta.nxg = vol->nx_g;
-  ta.xm = xm->dev;
-  ta.ym = ym->dev;
-  ta.K = (float)K;
-  ta.lut_mode = (xm->all_integral && ym->all_integral) ? 1 : 0;
-  const float spec_scale = (float)((double)nx * (double)ny);
-  const size_t slice_bytes = (size_t)vol->ny_g * vol->nx_g;
-  for (int k = 0; k < count; ++k) {
-    const int sl = first + k;
-    ta.ids = vol->ids + (size_t)sl * slice_bytes;
-    const bool last = k == count - 1;
-    ktime_begin(KT_ROW, s);
-    if (!fused_row(nx, k == 0 ? ROW_FIRST : ROW_MID, k == 0 ? u : U, U, ny, tx, ta, s))
-      return fail(RSIM_E_HIP, "no fused row kernel for nx=%d", nx);
-    ktime_end(KT_ROW, s);
-    RSIM_LAUNCHED();
-    // last slice: W -> u (and the spectrum -> U); otherwise in place in U
-    ktime_begin(KT_COL, s);
-    if (!fused_col(ny, last && want_spec, U, last ? u : U, U, nx, ty, ct->Bx, ct->Ay, ct->Rmax,
-                   spec_scale, s))
-      return fail(RSIM_E_HIP, "no fused column kernel for ny=%d", ny);
-    ktime_end(KT_COL, s);
-    RSIM_LAUNCHED();
-    const bool cancel = !last && cancel_requested();
-    if (cancel || last) {
-      // row IFFT of W; after a cancel this leaves u consistent at a slice
-      // boundary (U then does not hold FFT2(u))
-      ktime_begin(KT_ROW, s);
-      fused_row(nx, ROW_LAST, last ? u : U, u, ny, tx, ta, s);
-      ktime_end(KT_ROW, s);
-      RSIM_LAUNCHED();
-      if (cancel) return fail(RSIM_E_CANCELLED, "multislice cancelled after slice %d", sl);
-    }
-    if (g_progress_sync()) RSIM_HIP(hipStreamSynchronize(s));
-    report_progress("sample_slice", k + 1, total);
-  }
+  // butterflies in FP64 (FP64 twiddle tables) unless FP32 math is requested
+  if (flags & RSIM_FLAG_MATH_FP32)
+    return run_fused<float>(u, U, nx, ny, dx, dy, wl, freq, vol, px, py, pz, xpos, ypos, first,
+                            count, want_spec, s);
+  return run_fused<double>(u, U, nx, ny, dx, dy, wl, freq, vol, px, py, pz, xpos, ypos, first,
+                           count, want_spec, s);
   return RSIM_OK;
 }
 

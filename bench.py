@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--cpu-slices", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 3 times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; the multislice engine has no "
@@ -92,6 +94,17 @@ def main():
     lib.load()
     if lib.device_count() < 1:
         raise RuntimeError("no HIP device visible")
+    if args.calibrate_copy:
+        nbytes = args.calibrate_copy << 20
+        a, b = C.c_void_p(), C.c_void_p()
+        lib.call("rsim_malloc", C.byref(a), nbytes)
+        lib.call("rsim_malloc", C.byref(b), nbytes)
+        lib.call("rsim_memset", a, 1, nbytes, None)
+        for _ in range(3):
+            lib.call("rsim_memcpy_d2d", b, a, nbytes, None)
+        lib.call("rsim_synchronize", None)
+        print(json.dumps({"copy_bytes_per_launch": nbytes, "launches": 3}), flush=True)
+        return
     n, nz = args.n, args.slices
     dx = 0.5e-6
     wl = 0.5e-10
@@ -175,6 +188,16 @@ def main():
         "reference_published_gpu_slice_steps_per_s": 23.1,
     }
 
+    # ---- fused with FP32 butterflies (reported beside the default) ---------
+    t0 = time.perf_counter()
+    for _ in range(max(1, args.steps // 2)):
+        run_multislice(lib, u, U, n, vol, lib.RSIM_FUSED, dx, wl, pz, nz,
+                       flags=lib.RSIM_FLAG_MATH_FP32, stream=s)
+    lib.call("rsim_synchronize", s)
+    t32 = time.perf_counter() - t0
+    result["fused_fp32_math_slice_steps_per_s"] = round(max(1, args.steps // 2) * nz / t32, 2)
+    log(f"fused c64 with FP32 butterflies: {result['fused_fp32_math_slice_steps_per_s']:.1f} slice-steps/s")
+
     # ---- plain complex128 GPU path ---------------------------------------
     u2 = rs.DeviceVector.from_numpy(np.ones(N, dtype=np.complex128))
     U2 = rs.DeviceVector(N, np.complex128)
@@ -214,7 +237,30 @@ def main():
         result["rel_l2_detector_fused_vs_c128"] = err_det
         result["detector_shape"] = list(d1.shape)
         log(f"parity vs FP64 GPU path: field {err_field:.3e}, detector {err_det:.3e}")
+        # the fused engine with FP32 butterflies
+        u.upload(ones64)
+        run_multislice(lib, u, U, n, vol, lib.RSIM_FUSED, dx, wl, pz, nz,
+                       flags=lib.RSIM_FLAG_MATH_FP32, stream=s)
+        lib.call("rsim_synchronize", s)
+        a = u.to_numpy()
+        result["rel_l2_field_fused_fp32math_vs_c128"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        rs.propagate_2d(u, U, dx, dx, wl, 0.5, n, 0.4 / dx, n, n, skip_fft=True)
+        d3 = rs.square_and_downsample_2d(u, p, z_det)
+        result["rel_l2_detector_fused_fp32math_vs_c128"] = float(np.linalg.norm(d3 - d2) / np.linalg.norm(d2))
+        log(f"FP32-butterfly fused vs FP64 GPU path: field "
+            f"{result['rel_l2_field_fused_fp32math_vs_c128']:.3e}, detector "
+            f"{result['rel_l2_detector_fused_fp32math_vs_c128']:.3e}")
+        # diagnostic: the unfused kernels in complex64 (FP32 FFT, FP64 pointwise)
+        u.upload(ones64)
+        run_multislice(lib, u, U, n, vol, lib.RSIM_PLAIN, dx, wl, pz, nz, stream=s)
+        lib.call("rsim_synchronize", s)
+        a = u.to_numpy()
+        result["rel_l2_field_plain_c64_vs_c128"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        log(f"plain c64 vs c128 field: {result['rel_l2_field_plain_c64_vs_c128']:.3e}")
         del a, b
+    rv, cv = C.c_int(-1), C.c_int(-1)
+    lib.call("rsim_fused_schedule", n, n, C.byref(rv), C.byref(cv))
+    result["fused_launch_variants"] = {"row": rv.value, "col": cv.value}
 
     # ---- CPU baseline: the FP64 oracle on a bounded sample ----------------
     if not args.no_cpu:

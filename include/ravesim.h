@@ -40,7 +40,10 @@ enum { RSIM_FORWARD = -1, RSIM_INVERSE = 1 };
  * FUSED = complex64 fused row/column kernels with FP64 phase tables. */
 enum { RSIM_PLAIN = 0, RSIM_FUSED = 1 };
 /* rsim_sample_apply_2d flags */
-enum { RSIM_FLAG_NO_SPECTRUM = 1 };
+enum {
+  RSIM_FLAG_NO_SPECTRUM = 1, /* do not materialise U = FFT2(u) */
+  RSIM_FLAG_MATH_FP32 = 2    /* FUSED: FP32 butterflies (default: FP64 butterflies, FP32 data) */
+};
 
 typedef struct rsim_volume rsim_volume;
 
@@ -148,6 +151,13 @@ int rsim_sample_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx,
                          double pixel_size_x, double pixel_size_y, double pixel_size_z,
                          double x_pos, double y_pos, int first_slice, int n_slices, int engine,
                          int flags, void* stream);
+
+/* Launch-shape autotuner of the fused engine (no counterpart in the
+ * reference): the first RSIM_FUSED call for an (nx, ny) times each row-pass
+ * and column-pass variant on a scratch field and keeps the fastest; the
+ * environment variable RAVESIM_FUSED_VARIANT="row,col" overrides it.  Returns
+ * the variants in use for (nx, ny), or -1 before the first fused call. */
+int rsim_fused_schedule(int nx, int ny, int* row_variant, int* col_variant);
 
 /* ---- Detector ----
  * Replaces square_and_downsample_2d with the area_v1 integrator
