@@ -30,6 +30,7 @@ RSIM_PLAIN = 0
 RSIM_FUSED = 1
 RSIM_FLAG_NO_SPECTRUM = 1
 RSIM_FLAG_MATH_FP32 = 2
+RSIM_FLAG_MATH_FP64 = 4
 
 _vp = C.c_void_p
 _i = C.c_int

@@ -171,6 +171,97 @@ __device__ __forceinline__ void dft16(cpx<R>* v) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Compensated single precision: constants and twiddles carried as FP32
+// hi + lo pairs (value = hi + lo to ~2^-48), products formed with FMA so that
+// every product carries only its own rounding.  FP32-rounded rotation
+// constants make |W| != 1 by ~1e-8, a bias that repeats identically in every
+// transform and compounds linearly over a multislice run; compensation
+// removes it while keeping FP32 data and FP32 register pressure.
+// ---------------------------------------------------------------------------
+struct twc {
+  float x, y, lx, ly;   // hi.re, hi.im, lo.re, lo.im
+};
+
+// a * (c + i s) with c = ch + cl, s = sh + sl
+__device__ __forceinline__ cpx<float> cmul_comp(cpx<float> a, float ch, float cl, float sh,
+                                                float sl) {
+  const float re_lo = a.x * cl - a.y * sl;
+  const float im_lo = a.x * sl + a.y * cl;
+  return mk(__builtin_fmaf(a.x, ch, __builtin_fmaf(-a.y, sh, re_lo)),
+            __builtin_fmaf(a.x, sh, __builtin_fmaf(a.y, ch, im_lo)));
+}
+template <int DIR>
+__device__ __forceinline__ cpx<float> twmul_comp(cpx<float> a, const twc& w) {
+  // forward: a * w; inverse: a * conj(w)
+  if (DIR < 0) return cmul_comp(a, w.x, w.lx, w.y, w.ly);
+  return cmul_comp(a, w.x, w.lx, -w.y, -w.ly);
+}
+
+// hi/lo splits of the rotation constants (exact decimal values rounded in
+// two steps: hi = fl32(v), lo = fl32(v - hi))
+constexpr double kC1 = 0.92387953251128675612818318939679;  // cos(pi/8)
+constexpr double kS1 = 0.38268343236508977172845998403040;  // sin(pi/8)
+constexpr double kH = 0.70710678118654752440084436210485;   // sqrt(1/2)
+#define RSIM_HI(v) ((float)(v))
+#define RSIM_LO(v) ((float)((v) - (double)(float)(v)))
+
+template <int DIR>
+__device__ __forceinline__ void dft8_comp(cpx<float>* v) {
+  const float hh = RSIM_HI(kH), hl = RSIM_LO(kH);
+  const float d = (float)DIR;
+  cpx<float> e[4] = {v[0], v[2], v[4], v[6]};
+  cpx<float> o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<DIR>(e);
+  dft4<DIR>(o);
+  cpx<float> o1 = cmul_comp(o[1], hh, hl, d * hh, d * hl);
+  cpx<float> o2 = rot<DIR>(o[2]);
+  cpx<float> o3 = rot<DIR>(cmul_comp(o[3], hh, hl, d * hh, d * hl));
+  v[0] = e[0] + o[0];
+  v[4] = e[0] - o[0];
+  v[1] = e[1] + o1;
+  v[5] = e[1] - o1;
+  v[2] = e[2] + o2;
+  v[6] = e[2] - o2;
+  v[3] = e[3] + o3;
+  v[7] = e[3] - o3;
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft16_comp(cpx<float>* v) {
+  const float ch = RSIM_HI(kC1), cl = RSIM_LO(kC1);
+  const float sh = RSIM_HI(kS1), sl = RSIM_LO(kS1);
+  const float hh = RSIM_HI(kH), hl = RSIM_LO(kH);
+  const float d = (float)DIR;
+  cpx<float> y[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    y[n2][0] = v[n2];
+    y[n2][1] = v[4 + n2];
+    y[n2][2] = v[8 + n2];
+    y[n2][3] = v[12 + n2];
+    dft4<DIR>(y[n2]);
+  }
+  y[1][1] = cmul_comp(y[1][1], ch, cl, d * sh, d * sl);
+  y[1][2] = cmul_comp(y[1][2], hh, hl, d * hh, d * hl);
+  y[1][3] = cmul_comp(y[1][3], sh, sl, d * ch, d * cl);
+  y[2][1] = cmul_comp(y[2][1], hh, hl, d * hh, d * hl);
+  y[2][2] = rot<DIR>(y[2][2]);
+  y[2][3] = rot<DIR>(cmul_comp(y[2][3], hh, hl, d * hh, d * hl));
+  y[3][1] = cmul_comp(y[3][1], sh, sl, d * ch, d * cl);
+  y[3][2] = rot<DIR>(cmul_comp(y[3][2], hh, hl, d * hh, d * hl));
+  y[3][3] = cmul_comp(y[3][3], -ch, -cl, -d * sh, -d * sl);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    cpx<float> z[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+    dft4<DIR>(z);
+    v[k1] = z[0];
+    v[k1 + 4] = z[1];
+    v[k1 + 8] = z[2];
+    v[k1 + 12] = z[3];
+  }
+}
+
 template <int RADIX, int DIR, typename R>
 __device__ __forceinline__ void dft(cpx<R>* v) {
   if constexpr (RADIX == 2) dft2<DIR>(v);
@@ -180,6 +271,14 @@ __device__ __forceinline__ void dft(cpx<R>* v) {
   else if constexpr (RADIX == 8) dft8<DIR>(v);
   else if constexpr (RADIX == 16) dft16<DIR>(v);
 }
+
+template <int RADIX, int DIR>
+__device__ __forceinline__ void dft_comp(cpx<float>* v) {
+  if constexpr (RADIX == 8) dft8_comp<DIR>(v);
+  else if constexpr (RADIX == 16) dft16_comp<DIR>(v);
+  else dft<RADIX, DIR>(v);
+}
+
 
 // ---------------------------------------------------------------------------
 // Buffer-resource global I/O: the base address lives in SGPRs (a V#), each

@@ -18,6 +18,8 @@
 //   and pointwise work (transmission, chirp, scaling) is applied in registers.
 #pragma once
 
+#include <type_traits>
+
 #include "rsim_device.hpp"
 
 namespace rsim {
@@ -148,11 +150,18 @@ struct RegFac {
   static constexpr int last_radix() { return radix(NS - 1); }
 };
 
-// Twiddle tables for a RegFac: table s (s >= 1) holds W_{L_s R_s}^m for
-// m in [0, L_s R_s), forward sign.
+// Twiddle tables for a RegFac: table s (s >= 1) holds, at [k*R_s + j],
+// W_{L_s R_s}^(j k) for k in [0, L_s), j in [0, R_s), forward sign -- the R_s
+// twiddles of one butterfly are contiguous, so their addresses are one base
+// plus compile-time offsets.
 template <typename R>
 struct StageTw {
   const cpx<R>* t[4];
+};
+// compensated single-precision tables: FP32 hi + lo per twiddle
+template <>
+struct StageTw<twc> {
+  const twc* t[4];
 };
 
 // LDS padding code: PS = (shift << 8) | k inserts k pad slots after every
@@ -232,11 +241,82 @@ __device__ __forceinline__ void reg_exchange(cpx<R> (&v)[F::E], void* lds_raw, i
 // systematically (about 1e-7 per transform pair), which compounds linearly
 // over hundreds of slices, while rounding only the stage outputs grows as a
 // random walk.
+__device__ __forceinline__ cpx<double> to64(cpx<float> a) { return mk((double)a.x, (double)a.y); }
+__device__ __forceinline__ cpx<float> to32(cpx<double> a) { return mk((float)a.x, (float)a.y); }
+
+// Radix-16 butterfly (4 x 4) on FP32 data with each radix-4 layer evaluated in
+// FP64 and rounded to FP32 between the layers: FP64 accuracy per operation
+// with only four complex doubles live at a time.  T/k: stage twiddles
+// W^(j k) applied to input j (TWID = false for the first stage).
+template <int DIR, bool TWID>
+__device__ __forceinline__ void dft16_l64(cpx<float>* v, const cpx<double>* T, int k) {
+  const double c1 = 0.92387953251128675612818318939679;  // cos(pi/8)
+  const double s1 = 0.38268343236508977172845998403040;  // sin(pi/8)
+  const double h = 0.70710678118654752440084436210485;
+  const double d = (double)DIR;
+  cpx<float> y[16];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) {
+    cpx<double> x[4];
+#pragma unroll
+    for (int n1 = 0; n1 < 4; ++n1) {
+      x[n1] = to64(v[n2 + 4 * n1]);
+      if constexpr (TWID) {
+        if (n2 + 4 * n1 > 0) x[n1] = twmul<DIR>(x[n1], T[k * 16 + n2 + 4 * n1]);
+      }
+    }
+    dft4<DIR>(x);
+    if (n2 == 1) {
+      x[1] = cmulcs(x[1], c1, d * s1);
+      x[2] = cmulcs(x[2], h, d * h);
+      x[3] = cmulcs(x[3], s1, d * c1);
+    } else if (n2 == 2) {
+      x[1] = cmulcs(x[1], h, d * h);
+      x[2] = rot<DIR>(x[2]);
+      x[3] = rot<DIR>(cmulcs(x[3], h, d * h));
+    } else if (n2 == 3) {
+      x[1] = cmulcs(x[1], s1, d * c1);
+      x[2] = rot<DIR>(cmulcs(x[2], h, d * h));
+      x[3] = cmulcs(x[3], -c1, -d * s1);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) y[n2 * 4 + k1] = to32(x[k1]);
+    // one group's twiddle loads in flight at a time (register pressure)
+    if constexpr (TWID) compiler_fence();
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    cpx<double> z[4] = {to64(y[k1]), to64(y[4 + k1]), to64(y[8 + k1]), to64(y[12 + k1])};
+    dft4<DIR>(z);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = to32(z[k2]);
+  }
+}
+
 template <class F, int S, int DIR, typename R, typename TW>
 __device__ __forceinline__ void reg_stage(cpx<R> (&v)[F::E], const StageTw<TW>& tw, int t) {
   constexpr int RR = F::radix(S);
   constexpr int LS = F::stride(S);
-  if constexpr (sizeof(TW) > sizeof(R)) {
+  if constexpr (std::is_same<TW, twc>::value) {
+    if constexpr (LS > 1) {
+      const twc* T = tw.t[S];
+#pragma unroll
+      for (int i = 0; i < F::E / RR; ++i) {
+        const int b = t + i * F::TC;
+        const int k = b % LS;
+#pragma unroll
+        for (int j = 1; j < RR; ++j) v[i * RR + j] = twmul_comp<DIR>(v[i * RR + j], T[k * RR + j]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < F::E / RR; ++i) dft_comp<RR, DIR>(&v[i * RR]);
+  } else if constexpr (sizeof(TW) > sizeof(R) && RR == 16) {
+#pragma unroll
+    for (int i = 0; i < F::E / 16; ++i) {
+      const int b = t + i * F::TC;
+      dft16_l64<DIR, (LS > 1)>(&v[i * 16], tw.t[S], b % LS);
+    }
+  } else if constexpr (sizeof(TW) > sizeof(R)) {
 #pragma unroll
     for (int i = 0; i < F::E / RR; ++i) {
       cpx<TW> x[RR];
@@ -247,7 +327,7 @@ __device__ __forceinline__ void reg_stage(cpx<R> (&v)[F::E], const StageTw<TW>& 
         const int b = t + i * F::TC;
         const int k = b % LS;
 #pragma unroll
-        for (int j = 1; j < RR; ++j) x[j] = twmul<DIR>(x[j], T[j * k]);
+        for (int j = 1; j < RR; ++j) x[j] = twmul<DIR>(x[j], T[k * RR + j]);
       }
       dft<RR, DIR>(x);
 #pragma unroll
@@ -261,7 +341,7 @@ __device__ __forceinline__ void reg_stage(cpx<R> (&v)[F::E], const StageTw<TW>& 
         const int b = t + i * F::TC;
         const int k = b % LS;
 #pragma unroll
-        for (int j = 1; j < RR; ++j) v[i * RR + j] = twmul<DIR>(v[i * RR + j], T[j * k]);
+        for (int j = 1; j < RR; ++j) v[i * RR + j] = twmul<DIR>(v[i * RR + j], T[k * RR + j]);
       }
     }
 #pragma unroll

@@ -28,7 +28,7 @@ struct RowShape {
   using F = typename Sched<N>::F;
   static constexpr int C = (256 / F::TC) > 0 ? (256 / F::TC) : 1;
   static constexpr bool SPLIT = (size_t)F::N * C * sizeof(cpx<R>) > 65536;
-  static constexpr int PS = pad_code(5, 1);
+  static constexpr int PS = pad_code(4 + (C >= 2) + (C >= 4) + (C >= 8) + (C >= 16), C);
 };
 template <typename R, int N>
 struct ColShape {
@@ -40,7 +40,7 @@ struct ColShape {
   static constexpr int C1 = Cmax < Cw ? Cmax : Cw;
   static constexpr int C = C1 < Clds ? C1 : Clds;
   static constexpr bool SPLIT = true;
-  static constexpr int PS = C >= 2 ? pad_code(4 + (C >= 2) + (C >= 4) + (C >= 8) + (C >= 16), C) : pad_code(5, 1);
+  static constexpr int PS = pad_code(4 + (C >= 2) + (C >= 4) + (C >= 8) + (C >= 16), C);
 };
 
 // Host-side description of a schedule (for building twiddle tables).

@@ -66,11 +66,32 @@ void twiddle(long long m, long long M, double* re, double* im) {
   *im = -s;
 }
 
-int upload_table(int dtype, long long M, void** out) {
+// W_M^m for m in [0, M) when R == 0; otherwise the [k][j] layout of a
+// register-FFT stage: entry k*R + j = W_M^(j*k), M = LS*R.
+int upload_table(int dtype, long long M, void** out, int R = 0) {
   std::vector<double> d(2 * M);
-  for (long long m = 0; m < M; ++m) twiddle(m, M, &d[2 * m], &d[2 * m + 1]);
+  if (R == 0) {
+    for (long long m = 0; m < M; ++m) twiddle(m, M, &d[2 * m], &d[2 * m + 1]);
+  } else {
+    const long long LS = M / R;
+    for (long long k = 0; k < LS; ++k)
+      for (long long j = 0; j < R; ++j)
+        twiddle(j * k, M, &d[2 * (k * R + j)], &d[2 * (k * R + j) + 1]);
+  }
   size_t bytes;
-  if (dtype == RSIM_C128) {
+  if (dtype == kTwComp) {
+    std::vector<float> f(4 * M);
+    for (long long i = 0; i < M; ++i) {
+      const float hr = (float)d[2 * i], hi = (float)d[2 * i + 1];
+      f[4 * i] = hr;
+      f[4 * i + 1] = hi;
+      f[4 * i + 2] = (float)(d[2 * i] - (double)hr);
+      f[4 * i + 3] = (float)(d[2 * i + 1] - (double)hi);
+    }
+    bytes = 16 * M;
+    RSIM_HIP(hipMalloc(out, bytes));
+    RSIM_HIP(hipMemcpy(*out, f.data(), bytes, hipMemcpyHostToDevice));
+  } else if (dtype == RSIM_C128) {
     bytes = 16 * M;
     RSIM_HIP(hipMalloc(out, bytes));
     RSIM_HIP(hipMemcpy(*out, d.data(), bytes, hipMemcpyHostToDevice));
@@ -99,7 +120,7 @@ int get_tables(int dtype, int N, Tables** out) {
     t.reg = true;
     int ls = 1;
     for (int s = 0; s < si.nst; ++s) {
-      if (s > 0) RSIM_CHECK(upload_table(dtype, (long long)ls * si.radix[s], &t.stage[s]));
+      if (s > 0) RSIM_CHECK(upload_table(dtype, (long long)ls * si.radix[s], &t.stage[s], si.radix[s]));
       ls *= si.radix[s];
     }
   }

@@ -173,9 +173,10 @@ struct Shape {
   using F = typename FacE<N, E>::F;
   static constexpr int CC = C;
   static constexpr bool SPLIT = SPLIT_;
-  // rows: 1 pad slot per 32; columns: C pad slots per 16*C (bank model)
-  static constexpr int PS = C == 1 ? pad_code(5, 1)
-                                   : pad_code(4 + (C >= 2) + (C >= 4) + (C >= 8) + (C >= 16), C);
+  // C pad slots per 16*C slots: spreads the strided stage writes over the
+  // banks and keeps every exchange address linear in the element index (the
+  // per-element part folds into the instruction offset)
+  static constexpr int PS = pad_code(4 + (C >= 2) + (C >= 4) + (C >= 8) + (C >= 16), C);
   static constexpr int threads = F::TC * C;
 };
 
@@ -291,6 +292,17 @@ bool col_go(int v, bool spec_out, const void* in, void* out, void* spec, int nx,
                          const StageTw<double>& tw, const double2* Bx, const double2* Ay,         \
                          const int* Rmax, float ss, hipStream_t s) {                              \
     return col_go_impl<N, double>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);       \
+  }                                                                                               \
+  template <>                                                                                     \
+  bool row_go<N, twc>(int v, int mode, bool lut, const void* in, void* out, int ny,               \
+                      const StageTw<twc>& tw, const TransArgsF& ta, hipStream_t s) {              \
+    return row_go_impl<N, twc>(v, mode, lut, in, out, ny, tw, ta, s);                             \
+  }                                                                                               \
+  template <>                                                                                     \
+  bool col_go<N, twc>(int v, bool spec_out, const void* in, void* out, void* spec, int nx,        \
+                      const StageTw<twc>& tw, const double2* Bx, const double2* Ay,               \
+                      const int* Rmax, float ss, hipStream_t s) {                                 \
+    return col_go_impl<N, twc>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);          \
   }
 
 }  // namespace rsim

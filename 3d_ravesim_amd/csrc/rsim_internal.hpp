@@ -41,6 +41,8 @@ int current_device();
 size_t max_lds_per_block();
 
 inline size_t elem_bytes(int dtype) { return dtype == RSIM_C128 ? 16 : 8; }
+// internal table kind: compensated FP32 (hi + lo) twiddles
+constexpr int kTwComp = 2;
 inline bool valid_dtype(int dtype) { return dtype == RSIM_C64 || dtype == RSIM_C128; }
 
 // ---- FFT (rsim_fft_host.hip) ----

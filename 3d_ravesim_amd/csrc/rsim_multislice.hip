@@ -230,7 +230,7 @@ int fused_schedule(int nx, int ny, bool lut, const StageTw<TW>& tx, const StageT
     }
   }
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  const int math = sizeof(TW) == 8 ? 2 : 0;
+  const int math = std::is_same<TW, twc>::value ? 4 : sizeof(TW) == 8 ? 2 : 0;
   auto key = std::make_tuple(current_device(), nx, ny, (lut ? 1 : 0) | math);
   auto it = g_tune.find(key);
   if (it != g_tune.end()) {
@@ -251,7 +251,7 @@ int fused_schedule(int nx, int ny, bool lut, const StageTw<TW>& tx, const StageT
     if (ms > 0.f && ms < tr_best) { tr_best = ms; best_r = v; }
     if (verbose)
       std::fprintf(stderr, "[rsim tune] %dx%d math=%s row variant %d: %.1f us\n", nx, ny,
-                   math ? "fp64" : "fp32", v, ms * 1e3f / 3);
+                   math == 4 ? "comp32" : math ? "fp64" : "fp32", v, ms * 1e3f / 3);
   }
   for (int v = 0; v < kColVariants; ++v) {
     float ms = -1.f;
@@ -262,7 +262,7 @@ int fused_schedule(int nx, int ny, bool lut, const StageTw<TW>& tx, const StageT
     if (ms > 0.f && ms < tc_best) { tc_best = ms; best_c = v; }
     if (verbose)
       std::fprintf(stderr, "[rsim tune] %dx%d math=%s col variant %d: %.1f us\n", nx, ny,
-                   math ? "fp64" : "fp32", v, ms * 1e3f / 3);
+                   math == 4 ? "comp32" : math ? "fp64" : "fp32", v, ms * 1e3f / 3);
   }
   RSIM_HIP(hipStreamSynchronize(s));
   RSIM_HIP(hipFree(tmp));
@@ -328,15 +328,15 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
                      double xpos, double ypos, int first, int count, bool want_spec,
                      hipStream_t s) {
   const int total = count;
-  const int tdtype = sizeof(TW) == 8 ? RSIM_C128 : RSIM_C64;
+  const int tdtype = std::is_same<TW, twc>::value ? kTwComp : sizeof(TW) == 8 ? RSIM_C128 : RSIM_C64;
   const void* twx[4];
   const void* twy[4];
   if (!reg_stage_tables(tdtype, nx, twx) || !reg_stage_tables(tdtype, ny, twy))
     return fail(RSIM_E_HIP, "missing FFT schedule tables");
   StageTw<TW> tx, ty;
   for (int i = 0; i < 4; ++i) {
-    tx.t[i] = (const cpx<TW>*)twx[i];
-    ty.t[i] = (const cpx<TW>*)twy[i];
+    tx.t[i] = (decltype(tx.t[i]))twx[i];
+    ty.t[i] = (decltype(ty.t[i]))twy[i];
   }
   const ChirpTables* ct = nullptr;
   RSIM_CHECK(chirp_tables(nx, ny, dx, dy, wl, pz, freq, &ct));
@@ -407,12 +407,15 @@ int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double 
     return RSIM_OK;
   }
   // ---- fused complex64 engine ----
-  // butterflies in FP64 (FP64 twiddle tables) unless FP32 math is requested
+  // butterfly arithmetic: compensated FP32 (default), plain FP32 or FP64
   if (flags & RSIM_FLAG_MATH_FP32)
     return run_fused<float>(u, U, nx, ny, dx, dy, wl, freq, vol, px, py, pz, xpos, ypos, first,
                             count, want_spec, s);
-  return run_fused<double>(u, U, nx, ny, dx, dy, wl, freq, vol, px, py, pz, xpos, ypos, first,
-                           count, want_spec, s);
+  if (flags & RSIM_FLAG_MATH_FP64)
+    return run_fused<double>(u, U, nx, ny, dx, dy, wl, freq, vol, px, py, pz, xpos, ypos, first,
+                             count, want_spec, s);
+  return run_fused<twc>(u, U, nx, ny, dx, dy, wl, freq, vol, px, py, pz, xpos, ypos, first,
+                        count, want_spec, s);
   return RSIM_OK;
 }
 

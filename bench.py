@@ -170,7 +170,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"voxelized multislice {n}x{n}, {nz} slices x 0.5um, 200 seeded spheres (seed 2), "
-                        "3 materials, lambda 0.5A, fused complex64 engine",
+                        "3 materials, lambda 0.5A, fused complex64 engine (compensated FP32 butterflies)",
             "nx": n, "ny": n, "slices": nz, "dx_m": dx, "wl_m": wl, "dz_m": pz,
             "parallelism": "single GPU",
         },
@@ -188,15 +188,18 @@ def main():
         "reference_published_gpu_slice_steps_per_s": 23.1,
     }
 
-    # ---- fused with FP32 butterflies (reported beside the default) ---------
-    t0 = time.perf_counter()
-    for _ in range(max(1, args.steps // 2)):
-        run_multislice(lib, u, U, n, vol, lib.RSIM_FUSED, dx, wl, pz, nz,
-                       flags=lib.RSIM_FLAG_MATH_FP32, stream=s)
-    lib.call("rsim_synchronize", s)
-    t32 = time.perf_counter() - t0
-    result["fused_fp32_math_slice_steps_per_s"] = round(max(1, args.steps // 2) * nz / t32, 2)
-    log(f"fused c64 with FP32 butterflies: {result['fused_fp32_math_slice_steps_per_s']:.1f} slice-steps/s")
+    # ---- the other butterfly-arithmetic modes, reported beside the default --
+    MODES = {"fp32": lib.RSIM_FLAG_MATH_FP32, "fp64": lib.RSIM_FLAG_MATH_FP64}
+    for name, fl in MODES.items():
+        run_multislice(lib, u, U, n, vol, lib.RSIM_FUSED, dx, wl, pz, 2, flags=fl, stream=s)
+        lib.call("rsim_synchronize", s)
+        k = max(1, args.steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(k):
+            run_multislice(lib, u, U, n, vol, lib.RSIM_FUSED, dx, wl, pz, nz, flags=fl, stream=s)
+        lib.call("rsim_synchronize", s)
+        result[f"fused_{name}_math_slice_steps_per_s"] = round(k * nz / (time.perf_counter() - t0), 2)
+        log(f"fused c64, {name} butterflies: {result[f'fused_{name}_math_slice_steps_per_s']:.1f} slice-steps/s")
 
     # ---- plain complex128 GPU path ---------------------------------------
     u2 = rs.DeviceVector.from_numpy(np.ones(N, dtype=np.complex128))
@@ -237,19 +240,18 @@ def main():
         result["rel_l2_detector_fused_vs_c128"] = err_det
         result["detector_shape"] = list(d1.shape)
         log(f"parity vs FP64 GPU path: field {err_field:.3e}, detector {err_det:.3e}")
-        # the fused engine with FP32 butterflies
-        u.upload(ones64)
-        run_multislice(lib, u, U, n, vol, lib.RSIM_FUSED, dx, wl, pz, nz,
-                       flags=lib.RSIM_FLAG_MATH_FP32, stream=s)
-        lib.call("rsim_synchronize", s)
-        a = u.to_numpy()
-        result["rel_l2_field_fused_fp32math_vs_c128"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
-        rs.propagate_2d(u, U, dx, dx, wl, 0.5, n, 0.4 / dx, n, n, skip_fft=True)
-        d3 = rs.square_and_downsample_2d(u, p, z_det)
-        result["rel_l2_detector_fused_fp32math_vs_c128"] = float(np.linalg.norm(d3 - d2) / np.linalg.norm(d2))
-        log(f"FP32-butterfly fused vs FP64 GPU path: field "
-            f"{result['rel_l2_field_fused_fp32math_vs_c128']:.3e}, detector "
-            f"{result['rel_l2_detector_fused_fp32math_vs_c128']:.3e}")
+        for name, fl in MODES.items():
+            u.upload(ones64)
+            run_multislice(lib, u, U, n, vol, lib.RSIM_FUSED, dx, wl, pz, nz, flags=fl, stream=s)
+            lib.call("rsim_synchronize", s)
+            a = u.to_numpy()
+            result[f"rel_l2_field_fused_{name}_math_vs_c128"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+            rs.propagate_2d(u, U, dx, dx, wl, 0.5, n, 0.4 / dx, n, n, skip_fft=True)
+            d3 = rs.square_and_downsample_2d(u, p, z_det)
+            result[f"rel_l2_detector_fused_{name}_math_vs_c128"] = float(np.linalg.norm(d3 - d2) / np.linalg.norm(d2))
+            log(f"fused {name} butterflies vs FP64 GPU path: field "
+                f"{result[f'rel_l2_field_fused_{name}_math_vs_c128']:.3e}, detector "
+                f"{result[f'rel_l2_detector_fused_{name}_math_vs_c128']:.3e}")
         # diagnostic: the unfused kernels in complex64 (FP32 FFT, FP64 pointwise)
         u.upload(ones64)
         run_multislice(lib, u, U, n, vol, lib.RSIM_PLAIN, dx, wl, pz, nz, stream=s)

@@ -42,7 +42,8 @@ enum { RSIM_PLAIN = 0, RSIM_FUSED = 1 };
 /* rsim_sample_apply_2d flags */
 enum {
   RSIM_FLAG_NO_SPECTRUM = 1, /* do not materialise U = FFT2(u) */
-  RSIM_FLAG_MATH_FP32 = 2    /* FUSED: FP32 butterflies (default: FP64 butterflies, FP32 data) */
+  RSIM_FLAG_MATH_FP32 = 2,   /* FUSED: plain FP32 butterflies */
+  RSIM_FLAG_MATH_FP64 = 4    /* FUSED: FP64 butterflies (default: compensated FP32, hi+lo constants) */
 };
 
 typedef struct rsim_volume rsim_volume;
