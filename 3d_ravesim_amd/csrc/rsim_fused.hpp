@@ -24,23 +24,33 @@ struct TransArgsF {
   const float4* lut;      // per material (hi.re, hi.im, lo.re, lo.im) of the FP64 factor
   const float2* dbf;      // per material (delta, beta), bilinear mode
   float K;                // 2*pi*pz/wl
+  // LUT mode (integral voxel map): voxel x = ax*x + cx for x in [xlo, xhi)
+  int ax, cx, xlo, xhi, ay, cy, ylo, yhi;
 };
 
-// Transmission of one element; LUT: nearest-voxel lookup of the hi+lo split
-// factor (exact when the voxel map is integral), else bilinear delta-beta
-// interpolation and an FP32 exponential.
+// Material LUT staged in LDS (256 entries); filled by the whole workgroup.
+__device__ __forceinline__ void lut_to_lds(float4* lds_lut, const float4* lut) {
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) lds_lut[i] = lut[i];
+}
+
+// Transmission of one element.  LUT: the voxel map is integral, hence affine
+// (x0 = ax*x + cx); the factor is a per-material lookup (LDS) of the FP64
+// value split hi+lo.  Otherwise bilinear delta-beta interpolation from the
+// four corners and an FP32 exponential.  row0 = ids row of this field row.
 template <bool LUT>
 __device__ __forceinline__ cpx<float> transmit(cpx<float> v, const TransArgsF& a, int x, int y0,
-                                               float fy, bool iny) {
-  if (!(iny && a.xm.inside[x])) return v;
-  const int x0 = a.xm.i0[x];
-  const uint8_t* r0 = a.ids + (long long)y0 * a.nxg + x0;
+                                               float fy, bool iny, const uint8_t* row0,
+                                               const float4* lds_lut) {
   if constexpr (LUT) {
-    const float4 t = a.lut[r0[0]];
+    if (!(iny && x >= a.xlo && x < a.xhi)) return v;
+    const float4 t = lds_lut[row0[a.ax * x + a.cx]];
     const float rr = v.x * t.x - v.y * t.y + (v.x * t.z - v.y * t.w);
     const float ii = v.x * t.y + v.y * t.x + (v.x * t.w + v.y * t.z);
     return mk(rr, ii);
   } else {
+    if (!(iny && a.xm.inside[x])) return v;
+    const int x0 = a.xm.i0[x];
+    const uint8_t* r0 = a.ids + (long long)y0 * a.nxg + x0;
     const uint8_t* r1 = r0 + a.nxg;
     const float fx = a.xm.fracf[x];
     const float2 d00 = a.dbf[r0[0]], d01 = a.dbf[r0[1]], d10 = a.dbf[r1[0]], d11 = a.dbf[r1[1]];
@@ -55,6 +65,30 @@ __device__ __forceinline__ cpx<float> transmit(cpx<float> v, const TransArgsF& a
   }
 }
 
+// Per-row transmission state.
+struct RowT {
+  int y0;
+  float fy;
+  bool iny;
+  const uint8_t* row0;
+};
+template <bool LUT>
+__device__ __forceinline__ RowT row_state(const TransArgsF& a, int y) {
+  RowT r;
+  if constexpr (LUT) {
+    r.iny = y >= a.ylo && y < a.yhi;
+    r.y0 = a.ay * y + a.cy;
+    r.fy = 0.f;
+    r.row0 = a.ids + (long long)(r.iny ? r.y0 : 0) * a.nxg;
+  } else {
+    r.y0 = a.ym.i0[y];
+    r.fy = a.ym.fracf[y];
+    r.iny = a.ym.inside[y];
+    r.row0 = a.ids;
+  }
+  return r;
+}
+
 enum { ROW_FIRST = 0, ROW_MID = 1, ROW_LAST = 2 };
 
 template <class F, int C, int PS, bool SPLIT, int MODE, bool LUT, typename TW>
@@ -63,6 +97,7 @@ __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(const cpx<float>* in,
                                                            StageTw<TW> tw, TransArgsF ta) {
   constexpr int N = F::N;
   __shared__ unsigned char lds[LdsSize<F, C, PS>::slots * (SPLIT ? 4 : 8)];
+  __shared__ float4 lds_lut[(MODE != ROW_LAST && LUT) ? 256 : 1];
   const int tid = threadIdx.x;
   const int c = tid % C;
   const int t = tid / C;
@@ -71,28 +106,26 @@ __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(const cpx<float>* in,
   const int yy = live ? y : 0;
   const Gio<float> gi(in, nbytes), go(out, nbytes);
   const uint32_t vo = ((uint32_t)yy * N + t) * 8u;
+  if constexpr (MODE != ROW_LAST && LUT) lut_to_lds(lds_lut, ta.lut);
   cpx<float> v[F::E];
 #pragma unroll
   for (int e = 0; e < F::E; ++e) v[e] = gi.ld(vo, first_off<F>(e, 8u));
   if constexpr (MODE == ROW_FIRST) {
-    const int y0 = ta.ym.i0[yy];
-    const float fy = ta.ym.fracf[yy];
-    const bool iny = ta.ym.inside[yy];
+    const RowT r = row_state<LUT>(ta, yy);
+    if constexpr (LUT) __syncthreads();
 #pragma unroll
     for (int e = 0; e < F::E; ++e) {
-      v[e] = transmit<LUT>(v[e], ta, first_pos<F>(t, e), y0, fy, iny);
+      v[e] = transmit<LUT>(v[e], ta, first_pos<F>(t, e), r.y0, r.fy, r.iny, r.row0, lds_lut);
       if ((e & 3) == 3) compiler_fence();
     }
     reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
   } else if constexpr (MODE == ROW_MID) {
-    reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);
+    reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);  // its barriers publish lds_lut
     compiler_fence();
-    const int y0 = ta.ym.i0[yy];
-    const float fy = ta.ym.fracf[yy];
-    const bool iny = ta.ym.inside[yy];
+    const RowT r = row_state<LUT>(ta, yy);
 #pragma unroll
     for (int e = 0; e < F::E; ++e) {
-      v[e] = transmit<LUT>(v[e], ta, last_pos<F>(t, e), y0, fy, iny);
+      v[e] = transmit<LUT>(v[e], ta, last_pos<F>(t, e), r.y0, r.fy, r.iny, r.row0, lds_lut);
       if ((e & 3) == 3) compiler_fence();
     }
     reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);

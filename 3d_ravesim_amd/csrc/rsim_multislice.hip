@@ -351,7 +351,15 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
   ta.xm = xm->dev;
   ta.ym = ym->dev;
   ta.K = (float)K;
-  const bool lut = xm->all_integral && ym->all_integral;
+  const bool lut = xm->all_integral && ym->all_integral && xm->affine && ym->affine;
+  ta.ax = xm->a;
+  ta.cx = xm->c;
+  ta.xlo = xm->lo;
+  ta.xhi = xm->hi;
+  ta.ay = ym->a;
+  ta.cy = ym->c;
+  ta.ylo = ym->lo;
+  ta.yhi = ym->hi;
   const float spec_scale = (float)((double)nx * (double)ny);
   const size_t slice_bytes = (size_t)vol->ny_g * vol->nx_g;
   ta.ids = vol->ids + (size_t)first * slice_bytes;

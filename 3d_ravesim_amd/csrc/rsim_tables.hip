@@ -13,6 +13,9 @@
 namespace rsim {
 namespace {
 
+// |idx - nearest integer| (voxel units) below which an axis counts as integral
+constexpr double kSnap = 1e-8;
+
 uint64_t bits(double v) {
   uint64_t b;
   std::memcpy(&b, &v, 8);
@@ -61,7 +64,7 @@ int axis_map(int n, double spacing, double scale, double pos, int grid_len, doub
     *out = &it->second;
     return RSIM_OK;
   }
-  std::vector<int> i0(n);
+  std::vector<int> i0(n), inear(n);
   std::vector<double> fr(n);
   std::vector<float> frf(n);
   std::vector<uint8_t> ins(n);
@@ -78,14 +81,45 @@ int axis_map(int n, double spacing, double scale, double pos, int grid_len, doub
     long long c = fraw < 0 ? 0 : fraw;
     if (c > grid_len - 2) c = grid_len - 2;
     i0[i] = (int)c;
+    // Nearest voxel.  Equal pitches leave rounding residues of ~1e-12 voxel
+    // in idx (e.g. 1202 of 4096 positions at 0.5 um); within kSnap the
+    // bilinear weight of the second corner is below 1e-8, far under FP32
+    // resolution, so the axis is treated as integral.
+    const double r = std::nearbyint(idx);
+    inear[i] = 0;
     if (ins[i]) {
+      inear[i] = (int)r;
       any = true;
-      if (fr[i] != 0.0) integral = false;
+      if (std::fabs(idx - r) > kSnap) integral = false;
     }
   }
   AxisMap m;
   m.all_integral = integral;
   m.any_inside = any;
+  m.affine = false;
+  m.a = m.c = m.lo = m.hi = 0;
+  if (integral) {
+    int lo = -1, hi = -1;
+    for (int i = 0; i < n; ++i)
+      if (ins[i]) {
+        if (lo < 0) lo = i;
+        hi = i + 1;
+      }
+    bool ok = true;
+    if (lo < 0) {
+      lo = hi = 0;  // nothing inside: empty range
+    } else {
+      for (int i = lo; i < hi; ++i) ok = ok && ins[i];
+      const int a = hi - lo > 1 ? inear[lo + 1] - inear[lo] : 1;
+      const int c = inear[lo] - a * lo;
+      for (int i = lo; i < hi && ok; ++i) ok = inear[i] == a * i + c && inear[i] < grid_len;
+      m.a = a;
+      m.c = c;
+    }
+    m.affine = ok;
+    m.lo = lo;
+    m.hi = hi;
+  }
   const size_t bytes = n * (4 + 8 + 4 + 1) + 64;
   unsigned char* d = nullptr;
   RSIM_HIP(hipMalloc((void**)&d, bytes));

@@ -22,8 +22,13 @@ struct AxisMapDev {
 };
 struct AxisMap {
   AxisMapDev dev;
-  bool all_integral;      // every frac == 0 (nearest-voxel fast path is exact)
+  bool all_integral;      // every inside idx within 1e-8 of an integer
   bool any_inside;
+  // when all_integral: nearest voxel = a*i + c on the inside range [lo, hi)
+  // (an integral map of a uniform grid is affine); affine == false if that
+  // check fails
+  bool affine;
+  int a, c, lo, hi;
 };
 int axis_map(int n, double spacing, double scale, double pos, int grid_len, double pixel,
              const AxisMap** out);

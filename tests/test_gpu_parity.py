@@ -223,12 +223,19 @@ def test_transmission_matches_oracle(aligned):
 
 # ---------------------------------------------------------- multislice ----
 
-def _sample_case(n=256, nz=8, seed=4, aligned=True):
+def _sample_case(n=256, nz=8, seed=4, aligned="unit"):
+    """aligned: "unit" (voxel pitch == field pitch 0.5 um: the voxel
+    coordinates carry ~1e-13 rounding residues), "exact" (pitch 2**-21 m:
+    exactly integral coordinates), "half" (voxel pitch dx/2: affine map
+    x0 = 2x + c) or "bilinear" (non-integral map)."""
     rng = np.random.default_rng(seed)
-    dx = 0.5e-6
-    if aligned:
+    dx = 2.0 ** -21 if aligned == "exact" else 0.5e-6
+    if aligned in ("unit", "exact"):
         grid = _spheres_grid(rng, nz, n, n, 20, 3)
         px = dx
+    elif aligned == "half":
+        grid = _spheres_grid(rng, nz, 2 * n + 6, 2 * n + 2, 20, 3)
+        px = dx / 2
     else:
         grid = _spheres_grid(rng, nz, n // 2 + 3, n // 2 + 5, 10, 3)
         px = 2.13 * dx
@@ -239,7 +246,7 @@ def _sample_case(n=256, nz=8, seed=4, aligned=True):
 
 
 @pytest.mark.parametrize("engine", ["plain", "fused"])
-@pytest.mark.parametrize("aligned", [True, False])
+@pytest.mark.parametrize("aligned", ["unit", "exact", "half", "bilinear"])
 def test_sample_apply_matches_oracle(engine, aligned):
     n = 256
     grid, spec, u0, dx = _sample_case(n, 8, aligned=aligned)
