@@ -107,6 +107,8 @@ struct KEv {
 };
 std::mutex g_kt_mu;
 bool g_kt_on = false;
+int g_kt_every = 1;
+long long g_kt_seen[3] = {0, 0, 0};
 std::vector<KEv> g_kt_done;
 std::vector<hipEvent_t> g_kt_pool;
 KEv g_kt_open[3];
@@ -129,6 +131,9 @@ bool ktime_on() { return g_kt_on; }
 void ktime_begin(int kind, hipStream_t s) {
   if (!g_kt_on) return;
   std::lock_guard<std::mutex> lk(g_kt_mu);
+  // sample one launch in g_kt_every of each kind: an event pair costs a few
+  // microseconds of dispatch gap, which would skew the run it measures
+  if (g_kt_seen[kind]++ % g_kt_every != 0) return;
   KEv ev{kind, kt_event(), kt_event()};
   hipEventRecord(ev.a, s);
   g_kt_open[kind] = ev;
@@ -144,9 +149,12 @@ void ktime_end(int kind, hipStream_t s) {
   g_kt_has_open[kind] = false;
 }
 
-int ktime_enable(int on) {
+int ktime_enable(int every) {
+  if (every < 0) return fail(RSIM_E_INVALID_ARG, "rsim_kernel_timing: every must be >= 0");
   std::lock_guard<std::mutex> lk(g_kt_mu);
-  g_kt_on = on != 0;
+  g_kt_on = every != 0;
+  g_kt_every = every > 0 ? every : 1;
+  for (int k = 0; k < 3; ++k) g_kt_seen[k] = 0;
   return RSIM_OK;
 }
 
@@ -290,7 +298,7 @@ int rsim_release_caches(void) {
   return RSIM_OK;
 }
 
-int rsim_kernel_timing(int enable) { return ktime_enable(enable); }
+int rsim_kernel_timing(int every) { return ktime_enable(every); }
 
 int rsim_kernel_timing_read(double* ms3, int* launches3) { return ktime_read(ms3, launches3); }
 

@@ -424,7 +424,6 @@ int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double 
                              count, want_spec, s);
   return run_fused<twc>(u, U, nx, ny, dx, dy, wl, freq, vol, px, py, pz, xpos, ypos, first,
                         count, want_spec, s);
-  return RSIM_OK;
 }
 
 }  // namespace rsim

@@ -32,12 +32,30 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "slice-steps/s at 4096² (c64 fused vs c128 plain, rel err≤1e-5); % HBM peak"
+TRAFFIC_FILE = "profiles/ms_kernels_4096_hbm_traffic.md"
+KT_EVERY = 16  # one launch in 16 of each kind carries a HIP event pair
 HBM_PEAK_GBS = 8000.0
 DB = np.array([0, 3.5e-7 + 2e-10j, 1.1e-6 + 3e-9j, 2.6e-7 + 1.5e-10j])
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` (default comp32 engine) from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary, scaled by the copy
+    calibration recorded there; None when the file holds no figure for it."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_FILE)
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) >= 4 and parts[0] == "traffic" and parts[1] == kernel and parts[2] == "comp32":
+                    return float(parts[-1])
+    except OSError:
+        pass
+    return None
 
 
 def make_volume(rs, nz, n, seed=2, nsph=200, voxel=0.5e-6):
@@ -127,7 +145,9 @@ def main():
         run_multislice(lib, u, U, n, vol, lib.RSIM_FUSED, dx, wl, pz, nz, stream=s)
     lib.call("rsim_synchronize", s)
     lib.call("rsim_kernel_timing_read", None, None)
-    lib.call("rsim_kernel_timing", 1)
+    # HIP events bracket one launch in KT_EVERY of each kind on the run's own
+    # stream; timing every launch would add a dispatch gap to each of them
+    lib.call("rsim_kernel_timing", KT_EVERY)
     lib.call("rsim_synchronize", s)
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -144,16 +164,16 @@ def main():
     log(f"fused c64: {value:.1f} slice-steps/s, {ms_per_step:.2f} ms per {nz}-slice step")
 
     # algorithmic bytes (65 B/point/slice-step: 4 field passes x 16 B + 1 B id),
-    # split over the fused kernels: row pass 33 B/pt, column pass 32 B/pt
+    # split over the fused kernels: row pass 33 B/pt (IFFT_x read+write,
+    # FFT_x read+write, voxel id), column pass 32 B/pt (FFT_y, IFFT_y)
     row_ms, col_ms = ms3[0], ms3[1]
-    row_n, col_n = cnt3[0], cnt3[1]
-    row_bytes = 33.0 * N * slice_steps
-    col_bytes = 32.0 * N * slice_steps
-    dom = "column" if col_ms >= row_ms else "row"
-    d_ms, d_n, d_bytes = (col_ms, col_n, col_bytes) if dom == "column" else (row_ms, row_n, row_bytes)
-    achieved = d_bytes / (d_ms * 1e-3) / 1e9 if d_ms > 0 else 0.0
+    row_n, col_n = max(cnt3[0], 1), max(cnt3[1], 1)
+    row_us, col_us = 1e3 * row_ms / row_n, 1e3 * col_ms / col_n
+    dom = "column" if col_us * 33.0 >= row_us * 32.0 else "row"
+    d_us, d_n, d_bpl = (col_us, col_n, 32.0 * N) if dom == "column" else (row_us, row_n, 33.0 * N)
+    achieved = d_bpl / (d_us * 1e-6) / 1e9 if d_us > 0 else 0.0
     step_gbs = 65.0 * N * slice_steps / t_fused / 1e9
-    log(f"kernels: row {row_ms / max(row_n, 1) * 1e3:.1f} us x{row_n}, col {col_ms / max(col_n, 1) * 1e3:.1f} us x{col_n}")
+    log(f"kernels (sampled): row {row_us:.1f} us x{row_n}, col {col_us:.1f} us x{col_n}")
 
     result = {
         "metric": METRIC,
@@ -177,14 +197,15 @@ def main():
         "roofline": {
             "bound": "hbm", "kernel": f"ms_{'col' if dom == 'column' else 'row'}_kernel<{n}>",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "avg_launch_us": round(d_ms / max(d_n, 1) * 1e3, 2),
-            "alg_bytes_per_launch": (32.0 if dom == "column" else 33.0) * N,
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(f"ms_{'col' if dom == 'column' else 'row'}_kernel<{n}>"),
+            "traffic_source": TRAFFIC_FILE,
+            "avg_launch_us": round(d_us, 2), "timed_launches": d_n,
+            "alg_bytes_per_launch": d_bpl,
         },
         "step_hbm_GBs_65B_model": round(step_gbs, 1),
         "step_hbm_frac_65B_model": round(step_gbs / HBM_PEAK_GBS, 4),
-        "kernel_avg_us": {"row": round(row_ms / max(row_n, 1) * 1e3, 2),
-                          "col": round(col_ms / max(col_n, 1) * 1e3, 2)},
+        "kernel_avg_us": {"row": round(row_us, 2), "col": round(col_us, 2)},
         "reference_published_gpu_slice_steps_per_s": 23.1,
     }
 

@@ -75,12 +75,14 @@ int rsim_request_cancel(void);
 int rsim_clear_cancel(void);
 
 /* Kernel timing (the reference's CudaTimerSection, fast-wave/src/simulation.cpp:14-33):
- * while enabled, every row pass, column pass and pointwise launch of the FFT
- * and multislice engines is bracketed by hipEvents on its own stream.
+ * every > 0 brackets one launch in `every` of each kind (row pass, column
+ * pass, pointwise launch of the FFT and multislice engines) with hipEvents on
+ * its own stream (every = 1: all launches); 0 disables.  Each event pair adds
+ * a few microseconds of dispatch gap, hence the sampling.
  * rsim_kernel_timing_read waits for them and returns, per kind
  * (0 = row pass, 1 = column pass, 2 = pointwise), the summed kernel time in
- * ms and the launch count since the last read, then resets. */
-int rsim_kernel_timing(int enable);
+ * ms and the number of timed launches since the last read, then resets. */
+int rsim_kernel_timing(int every);
 int rsim_kernel_timing_read(double* ms3, int* launches3);
 
 /* ---- FFT ----

@@ -38,6 +38,9 @@ def test_invalid_arguments_map_to_value_error_without_a_device():
     assert rc == lib.RSIM_E_INVALID_ARG and b"vacuum" in L.rsim_last_error()
     rc = L.rsim_volume_create(C.byref(vol), ids.ctypes.data, 3, 1, 2, 2, db.ctypes.data_as(lib._dp), 1)
     assert rc == lib.RSIM_E_INVALID_ARG
+    with pytest.raises(ValueError):
+        lib.call("rsim_kernel_timing", -1)
+    assert L.rsim_kernel_timing(16) == 0 and L.rsim_kernel_timing(0) == 0
 
 
 def test_cancel_flag_roundtrip_and_progress_registration():
