@@ -79,6 +79,14 @@ SIGNATURES = {
     "rsim_detector_area_v1": [_vp, _i, _i, _i, _d, _d, _i, _i, _d, _d, _d, _dp, _vp],
     "rsim_detector_area_v1_accumulate": [_vp, _i, _i, _i, _d, _d, _i, _i, _d, _d, _d, _d, _vp,
                                          _vp],
+    "rsim_transmission_db_2d": [_vp, _i, _i, _i, _d, _d, _vp, _vp, _vp],
+    "rsim_multislice_db_2d": [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _vp, _vp, _d, _i, _vp, _i,
+                              _vp],
+    "rsim_multislice_adjoint_workspace": [_i, _i, _i, _i, _i, C.POINTER(_sz), C.POINTER(_sz)],
+    "rsim_multislice_db_adjoint_2d": [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _vp, _vp, _d, _i,
+                                      _vp, _i, _vp, _vp, _vp, _vp],
+    "rsim_detector_area_v1_adjoint": [_vp, _vp, _i, _i, _i, _d, _d, _i, _i, _d, _d, _d, _vp,
+                                      _vp],
 }
 _RESTYPES = {"rsim_last_error": C.c_char_p, "rsim_version": C.c_char_p}
 

@@ -1,0 +1,159 @@
+"""Differentiable multislice: a torch.autograd.Function over the HIP adjoint.
+
+Forward: a plane (or given) incident field passes ``n`` slices of per-voxel
+float32 delta/beta (one voxel per field pixel), each slice applying
+big-wave's material_factor (big-wave/optical_element.py:39-42) and
+propagate_2d (big-wave/propagation.py:396-442); then a free-space leg over
+``z_leg`` and the area_v1 detector (big-wave/propagation.py:995-1130).
+Backward: the hand-written adjoint (rsim_multislice_db_adjoint_2d) with
+sqrt(n) slice checkpointing returns dL/d(delta, beta) per voxel.
+
+PyTorch is used for device memory, streams and the autograd graph only; all
+arithmetic runs in the HIP library on torch's current stream.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib as lib
+
+
+def _code(dtype: torch.dtype) -> int:
+    if dtype == torch.complex64:
+        return lib.RSIM_C64
+    if dtype == torch.complex128:
+        return lib.RSIM_C128
+    raise ValueError(f"field dtype must be complex64 or complex128, got {dtype}")
+
+
+_SIDE: dict = {}
+
+
+class _OnStream:
+    """Stream for library calls, ordered with torch's current stream.
+
+    A non-default current stream is used directly.  torch's default stream has
+    handle 0, which the C ABI reads as "the library's own stream", so then the
+    calls run on a side stream that first waits for the current stream and
+    that the current stream waits for afterwards.
+    """
+
+    def __enter__(self) -> C.c_void_p:
+        cur = torch.cuda.current_stream()
+        self.cur, self.side = cur, None
+        if cur.cuda_stream != 0:
+            return C.c_void_p(cur.cuda_stream)
+        side = _SIDE.get(cur.device)
+        if side is None:
+            side = _SIDE[cur.device] = torch.cuda.Stream(device=cur.device)
+        side.wait_stream(cur)
+        self.side = side
+        return C.c_void_p(side.cuda_stream)
+
+    def __exit__(self, *exc) -> None:
+        if self.side is not None:
+            self.cur.wait_stream(self.side)
+
+
+def _p(t: torch.Tensor) -> C.c_void_p:
+    return C.c_void_p(t.data_ptr())
+
+
+@dataclass(frozen=True)
+class MultisliceGeometry:
+    """Geometry of the differentiable path (SI units)."""
+    dx: float
+    wl: float
+    pz: float                 # slice thickness
+    cutoff: float             # circular cutoff frequency f (kx^2 + ky^2 <= 2 f^2)
+    z_leg: float              # free space after the last slice
+    out_x: int                # detector pixels
+    out_y: int
+    ds_x: float               # detector pixel size
+    ds_y: float
+    z_det: float              # current_z of area_v1's obliquity factor
+    dy: float = 0.0           # 0: dy = dx
+    ckpt_every: int = 0       # 0: ceil(sqrt(n_slices))
+
+    @property
+    def dyy(self) -> float:
+        return self.dy if self.dy != 0.0 else self.dx
+
+
+def _check_volume(name: str, t: torch.Tensor) -> None:
+    if t.dtype != torch.float32 or t.dim() != 3 or not t.is_cuda or not t.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous float32 CUDA tensor (n_slices, ny, nx)")
+
+
+class MultisliceIntensity(torch.autograd.Function):
+    """I = area_v1(|P_leg prod_s P T_s u0|^2) as a function of (delta, beta)."""
+
+    @staticmethod
+    def forward(ctx, delta, beta, u0, geom: MultisliceGeometry, dtype: torch.dtype):
+        _check_volume("delta", delta)
+        _check_volume("beta", beta)
+        if delta.shape != beta.shape:
+            raise ValueError("delta and beta must have the same shape")
+        n, ny, nx = delta.shape
+        code = _code(dtype)
+        every = geom.ckpt_every or max(1, math.ceil(math.sqrt(n)))
+        dev = delta.device
+        u = u0.to(device=dev, dtype=dtype).reshape(ny, nx).contiguous().clone()
+        U = torch.empty_like(u)
+        ck_b, wk_b = C.c_size_t(), C.c_size_t()
+        lib.call("rsim_multislice_adjoint_workspace", nx, ny, code, n, every, C.byref(ck_b),
+                 C.byref(wk_b))
+        ckpt = torch.empty(ck_b.value // u.element_size(), dtype=dtype, device=dev)
+        img = torch.zeros((geom.out_y, geom.out_x), dtype=torch.float64, device=dev)
+        with _OnStream() as s:
+            lib.call("rsim_multislice_db_2d", _p(u), _p(U), nx, ny, code, geom.dx, geom.dyy,
+                     geom.wl, geom.cutoff, _p(delta), _p(beta), geom.pz, n, _p(ckpt), every, s)
+            lib.call("rsim_propagate_2d", _p(u), _p(U), nx, ny, code, geom.dx, geom.dyy, geom.wl,
+                     geom.z_leg, geom.cutoff, 0, s)
+            lib.call("rsim_detector_area_v1_accumulate", _p(u), nx, ny, code, geom.dx, geom.dyy,
+                     geom.out_x, geom.out_y, geom.ds_x, geom.ds_y, geom.z_det, 1.0, _p(img), s)
+        ctx.save_for_backward(delta, beta, ckpt, u)
+        ctx.geom, ctx.every, ctx.code, ctx.work_bytes = geom, every, code, wk_b.value
+        return img
+
+    @staticmethod
+    def backward(ctx, grad_img):
+        delta, beta, ckpt, v = ctx.saved_tensors
+        geom, every, code = ctx.geom, ctx.every, ctx.code
+        n, ny, nx = delta.shape
+        gimg = grad_img.to(dtype=torch.float64).contiguous()
+        g = torch.empty_like(v)
+        U = torch.empty_like(v)
+        work = torch.empty(max(ctx.work_bytes // v.element_size(), 1), dtype=v.dtype,
+                           device=v.device)
+        gd = torch.empty_like(delta)
+        gb = torch.empty_like(beta)
+        with _OnStream() as s:
+            lib.call("rsim_detector_area_v1_adjoint", _p(v), _p(g), nx, ny, code, geom.dx,
+                     geom.dyy, geom.out_x, geom.out_y, geom.ds_x, geom.ds_y, geom.z_det,
+                     _p(gimg), s)
+            # adjoint of the leg: propagation over -z_leg (conjugate chirp, same mask)
+            lib.call("rsim_propagate_2d", _p(g), _p(U), nx, ny, code, geom.dx, geom.dyy, geom.wl,
+                     -geom.z_leg, geom.cutoff, 0, s)
+            lib.call("rsim_multislice_db_adjoint_2d", _p(g), _p(U), nx, ny, code, geom.dx,
+                     geom.dyy, geom.wl, geom.cutoff, _p(delta), _p(beta), geom.pz, n, _p(ckpt),
+                     every, _p(work), _p(gd), _p(gb), s)
+        return gd, gb, None, None, None
+
+
+def multislice_intensity(delta: torch.Tensor, beta: torch.Tensor, geom: MultisliceGeometry,
+                         u0: torch.Tensor | None = None,
+                         dtype: torch.dtype = torch.complex64) -> torch.Tensor:
+    """Detector image (out_y, out_x) float64, differentiable in delta and beta.
+
+    ``u0`` is the incident field (default: unit plane wave).  ``dtype`` is the
+    field precision (complex64: FP32 FFTs; complex128: FP64 throughout).
+    """
+    if u0 is None:
+        u0 = torch.ones(delta.shape[1:], dtype=dtype, device=delta.device)
+    return MultisliceIntensity.apply(delta, beta, u0, geom, dtype)

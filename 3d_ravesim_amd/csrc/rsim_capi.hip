@@ -1,5 +1,6 @@
 // extern "C" entry points of libravesim_hip.so (declared in include/ravesim.h).
 #include <atomic>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -26,6 +27,18 @@ int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double 
                  double freq, const rsim_volume* vol, double px, double py, double pz,
                  double xpos, double ypos, int first, int count, int engine, int flags,
                  hipStream_t s);
+int detector_area_v1_adjoint(const void* v, void* g, int nx, int ny, int dtype, double dx,
+                             double dy, int ox, int oy, double dsx, double dsy, double z,
+                             const double* gimg_dev, hipStream_t s);
+int transmission_db_2d(void* u, int nx, int ny, int dtype, double wl, double pz,
+                       const float* delta, const float* beta, hipStream_t s);
+int multislice_db(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
+                  double freq, const float* delta, const float* beta, double pz, int n,
+                  void* ckpt, int every, hipStream_t s);
+int multislice_db_adjoint(void* g, void* U, int nx, int ny, int dtype, double dx, double dy,
+                          double wl, double freq, const float* delta, const float* beta,
+                          double pz, int n, const void* ckpt, int every, void* work,
+                          float* gdelta, float* gbeta, hipStream_t s);
 int rsim_release_tables();
 int release_tables();
 int release_multislice_tables();
@@ -538,6 +551,76 @@ int rsim_detector_area_v1_accumulate(const void* u, int nx, int ny, int dtype, d
     return fail(RSIM_E_INVALID_ARG, "2D detector grid and pixel spacings must be positive");
   return detector_area_v1(u, nx, ny, dtype, dx, dy, out_x, out_y, ds_x, ds_y, z, weight, true,
                           out_dev, resolve_stream(stream));
+}
+
+int rsim_detector_area_v1_adjoint(const void* v, void* g, int nx, int ny, int dtype, double dx,
+                                  double dy, int out_x, int out_y, double ds_x, double ds_y,
+                                  double z, const double* grad_img, void* stream) {
+  RSIM_CHECK(check_field(v, nx, ny, dtype, "v"));
+  RSIM_CHECK(check_field(g, nx, ny, dtype, "g"));
+  if (!grad_img) return fail(RSIM_E_INVALID_ARG, "grad_img is NULL");
+  if (!(dx > 0) || !(dy > 0) || !(ds_x > 0) || !(ds_y > 0))
+    return fail(RSIM_E_INVALID_ARG, "2D detector grid and pixel spacings must be positive");
+  return detector_area_v1_adjoint(v, g, nx, ny, dtype, dx, dy, out_x, out_y, ds_x, ds_y, z,
+                                  grad_img, resolve_stream(stream));
+}
+
+static int check_db_args(int nx, int ny, double dx, double dy, double wl, double pz,
+                         const float* delta, const float* beta, int n, int every) {
+  RSIM_CHECK(check_fft_sizes(nx, ny));
+  if (!(dx > 0) || !(dy > 0) || !(wl > 0)) return fail(RSIM_E_INVALID_ARG, "dx, dy and wl must be positive");
+  if (!(pz > 0) || !std::isfinite(pz)) return fail(RSIM_E_INVALID_ARG, "pz must be positive and finite");
+  if (!delta || !beta) return fail(RSIM_E_INVALID_ARG, "delta and beta volumes must not be NULL");
+  if (n < 1) return fail(RSIM_E_INVALID_ARG, "n_slices must be >= 1");
+  if (every < 1) return fail(RSIM_E_INVALID_ARG, "ckpt_every must be >= 1");
+  return RSIM_OK;
+}
+
+int rsim_transmission_db_2d(void* u, int nx, int ny, int dtype, double wl, double pz,
+                            const float* delta, const float* beta, void* stream) {
+  RSIM_CHECK(check_field(u, nx, ny, dtype, "u"));
+  if (!delta || !beta) return fail(RSIM_E_INVALID_ARG, "delta and beta must not be NULL");
+  if (!(wl > 0) || !std::isfinite(pz)) return fail(RSIM_E_INVALID_ARG, "wl must be positive, pz finite");
+  return transmission_db_2d(u, nx, ny, dtype, wl, pz, delta, beta, resolve_stream(stream));
+}
+
+int rsim_multislice_db_2d(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
+                          double wl, double cutoff_freq, const float* delta, const float* beta,
+                          double pz, int n_slices, void* ckpt, int ckpt_every, void* stream) {
+  RSIM_CHECK(check_field(u, nx, ny, dtype, "u"));
+  RSIM_CHECK(check_field(U, nx, ny, dtype, "U"));
+  RSIM_CHECK(check_db_args(nx, ny, dx, dy, wl, pz, delta, beta, n_slices, ckpt ? ckpt_every : 1));
+  return multislice_db(u, U, nx, ny, dtype, dx, dy, wl, cutoff_freq, delta, beta, pz, n_slices,
+                       ckpt, ckpt ? ckpt_every : 1, resolve_stream(stream));
+}
+
+int rsim_multislice_adjoint_workspace(int nx, int ny, int dtype, int n_slices, int ckpt_every,
+                                      size_t* ckpt_bytes, size_t* work_bytes) {
+  if (!valid_dtype(dtype)) return fail(RSIM_E_INVALID_ARG, "unknown dtype %d", dtype);
+  if (nx < 1 || ny < 1 || n_slices < 1 || ckpt_every < 1)
+    return fail(RSIM_E_INVALID_ARG, "sizes, n_slices and ckpt_every must be >= 1");
+  const size_t fb = (size_t)nx * ny * elem_bytes(dtype);
+  const int seg = ckpt_every < n_slices ? ckpt_every : n_slices;
+  if (ckpt_bytes) *ckpt_bytes = (size_t)((n_slices + ckpt_every - 1) / ckpt_every) * fb;
+  if (work_bytes) *work_bytes = (size_t)(seg - 1) * fb;
+  return RSIM_OK;
+}
+
+int rsim_multislice_db_adjoint_2d(void* g, void* U, int nx, int ny, int dtype, double dx,
+                                  double dy, double wl, double cutoff_freq, const float* delta,
+                                  const float* beta, double pz, int n_slices, const void* ckpt,
+                                  int ckpt_every, void* work, float* grad_delta,
+                                  float* grad_beta, void* stream) {
+  RSIM_CHECK(check_field(g, nx, ny, dtype, "g"));
+  RSIM_CHECK(check_field(U, nx, ny, dtype, "U"));
+  RSIM_CHECK(check_db_args(nx, ny, dx, dy, wl, pz, delta, beta, n_slices, ckpt_every));
+  if (!ckpt) return fail(RSIM_E_INVALID_ARG, "ckpt is NULL");
+  if (!grad_delta || !grad_beta) return fail(RSIM_E_INVALID_ARG, "gradient outputs must not be NULL");
+  if (!work && ckpt_every > 1 && n_slices > 1)
+    return fail(RSIM_E_INVALID_ARG, "work is NULL but ckpt_every > 1");
+  return multislice_db_adjoint(g, U, nx, ny, dtype, dx, dy, wl, cutoff_freq, delta, beta, pz,
+                               n_slices, ckpt, ckpt_every, work, grad_delta, grad_beta,
+                               resolve_stream(stream));
 }
 
 }  // extern "C"

@@ -1,7 +1,9 @@
 // Pointwise and reduction kernels of the plain engine, evaluated in FP64 in
 // the reference's operation order (contraction disabled) so the complex128
 // path tracks big-wave's NumpyVector arithmetic to round-off.
+#include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "rsim_device.hpp"
@@ -244,82 +246,209 @@ __global__ void __launch_bounds__(256) area_v1_kernel(const cpx<R>* __restrict__
   else out[o] = acc;
 }
 
-int detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double dy, int ox,
-                     int oy, double dsx, double dsy, double z, double weight, bool accumulate,
-                     double* out_dev, hipStream_t s) {
+namespace {
+
+// Host plan of the per-axis overlaps (propagation.py:949-977).
+struct DetPlanHost {
+  std::vector<int> xi0, xi1, jlo, jhi;
+  std::vector<double> xf0, xf1, xdet, ylo, yhi, ydet;
+  double gyl;
+};
+
+void det_plan(int nx, int ny, double dx, double dy, int ox, int oy, double dsx, double dsy,
+              DetPlanHost* P) {
+  P->xdet.resize(ox);
+  P->ydet.resize(oy);
+  for (int p = 0; p < ox; ++p) P->xdet[p] = ((double)p - (double)(ox / 2)) * dsx;
+  for (int q = 0; q < oy; ++q) P->ydet[q] = ((double)q - (double)(oy / 2)) * dsy;
+  P->xi0.resize(ox); P->xi1.resize(ox); P->xf0.resize(ox); P->xf1.resize(ox);
+  P->jlo.resize(oy); P->jhi.resize(oy); P->ylo.resize(oy); P->yhi.resize(oy);
+  const double gl = (-(double)(nx / 2) - 0.5) * dx;
+  const double gr = gl + nx * dx;
+  for (int p = 0; p < ox; ++p) {
+    const double lo = P->xdet[p] - dsx * 0.5, hi = P->xdet[p] + dsx * 0.5;
+    const double clo = std::min(std::max(lo, gl), gr);
+    double chi = std::min(std::max(hi, gl), gr);
+    chi = std::max(chi, clo);
+    const double slo = (clo - gl) / dx, shi = (chi - gl) / dx;
+    long long a = (long long)std::floor(slo), b = (long long)std::floor(shi);
+    double fa = slo - (double)a, fb = shi - (double)b;
+    if (a >= nx) { a = nx; fa = 0.0; }
+    if (b >= nx) { b = nx; fb = 0.0; }
+    P->xi0[p] = (int)a; P->xi1[p] = (int)b; P->xf0[p] = fa; P->xf1[p] = fb;
+  }
+  P->gyl = (-(double)(ny / 2) - 0.5) * dy;
+  for (int q = 0; q < oy; ++q) {
+    P->ylo[q] = P->ydet[q] - dsy * 0.5;
+    P->yhi[q] = P->ydet[q] + dsy * 0.5;
+    // rows whose cell [gyl + j dy, gyl + j dy + dy] can overlap (checked exactly in the kernel)
+    long long a = (long long)std::floor((P->ylo[q] - P->gyl) / dy) - 1;
+    long long b = (long long)std::floor((P->yhi[q] - P->gyl) / dy) + 2;
+    a = std::max<long long>(0, std::min<long long>(a, ny));
+    b = std::max<long long>(0, std::min<long long>(b, ny));
+    P->jlo[q] = (int)a;
+    P->jhi[q] = (int)std::max(a, b);
+  }
+}
+
+// One device blob: int arrays, then (8-byte aligned) double arrays.
+struct Blob {
+  std::vector<unsigned char> bytes;
+  std::vector<size_t> ioff, doff;
+};
+Blob make_blob(const std::vector<const std::vector<int>*>& iv,
+               const std::vector<const std::vector<double>*>& dv) {
+  Blob b;
+  size_t o = 0;
+  for (auto* v : iv) { b.ioff.push_back(o); o += v->size() * 4; }
+  o = (o + 7) / 8 * 8;
+  for (auto* v : dv) { b.doff.push_back(o); o += v->size() * 8; }
+  b.bytes.resize(std::max<size_t>(o, 8));
+  for (size_t k = 0; k < iv.size(); ++k)
+    if (!iv[k]->empty()) std::memcpy(b.bytes.data() + b.ioff[k], iv[k]->data(), iv[k]->size() * 4);
+  for (size_t k = 0; k < dv.size(); ++k)
+    if (!dv[k]->empty()) std::memcpy(b.bytes.data() + b.doff[k], dv[k]->data(), dv[k]->size() * 8);
+  return b;
+}
+
+int check_detector(int ox, int oy, double z) {
   if (ox <= 0 || oy <= 0) return fail(RSIM_E_INVALID_ARG, "detector output must be non-empty");
   if (!std::isfinite(z) || z == 0.0)
     return fail(RSIM_E_INVALID_ARG, "detector current_z must be finite and non-zero");
-  // per-axis overlap plans (propagation.py:949-977)
-  std::vector<double> xdet(ox), ydet(oy);
-  for (int p = 0; p < ox; ++p) xdet[p] = ((double)p - (double)(ox / 2)) * dsx;
-  for (int q = 0; q < oy; ++q) ydet[q] = ((double)q - (double)(oy / 2)) * dsy;
-  std::vector<int> xi0(ox), xi1(ox), jlo(oy), jhi(oy);
-  std::vector<double> xf0(ox), xf1(ox), ylo(oy), yhi(oy);
-  {
-    const double gl = (-(double)(nx / 2) - 0.5) * dx;
-    const double gr = gl + nx * dx;
-    for (int p = 0; p < ox; ++p) {
-      const double lo = xdet[p] - dsx * 0.5, hi = xdet[p] + dsx * 0.5;
-      const double clo = std::min(std::max(lo, gl), gr);
-      double chi = std::min(std::max(hi, gl), gr);
-      chi = std::max(chi, clo);
-      const double slo = (clo - gl) / dx, shi = (chi - gl) / dx;
-      long long a = (long long)std::floor(slo), b = (long long)std::floor(shi);
-      double fa = slo - (double)a, fb = shi - (double)b;
-      if (a >= nx) { a = nx; fa = 0.0; }
-      if (b >= nx) { b = nx; fb = 0.0; }
-      xi0[p] = (int)a; xi1[p] = (int)b; xf0[p] = fa; xf1[p] = fb;
-    }
-  }
-  const double gyl = (-(double)(ny / 2) - 0.5) * dy;
-  for (int q = 0; q < oy; ++q) {
-    ylo[q] = ydet[q] - dsy * 0.5;
-    yhi[q] = ydet[q] + dsy * 0.5;
-    // rows whose cell [gyl + j dy, gyl + j dy + dy] can overlap (checked exactly in the kernel)
-    long long a = (long long)std::floor((ylo[q] - gyl) / dy) - 1;
-    long long b = (long long)std::floor((yhi[q] - gyl) / dy) + 2;
-    a = std::max<long long>(0, std::min<long long>(a, ny));
-    b = std::max<long long>(0, std::min<long long>(b, ny));
-    jlo[q] = (int)a;
-    jhi[q] = (int)std::max(a, b);
-  }
-  // one device blob for the plan
-  const size_t nI = 2 * (size_t)ox + 2 * (size_t)oy;
-  const size_t nD = 3 * (size_t)ox + 3 * (size_t)oy;
-  std::vector<unsigned char> blob(nI * 4 + 8 + nD * 8);
-  int* bi = (int*)blob.data();
-  std::copy(xi0.begin(), xi0.end(), bi);
-  std::copy(xi1.begin(), xi1.end(), bi + ox);
-  std::copy(jlo.begin(), jlo.end(), bi + 2 * ox);
-  std::copy(jhi.begin(), jhi.end(), bi + 2 * ox + oy);
-  const size_t doff = ((nI * 4 + 7) / 8) * 8;
-  double* bd = (double*)(blob.data() + doff);
-  std::copy(xf0.begin(), xf0.end(), bd);
-  std::copy(xf1.begin(), xf1.end(), bd + ox);
-  std::copy(xdet.begin(), xdet.end(), bd + 2 * ox);
-  std::copy(ylo.begin(), ylo.end(), bd + 3 * ox);
-  std::copy(yhi.begin(), yhi.end(), bd + 3 * ox + oy);
-  std::copy(ydet.begin(), ydet.end(), bd + 3 * ox + 2 * oy);
+  return RSIM_OK;
+}
+
+}  // namespace
+
+int detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double dy, int ox,
+                     int oy, double dsx, double dsy, double z, double weight, bool accumulate,
+                     double* out_dev, hipStream_t s) {
+  RSIM_CHECK(check_detector(ox, oy, z));
+  DetPlanHost P;
+  det_plan(nx, ny, dx, dy, ox, oy, dsx, dsy, &P);
+  Blob b = make_blob({&P.xi0, &P.xi1, &P.jlo, &P.jhi},
+                     {&P.xf0, &P.xf1, &P.xdet, &P.ylo, &P.yhi, &P.ydet});
   unsigned char* d = nullptr;
-  RSIM_HIP(hipMallocAsync((void**)&d, blob.size(), s));
-  RSIM_HIP(hipMemcpyAsync(d, blob.data(), blob.size(), hipMemcpyHostToDevice, s));
+  RSIM_HIP(hipMallocAsync((void**)&d, b.bytes.size(), s));
+  RSIM_HIP(hipMemcpyAsync(d, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice, s));
   DetPlanDev pl;
-  const int* di = (const int*)d;
-  const double* dd = (const double*)(d + doff);
-  pl.xi0 = di; pl.xi1 = di + ox; pl.jlo = di + 2 * ox; pl.jhi = di + 2 * ox + oy;
-  pl.xf0 = dd; pl.xf1 = dd + ox; pl.xdet = dd + 2 * ox;
-  pl.ylo = dd + 3 * ox; pl.yhi = dd + 3 * ox + oy; pl.ydet = dd + 3 * ox + 2 * oy;
+  pl.xi0 = (const int*)(d + b.ioff[0]); pl.xi1 = (const int*)(d + b.ioff[1]);
+  pl.jlo = (const int*)(d + b.ioff[2]); pl.jhi = (const int*)(d + b.ioff[3]);
+  pl.xf0 = (const double*)(d + b.doff[0]); pl.xf1 = (const double*)(d + b.doff[1]);
+  pl.xdet = (const double*)(d + b.doff[2]); pl.ylo = (const double*)(d + b.doff[3]);
+  pl.yhi = (const double*)(d + b.doff[4]); pl.ydet = (const double*)(d + b.doff[5]);
   dim3 grid((ox + 15) / 16, (oy + 15) / 16);
   if (dtype == RSIM_C64)
     hipLaunchKernelGGL(area_v1_kernel<float>, grid, dim3(256), 0, s, (const cpx<float>*)u, nx, ny,
-                       ox, oy, pl, dx, dy, gyl, z, weight, accumulate ? 1 : 0, out_dev);
+                       ox, oy, pl, dx, dy, P.gyl, z, weight, accumulate ? 1 : 0, out_dev);
   else
     hipLaunchKernelGGL(area_v1_kernel<double>, grid, dim3(256), 0, s, (const cpx<double>*)u, nx,
-                       ny, ox, oy, pl, dx, dy, gyl, z, weight, accumulate ? 1 : 0, out_dev);
+                       ny, ox, oy, pl, dx, dy, P.gyl, z, weight, accumulate ? 1 : 0, out_dev);
   RSIM_LAUNCHED();
   RSIM_HIP(hipFreeAsync(d, s));
   RSIM_HIP(hipStreamSynchronize(s));   // host plan vectors must outlive the copy
+  return RSIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of area_v1.  The forward is I[q,p] = cos(q,p) * sum_{j,i} oy(q,j)
+// wx(p,i) |v[j,i]|^2 with the column weight wx(p,i) = dx * ([i0 <= i < i1] +
+// f1 [i == i1] - f0 [i == i0]) and the row overlap oy(q,j) of area_v1_kernel.
+// Given gI = dL/dI (real), dL/dv* = v * G with G[j,i] = sum_{q,p} gI[q,p]
+// cos(q,p) oy(q,j) wx(p,i).  Each field cell gathers from the detector
+// pixels whose column / row ranges contain it: p in [plo[i], phi[i]), q in
+// [qlo[j], qhi[j]) (both ranges are monotone in the cell index).
+// ---------------------------------------------------------------------------
+struct DetAdjDev {
+  const int* plo;
+  const int* phi;
+  const int* qlo;
+  const int* qhi;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(256) area_v1_adjoint_kernel(
+    const cpx<R>* __restrict__ v, cpx<R>* __restrict__ g, int nx, int ny, int ox, DetPlanDev pl,
+    DetAdjDev ap, double dx, double dy, double gyl, double z, const double* __restrict__ gimg) {
+  const long long n = (long long)nx * ny;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int j = (int)(idx / nx);
+    const int i = (int)(idx - (long long)j * nx);
+    const double cell_lo = gyl + j * dy;
+    const double cell_hi = cell_lo + dy;
+    double G = 0.0;
+    for (int q = ap.qlo[j]; q < ap.qhi[j]; ++q) {
+      const double ov = fmin(cell_hi, pl.yhi[q]) - fmax(cell_lo, pl.ylo[q]);
+      if (!(ov > 0.0)) continue;
+      const double yd = pl.ydet[q];
+      double h = 0.0;
+      for (int p = ap.plo[i]; p < ap.phi[i]; ++p) {
+        const int i0 = pl.xi0[p], i1 = pl.xi1[p];
+        double w = (i >= i0 && i < i1) ? dx : 0.0;
+        if (i == i1) w += pl.xf1[p] * dx;
+        if (i == i0) w -= pl.xf0[p] * dx;
+        const double xd = pl.xdet[p];
+        const double cosine = z / sqrt(xd * xd + yd * yd + z * z);
+        h += gimg[(long long)q * ox + p] * cosine * w;
+      }
+      G += ov * h;
+    }
+    const cpx<R> a = v[idx];
+    g[idx] = mk((R)((double)a.x * G), (R)((double)a.y * G));
+  }
+}
+
+int detector_area_v1_adjoint(const void* v, void* g, int nx, int ny, int dtype, double dx,
+                             double dy, int ox, int oy, double dsx, double dsy, double z,
+                             const double* gimg_dev, hipStream_t s) {
+  RSIM_CHECK(check_detector(ox, oy, z));
+  DetPlanHost P;
+  det_plan(nx, ny, dx, dy, ox, oy, dsx, dsy, &P);
+  // pixels containing column i: xi0[p] <= i <= xi1[p]; rows: jlo[q] <= j < jhi[q]
+  std::vector<int> plo(nx), phi(nx), qlo(ny), qhi(ny);
+  {
+    int a = 0, b = 0;
+    for (int i = 0; i < nx; ++i) {
+      while (a < ox && P.xi1[a] < i) ++a;
+      while (b < ox && P.xi0[b] <= i) ++b;
+      plo[i] = a;
+      phi[i] = std::max(a, b);
+    }
+    a = b = 0;
+    for (int j = 0; j < ny; ++j) {
+      while (a < oy && P.jhi[a] <= j) ++a;
+      while (b < oy && P.jlo[b] <= j) ++b;
+      qlo[j] = a;
+      qhi[j] = std::max(a, b);
+    }
+  }
+  Blob b = make_blob({&P.xi0, &P.xi1, &plo, &phi, &qlo, &qhi},
+                     {&P.xf0, &P.xf1, &P.xdet, &P.ylo, &P.yhi, &P.ydet});
+  unsigned char* d = nullptr;
+  RSIM_HIP(hipMallocAsync((void**)&d, b.bytes.size(), s));
+  RSIM_HIP(hipMemcpyAsync(d, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice, s));
+  DetPlanDev pl{};
+  pl.xi0 = (const int*)(d + b.ioff[0]); pl.xi1 = (const int*)(d + b.ioff[1]);
+  pl.xf0 = (const double*)(d + b.doff[0]); pl.xf1 = (const double*)(d + b.doff[1]);
+  pl.xdet = (const double*)(d + b.doff[2]); pl.ylo = (const double*)(d + b.doff[3]);
+  pl.yhi = (const double*)(d + b.doff[4]); pl.ydet = (const double*)(d + b.doff[5]);
+  DetAdjDev ap;
+  ap.plo = (const int*)(d + b.ioff[2]); ap.phi = (const int*)(d + b.ioff[3]);
+  ap.qlo = (const int*)(d + b.ioff[4]); ap.qhi = (const int*)(d + b.ioff[5]);
+  const long long n = (long long)nx * ny;
+  const int blocks = (int)std::min<long long>((n + 255) / 256, 65536);
+  if (dtype == RSIM_C64)
+    hipLaunchKernelGGL(area_v1_adjoint_kernel<float>, dim3(blocks), dim3(256), 0, s,
+                       (const cpx<float>*)v, (cpx<float>*)g, nx, ny, ox, pl, ap, dx, dy, P.gyl, z,
+                       gimg_dev);
+  else
+    hipLaunchKernelGGL(area_v1_adjoint_kernel<double>, dim3(blocks), dim3(256), 0, s,
+                       (const cpx<double>*)v, (cpx<double>*)g, nx, ny, ox, pl, ap, dx, dy, P.gyl,
+                       z, gimg_dev);
+  RSIM_LAUNCHED();
+  RSIM_HIP(hipFreeAsync(d, s));
+  RSIM_HIP(hipStreamSynchronize(s));
   return RSIM_OK;
 }
 

@@ -176,6 +176,45 @@ int rsim_detector_area_v1_accumulate(const void* u, int nx, int ny, int dtype, d
                                      int out_x, int out_y, double ds_x, double ds_y, double z,
                                      double weight, double* out_dev, void* stream);
 
+/* ---- Differentiable multislice (per-voxel delta/beta volumes) ----
+ * The reference has no adjoint; these entry points are the gradient path of
+ * the same physics.  Volumes are device float32 arrays (n_slices, ny, nx),
+ * C-order, one voxel per field pixel: the direct per-voxel form of
+ * fast-wave's complex delta-beta grid (fast-wave/fwcuda/kernels.cu:411-491).
+ * Slice s multiplies u by material_factor(delta_s + i beta_s, pz, wl)
+ * = exp(-2 pi pz/wl (beta_s + i delta_s)) (big-wave/optical_element.py:39-42)
+ * and then runs propagate_2d over pz (big-wave/propagation.py:396-442). */
+
+/* One transmission slice (the voxel multiply of Sample._apply_2d,
+ * big-wave/optical_element.py:580-582, with per-voxel delta/beta). */
+int rsim_transmission_db_2d(void* u, int nx, int ny, int dtype, double wl, double pz,
+                            const float* delta, const float* beta, void* stream);
+/* Forward multislice over n_slices.  U is scratch (holds FFT2(u) on return).
+ * ckpt (optional): receives u before slice k for every k % ckpt_every == 0,
+ * ceil(n_slices / ckpt_every) fields. */
+int rsim_multislice_db_2d(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
+                          double wl, double cutoff_freq, const float* delta, const float* beta,
+                          double pz, int n_slices, void* ckpt, int ckpt_every, void* stream);
+/* Bytes of the checkpoint and work buffers for (n_slices, ckpt_every);
+ * ckpt_every = ceil(sqrt(n_slices)) minimises their sum. */
+int rsim_multislice_adjoint_workspace(int nx, int ny, int dtype, int n_slices, int ckpt_every,
+                                      size_t* ckpt_bytes, size_t* work_bytes);
+/* Adjoint multislice.  On entry g = dL/du* after the last slice, on return
+ * dL/du* before the first.  grad_delta / grad_beta (n_slices, ny, nx) receive
+ * dL/d delta and dL/d beta.  ckpt as written by rsim_multislice_db_2d with the
+ * same ckpt_every; work holds the rebuilt fields of one segment; U is scratch. */
+int rsim_multislice_db_adjoint_2d(void* g, void* U, int nx, int ny, int dtype, double dx,
+                                  double dy, double wl, double cutoff_freq, const float* delta,
+                                  const float* beta, double pz, int n_slices, const void* ckpt,
+                                  int ckpt_every, void* work, float* grad_delta,
+                                  float* grad_beta, void* stream);
+/* Adjoint of rsim_detector_area_v1 (big-wave/propagation.py:995-1130): given
+ * grad_img = dL/dI (out_y, out_x) float64 on the device, writes
+ * g = dL/dv* = v * A^T grad_img, where I = A |v|^2. */
+int rsim_detector_area_v1_adjoint(const void* v, void* g, int nx, int ny, int dtype, double dx,
+                                  double dy, int out_x, int out_y, double ds_x, double ds_y,
+                                  double z, const double* grad_img, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,6 +32,15 @@ Pinning (what checks each function against something other than itself):
 * ``area_v1``                   -- brute-force overlap loops
   (tests/big_wave_2d/test_p3.py:85-123) and uniform-field area preservation
   (test_p3.py:222-243).
+* ``area_v1_matrices`` / ``area_v1_adjoint`` -- the dense form reproduces
+  ``area_v1`` (itself pinned by brute force); the adjoint passes the
+  dot-product test <gI, A x> = <A^T gI, x>.
+* ``multislice_db_2d`` / ``multislice_db_intensity`` -- equals
+  ``sample_apply_2d`` on an integral-pitch id volume carrying the same
+  delta/beta per voxel.
+* ``multislice_db_gradient``    -- central finite differences of
+  ``multislice_db_intensity`` (FP64) and complex128 torch autograd of an
+  independent torch forward, on small volumes.
 * ``run_point_source_sample``   -- the reference's own end-to-end golden
   fingerprints tests/big_wave_2d/p5_results/{vacuum,sample}_convergence_4096.json
   (inputs in tests/big_wave_2d/p5_benchmark.py:69-185).
@@ -355,6 +364,94 @@ def run_point_source_sample(nx, ny, dx, dy, wl, source_xyz, cutoff_freq, z_detec
                             workers=workers)
     det = area_v1(u, dx, dy, det_size_x, det_size_y, ds_x, ds_y, z_detector)
     return (det, u) if return_field else det
+
+
+# ---------------------------------------------------------------------------
+# Per-voxel delta/beta multislice and its adjoint (the gradient path; the
+# reference has no adjoint).  The forward restates Sample._apply_2d
+# (optical_element.py:533-596) with the voxel grid equal to the field grid and
+# delta/beta given per voxel, as fast-wave's per-voxel complex grid
+# (fast-wave/fwcuda/kernels.cu:411-491).
+# ---------------------------------------------------------------------------
+
+def area_v1_matrices(nx, ny, dx, dy, size_x, size_y, ds_x, ds_y, z):
+    """(Wy, Wx, cosine) with area_v1(u) == cosine * (Wy @ |u|^2 @ Wx.T).
+
+    Wx[p, i] is the x overlap weight of cell i in pixel p as area_v1's prefix
+    difference applies it (propagation.py:980-992); Wy[q, j] the y overlap.
+    """
+    out_y, out_x = detector_shape(size_x, size_y, ds_x, ds_y)
+    x_det = (np.arange(out_x, dtype=np.float64) - out_x // 2) * ds_x
+    y_det = (np.arange(out_y, dtype=np.float64) - out_y // 2) * ds_y
+    xi0, xf0, xi1, xf1, _, _ = area_axis_plan(x_det, ds_x, nx, dx)
+    _, _, _, _, y_lo, y_hi = area_axis_plan(y_det, ds_y, ny, dy)
+    i = np.arange(nx)
+    Wx = dx * ((i[None, :] < xi1[:, None]).astype(np.float64) - (i[None, :] < xi0[:, None]))
+    Wx += np.where(i[None, :] == xi1[:, None], xf1[:, None] * dx, 0.0)
+    Wx -= np.where(i[None, :] == xi0[:, None], xf0[:, None] * dx, 0.0)
+    grid_y_left = (-(ny // 2) - 0.5) * dy
+    cell_lo = grid_y_left + np.arange(ny) * dy
+    cell_hi = cell_lo + dy
+    Wy = np.minimum(cell_hi[None, :], y_hi[:, None]) - np.maximum(cell_lo[None, :], y_lo[:, None])
+    Wy = np.where(Wy > 0, Wy, 0.0)
+    cosine = z / np.sqrt(np.square(x_det)[None, :] + np.square(y_det)[:, None] + z * z)
+    return Wy, Wx, cosine
+
+
+def area_v1_adjoint(v2d, grad_img, dx, dy, size_x, size_y, ds_x, ds_y, z):
+    """dL/dv* = v * A^T dL/dI for I = area_v1(v) = A |v|^2."""
+    ny, nx = v2d.shape
+    Wy, Wx, cosine = area_v1_matrices(nx, ny, dx, dy, size_x, size_y, ds_x, ds_y, z)
+    return v2d * (Wy.T @ (cosine * grad_img) @ Wx)
+
+
+def multislice_db_2d(u2d, delta, beta, dx, dy, wl, pz, cutoff_freq, workers=1, keep=False):
+    """Per slice s: u *= material_factor(delta_s + i beta_s, pz, wl); propagate_2d(pz).
+
+    With keep=True also returns the field before each slice.
+    """
+    u = u2d.astype(np.complex128)
+    before = []
+    for s in range(delta.shape[0]):
+        if keep:
+            before.append(u)
+        t = material_factor(delta[s].astype(np.float64) + 1j * beta[s].astype(np.float64), pz, wl)
+        u, _ = propagate_2d(u * t, dx, dy, wl, pz, cutoff_freq, workers=workers)
+    return (u, before) if keep else u
+
+
+def multislice_db_intensity(u2d, delta, beta, dx, dy, wl, pz, cutoff_freq, z_leg,
+                            size_x, size_y, ds_x, ds_y, z_det, workers=1):
+    """Slices, a free-space leg over z_leg, then area_v1 at z_det."""
+    u = multislice_db_2d(u2d, delta, beta, dx, dy, wl, pz, cutoff_freq, workers)
+    v, _ = propagate_2d(u, dx, dy, wl, z_leg, cutoff_freq, workers=workers)
+    return area_v1(v, dx, dy, size_x, size_y, ds_x, ds_y, z_det)
+
+
+def multislice_db_gradient(u2d, delta, beta, dx, dy, wl, pz, cutoff_freq, z_leg,
+                           size_x, size_y, ds_x, ds_y, z_det, grad_img, workers=1):
+    """(dL/d delta, dL/d beta) for a loss L with dL/dI = grad_img.
+
+    Reverse-mode Wirtinger calculus for real L: with g = dL/du* after a
+    slice, h = P^H g (propagation over -pz), w = t_s u_s,
+    dL/d delta_s = 2 K Im(conj(h) w), dL/d beta_s = -2 K Re(conj(h) w),
+    K = 2 pi pz / wl, and g <- conj(t_s) h.
+    """
+    u, before = multislice_db_2d(u2d, delta, beta, dx, dy, wl, pz, cutoff_freq, workers, keep=True)
+    v, _ = propagate_2d(u, dx, dy, wl, z_leg, cutoff_freq, workers=workers)
+    g = area_v1_adjoint(v, grad_img, dx, dy, size_x, size_y, ds_x, ds_y, z_det)
+    g, _ = propagate_2d(g, dx, dy, wl, -z_leg, cutoff_freq, workers=workers)
+    K = 2 * np.pi * pz / wl
+    gd = np.zeros(delta.shape, np.float64)
+    gb = np.zeros(beta.shape, np.float64)
+    for s in range(delta.shape[0] - 1, -1, -1):
+        h, _ = propagate_2d(g, dx, dy, wl, -pz, cutoff_freq, workers=workers)
+        t = material_factor(delta[s].astype(np.float64) + 1j * beta[s].astype(np.float64), pz, wl)
+        c = np.conj(h) * (t * before[s])
+        gd[s] = 2 * K * c.imag
+        gb[s] = -2 * K * c.real
+        g = np.conj(t) * h
+    return gd, gb
 
 
 def relative_l2(actual, expected) -> float:

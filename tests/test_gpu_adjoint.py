@@ -1,0 +1,242 @@
+"""GPU parity of the differentiable multislice (per-voxel delta/beta) and its adjoint.
+
+Every case runs the HIP library through the C ABI (directly, or through the
+torch.autograd.Function wrapper on torch-owned device memory) and compares
+with the FP64 oracle or the complex128 CPU torch autograd reference.
+"""
+
+import ctypes as C
+import importlib
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from _adjoint_ref import case, loss_and_grad_img, oracle_gradient, torch_gradient
+from _rs import rs
+
+pytestmark = pytest.mark.gpu
+lib = rs._lib
+AG = importlib.import_module("3d_ravesim_amd.autograd")
+TORCH_C = {np.complex64: torch.complex64, np.complex128: torch.complex128}
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def P(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def S():
+    h = torch.cuda.current_stream().cuda_stream
+    assert h != 0, "tests run on a non-default torch stream (0 selects the library's own)"
+    return C.c_void_p(h)
+
+
+@pytest.fixture(autouse=True)
+def _torch_stream():
+    """Run each test's torch work and library calls on one non-default stream."""
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        yield
+    torch.cuda.synchronize()
+
+
+def code(dt):
+    return lib.RSIM_C64 if dt == np.complex64 else lib.RSIM_C128
+
+
+def field(a, dt):
+    return torch.from_numpy(np.ascontiguousarray(a.astype(dt))).cuda()
+
+
+def vol(a):
+    return torch.from_numpy(np.ascontiguousarray(a.astype(np.float32))).cuda()
+
+
+def rel(a, e):
+    return O.relative_l2(a, e)
+
+
+@pytest.mark.parametrize("dt,tol", [(np.complex128, 1e-14), (np.complex64, 2e-7)])
+def test_transmission_db_matches_oracle(dt, tol):
+    delta, beta, u0, geo = case(n=64, nz=1, seed=11)
+    say("transmission_db", dt.__name__)
+    u = field(u0, dt)
+    d, b = vol(delta), vol(beta)   # held: a temporary's memory could be reused at once
+    lib.call("rsim_transmission_db_2d", P(u), 64, 64, code(dt), geo["wl"], geo["pz"], P(d), P(b),
+             S())
+    t = O.material_factor(delta[0].astype(np.float64) + 1j * beta[0].astype(np.float64),
+                          geo["pz"], geo["wl"])
+    assert rel(u.cpu().numpy(), u0 * t) < tol
+
+
+@pytest.mark.parametrize("shape,ds,size", [((7, 11), (1.3, 1.7), (9.0, 6.5)),
+                                           ((64, 64), (4.0, 4.0), (64.0, 64.0)),
+                                           ((12, 10), (0.6, 0.45), (8.0, 10.5))])
+@pytest.mark.parametrize("dt,tol", [(np.complex128, 1e-13), (np.complex64, 1e-6)])
+def test_area_v1_adjoint_matches_oracle(shape, ds, size, dt, tol):
+    rng = np.random.default_rng(12)
+    ny, nx = shape
+    v = rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    oy, ox = O.detector_shape(size[0], size[1], ds[0], ds[1])
+    gI = rng.normal(size=(oy, ox))
+    exp = O.area_v1_adjoint(v.astype(dt).astype(np.complex128), gI, 1.0, 1.1, size[0], size[1],
+                            ds[0], ds[1], 5.0)
+    say("area_v1_adjoint", shape, ds, dt.__name__)
+    vd = field(v, dt)
+    g = torch.empty_like(vd)
+    gi = torch.from_numpy(gI).cuda()
+    lib.call("rsim_detector_area_v1_adjoint", P(vd), P(g), nx, ny, code(dt), 1.0, 1.1, ox, oy,
+             ds[0], ds[1], 5.0, P(gi), S())
+    assert rel(g.cpu().numpy(), exp) < tol
+
+
+@pytest.mark.parametrize("dt,tol", [(np.complex128, 1e-12), (np.complex64, 3e-6)])
+def test_multislice_db_forward_and_checkpoints_match_oracle(dt, tol):
+    delta, beta, u0, geo = case(n=128, nz=6, seed=13)
+    exp, before = O.multislice_db_2d(u0, delta, beta, geo["dx"], geo["dy"], geo["wl"], geo["pz"],
+                                     geo["cutoff_freq"], keep=True)
+    say("multislice_db forward", dt.__name__)
+    u = field(u0, dt)
+    U = torch.empty_like(u)
+    every = 4
+    ck = torch.empty((2, 128, 128), dtype=u.dtype, device=u.device)
+    d, b = vol(delta), vol(beta)
+    lib.call("rsim_multislice_db_2d", P(u), P(U), 128, 128, code(dt), geo["dx"], geo["dy"],
+             geo["wl"], geo["cutoff_freq"], P(d), P(b), geo["pz"], 6, P(ck), every, S())
+    assert rel(u.cpu().numpy(), exp) < tol
+    assert rel(ck[0].cpu().numpy(), before[0]) < tol
+    assert rel(ck[1].cpu().numpy(), before[4]) < tol
+    assert rel(U.cpu().numpy(), O.fft2(exp)) < tol
+
+
+def _run_adjoint(delta, beta, u0, geo, gI, dt, every):
+    """HIP forward (with checkpoints), leg, detector adjoint, leg adjoint, slice adjoint."""
+    nz, ny, nx = delta.shape
+    c = code(dt)
+    d, b = vol(delta), vol(beta)
+    u = field(u0, dt)
+    U = torch.empty_like(u)
+    ckb, wkb = C.c_size_t(), C.c_size_t()
+    lib.call("rsim_multislice_adjoint_workspace", nx, ny, c, nz, every, C.byref(ckb), C.byref(wkb))
+    es = u.element_size()
+    ck = torch.empty(ckb.value // es, dtype=u.dtype, device=u.device)
+    work = torch.empty(max(wkb.value // es, 1), dtype=u.dtype, device=u.device)
+    s = S()
+    lib.call("rsim_multislice_db_2d", P(u), P(U), nx, ny, c, geo["dx"], geo["dy"], geo["wl"],
+             geo["cutoff_freq"], P(d), P(b), geo["pz"], nz, P(ck), every, s)
+    lib.call("rsim_propagate_2d", P(u), P(U), nx, ny, c, geo["dx"], geo["dy"], geo["wl"],
+             geo["z_leg"], geo["cutoff_freq"], 0, s)
+    oy, ox = gI.shape
+    g = torch.empty_like(u)
+    gi = torch.from_numpy(gI).cuda()
+    lib.call("rsim_detector_area_v1_adjoint", P(u), P(g), nx, ny, c, geo["dx"], geo["dy"], ox, oy,
+             geo["ds_x"], geo["ds_y"], geo["z_det"], P(gi), s)
+    lib.call("rsim_propagate_2d", P(g), P(U), nx, ny, c, geo["dx"], geo["dy"], geo["wl"],
+             -geo["z_leg"], geo["cutoff_freq"], 0, s)
+    gd, gb = torch.empty_like(d), torch.empty_like(b)
+    lib.call("rsim_multislice_db_adjoint_2d", P(g), P(U), nx, ny, c, geo["dx"], geo["dy"],
+             geo["wl"], geo["cutoff_freq"], P(d), P(b), geo["pz"], nz, P(ck), every, P(work),
+             P(gd), P(gb), s)
+    torch.cuda.synchronize()
+    return gd.cpu().numpy(), gb.cpu().numpy()
+
+
+def _target(delta, beta, u0, geo):
+    return O.multislice_db_intensity(u0, delta * 1.05, beta * 0.96, geo["dx"], geo["dy"],
+                                     geo["wl"], geo["pz"], geo["cutoff_freq"], geo["z_leg"],
+                                     geo["size_x"], geo["size_y"], geo["ds_x"], geo["ds_y"],
+                                     geo["z_det"])
+
+
+@pytest.mark.parametrize("every", [1, 3, 7])
+@pytest.mark.parametrize("dt,tol", [(np.complex128, 1e-9), (np.complex64, 1e-4)])
+def test_adjoint_gradient_matches_oracle(every, dt, tol):
+    delta, beta, u0, geo = case(n=96, nz=7, seed=14)
+    target = _target(delta, beta, u0, geo)
+    _, gI = loss_and_grad_img(delta, beta, u0, geo, target)
+    ed, eb = oracle_gradient(delta, beta, u0, geo, gI)
+    say("adjoint gradient", dt.__name__, "every", every)
+    gd, gb = _run_adjoint(delta, beta, u0, geo, gI, dt, every)
+    assert rel(gd, ed) < tol
+    assert rel(gb, eb) < tol
+
+
+def test_checkpoint_interval_does_not_change_gradients():
+    delta, beta, u0, geo = case(n=64, nz=9, seed=15)
+    target = _target(delta, beta, u0, geo)
+    _, gI = loss_and_grad_img(delta, beta, u0, geo, target)
+    say("checkpoint invariance")
+    ref = _run_adjoint(delta, beta, u0, geo, gI, np.complex128, 1)
+    for every in (2, 3, 9, 20):
+        got = _run_adjoint(delta, beta, u0, geo, gI, np.complex128, every)
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), every
+
+
+def _geometry(geo, n, every=0):
+    oy, ox = O.detector_shape(geo["size_x"], geo["size_y"], geo["ds_x"], geo["ds_y"])
+    return AG.MultisliceGeometry(dx=geo["dx"], dy=geo["dy"], wl=geo["wl"], pz=geo["pz"],
+                                 cutoff=geo["cutoff_freq"], z_leg=geo["z_leg"], out_x=ox, out_y=oy,
+                                 ds_x=geo["ds_x"], ds_y=geo["ds_y"], z_det=geo["z_det"],
+                                 ckpt_every=every)
+
+
+@pytest.mark.parametrize("dt,tol", [(np.complex128, 1e-9), (np.complex64, 1e-4)])
+def test_autograd_function_matches_cpu_complex128_autograd(dt, tol):
+    delta, beta, u0, geo = case(n=64, nz=6, seed=16)
+    target = _target(delta, beta, u0, geo)
+    td, tb, tI = torch_gradient(delta, beta, u0, geo, target)
+    say("autograd.Function", dt.__name__)
+    d = vol(delta).requires_grad_(True)
+    b = vol(beta).requires_grad_(True)
+    I = AG.multislice_intensity(d, b, _geometry(geo, 64), u0=field(u0, dt), dtype=TORCH_C[dt])
+    loss = ((I - torch.from_numpy(target).cuda()) ** 2).sum()
+    loss.backward()
+    assert rel(I.detach().cpu().numpy(), tI) < tol
+    assert rel(d.grad.cpu().numpy(), td) < tol
+    assert rel(b.grad.cpu().numpy(), tb) < tol
+
+
+def test_gradient_matches_central_differences_on_gpu():
+    """dL/d(delta, beta) of the complex128 HIP adjoint vs central differences of
+    the complex128 HIP forward at 8 random voxels."""
+    delta, beta, u0, geo = case(n=64, nz=5, seed=17)
+    target = torch.from_numpy(_target(delta, beta, u0, geo)).cuda()
+    G = _geometry(geo, 64, every=2)
+    u = field(u0, np.complex128)
+    d = vol(delta).requires_grad_(True)
+    b = vol(beta).requires_grad_(True)
+    say("gpu finite differences (autograd on torch's default stream)")
+    with torch.cuda.stream(torch.cuda.default_stream()):
+        loss = ((AG.multislice_intensity(d, b, G, u0=u, dtype=torch.complex128) - target) ** 2).sum()
+        loss.backward()
+    torch.cuda.synchronize()   # the gradients were produced on the default stream
+    gd, gb = d.grad.cpu().numpy(), b.grad.cpu().numpy()
+    rng = np.random.default_rng(18)
+
+    def L(dd, bb):
+        with torch.no_grad():
+            I = AG.multislice_intensity(dd, bb, G, u0=u, dtype=torch.complex128)
+            return float(((I - target) ** 2).sum())
+
+    for _ in range(8):
+        s, y, x = (int(rng.integers(0, k)) for k in delta.shape)
+        for which, grad, h in ((0, gd, 2e-8), (1, gb, 2e-11)):
+            dp, bp = d.detach().clone(), b.detach().clone()
+            dm, bm = d.detach().clone(), b.detach().clone()
+            tgt_p, tgt_m = (dp, dm) if which == 0 else (bp, bm)
+            # float32 volumes: use the exactly representable perturbed values
+            v0 = float(tgt_p[s, y, x])
+            vp = float(np.float32(v0 + h))
+            vm = float(np.float32(v0 - h))
+            tgt_p[s, y, x] = vp
+            tgt_m[s, y, x] = vm
+            fd = (L(dp, bp) - L(dm, bm)) / (vp - vm)
+            scale = np.abs(grad).max()
+            assert abs(fd - grad[s, y, x]) < 2e-4 * scale, (which, s, y, x, fd, grad[s, y, x])
