@@ -110,6 +110,14 @@ def load() -> C.CDLL:
             raise ImportError(
                 f"libravesim_hip.so not found at {LIB_PATH}; build it with "
                 "`python 3d_ravesim_amd/build.py` (the engine has no CPU fallback)")
+        # PyTorch-ROCm wheels carry their own HIP/HSA runtime.  Loading torch
+        # first lets the library bind to that runtime, so one runtime serves
+        # the process and torch's streams and allocations are valid here; the
+        # reverse order leaves torch unable to find the device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(LIB_PATH)
         for name, args in SIGNATURES.items():
             fn = getattr(lib, name)

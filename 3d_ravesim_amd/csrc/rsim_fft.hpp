@@ -398,11 +398,13 @@ __device__ __forceinline__ uint32_t last_off(int e, uint32_t es_b) {
 // Plain single-direction transform along one axis, C sequences per WG.
 // Batch b starts at in + b*bs (elements); sequence q of it at q*ss_b bytes,
 // its element p at p*es_b bytes; nbytes bounds one batch.
-template <typename R, class F, int DIR, int C, int PS, bool SPLIT>
+// TW: twiddle / butterfly arithmetic (R, or twc: FP32 data with hi+lo
+// compensated constants and twiddles).
+template <typename R, class F, int DIR, int C, int PS, bool SPLIT, typename TW = R>
 __global__ void __launch_bounds__(F::TC * C) fft_reg_kernel(const cpx<R>* in, cpx<R>* out,
                                                             uint32_t es_b, uint32_t ss_b,
                                                             long long bs, uint32_t nbytes,
-                                                            int nseq, StageTw<R> tw, R scale) {
+                                                            int nseq, StageTw<TW> tw, R scale) {
   __shared__ unsigned char lds[LdsSize<F, C, PS>::slots * (SPLIT ? sizeof(R) : sizeof(cpx<R>))];
   const int tid = threadIdx.x;
   const int c = tid % C;

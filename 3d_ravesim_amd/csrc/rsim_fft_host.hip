@@ -197,8 +197,11 @@ int fft_axis(int dtype, int dir, const void* in, void* out, int N, long long es,
   const bool col = es != 1;
   const int kind = col ? KT_COL : KT_ROW;
   if (t->reg) {
+    // complex64 register transforms take the compensated (hi + lo) tables
+    Tables* tr = t;
+    if (dtype == RSIM_C64) RSIM_CHECK(get_tables(kTwComp, N, &tr));
     ktime_begin(kind, s);
-    const void* tw[4] = {t->stage[0], t->stage[1], t->stage[2], t->stage[3]};
+    const void* tw[4] = {tr->stage[0], tr->stage[1], tr->stage[2], tr->stage[3]};
     bool ok = dtype == RSIM_C64
                   ? launch_reg_fft_c64(dir, N, col, in, out, es, ss, bs, nseq, batch, tw, scale, s)
                   : launch_reg_fft_c128(dir, N, col, in, out, es, ss, bs, nseq, batch, tw, scale, s);

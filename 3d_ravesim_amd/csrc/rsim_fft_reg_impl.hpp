@@ -9,18 +9,26 @@
 
 namespace rsim {
 
+// complex64 transforms use compensated FP32 butterflies (twc tables): plain
+// FP32 roundings of the radix-16 constants and twiddles bias |W| the same way
+// in every transform, which accumulates over a multislice run.
+template <typename R>
+using RegTw = typename std::conditional<std::is_same<R, float>::value, twc, R>::type;
+
 template <typename R, int N, int DIR, bool COL>
 static void launch_one(const void* in, void* out, long long es, long long ss, long long bs,
                        int nseq, int batch, const void* const* tw, double scale, hipStream_t s) {
   using F = typename Sched<N>::F;
-  StageTw<R> t;
-  for (int i = 0; i < 4; ++i) t.t[i] = reinterpret_cast<const cpx<R>*>(tw[i]);
+  using TW = RegTw<R>;
+  StageTw<TW> t;
+  using TP = typename std::remove_reference<decltype(t.t[0])>::type;
+  for (int i = 0; i < 4; ++i) t.t[i] = reinterpret_cast<TP>(tw[i]);
   const long long esz = sizeof(cpx<R>);
   const uint32_t es_b = (uint32_t)(es * esz), ss_b = (uint32_t)(ss * esz);
   const uint32_t nbytes = (uint32_t)(((long long)(nseq - 1) * ss + (long long)(N - 1) * es + 1) * esz);
   using S = typename std::conditional<COL, ColShape<R, N>, RowShape<R, N>>::type;
   dim3 grid((nseq + S::C - 1) / S::C, batch);
-  hipLaunchKernelGGL((fft_reg_kernel<R, F, DIR, S::C, S::PS, S::SPLIT>), grid, dim3(F::TC * S::C), 0,
+  hipLaunchKernelGGL((fft_reg_kernel<R, F, DIR, S::C, S::PS, S::SPLIT, TW>), grid, dim3(F::TC * S::C), 0,
                      s, reinterpret_cast<const cpx<R>*>(in), reinterpret_cast<cpx<R>*>(out), es_b,
                      ss_b, bs, nbytes, nseq, t, (R)scale);
 }

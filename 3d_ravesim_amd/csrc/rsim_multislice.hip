@@ -334,9 +334,10 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
   if (!reg_stage_tables(tdtype, nx, twx) || !reg_stage_tables(tdtype, ny, twy))
     return fail(RSIM_E_HIP, "missing FFT schedule tables");
   StageTw<TW> tx, ty;
+  using TP = typename std::remove_reference<decltype(tx.t[0])>::type;
   for (int i = 0; i < 4; ++i) {
-    tx.t[i] = (decltype(tx.t[i]))twx[i];
-    ty.t[i] = (decltype(ty.t[i]))twy[i];
+    tx.t[i] = reinterpret_cast<TP>(twx[i]);
+    ty.t[i] = reinterpret_cast<TP>(twy[i]);
   }
   const ChirpTables* ct = nullptr;
   RSIM_CHECK(chirp_tables(nx, ny, dx, dy, wl, pz, freq, &ct));

@@ -98,6 +98,10 @@ def main():
     ap.add_argument("--cpu-slices", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--workload", choices=["multislice", "adjoint"], default="multislice",
+                    help="multislice: the headline 4096^2 fused run; adjoint: forward + "
+                         "backward of the differentiable path at 2048^2 x 128")
+    ap.add_argument("--fd-voxels", type=int, default=64)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -112,6 +116,8 @@ def main():
     lib.load()
     if lib.device_count() < 1:
         raise RuntimeError("no HIP device visible")
+    if args.workload == "adjoint":
+        return adjoint_main(args, rs)
     if args.calibrate_copy:
         nbytes = args.calibrate_copy << 20
         a, b = C.c_void_p(), C.c_void_p()
@@ -315,6 +321,121 @@ def main():
 
     lib.call("rsim_stream_destroy", s)
     print(json.dumps(result), flush=True)
+
+
+def smooth_volume(torch, shape, sigma, seed, device):
+    """Gaussian-filtered (periodic, sigma voxels on every axis) seeded normal
+    noise, min-max scaled to [0, 1]; float32 on the device."""
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    x = torch.randn(shape, generator=g, dtype=torch.float32).to(device)
+    X = torch.fft.rfftn(x)
+    f = [torch.fft.fftfreq(n, device=device) for n in shape[:-1]] + [torch.fft.rfftfreq(shape[-1], device=device)]
+    k2 = f[0][:, None, None] ** 2 + f[1][None, :, None] ** 2 + f[2][None, None, :] ** 2
+    x = torch.fft.irfftn(X * torch.exp(-2 * math.pi ** 2 * sigma ** 2 * k2), s=shape)
+    del X, k2
+    lo, hi = x.min(), x.max()
+    return ((x - lo) / (hi - lo)).contiguous()
+
+
+def adjoint_main(args, rs):
+    """Differentiable multislice at 2048^2 x 128 (float32 delta, beta).
+
+    delta in [0, 1e-6], beta in [0, 1e-9] from Gaussian-filtered (sigma = 4
+    voxels) seeded normal noise (seed 4); pz = 0.5 um, lambda = 0.5 A, 0.2 m leg,
+    area_v1 4x4.  Target: the image of a volume perturbed by +5 % of a second
+    smooth field (seed 5).  Loss ||I - I_t||^2.  Times forward and forward +
+    backward (complex64, sqrt(128) checkpointing), checks the gradient against
+    FP64 (complex128) central differences at --fd-voxels random voxels (seed 6)
+    and against the complex128 adjoint.
+    """
+    import torch
+    AG = importlib.import_module("3d_ravesim_amd.autograd")
+    n, nz = 2048, 128
+    dx, wl, pz, z_leg = 0.5e-6, 0.5e-10, 0.5e-6, 0.2
+    dev = torch.device("cuda:0")
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        delta = smooth_volume(torch, (nz, n, n), 4.0, 4, dev) * 1e-6
+        beta = (delta * 1e-3).contiguous()
+        pert = smooth_volume(torch, (nz, n, n), 4.0, 5, dev)
+        G = AG.MultisliceGeometry(dx=dx, wl=wl, pz=pz, cutoff=0.4 / dx, z_leg=z_leg,
+                                  out_x=n // 4, out_y=n // 4, ds_x=4 * dx, ds_y=4 * dx,
+                                  z_det=nz * pz + z_leg)
+        with torch.no_grad():
+            target = AG.multislice_intensity((delta * (1 + 0.05 * pert)).contiguous(),
+                                             (beta * (1 + 0.05 * pert)).contiguous(), G,
+                                             dtype=torch.complex128)
+        del pert
+
+        def fwd_bwd(dt):
+            d = delta.clone().requires_grad_(True)
+            b = beta.clone().requires_grad_(True)
+            loss = ((AG.multislice_intensity(d, b, G, dtype=dt) - target) ** 2).sum()
+            loss.backward()
+            return d.grad, b.grad, float(loss.detach())
+
+        fwd_bwd(torch.complex64)
+        torch.cuda.synchronize()
+        reps = max(1, args.steps // 5)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            with torch.no_grad():
+                AG.multislice_intensity(delta, beta, G, dtype=torch.complex64)
+        torch.cuda.synchronize()
+        t_fwd = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            gd, gb, loss = fwd_bwd(torch.complex64)
+        torch.cuda.synchronize()
+        t_fb = (time.perf_counter() - t0) / reps
+        log(f"adjoint 2048^2 x 128 c64: forward {t_fwd * 1e3:.1f} ms, forward+backward {t_fb * 1e3:.1f} ms")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gd64, gb64, _ = fwd_bwd(torch.complex128)
+        torch.cuda.synchronize()
+        t_fb128 = time.perf_counter() - t0
+        log(f"complex128 forward+backward {t_fb128 * 1e3:.1f} ms")
+        rel = lambda a, b: float(torch.linalg.vector_norm((a - b).double()) / torch.linalg.vector_norm(b.double()))  # noqa: E731
+        e_d, e_b = rel(gd, gd64), rel(gb, gb64)
+        log(f"c64 vs c128 adjoint gradient: delta {e_d:.3e}, beta {e_b:.3e}")
+
+        def L(dd, bb):
+            with torch.no_grad():
+                return float(((AG.multislice_intensity(dd, bb, G, dtype=torch.complex128) - target) ** 2).sum())
+
+        rng = np.random.default_rng(6)
+        worst = {"delta": 0.0, "beta": 0.0}
+        for _ in range(args.fd_voxels):
+            s_, y, x = (int(rng.integers(0, k)) for k in (nz, n, n))
+            for name, vol_, grad, h in (("delta", delta, gd64, 5e-8), ("beta", beta, gb64, 5e-11)):
+                v0 = float(vol_[s_, y, x])
+                vp, vm = float(np.float32(v0 + h)), float(np.float32(v0 - h))
+                vol_[s_, y, x] = vp
+                lp = L(delta, beta)
+                vol_[s_, y, x] = vm
+                lm = L(delta, beta)
+                vol_[s_, y, x] = v0
+                fd = (lp - lm) / (vp - vm)
+                scale = float(grad.abs().max())
+                worst[name] = max(worst[name], abs(fd - float(grad[s_, y, x])) / scale)
+        log(f"FD check ({args.fd_voxels} voxels, FP64): max |fd - grad| / max|grad|: "
+            f"delta {worst['delta']:.3e}, beta {worst['beta']:.3e}")
+    ckb, wkb = C.c_size_t(), C.c_size_t()
+    every = math.ceil(math.sqrt(nz))
+    rs._lib.call("rsim_multislice_adjoint_workspace", n, n, rs._lib.RSIM_C64, nz, every,
+                 C.byref(ckb), C.byref(wkb))
+    print(json.dumps({
+        "metric": "differentiable multislice forward+backward ms (2048^2 x 128, c64)",
+        "value": round(t_fb * 1e3, 2), "unit": "ms", "n_gpus": 1, "steps": reps,
+        "higher_is_better": False, "dtype": "c64", "data": "synthetic",
+        "forward_ms": round(t_fwd * 1e3, 2), "c128_forward_backward_ms": round(t_fb128 * 1e3, 2),
+        "slice_steps_per_s_forward_backward": round(nz / t_fb, 1),
+        "checkpoint_every": every, "checkpoint_MiB": ckb.value >> 20, "work_MiB": wkb.value >> 20,
+        "rel_l2_grad_delta_c64_vs_c128": e_d, "rel_l2_grad_beta_c64_vs_c128": e_b,
+        "fd_voxels": args.fd_voxels, "fd_max_err_over_max_grad": worst, "loss": loss,
+        "config": {"workload": "2048x2048x128 float32 delta/beta (smooth seeded noise, seed 4), "
+                               "lambda 0.5A, pz 0.5um, 0.2 m leg, area_v1 4x4, target +5% (seed 5)"},
+    }), flush=True)
 
 
 if __name__ == "__main__":
