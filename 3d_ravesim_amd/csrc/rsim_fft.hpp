@@ -38,6 +38,12 @@ struct AxisDesc {
   int C;                     // sequences per workgroup tile
   double scale;              // output multiplier (1 or 1/N)
   const void* tw;            // W_N^m, m in [0, N), forward sign, in the transform's precision
+  // Optional post-twiddle (a four-step transform's middle factor): output
+  // element pos of a sequence is multiplied by W_P^(i*pos) (conjugated for the
+  // inverse), i = the sequence index (ptw_mode 1) or the batch index (2).
+  int ptw_mode;
+  long long ptw_P;
+  const void* ptw;           // W_P^m, m in [0, P)
 };
 
 template <int RR, int DIR, typename R>
@@ -120,6 +126,10 @@ __global__ void __launch_bounds__(256) fft_lds_kernel(const cpx<R>* __restrict__
     const int seq = seq0 + c;
     if (seq < d.nseq) {
       cpx<R> v = src[pos * C + c];
+      if (d.ptw_mode) {
+        const long long i = d.ptw_mode == 1 ? seq : (long long)blockIdx.y;
+        v = twmul<DIR>(v, reinterpret_cast<const cpx<R>*>(d.ptw)[i * pos]);
+      }
       if (d.scale != 1.0) v = cscale(v, sc);
       out[boff_out + (long long)seq * d.ss_out + (long long)pos * d.es_out] = v;
     }

@@ -27,6 +27,8 @@ struct Tables {
 
 std::mutex g_mu;
 std::map<std::tuple<int, int, int>, Tables> g_tables;  // (device, dtype, N)
+// scratch field of the out-of-LDS schedule (per device, grown on demand)
+std::map<int, std::pair<void*, size_t>> g_scratch;
 
 // exp(-2*pi*i*m/M) with the angle reduced to the first octant so the FP64
 // value is correct to a few ulp for every m.
@@ -179,6 +181,156 @@ int rsim_release_tables() {
       ++it;
     }
   }
+  auto sc = g_scratch.find(dev);
+  if (sc != g_scratch.end()) {
+    hipFree(sc->second.first);
+    g_scratch.erase(sc);
+  }
+  return RSIM_OK;
+}
+
+namespace {
+
+// Stage list of the generic kernel for N.
+int axis_desc(int N, const Tables* t, AxisDesc* d) {
+  std::vector<int> r;
+  if (!factorize(N, &r)) return fail(RSIM_E_INVALID_ARG, "FFT length %d has a prime factor > 5", N);
+  if ((int)r.size() > kMaxStages) return fail(RSIM_E_UNSUPPORTED, "FFT length %d needs too many stages", N);
+  d->N = N;
+  d->nst = (r.size() == 1 && r[0] == 1) ? 0 : (int)r.size();
+  int ls = 1;
+  for (int i = 0; i < d->nst; ++i) {
+    d->radix[i] = r[i];
+    d->ls[i] = ls;
+    ls *= r[i];
+  }
+  d->tw = t->full;
+  return RSIM_OK;
+}
+
+int launch_lds(int dtype, int dir, const void* in, void* out, const AxisDesc& d, int batch,
+               hipStream_t s) {
+  const size_t shmem = (size_t)2 * d.C * d.N * elem_bytes(dtype);
+  if (shmem > max_lds_per_block())
+    return fail(RSIM_E_UNSUPPORTED, "FFT length %d exceeds the single-pass LDS capacity", d.N);
+  if (batch > 65535) return fail(RSIM_E_UNSUPPORTED, "FFT batch %d exceeds the grid limit", batch);
+  dim3 grid((d.nseq + d.C - 1) / d.C, batch);
+  dim3 block(256);
+  if (dtype == RSIM_C64) {
+    if (dir < 0) {
+      hipFuncSetAttribute((const void*)fft_lds_kernel<float, -1>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+      hipLaunchKernelGGL((fft_lds_kernel<float, -1>), grid, block, shmem, s,
+                         (const cpx<float>*)in, (cpx<float>*)out, d);
+    } else {
+      hipFuncSetAttribute((const void*)fft_lds_kernel<float, 1>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+      hipLaunchKernelGGL((fft_lds_kernel<float, 1>), grid, block, shmem, s,
+                         (const cpx<float>*)in, (cpx<float>*)out, d);
+    }
+  } else {
+    if (dir < 0) {
+      hipFuncSetAttribute((const void*)fft_lds_kernel<double, -1>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+      hipLaunchKernelGGL((fft_lds_kernel<double, -1>), grid, block, shmem, s,
+                         (const cpx<double>*)in, (cpx<double>*)out, d);
+    } else {
+      hipFuncSetAttribute((const void*)fft_lds_kernel<double, 1>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+      hipLaunchKernelGGL((fft_lds_kernel<double, 1>), grid, block, shmem, s,
+                         (const cpx<double>*)in, (cpx<double>*)out, d);
+    }
+  }
+  RSIM_LAUNCHED();
+  return RSIM_OK;
+}
+
+int axis_scratch(size_t bytes, void** out, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto& e = g_scratch[current_device()];
+  if (e.second < bytes) {
+    if (e.first) {
+      RSIM_HIP(hipStreamSynchronize(s));
+      RSIM_HIP(hipFree(e.first));
+      e.first = nullptr;
+      e.second = 0;
+    }
+    RSIM_HIP(hipMalloc(&e.first, bytes));
+    e.second = bytes;
+  }
+  *out = e.first;
+  return RSIM_OK;
+}
+
+// Largest divisor of n that is <= sqrt(n) and leaves both factors
+// factorizable into radices 2..16, 3, 5 (0 if none).
+int split_length(int n) {
+  std::vector<int> r;
+  for (int a = (int)std::sqrt((double)n); a >= 2; --a)
+    if (n % a == 0 && factorize(a, &r) && factorize(n / a, &r)) return a;
+  return 0;
+}
+
+}  // namespace
+
+// Out-of-LDS transform of length N = N1*N2 (four-step): x[N2*n1 + n2]
+//   step 1: Y[k1, n2] = W_N^(n2 k1) * sum_n1 x[N2 n1 + n2] W_N1^(n1 k1)
+//   step 2: X[k1 + N1 k2] = sum_n2 Y[k1, n2] W_N2^(n2 k2)
+// through a scratch copy of the axis data (so in == out is allowed).  Rows
+// (es == 1) keep the sequence index on the contiguous n2; columns (ss == 1)
+// keep it on the contiguous column index and put n2 / k1 on the grid's
+// batch dimension.  One batch of sequences only.
+int big_axis(int dtype, int dir, const void* in, void* out, int N, long long es, long long ss,
+             int nseq, int batch, long long bs, double scale, hipStream_t s) {
+  if (batch != 1) return fail(RSIM_E_UNSUPPORTED, "out-of-LDS FFT of length %d needs batch 1", N);
+  const bool rows = es == 1;
+  if (!rows && ss != 1) return fail(RSIM_E_UNSUPPORTED, "out-of-LDS FFT layout (es=%lld, ss=%lld)", es, ss);
+  const int N2 = split_length(N);
+  if (N2 == 0) return fail(RSIM_E_UNSUPPORTED, "FFT length %d has no out-of-LDS split", N);
+  const int N1 = N / N2;
+  Tables *t1 = nullptr, *t2 = nullptr, *tN = nullptr;
+  RSIM_CHECK(get_tables(dtype, N1, &t1));
+  RSIM_CHECK(get_tables(dtype, N2, &t2));
+  RSIM_CHECK(get_tables(dtype, N, &tN));
+  const size_t eb = elem_bytes(dtype);
+  void* scr = nullptr;
+  RSIM_CHECK(axis_scratch((size_t)nseq * N * eb, &scr, s));
+  const int kind = rows ? KT_ROW : KT_COL;
+  AxisDesc a{}, b{};
+  RSIM_CHECK(axis_desc(N1, t1, &a));
+  RSIM_CHECK(axis_desc(N2, t2, &b));
+  a.scale = 1.0;
+  b.scale = scale;
+  a.ptw_P = N;
+  a.ptw = tN->full;
+  a.C = b.C = 16;
+  while (a.C > 1 && (size_t)2 * a.C * N1 * eb > max_lds_per_block()) a.C /= 2;
+  while (b.C > 1 && (size_t)2 * b.C * N2 * eb > max_lds_per_block()) b.C /= 2;
+  int ga, gb;
+  if (rows) {
+    // step 1: sequences n2 (stride 1), elements n1 (stride N2), batch q (row)
+    a.es_in = N2; a.ss_in = 1; a.bs_in = ss; a.nseq = N2;
+    a.es_out = N2; a.ss_out = 1; a.bs_out = N;       // scratch[q][N2 k1 + n2]
+    a.ptw_mode = 1;
+    // step 2: sequences k1 (stride N2), elements n2 (stride 1), batch q
+    b.es_in = 1; b.ss_in = N2; b.bs_in = N; b.nseq = N1;
+    b.es_out = N1; b.ss_out = 1; b.bs_out = ss;      // out[q][k1 + N1 k2]
+    ga = gb = nseq;
+  } else {
+    // step 1: sequences q (stride 1), elements n1 (stride es*N2), batch n2
+    a.es_in = es * N2; a.ss_in = 1; a.bs_in = es; a.nseq = nseq;
+    a.es_out = (long long)N2 * nseq; a.ss_out = 1; a.bs_out = nseq;  // scratch[N2 k1 + n2][q]
+    a.ptw_mode = 2;
+    // step 2: sequences q, elements n2 (stride nseq), batch k1
+    b.es_in = nseq; b.ss_in = 1; b.bs_in = (long long)N2 * nseq; b.nseq = nseq;
+    b.es_out = es * N1; b.ss_out = 1; b.bs_out = es;  // out[(k1 + N1 k2) es + q]
+    ga = N2;
+    gb = N1;
+  }
+  ktime_begin(kind, s);
+  RSIM_CHECK(launch_lds(dtype, dir, in, scr, a, ga, s));
+  RSIM_CHECK(launch_lds(dtype, dir, scr, out, b, gb, s));
+  ktime_end(kind, s);
   return RSIM_OK;
 }
 
@@ -211,69 +363,29 @@ int fft_axis(int dtype, int dir, const void* in, void* out, int N, long long es,
       return RSIM_OK;
     }
   }
-  // Generic LDS kernel.
-  std::vector<int> r;
-  if (!factorize(N, &r)) return fail(RSIM_E_INVALID_ARG, "FFT length %d has a prime factor > 5", N);
-  if ((int)r.size() > kMaxStages) return fail(RSIM_E_UNSUPPORTED, "FFT length %d needs too many stages", N);
+  if ((size_t)2 * N * elem_bytes(dtype) > max_lds_per_block())
+    return big_axis(dtype, dir, in, out, N, es, ss, nseq, batch, bs, scale, s);
   AxisDesc d{};
-  d.N = N;
-  d.nst = (r.size() == 1 && r[0] == 1) ? 0 : (int)r.size();
-  int ls = 1;
-  for (int i = 0; i < d.nst; ++i) {
-    d.radix[i] = r[i];
-    d.ls[i] = ls;
-    ls *= r[i];
-  }
+  RSIM_CHECK(axis_desc(N, t, &d));
   d.es_in = d.es_out = es;
   d.ss_in = d.ss_out = ss;
   d.bs_in = d.bs_out = bs;
   d.nseq = nseq;
   d.scale = scale;
-  d.tw = t->full;
   const size_t eb = elem_bytes(dtype);
   const size_t lds_cap = max_lds_per_block();
   // tile: enough sequences for >= 64 B contiguous runs on strided axes,
   // bounded by LDS (two ping-pong buffers)
   int C = col ? 8 : 1;
   while (C > 1 && (size_t)2 * C * N * eb > lds_cap) C /= 2;
-  if ((size_t)2 * N * eb > lds_cap)
-    return fail(RSIM_E_UNSUPPORTED, "FFT length %d exceeds the single-pass LDS capacity", N);
   if (!col) {
     // several short rows per workgroup
     while (C * N < 1024 && C < 64 && (size_t)4 * C * N * eb <= lds_cap) C *= 2;
   }
   d.C = C;
-  const size_t shmem = (size_t)2 * C * N * eb;
-  dim3 grid((nseq + C - 1) / C, batch);
-  dim3 block(256);
   ktime_begin(kind, s);
-  if (dtype == RSIM_C64) {
-    if (dir < 0) {
-      hipFuncSetAttribute((const void*)fft_lds_kernel<float, -1>,
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-      hipLaunchKernelGGL((fft_lds_kernel<float, -1>), grid, block, shmem, s,
-                         (const cpx<float>*)in, (cpx<float>*)out, d);
-    } else {
-      hipFuncSetAttribute((const void*)fft_lds_kernel<float, 1>,
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-      hipLaunchKernelGGL((fft_lds_kernel<float, 1>), grid, block, shmem, s,
-                         (const cpx<float>*)in, (cpx<float>*)out, d);
-    }
-  } else {
-    if (dir < 0) {
-      hipFuncSetAttribute((const void*)fft_lds_kernel<double, -1>,
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-      hipLaunchKernelGGL((fft_lds_kernel<double, -1>), grid, block, shmem, s,
-                         (const cpx<double>*)in, (cpx<double>*)out, d);
-    } else {
-      hipFuncSetAttribute((const void*)fft_lds_kernel<double, 1>,
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-      hipLaunchKernelGGL((fft_lds_kernel<double, 1>), grid, block, shmem, s,
-                         (const cpx<double>*)in, (cpx<double>*)out, d);
-    }
-  }
+  RSIM_CHECK(launch_lds(dtype, dir, in, out, d, batch, s));
   ktime_end(kind, s);
-  RSIM_LAUNCHED();
   return RSIM_OK;
 }
 

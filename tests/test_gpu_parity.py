@@ -77,7 +77,9 @@ def test_fft_known_answers_from_fwtest():
 
 FFT_SIZES = [(1, 2), (2, 3), (5, 6), (8, 8), (16, 12), (30, 20), (64, 32), (72, 72), (128, 256),
              (256, 256), (512, 128), (1024, 1024), (2048, 512), (4096, 256), (256, 4096),
-             (8192, 64), (64, 8192), (3072, 96), (96, 3072), (1000, 10)]
+             (8192, 64), (64, 8192), (3072, 96), (96, 3072), (1000, 10),
+             # beyond one LDS pass (c128 > 5120, c64 > 10240): four-step schedule
+             (8, 6144), (6144, 8), (6, 16384), (16384, 6), (5, 12000), (24, 10240)]
 
 
 @pytest.mark.parametrize("ny,nx", FFT_SIZES)
@@ -93,13 +95,14 @@ def test_fft2_matches_oracle(ny, nx):
         assert rel(gpu_fft2(xf, 1), ref_i) < tol
 
 
-def test_fft_in_place():
+@pytest.mark.parametrize("ny,nx", [(256, 512), (16, 6144), (6144, 16)])
+def test_fft_in_place(ny, nx):
     rng = np.random.default_rng(1)
-    x = rng.normal(size=(256, 512)) + 1j * rng.normal(size=(256, 512))
+    x = rng.normal(size=(ny, nx)) + 1j * rng.normal(size=(ny, nx))
     a = dev(x)
-    say("fft2 in place")
-    a.fft2(a, 512, 256)
-    assert rel(host(a, 256, 512), O.fft2(x)) < 1e-13
+    say("fft2 in place", ny, nx)
+    a.fft2(a, nx, ny)
+    assert rel(host(a, ny, nx), O.fft2(x)) < 1e-13
 
 
 # ------------------------------------------------- cutoff / propagation ----
