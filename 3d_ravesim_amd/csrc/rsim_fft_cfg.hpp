@@ -19,6 +19,10 @@ template <> struct Sched<1024> { using F = RegFac<1024, 16, 16, 4, 16>; };
 template <> struct Sched<2048> { using F = RegFac<2048, 16, 16, 8, 16>; };
 template <> struct Sched<4096> { using F = RegFac<4096, 16, 16, 16, 16>; };
 template <> struct Sched<8192> { using F = RegFac<8192, 16, 16, 2, 16, 16>; };
+// row pass of 16384-point axes (1024 threads per row)
+template <> struct Sched<16384> { using F = RegFac<16384, 16, 16, 4, 16, 16>; };
+// sub-transform of the three-pass 16384-point column schedule (128 x 128)
+template <> struct Sched<128> { using F = RegFac<128, 8, 8, 2, 8>; };
 
 // Launch shape along the contiguous axis (rows) and the strided axis
 // (columns).  Columns put C neighbouring columns in one workgroup so every
@@ -71,6 +75,8 @@ inline bool sched_info(int N, SchedInfo* si) {
     RSIM_SI(2048)
     RSIM_SI(4096)
     RSIM_SI(8192)
+    RSIM_SI(16384)
+    RSIM_SI(128)
     default: return false;
   }
 #undef RSIM_SI

@@ -161,6 +161,12 @@ bool fft_size_supported(int n) {
   return factorize(n, &r);
 }
 
+const void* fft_full_table(int dtype, int N) {
+  Tables* t = nullptr;
+  if (get_tables(dtype, N, &t) != RSIM_OK) return nullptr;
+  return t->full;
+}
+
 bool reg_stage_tables(int dtype, int N, const void* out[4]) {
   Tables* t = nullptr;
   if (get_tables(dtype, N, &t) != RSIM_OK || !t->reg) return false;

@@ -284,23 +284,13 @@ bool col_go(int v, bool spec_out, const void* in, void* out, void* spec, int nx,
     ColLaunch<Shape<N, E, C, SP>, TW>::go(spec_out, in, out, spec, nx, nb, tw, Bx, Ay, Rmax, ss, s);  \
     return true;
 
-#define RSIM_FUSED_INSTANTIATE(N)                                                                 \
+#define RSIM_FUSED_INSTANTIATE_ROWS(N)                                                            \
   template <int NN, typename TW>                                                                  \
   bool row_go_impl(int v, int mode, bool lut, const void* in, void* out, int ny,                  \
                    const StageTw<TW>& tw, const TransArgsF& ta, hipStream_t s) {                  \
     const uint32_t nb = (uint32_t)((long long)NN * ny * 8);                                      \
     switch (v) {                                                                                  \
       RSIM_ROW_VARIANTS(NN, RSIM_ROW_CASE)                                                        \
-      default: return false;                                                                      \
-    }                                                                                             \
-  }                                                                                               \
-  template <int NN, typename TW>                                                                  \
-  bool col_go_impl(int v, bool spec_out, const void* in, void* out, void* spec, int nx,           \
-                   const StageTw<TW>& tw, const double2* Bx, const double2* Ay, const int* Rmax,  \
-                   float ss, hipStream_t s) {                                                     \
-    const uint32_t nb = (uint32_t)((long long)NN * nx * 8);                                      \
-    switch (v) {                                                                                  \
-      RSIM_COL_VARIANTS(NN, RSIM_COL_CASE)                                                        \
       default: return false;                                                                      \
     }                                                                                             \
   }                                                                                               \
@@ -315,6 +305,23 @@ bool col_go(int v, bool spec_out, const void* in, void* out, void* spec, int nx,
     return row_go_impl<N, double>(v, mode, lut, in, out, ny, tw, ta, s);                          \
   }                                                                                               \
   template <>                                                                                     \
+  bool row_go<N, twc>(int v, int mode, bool lut, const void* in, void* out, int ny,               \
+                      const StageTw<twc>& tw, const TransArgsF& ta, hipStream_t s) {              \
+    return row_go_impl<N, twc>(v, mode, lut, in, out, ny, tw, ta, s);                             \
+  }
+
+#define RSIM_FUSED_INSTANTIATE_COLS(N)                                                            \
+  template <int NN, typename TW>                                                                  \
+  bool col_go_impl(int v, bool spec_out, const void* in, void* out, void* spec, int nx,           \
+                   const StageTw<TW>& tw, const double2* Bx, const double2* Ay, const int* Rmax,  \
+                   float ss, hipStream_t s) {                                                     \
+    const uint32_t nb = (uint32_t)((long long)NN * nx * 8);                                      \
+    switch (v) {                                                                                  \
+      RSIM_COL_VARIANTS(NN, RSIM_COL_CASE)                                                        \
+      default: return false;                                                                      \
+    }                                                                                             \
+  }                                                                                               \
+  template <>                                                                                     \
   bool col_go<N, float>(int v, bool spec_out, const void* in, void* out, void* spec, int nx,      \
                         const StageTw<float>& tw, const double2* Bx, const double2* Ay,           \
                         const int* Rmax, float ss, hipStream_t s) {                               \
@@ -327,15 +334,128 @@ bool col_go(int v, bool spec_out, const void* in, void* out, void* spec, int nx,
     return col_go_impl<N, double>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);       \
   }                                                                                               \
   template <>                                                                                     \
-  bool row_go<N, twc>(int v, int mode, bool lut, const void* in, void* out, int ny,               \
-                      const StageTw<twc>& tw, const TransArgsF& ta, hipStream_t s) {              \
-    return row_go_impl<N, twc>(v, mode, lut, in, out, ny, tw, ta, s);                             \
-  }                                                                                               \
-  template <>                                                                                     \
   bool col_go<N, twc>(int v, bool spec_out, const void* in, void* out, void* spec, int nx,        \
                       const StageTw<twc>& tw, const double2* Bx, const double2* Ay,               \
                       const int* Rmax, float ss, hipStream_t s) {                                 \
     return col_go_impl<N, twc>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);          \
   }
+
+#define RSIM_FUSED_INSTANTIATE(N) RSIM_FUSED_INSTANTIATE_ROWS(N) RSIM_FUSED_INSTANTIATE_COLS(N)
+
+// ---------------------------------------------------------------------------
+// Three-pass column schedule for ny = N1 * N2 beyond one LDS pass
+// (y = N2*n1 + n2), each pass on tiles of C neighbouring columns:
+//   A: FFT over n1 (rows N2 apart) for fixed n2, times W_ny^(n2 k1), in place
+//   B: for fixed k1 (N2 contiguous rows): FFT over n2 -> k2, times the chirp
+//      and cutoff at ky = k1 + N1 k2, IFFT over k2 -> n2, times W_ny^-(n2 k1)
+//   C: IFFT over k1 (rows N2 apart) for fixed n2, in place: natural order
+// The sub-transforms (Sched<128>) start and end with the same radix, so B
+// chains FFT -> chirp -> IFFT in registers.  Every row segment a wavefront
+// touches is C * 8 contiguous bytes.
+// ---------------------------------------------------------------------------
+template <typename TW>
+using BigTwElem = typename std::conditional<std::is_same<TW, twc>::value, twc, cpx<TW>>::type;
+
+template <int DIR>
+__device__ __forceinline__ cpx<float> big_tw(cpx<float> v, const twc* T, int i) {
+  return twmul_comp<DIR>(v, T[i]);
+}
+template <int DIR>
+__device__ __forceinline__ cpx<float> big_tw(cpx<float> v, const cpx<float>* T, int i) {
+  return twmul<DIR>(v, T[i]);
+}
+template <int DIR>
+__device__ __forceinline__ cpx<float> big_tw(cpx<float> v, const cpx<double>* T, int i) {
+  const cpx<double> w = twmul<DIR>(mk((double)v.x, (double)v.y), T[i]);
+  return mk((float)w.x, (float)w.y);
+}
+
+// Passes A (DIR = -1, with the W_ny twiddle) and C (DIR = +1).  grid.y = n2.
+template <class F, int C, int PS, bool SPLIT, int DIR, typename TW>
+__global__ void __launch_bounds__(F::TC * C) ms_colAC_kernel(cpx<float>* buf, int nx, int N2,
+                                                             uint32_t nbytes, StageTw<TW> tw,
+                                                             const BigTwElem<TW>* __restrict__ wN) {
+  __shared__ unsigned char lds[LdsSize<F, C, PS>::slots * (SPLIT ? 4 : 8)];
+  const int tid = threadIdx.x;
+  const int c = tid % C;
+  const int t = tid / C;
+  const int x = xcd_tile(blockIdx.x, gridDim.x) * C + c;
+  const bool live = x < nx;
+  const int xx = live ? x : 0;
+  const int n2 = blockIdx.y;
+  const Gio<float> g(buf, nbytes);
+  const uint32_t row_b = (uint32_t)nx * 8u;
+  const uint32_t es_b = (uint32_t)N2 * row_b;
+  const uint32_t vo = ((uint32_t)N2 * (uint32_t)t + (uint32_t)n2) * row_b + (uint32_t)xx * 8u;
+  cpx<float> v[F::E];
+#pragma unroll
+  for (int e = 0; e < F::E; ++e) v[e] = g.ld(vo, first_off<F>(e, es_b));
+  reg_fft<F, DIR, C, PS, SPLIT>(v, tw, lds, t, c);
+  if constexpr (DIR < 0) {
+#pragma unroll
+    for (int e = 0; e < F::E; ++e) v[e] = big_tw<-1>(v[e], wN, n2 * last_pos<F>(t, e));
+  }
+  if (live) {
+#pragma unroll
+    for (int e = 0; e < F::E; ++e) g.st(v[e], vo, last_off<F>(e, es_b));
+  }
+}
+
+// Pass B.  grid.y = k1; ny = N1 * F::N.
+template <class F, int C, int PS, bool SPLIT, typename TW>
+__global__ void __launch_bounds__(F::TC * C) ms_colB_kernel(cpx<float>* buf, int nx, int N1,
+                                                            uint32_t nbytes, StageTw<TW> tw,
+                                                            const BigTwElem<TW>* __restrict__ wN,
+                                                            const double2* __restrict__ Bx,
+                                                            const double2* __restrict__ Ay,
+                                                            const int* __restrict__ Rmax) {
+  constexpr int N2 = F::N;
+  __shared__ unsigned char lds[LdsSize<F, C, PS>::slots * (SPLIT ? 4 : 8)];
+  const int tid = threadIdx.x;
+  const int c = tid % C;
+  const int t = tid / C;
+  const int x = xcd_tile(blockIdx.x, gridDim.x) * C + c;
+  const bool live = x < nx;
+  const int xx = live ? x : 0;
+  const int k1 = blockIdx.y;
+  const int ny = N1 * N2;
+  const Gio<float> g(buf, nbytes);
+  const uint32_t row_b = (uint32_t)nx * 8u;
+  const uint32_t vo = ((uint32_t)N2 * (uint32_t)k1 + (uint32_t)t) * row_b + (uint32_t)xx * 8u;
+  cpx<float> v[F::E];
+#pragma unroll
+  for (int e = 0; e < F::E; ++e) v[e] = g.ld(vo, first_off<F>(e, row_b));
+  reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
+  compiler_fence();
+  const double2 bx = Bx[xx];
+  const int rmax = Rmax[xx];
+  const int half = (ny - 1) / 2 + 1;
+#pragma unroll
+  for (int e = 0; e < F::E; ++e) {
+    const int y = k1 + N1 * last_pos<F>(t, e);
+    const int sy = y < half ? y : y - ny;
+    const int ay = sy < 0 ? -sy : sy;
+    const double2 a = Ay[y];
+    const double mr = bx.x * a.x - bx.y * a.y;
+    const double mi = bx.x * a.y + bx.y * a.x;
+    const double vr = (double)v[e].x, vi = (double)v[e].y;
+    const cpx<float> w = mk((float)(vr * mr - vi * mi), (float)(vr * mi + vi * mr));
+    v[e] = ay > rmax ? mk(0.f, 0.f) : w;
+    if ((e & 3) == 3) compiler_fence();
+  }
+  reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);
+#pragma unroll
+  for (int e = 0; e < F::E; ++e) v[e] = big_tw<1>(v[e], wN, k1 * last_pos<F>(t, e));
+  if (live) {
+#pragma unroll
+    for (int e = 0; e < F::E; ++e) g.st(v[e], vo, last_off<F>(e, row_b));
+  }
+}
+
+// Launcher of the three passes on buf (in place); false if ny has no such
+// schedule.
+template <typename TW>
+bool big_col_go(void* buf, int nx, int ny, const StageTw<TW>& tw128, const BigTwElem<TW>* wN,
+                const double2* Bx, const double2* Ay, const int* Rmax, hipStream_t s);
 
 }  // namespace rsim

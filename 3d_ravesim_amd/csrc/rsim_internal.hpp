@@ -55,6 +55,8 @@ int fft2(int dtype, int dir, const void* in, void* out, int nx, int ny, hipStrea
 // Per-stage twiddle tables of the register schedule for N (nullptr entries
 // for stages without twiddles); false when N has no register schedule.
 bool reg_stage_tables(int dtype, int N, const void* out[4]);
+// W_N^m, m in [0, N), forward sign, in dtype's table format (nullptr on error).
+const void* fft_full_table(int dtype, int N);
 
 // ---- launchers implemented in the per-precision register FFT units ----
 bool launch_reg_fft_c64(int dir, int N, bool colmode, const void* in, void* out, long long es,

@@ -45,6 +45,7 @@ bool fused_row(int N, int v, int mode, bool lut, const void* in, void* out, int 
     case 2048: return row_go<2048, TW>(v, mode, lut, in, out, ny, tw, ta, s);
     case 4096: return row_go<4096, TW>(v, mode, lut, in, out, ny, tw, ta, s);
     case 8192: return row_go<8192, TW>(v, mode, lut, in, out, ny, tw, ta, s);
+    case 16384: return row_go<16384, TW>(v, mode, lut, in, out, ny, tw, ta, s);
     default: return false;
   }
 }
@@ -304,9 +305,19 @@ int release_multislice_tables() {
   return RSIM_OK;
 }
 
+// Row lengths with a fused row pass; column lengths with a single-pass
+// fused column kernel, or (kBigCol) the three-pass schedule.
+constexpr int kBigCol = 16384;
+static bool fused_row_len(int n) {
+  return n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096 || n == 8192 || n == 16384;
+}
+static bool fused_col_len(int n) {
+  return n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096 || n == 8192 || n == kBigCol;
+}
+
 bool fused_supported(int nx, int ny, int dtype) {
-  SchedInfo a, b;
-  return dtype == RSIM_C64 && sched_info(nx, &a) && sched_info(ny, &b);
+  return dtype == RSIM_C64 && fused_row_len(nx) && fused_col_len(ny) &&
+         (unsigned long long)nx * ny * 8ull <= 0xFFFFFFFFull;
 }
 
 static int propagate_plain(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
@@ -338,6 +349,18 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
   for (int i = 0; i < 4; ++i) {
     tx.t[i] = reinterpret_cast<TP>(twx[i]);
     ty.t[i] = reinterpret_cast<TP>(twy[i]);
+  }
+  // three-pass columns: 128-point sub-transform tables and the W_ny table
+  const bool big = ny == kBigCol;
+  StageTw<TW> t128{};
+  const BigTwElem<TW>* wN = nullptr;
+  if (big) {
+    const void* tw128[4];
+    if (!reg_stage_tables(tdtype, 128, tw128)) return fail(RSIM_E_HIP, "missing FFT schedule tables");
+    using TP = typename std::remove_reference<decltype(t128.t[0])>::type;
+    for (int i = 0; i < 4; ++i) t128.t[i] = reinterpret_cast<TP>(tw128[i]);
+    wN = reinterpret_cast<const BigTwElem<TW>*>(fft_full_table(tdtype, ny));
+    if (!wN) return fail(RSIM_E_HIP, "missing FFT twiddle table for %d", ny);
   }
   const ChirpTables* ct = nullptr;
   RSIM_CHECK(chirp_tables(nx, ny, dx, dy, wl, pz, freq, &ct));
@@ -375,11 +398,17 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
       return fail(RSIM_E_HIP, "no fused row kernel for nx=%d", nx);
     ktime_end(KT_ROW, s);
     RSIM_LAUNCHED();
-    // last slice: W -> u (and the spectrum -> U); otherwise in place in U
     ktime_begin(KT_COL, s);
-    if (!fused_col(ny, cv, last && want_spec, U, last ? u : U, U, nx, ty, ct->Bx, ct->Ay,
-                   ct->Rmax, spec_scale, s))
-      return fail(RSIM_E_HIP, "no fused column kernel for ny=%d", ny);
+    if (big) {
+      // in place in U; the spectrum of the final field is formed after the loop
+      if (!big_col_go<TW>(U, nx, ny, t128, wN, ct->Bx, ct->Ay, ct->Rmax, s))
+        return fail(RSIM_E_HIP, "no three-pass column schedule for ny=%d", ny);
+    } else {
+      // last slice: W -> u (and the spectrum -> U); otherwise in place in U
+      if (!fused_col(ny, cv, last && want_spec, U, last ? u : U, U, nx, ty, ct->Bx, ct->Ay,
+                     ct->Rmax, spec_scale, s))
+        return fail(RSIM_E_HIP, "no fused column kernel for ny=%d", ny);
+    }
     ktime_end(KT_COL, s);
     RSIM_LAUNCHED();
     const bool cancel = !last && cancel_requested();
@@ -387,9 +416,10 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
       // row IFFT of W; after a cancel this leaves u consistent at a slice
       // boundary (U then does not hold FFT2(u))
       ktime_begin(KT_ROW, s);
-      fused_row(nx, rv, ROW_LAST, lut, last ? u : U, u, ny, tx, ta, s);
+      fused_row(nx, rv, ROW_LAST, lut, (last && !big) ? u : U, u, ny, tx, ta, s);
       ktime_end(KT_ROW, s);
       RSIM_LAUNCHED();
+      if (big && last && want_spec) RSIM_CHECK(fft2(RSIM_C64, RSIM_FORWARD, u, U, nx, ny, s));
       if (cancel) return fail(RSIM_E_CANCELLED, "multislice cancelled after slice %d", sl);
     }
     if (g_progress_sync()) RSIM_HIP(hipStreamSynchronize(s));

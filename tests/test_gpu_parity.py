@@ -293,6 +293,33 @@ def test_fused_sizes_match_plain_c128(n):
     assert rel(res[np.complex64][1], res[np.complex128][1]) < 2e-6
 
 
+@pytest.mark.parametrize("ny,nx", [(1024, 16384), (16384, 1024), (16384, 16384)])
+def test_fused_16384_axes_match_plain_c128(ny, nx):
+    """16384-point rows (1024-thread register pass) and 16384-point columns
+    (three-pass 128 x 128 schedule) against the plain complex128 path (whose
+    16384-point axes use the four-step out-of-LDS transform)."""
+    nz = 3
+    rng = np.random.default_rng(ny + 3 * nx)
+    vol = rs.DeviceVolume.empty((nz, ny, nx), DB3 * 30)
+    sph = np.stack([rng.uniform(0, nz, 40), rng.uniform(0, ny, 40), rng.uniform(0, nx, 40),
+                    rng.uniform(20, 300, 40)], axis=1)
+    vol.paint_spheres(sph, rng.integers(1, 3, 40))
+    dx, wl = 0.5e-6, 0.5e-10
+    res = {}
+    for dt, eng in ((np.complex128, lib.RSIM_PLAIN), (np.complex64, lib.RSIM_FUSED)):
+        u = rs.DeviceVector.from_numpy(np.ones(nx * ny, dtype=dt))
+        U = rs.DeviceVector(nx * ny, dt)
+        say("fused 16384 axes", ny, nx, dt.__name__)
+        lib.call("rsim_sample_apply_2d", u.ptr, U.ptr, nx, ny, u.code, dx, dx, wl, 0.4 / dx,
+                 vol.handle, dx, dx, 0.5e-6, 0.0, 0.0, 0, nz, eng, 0, None)
+        res[dt] = (host(u, ny, nx), host(U, ny, nx))
+        u.drop()
+        U.drop()
+    vol.destroy()
+    assert rel(res[np.complex64][0], res[np.complex128][0]) < 2e-6
+    assert rel(res[np.complex64][1], res[np.complex128][1]) < 2e-6
+
+
 # ------------------------------------------------------------ detector ----
 
 def test_area_v1_matches_oracle_non_integer_ratio():
