@@ -31,6 +31,7 @@ RSIM_FUSED = 1
 RSIM_FLAG_NO_SPECTRUM = 1
 RSIM_FLAG_MATH_FP32 = 2
 RSIM_FLAG_MATH_FP64 = 4
+RSIM_FLAG_MATH_COMP32 = 8
 
 _vp = C.c_void_p
 _i = C.c_int

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace rsim {
 
 template <typename R>
@@ -110,17 +112,56 @@ __device__ __forceinline__ void dft5(cpx<R>* v) {
   v[3] = m2 - in2;
 }
 
+// hi/lo splits of the rotation constants (exact decimal values rounded in
+// two steps: hi = fl32(v), lo = fl32(v - hi))
+constexpr double kC1 = 0.92387953251128675612818318939679;  // cos(pi/8)
+constexpr double kS1 = 0.38268343236508977172845998403040;  // sin(pi/8)
+constexpr double kH = 0.70710678118654752440084436210485;   // sqrt(1/2)
+#define RSIM_HI(v) ((float)(v))
+#define RSIM_LO(v) ((float)((v) - (double)(float)(v)))
+
+// Rotation constants of the radix-8/16 butterflies.  FP32 roundings of
+// cos(pi/8) and sin(pi/8) leave |W16|^2 - 1 = -5.7e-8, the same in every
+// transform; over a multislice run (a forward and an inverse transform per
+// axis and slice) that biases the field norm linearly.  The inverse
+// direction therefore uses FP32 values within an ulp, chosen so that
+// |W_inverse|^2 * |W_forward|^2 - 1 = -4.0e-9: the pair's norm errors cancel.
+template <int DIR, typename R>
+struct RotK {
+  static constexpr R c1 = R(kC1);
+  static constexpr R s1 = R(kS1);
+};
+template <>
+struct RotK<1, float> {
+  static constexpr float c1 = 0.9238795638084412f;   // fl32(cos(pi/8)) + 1 ulp
+  static constexpr float s1 = 0.3826834261417389f;   // fl32(sin(pi/8))
+};
+
+// v * W8^(-DIR) = v * (1 + DIR i) / sqrt(2): the sum and difference first,
+// then one scaling by sqrt(1/2), carried hi + lo in FP32 (an FP32-rounded
+// sqrt(1/2) alone would change |v| by -1.7e-8 at every rotation).
+template <int DIR, typename R>
+__device__ __forceinline__ cpx<R> rot45(cpx<R> v) {
+  const R a = v.x - R(DIR) * v.y;
+  const R b = v.y + R(DIR) * v.x;
+  if constexpr (std::is_same<R, float>::value) {
+    const float hh = RSIM_HI(kH), hl = RSIM_LO(kH);
+    return mk(__builtin_fmaf(a, hh, a * hl), __builtin_fmaf(b, hh, b * hl));
+  } else {
+    return mk(a * R(kH), b * R(kH));
+  }
+}
+
 template <int DIR, typename R>
 __device__ __forceinline__ void dft8(cpx<R>* v) {
-  const R h = R(0.70710678118654752440084436210485);
   cpx<R> e[4] = {v[0], v[2], v[4], v[6]};
   cpx<R> o[4] = {v[1], v[3], v[5], v[7]};
   dft4<DIR>(e);
   dft4<DIR>(o);
   // W8^1 = (h, DIR*h), W8^2 = DIR*i, W8^3 = W8^2 * W8^1
-  cpx<R> o1 = cmulcs(o[1], h, R(DIR) * h);
+  cpx<R> o1 = rot45<DIR>(o[1]);
   cpx<R> o2 = rot<DIR>(o[2]);
-  cpx<R> o3 = rot<DIR>(cmulcs(o[3], h, R(DIR) * h));
+  cpx<R> o3 = rot<DIR>(rot45<DIR>(o[3]));
   v[0] = e[0] + o[0];
   v[4] = e[0] - o[0];
   v[1] = e[1] + o1;
@@ -135,9 +176,8 @@ template <int DIR, typename R>
 __device__ __forceinline__ void dft16(cpx<R>* v) {
   // 16 = 4 x 4: Y[n2][k1] = DFT4 over n1 of x[4 n1 + n2], twiddle W16^(n2 k1),
   // X[k1 + 4 k2] = DFT4 over n2.
-  const R c1 = R(0.92387953251128675612818318939679);  // cos(pi/8)
-  const R s1 = R(0.38268343236508977172845998403040);  // sin(pi/8)
-  const R h = R(0.70710678118654752440084436210485);
+  const R c1 = RotK<DIR, R>::c1;
+  const R s1 = RotK<DIR, R>::s1;
   cpx<R> y[4][4];
 #pragma unroll
   for (int n2 = 0; n2 < 4; ++n2) {
@@ -150,15 +190,15 @@ __device__ __forceinline__ void dft16(cpx<R>* v) {
   const R d = R(DIR);
   // n2 = 1: W^1, W^2, W^3
   y[1][1] = cmulcs(y[1][1], c1, d * s1);
-  y[1][2] = cmulcs(y[1][2], h, d * h);
+  y[1][2] = rot45<DIR>(y[1][2]);
   y[1][3] = cmulcs(y[1][3], s1, d * c1);
   // n2 = 2: W^2, W^4, W^6
-  y[2][1] = cmulcs(y[2][1], h, d * h);
+  y[2][1] = rot45<DIR>(y[2][1]);
   y[2][2] = rot<DIR>(y[2][2]);
-  y[2][3] = rot<DIR>(cmulcs(y[2][3], h, d * h));
+  y[2][3] = rot<DIR>(rot45<DIR>(y[2][3]));
   // n2 = 3: W^3, W^6, W^9 = -W^1
   y[3][1] = cmulcs(y[3][1], s1, d * c1);
-  y[3][2] = rot<DIR>(cmulcs(y[3][2], h, d * h));
+  y[3][2] = rot<DIR>(rot45<DIR>(y[3][2]));
   y[3][3] = cmulcs(y[3][3], -c1, -d * s1);
 #pragma unroll
   for (int k1 = 0; k1 < 4; ++k1) {
@@ -198,13 +238,6 @@ __device__ __forceinline__ cpx<float> twmul_comp(cpx<float> a, const twc& w) {
   return cmul_comp(a, w.x, w.lx, -w.y, -w.ly);
 }
 
-// hi/lo splits of the rotation constants (exact decimal values rounded in
-// two steps: hi = fl32(v), lo = fl32(v - hi))
-constexpr double kC1 = 0.92387953251128675612818318939679;  // cos(pi/8)
-constexpr double kS1 = 0.38268343236508977172845998403040;  // sin(pi/8)
-constexpr double kH = 0.70710678118654752440084436210485;   // sqrt(1/2)
-#define RSIM_HI(v) ((float)(v))
-#define RSIM_LO(v) ((float)((v) - (double)(float)(v)))
 
 template <int DIR>
 __device__ __forceinline__ void dft8_comp(cpx<float>* v) {

@@ -253,18 +253,21 @@ struct ColLaunch {
 };
 
 // Variant lists: X(N, id, E, C, SPLIT).  Variant 0 is the default.
+#define RSIM_ROW_C32(N) (256 / (N / 32) > 0 ? (256 / (N / 32) < 16 ? 256 / (N / 32) : 16) : 1)
 #define RSIM_ROW_VARIANTS(N, X)                                                           \
   X(N, 0, 16, (256 / (N / 16) > 0 ? 256 / (N / 16) : 1), ((size_t)N * (256 / (N / 16) > 0 ? 256 / (N / 16) : 1) * 8 > 65536)) \
   X(N, 1, 16, (512 / (N / 16) > 0 ? 512 / (N / 16) : 1), ((size_t)N * (512 / (N / 16) > 0 ? 512 / (N / 16) : 1) * 8 > 65536)) \
-  X(N, 2, 16, (256 / (N / 16) > 0 ? 256 / (N / 16) : 1), true)
+  X(N, 2, 16, (256 / (N / 16) > 0 ? 256 / (N / 16) : 1), true)                           \
+  X(N, 3, 32, RSIM_ROW_C32(N), ((size_t)N * RSIM_ROW_C32(N) * 8 > 65536))
 #define RSIM_COL_C16(N) (1024 / (N / 16) < 16 ? 1024 / (N / 16) : 16)
 #define RSIM_COL_VARIANTS(N, X)                                                           \
   X(N, 0, 16, RSIM_COL_C16(N), true)                                                      \
   X(N, 1, 16, RSIM_COL_C16(N), ((size_t)N * RSIM_COL_C16(N) * 8 > 163840 - 8192))        \
   X(N, 2, 32, (512 / (N / 32) < 16 ? 512 / (N / 32) : 16), true)                         \
-  X(N, 3, 16, (RSIM_COL_C16(N) / 2 > 0 ? RSIM_COL_C16(N) / 2 : 1), true)
-constexpr int kRowVariants = 3;
-constexpr int kColVariants = 4;
+  X(N, 3, 16, (RSIM_COL_C16(N) / 2 > 0 ? RSIM_COL_C16(N) / 2 : 1), true)                 \
+  X(N, 4, 32, (1024 / (N / 32) < 16 ? 1024 / (N / 32) : 16), true)
+constexpr int kRowVariants = 4;
+constexpr int kColVariants = 5;
 
 // Per-size launchers (explicitly instantiated in rsim_fused_<N>.hip).
 template <int N, typename TW>
@@ -353,8 +356,12 @@ bool col_go(int v, bool spec_out, const void* in, void* out, void* spec, int nx,
 // chains FFT -> chirp -> IFFT in registers.  Every row segment a wavefront
 // touches is C * 8 contiguous bytes.
 // ---------------------------------------------------------------------------
+// The A and B twiddles of an element are W and conj(W) of the same index in
+// every slice, so an FP32-rounded |W| != 1 would scale that element
+// identically in each slice; FP32 arithmetic modes therefore take the hi + lo
+// table for them (8 twiddles per thread and pass).
 template <typename TW>
-using BigTwElem = typename std::conditional<std::is_same<TW, twc>::value, twc, cpx<TW>>::type;
+using BigTwElem = typename std::conditional<std::is_same<TW, double>::value, cpx<double>, twc>::type;
 
 template <int DIR>
 __device__ __forceinline__ cpx<float> big_tw(cpx<float> v, const twc* T, int i) {

@@ -43,7 +43,7 @@ def log(*a):
 
 
 def measured_traffic(kernel):
-    """HBM bytes per dispatch of `kernel` (default comp32 engine) from the
+    """HBM bytes per dispatch of `kernel` (default FP32 engine) from the
     committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary, scaled by the copy
     calibration recorded there; None when the file holds no figure for it."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_FILE)
@@ -51,8 +51,10 @@ def measured_traffic(kernel):
         with open(path) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) >= 4 and parts[0] == "traffic" and parts[1] == kernel and parts[2] == "comp32":
-                    return float(parts[-1])
+                if len(parts) >= 4 and parts[0] == "traffic" and parts[1] == kernel and parts[2] == "fp32":
+                    for tok in parts[3:]:
+                        if tok.isdigit():
+                            return float(tok)
     except OSError:
         pass
     return None
@@ -98,10 +100,12 @@ def main():
     ap.add_argument("--cpu-slices", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
-    ap.add_argument("--workload", choices=["multislice", "adjoint"], default="multislice",
+    ap.add_argument("--workload", choices=["multislice", "adjoint", "large"], default="multislice",
                     help="multislice: the headline 4096^2 fused run; adjoint: forward + "
-                         "backward of the differentiable path at 2048^2 x 128")
+                         "backward of the differentiable path at 2048^2 x 128; large: "
+                         "16384^2 through 512 slices (three-pass column schedule)")
     ap.add_argument("--fd-voxels", type=int, default=64)
+    ap.add_argument("--large-slices", type=int, default=512)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -118,6 +122,8 @@ def main():
         raise RuntimeError("no HIP device visible")
     if args.workload == "adjoint":
         return adjoint_main(args, rs)
+    if args.workload == "large":
+        return large_main(args, rs)
     if args.calibrate_copy:
         nbytes = args.calibrate_copy << 20
         a, b = C.c_void_p(), C.c_void_p()
@@ -196,7 +202,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"voxelized multislice {n}x{n}, {nz} slices x 0.5um, 200 seeded spheres (seed 2), "
-                        "3 materials, lambda 0.5A, fused complex64 engine (compensated FP32 butterflies)",
+                        "3 materials, lambda 0.5A, fused complex64 engine (FP32 butterflies, norm-matched inverse constants)",
             "nx": n, "ny": n, "slices": nz, "dx_m": dx, "wl_m": wl, "dz_m": pz,
             "parallelism": "single GPU",
         },
@@ -216,7 +222,7 @@ def main():
     }
 
     # ---- the other butterfly-arithmetic modes, reported beside the default --
-    MODES = {"fp32": lib.RSIM_FLAG_MATH_FP32, "fp64": lib.RSIM_FLAG_MATH_FP64}
+    MODES = {"comp32": lib.RSIM_FLAG_MATH_COMP32, "fp64": lib.RSIM_FLAG_MATH_FP64}
     for name, fl in MODES.items():
         run_multislice(lib, u, U, n, vol, lib.RSIM_FUSED, dx, wl, pz, 2, flags=fl, stream=s)
         lib.call("rsim_synchronize", s)
@@ -435,6 +441,163 @@ def adjoint_main(args, rs):
         "fd_voxels": args.fd_voxels, "fd_max_err_over_max_grad": worst, "loss": loss,
         "config": {"workload": "2048x2048x128 float32 delta/beta (smooth seeded noise, seed 4), "
                                "lambda 0.5A, pz 0.5um, 0.2 m leg, area_v1 4x4, target +5% (seed 5)"},
+    }), flush=True)
+
+
+DB5 = np.array([0, 3.5e-7 + 2e-10j, 1.1e-6 + 3e-9j, 2.6e-7 + 1.5e-10j, 2.0e-6 + 5e-9j,
+                2.7e-10 + 1e-13j])   # vacuum, config-2 materials, a denser one, air (cracks)
+
+
+def paint_large_volume(torch, lib, vol, n, nz, seed, stream, bands=8):
+    """Cylinders along z (one set per z band) and air-filled cracks, painted
+    per band on the GPU and copied into the band's slices."""
+    ids = C.c_void_p()
+    lib.call("rsim_volume_info", vol.handle, C.byref(ids), None, None, None)
+    rng = np.random.default_rng(seed)
+    dev = torch.device("cuda:0")
+    per = (nz + bands - 1) // bands
+    for b in range(bands):
+        m = torch.zeros((n, n), dtype=torch.uint8, device=dev)
+        for _ in range(40):
+            cy, cx = rng.uniform(0, n, 2)
+            r = rng.uniform(40, 800)
+            mat = int(rng.integers(1, 5))
+            y0, y1 = max(0, int(cy - r)), min(n, int(cy + r) + 1)
+            x0, x1 = max(0, int(cx - r)), min(n, int(cx + r) + 1)
+            if y1 <= y0 or x1 <= x0:
+                continue
+            yy = torch.arange(y0, y1, device=dev, dtype=torch.float32)[:, None] + 0.5 - cy
+            xx = torch.arange(x0, x1, device=dev, dtype=torch.float32)[None, :] + 0.5 - cx
+            sub = m[y0:y1, x0:x1]
+            sub[yy * yy + xx * xx <= r * r] = mat
+        for _ in range(12):
+            # a straight crack: points within w/2 of a segment, filled with air
+            py, px = rng.uniform(0, n, 2)
+            ang = rng.uniform(0, np.pi)
+            half = rng.uniform(500, 3000)
+            w = rng.uniform(2, 6)
+            dy, dx_ = np.sin(ang), np.cos(ang)
+            y0 = max(0, int(py - abs(dy) * half - w)); y1 = min(n, int(py + abs(dy) * half + w) + 1)
+            x0 = max(0, int(px - abs(dx_) * half - w)); x1 = min(n, int(px + abs(dx_) * half + w) + 1)
+            if y1 <= y0 or x1 <= x0:
+                continue
+            yy = torch.arange(y0, y1, device=dev, dtype=torch.float32)[:, None] + 0.5 - py
+            xx = torch.arange(x0, x1, device=dev, dtype=torch.float32)[None, :] + 0.5 - px
+            along = yy * dy + xx * dx_
+            across = yy * dx_ - xx * dy
+            sub = m[y0:y1, x0:x1]
+            sub[(along.abs() <= half) & (across.abs() <= w / 2)] = 5
+        for z in range(b * per, min(nz, (b + 1) * per)):
+            lib.call("rsim_memcpy_d2d", C.c_void_p(ids.value + z * n * n), C.c_void_p(m.data_ptr()),
+                     n * n, stream)
+        torch.cuda.current_stream().synchronize()
+
+
+def large_main(args, rs):
+    """16384^2 complex64 through 512 slices of 0.25 um (seeded cylinders and
+    cracks, seed 5; 4 materials and air), lambda 0.4 A, then 1 m of free space
+    and a 4x4 area_v1 detector (4096^2).  Times the fused engine (three-pass
+    column schedule) and the plain complex128 path on the same volume and
+    reports field / detector relative L2 between them."""
+    import torch
+    lib = rs._lib
+    n, nz = 16384, args.large_slices
+    dx, wl, pz, leg = 0.25e-6, 0.4e-10, 0.25e-6, 1.0
+    N = n * n
+    st = torch.cuda.Stream()
+    sh = C.c_void_p(st.cuda_stream)
+    with torch.cuda.stream(st):
+        t0 = time.perf_counter()
+        vol = rs.DeviceVolume.empty((nz, n, n), DB5)
+        paint_large_volume(torch, lib, vol, n, nz, 5, sh)
+        log(f"volume {nz}x{n}x{n} ({nz * N / 2**30:.0f} GiB) painted in {time.perf_counter() - t0:.1f}s")
+        dev = torch.device("cuda:0")
+
+        def run(u, U, eng, k, code):
+            lib.call("rsim_sample_apply_2d", C.c_void_p(u.data_ptr()), C.c_void_p(U.data_ptr()), n, n,
+                     code, dx, dx, wl, 0.4 / dx, vol.handle, dx, dx, pz, 0.0, 0.0, 0, k, eng, 0, sh)
+
+        u = torch.ones(N, dtype=torch.complex64, device=dev)
+        U = torch.empty_like(u)
+        run(u, U, lib.RSIM_FUSED, 4, lib.RSIM_C64)     # autotune + warm-up
+        u.fill_(1)
+        lib.call("rsim_kernel_timing_read", None, None)
+        lib.call("rsim_kernel_timing", KT_EVERY)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run(u, U, lib.RSIM_FUSED, nz, lib.RSIM_C64)
+        torch.cuda.synchronize()
+        t_fused = time.perf_counter() - t0
+        lib.call("rsim_kernel_timing", 0)
+        ms3 = (C.c_double * 3)()
+        cnt3 = (C.c_int * 3)()
+        lib.call("rsim_kernel_timing_read", ms3, cnt3)
+        rate = nz / t_fused
+        gbs = 65.0 * N * rate / 1e9
+        row_us = 1e3 * ms3[0] / max(cnt3[0], 1)
+        col_us = 1e3 * ms3[1] / max(cnt3[1], 1)
+        log(f"fused c64 16384^2: {rate:.1f} slice-steps/s ({t_fused:.2f} s / {nz} slices), "
+            f"{gbs:.0f} GB/s on the 65 B/pt model; row {row_us:.0f} us, three column passes {col_us:.0f} us")
+        # the other FP32 arithmetic on the same volume (the default for a
+        # 512-slice volume is compensated FP32; this is plain norm-matched FP32)
+        alt_flag = lib.RSIM_FLAG_MATH_FP32 if nz > 256 else lib.RSIM_FLAG_MATH_COMP32
+        alt = "fp32" if nz > 256 else "comp32"
+        uc = torch.ones(N, dtype=torch.complex64, device=dev)
+        Uc = torch.empty_like(uc)
+        lib.call("rsim_sample_apply_2d", C.c_void_p(uc.data_ptr()), C.c_void_p(Uc.data_ptr()), n, n,
+                 lib.RSIM_C64, dx, dx, wl, 0.4 / dx, vol.handle, dx, dx, pz, 0.0, 0.0, 0, 4,
+                 lib.RSIM_FUSED, alt_flag, sh)
+        uc.fill_(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lib.call("rsim_sample_apply_2d", C.c_void_p(uc.data_ptr()), C.c_void_p(Uc.data_ptr()), n, n,
+                 lib.RSIM_C64, dx, dx, wl, 0.4 / dx, vol.handle, dx, dx, pz, 0.0, 0.0, 0, nz,
+                 lib.RSIM_FUSED, alt_flag, sh)
+        torch.cuda.synchronize()
+        rate_c = nz / (time.perf_counter() - t0)
+        log(f"fused c64 16384^2, {alt} butterflies: {rate_c:.1f} slice-steps/s")
+        u2 = torch.ones(N, dtype=torch.complex128, device=dev)
+        U2 = torch.empty_like(u2)
+        run(u2, U2, lib.RSIM_PLAIN, 2, lib.RSIM_C128)
+        u2.fill_(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run(u2, U2, lib.RSIM_PLAIN, nz, lib.RSIM_C128)
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t0
+        log(f"plain c128 16384^2: {nz / t_plain:.1f} slice-steps/s")
+        err_field = float(torch.linalg.vector_norm(u.to(torch.complex128) - u2) / torch.linalg.vector_norm(u2))
+        z_det = nz * pz + leg
+        oy = ox = n // 4
+        err_field_c = float(torch.linalg.vector_norm(uc.to(torch.complex128) - u2) / torch.linalg.vector_norm(u2))
+        imgs = []
+        for fu, fU, code in ((u, U, lib.RSIM_C64), (uc, Uc, lib.RSIM_C64), (u2, U2, lib.RSIM_C128)):
+            lib.call("rsim_propagate_2d", C.c_void_p(fu.data_ptr()), C.c_void_p(fU.data_ptr()), n, n,
+                     code, dx, dx, wl, leg, 0.4 / dx, 1, sh)
+            img = torch.zeros((oy, ox), dtype=torch.float64, device=dev)
+            lib.call("rsim_detector_area_v1_accumulate", C.c_void_p(fu.data_ptr()), n, n, code, dx, dx,
+                     ox, oy, 4 * dx, 4 * dx, z_det, 1.0, C.c_void_p(img.data_ptr()), sh)
+            imgs.append(img)
+        torch.cuda.synchronize()
+        err_det = float(torch.linalg.vector_norm(imgs[0] - imgs[2]) / torch.linalg.vector_norm(imgs[2]))
+        err_det_c = float(torch.linalg.vector_norm(imgs[1] - imgs[2]) / torch.linalg.vector_norm(imgs[2]))
+        log(f"fused c64 (default arithmetic) vs plain c128: field {err_field:.3e}, detector {err_det:.3e}; "
+            f"{alt}: field {err_field_c:.3e}, detector {err_det_c:.3e}")
+    vol.destroy()
+    print(json.dumps({
+        "metric": "slice-steps/s at 16384^2 (c64 fused, three-pass columns); % HBM peak",
+        "value": round(rate, 2), "unit": "slice-steps/s", "n_gpus": 1, "steps": 1,
+        "higher_is_better": True, "dtype": "c64", "data": "synthetic",
+        "step_hbm_GBs_65B_model": round(gbs, 1), "step_hbm_frac_65B_model": round(gbs / HBM_PEAK_GBS, 4),
+        "kernel_avg_us": {"row": round(row_us, 1), "column_three_passes": round(col_us, 1)},
+        "plain_c128_slice_steps_per_s": round(nz / t_plain, 2), "fused_over_plain": round(t_plain / t_fused, 2),
+        "rel_l2_field_fused_vs_c128": err_field, "rel_l2_detector_fused_vs_c128": err_det,
+        "arithmetic": "comp32" if nz > 256 else "fp32",
+        f"fused_{alt}_slice_steps_per_s": round(rate_c, 2),
+        f"rel_l2_field_fused_{alt}_vs_c128": err_field_c, f"rel_l2_detector_fused_{alt}_vs_c128": err_det_c,
+        "detector_shape": [oy, ox],
+        "config": {"workload": f"16384x16384, {nz} slices x 0.25um, seeded cylinders + cracks (seed 5), "
+                               "4 materials + air, lambda 0.4A, 1 m leg, area_v1 4x4"},
     }), flush=True)
 
 

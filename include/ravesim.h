@@ -42,8 +42,15 @@ enum { RSIM_PLAIN = 0, RSIM_FUSED = 1 };
 /* rsim_sample_apply_2d flags */
 enum {
   RSIM_FLAG_NO_SPECTRUM = 1, /* do not materialise U = FFT2(u) */
-  RSIM_FLAG_MATH_FP32 = 2,   /* FUSED: plain FP32 butterflies */
-  RSIM_FLAG_MATH_FP64 = 4    /* FUSED: FP64 butterflies (default: compensated FP32, hi+lo constants) */
+  /* FUSED butterfly arithmetic.  RSIM_FLAG_MATH_FP32: FP32 with norm-matched
+   * inverse rotation constants.  RSIM_FLAG_MATH_COMP32: FP32 data, hi+lo
+   * compensated constants and twiddles (16 % slower; its error grows as the
+   * square root of the slice count instead of linearly).
+   * RSIM_FLAG_MATH_FP64: FP64 butterflies.  Without any of them: FP32 for
+   * volumes of up to 256 slices, COMP32 for deeper ones. */
+  RSIM_FLAG_MATH_FP32 = 2,
+  RSIM_FLAG_MATH_FP64 = 4,
+  RSIM_FLAG_MATH_COMP32 = 8
 };
 
 typedef struct rsim_volume rsim_volume;

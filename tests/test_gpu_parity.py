@@ -409,25 +409,30 @@ def test_phase_stepping_replays_from_snapshot():
         assert rel(outs[k], exp) < 1e-11
 
 
-@pytest.mark.parametrize("n", [1024, 4096])
-@pytest.mark.parametrize("variant", [(r, c) for r in range(3) for c in range(4)])
-def test_every_fused_launch_shape_matches_plain_c128(n, variant, monkeypatch):
+N_ROW_VARIANTS, N_COL_VARIANTS = 4, 5
+VARIANT_CASES = ([(1024, 1024, (r, c)) for r in range(N_ROW_VARIANTS) for c in range(N_COL_VARIANTS)]
+                 + [(4096, 4096, (i % N_ROW_VARIANTS, i)) for i in range(N_COL_VARIANTS)]
+                 + [(256, 16384, (r, 0)) for r in range(N_ROW_VARIANTS)])
+
+
+@pytest.mark.parametrize("ny,nx,variant", VARIANT_CASES)
+def test_every_fused_launch_shape_matches_plain_c128(ny, nx, variant, monkeypatch):
     """Each row/column launch-shape variant the autotuner may pick, forced in turn."""
     nz = 2
-    grid = np.zeros((nz, n, n), np.uint8)
-    grid[:, n // 5: n // 2, n // 3: (2 * n) // 3] = 1
-    grid[1, n // 2: (3 * n) // 4, n // 4: n // 2] = 2
+    grid = np.zeros((nz, ny, nx), np.uint8)
+    grid[:, ny // 5: ny // 2, nx // 3: (2 * nx) // 3] = 1
+    grid[1, ny // 2: (3 * ny) // 4, nx // 4: nx // 2] = 2
     dx, wl = 0.5e-6, 0.5e-10
     vol = rs.DeviceVolume.from_numpy(grid, DB3 * 30)
     rng = np.random.default_rng(7)
-    u0 = np.exp(1j * rng.uniform(0, 0.2, size=(n, n)))
+    u0 = np.exp(1j * rng.uniform(0, 0.2, size=(ny, nx)))
     out = {}
     monkeypatch.setenv("RAVESIM_FUSED_VARIANT", f"{variant[0]},{variant[1]}")
     for dt, eng in ((np.complex128, lib.RSIM_PLAIN), (np.complex64, lib.RSIM_FUSED)):
-        u, U = dev(u0.astype(dt)), rs.DeviceVector(n * n, dt)
-        say("variant", n, variant, np.dtype(dt).name)
-        lib.call("rsim_sample_apply_2d", u.ptr, U.ptr, n, n, u.code, dx, dx, wl, 0.4 / dx, vol.handle,
+        u, U = dev(u0.astype(dt)), rs.DeviceVector(nx * ny, dt)
+        say("variant", ny, nx, variant, np.dtype(dt).name)
+        lib.call("rsim_sample_apply_2d", u.ptr, U.ptr, nx, ny, u.code, dx, dx, wl, 0.4 / dx, vol.handle,
                  dx, dx, 0.5e-6, 0.0, 0.0, 0, nz, eng, 0, None)
-        out[dt] = (host(u, n, n), host(U, n, n))
+        out[dt] = (host(u, ny, nx), host(U, ny, nx))
     assert rel(out[np.complex64][0], out[np.complex128][0]) < 1e-6
     assert rel(out[np.complex64][1], out[np.complex128][1]) < 1e-6
