@@ -69,12 +69,22 @@ __device__ __forceinline__ void dft2(cpx<R>* v) {
 
 template <int DIR, typename R>
 __device__ __forceinline__ void dft3(cpx<R>* v) {
+  // W3 = -1/2 + DIR i sqrt(3)/2; in FP32 the sqrt(3)/2 product is carried
+  // hi + lo (an FP32-rounded constant alone would make |W3| != 1 by 3e-8 in
+  // every transform)
+  constexpr double kS3 = 0.86602540378443864676372317075294;
   const R c = R(-0.5);
-  const R s = R(DIR) * R(0.86602540378443864676372317075294);
   cpx<R> a = v[1] + v[2];
   cpx<R> b = v[1] - v[2];
   cpx<R> m = mk(v[0].x + c * a.x, v[0].y + c * a.y);
-  cpx<R> t = mk(-s * b.y, s * b.x);  // i*s*b
+  cpx<R> sb;
+  if constexpr (std::is_same<R, float>::value) {
+    const float sh = (float)kS3, sl = (float)(kS3 - (double)(float)kS3);
+    sb = mk(__builtin_fmaf(b.x, sh, b.x * sl), __builtin_fmaf(b.y, sh, b.y * sl));
+  } else {
+    sb = mk(b.x * R(kS3), b.y * R(kS3));
+  }
+  const cpx<R> t = DIR < 0 ? mk(sb.y, -sb.x) : mk(-sb.y, sb.x);  // DIR * i * s * b
   v[0] = v[0] + a;
   v[1] = m + t;
   v[2] = m - t;
@@ -295,13 +305,53 @@ __device__ __forceinline__ void dft16_comp(cpx<float>* v) {
   }
 }
 
+// Prime-factor (Good-Thomas) radix-6 and radix-12 butterflies: for
+// N = N1 * N2 with coprime factors, x[(N2 n1 + N1 n2) mod N] -> DFT_N2 over n2
+// -> DFT_N1 over n1 -> X[k], k = k1 (mod N1), k = k2 (mod N2), with no
+// twiddles between the two sub-transforms.
+template <int DIR, typename R>
+__device__ __forceinline__ void dft6(cpx<R>* v) {
+  // N1 = 2, N2 = 3: inputs (3 n1 + 2 n2) mod 6, outputs (3 k1 + 4 k2) mod 6
+  cpx<R> a[3] = {v[0], v[2], v[4]};
+  cpx<R> b[3] = {v[3], v[5], v[1]};
+  dft3<DIR>(a);
+  dft3<DIR>(b);
+  v[0] = a[0] + b[0];
+  v[3] = a[0] - b[0];
+  v[4] = a[1] + b[1];
+  v[1] = a[1] - b[1];
+  v[2] = a[2] + b[2];
+  v[5] = a[2] - b[2];
+}
+
+template <int DIR, typename R>
+__device__ __forceinline__ void dft12(cpx<R>* v) {
+  // N1 = 4, N2 = 3: inputs (3 n1 + 4 n2) mod 12, outputs (9 k1 + 4 k2) mod 12
+  cpx<R> y[4][3];
+#pragma unroll
+  for (int n1 = 0; n1 < 4; ++n1) {
+#pragma unroll
+    for (int n2 = 0; n2 < 3; ++n2) y[n1][n2] = v[(3 * n1 + 4 * n2) % 12];
+    dft3<DIR>(y[n1]);
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < 3; ++k2) {
+    cpx<R> z[4] = {y[0][k2], y[1][k2], y[2][k2], y[3][k2]};
+    dft4<DIR>(z);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) v[(9 * k1 + 4 * k2) % 12] = z[k1];
+  }
+}
+
 template <int RADIX, int DIR, typename R>
 __device__ __forceinline__ void dft(cpx<R>* v) {
   if constexpr (RADIX == 2) dft2<DIR>(v);
   else if constexpr (RADIX == 3) dft3<DIR>(v);
   else if constexpr (RADIX == 4) dft4<DIR>(v);
   else if constexpr (RADIX == 5) dft5<DIR>(v);
+  else if constexpr (RADIX == 6) dft6<DIR>(v);
   else if constexpr (RADIX == 8) dft8<DIR>(v);
+  else if constexpr (RADIX == 12) dft12<DIR>(v);
   else if constexpr (RADIX == 16) dft16<DIR>(v);
 }
 

@@ -19,6 +19,9 @@ template <> struct Sched<1024> { using F = RegFac<1024, 16, 16, 4, 16>; };
 template <> struct Sched<2048> { using F = RegFac<2048, 16, 16, 8, 16>; };
 template <> struct Sched<4096> { using F = RegFac<4096, 16, 16, 16, 16>; };
 template <> struct Sched<8192> { using F = RegFac<8192, 16, 16, 2, 16, 16>; };
+// 2^a * 3 lengths: E = 24 elements per thread (a multiple of 8, 6 and 12)
+template <> struct Sched<3072> { using F = RegFac<3072, 24, 8, 6, 8, 8>; };
+template <> struct Sched<6144> { using F = RegFac<6144, 24, 8, 12, 8, 8>; };
 // row pass of 16384-point axes (1024 threads per row)
 template <> struct Sched<16384> { using F = RegFac<16384, 16, 16, 4, 16, 16>; };
 // sub-transform of the three-pass 16384-point column schedule (128 x 128)
@@ -75,6 +78,8 @@ inline bool sched_info(int N, SchedInfo* si) {
     RSIM_SI(2048)
     RSIM_SI(4096)
     RSIM_SI(8192)
+    RSIM_SI(3072)
+    RSIM_SI(6144)
     RSIM_SI(16384)
     RSIM_SI(128)
     default: return false;

@@ -252,20 +252,29 @@ struct ColLaunch {
   }
 };
 
-// Variant lists: X(N, id, E, C, SPLIT).  Variant 0 is the default.
-#define RSIM_ROW_C32(N) (256 / (N / 32) > 0 ? (256 / (N / 32) < 16 ? 256 / (N / 32) : 16) : 1)
+// Variant lists: X(N, id, E, C, SPLIT).  Variant 0 is the default.  E1 is
+// the schedule's own element count (16, or 24 for 2^a * 3 lengths), E2 = 2*E1.
+template <int N> constexpr int vE1() { return Sched<N>::F::E; }
+template <int N> constexpr int vE2() { return 2 * Sched<N>::F::E; }
+constexpr int vmax1(int v) { return v > 0 ? v : 1; }
+constexpr int vmin(int a, int b) { return a < b ? a : b; }
+constexpr bool vsplit(int n, int c, int cap) { return (long long)n * c * 8 > cap; }
+constexpr int vpow2floor(int v) { return v >= 16 ? 16 : v >= 8 ? 8 : v >= 4 ? 4 : v >= 2 ? 2 : 1; }
+// most sequences whose split (4-byte) exchange buffer fits LDS beside the LUT
+constexpr int vLdsC(int n) { return vpow2floor(vmax1((163840 - 8192) / (n * 4))); }
+template <int N> constexpr int vRowC(int E, int t) { return vmin(vmin(vmax1(t / (N / E)), 16), vLdsC(N)); }
+template <int N> constexpr int vColC(int E, int t) { return vmin(vmin(vmax1(t / (N / E)), 16), vLdsC(N)); }
 #define RSIM_ROW_VARIANTS(N, X)                                                           \
-  X(N, 0, 16, (256 / (N / 16) > 0 ? 256 / (N / 16) : 1), ((size_t)N * (256 / (N / 16) > 0 ? 256 / (N / 16) : 1) * 8 > 65536)) \
-  X(N, 1, 16, (512 / (N / 16) > 0 ? 512 / (N / 16) : 1), ((size_t)N * (512 / (N / 16) > 0 ? 512 / (N / 16) : 1) * 8 > 65536)) \
-  X(N, 2, 16, (256 / (N / 16) > 0 ? 256 / (N / 16) : 1), true)                           \
-  X(N, 3, 32, RSIM_ROW_C32(N), ((size_t)N * RSIM_ROW_C32(N) * 8 > 65536))
-#define RSIM_COL_C16(N) (1024 / (N / 16) < 16 ? 1024 / (N / 16) : 16)
+  X(N, 0, vE1<N>(), vRowC<N>(vE1<N>(), 256), vsplit(N, vRowC<N>(vE1<N>(), 256), 65536))    \
+  X(N, 1, vE1<N>(), vRowC<N>(vE1<N>(), 512), vsplit(N, vRowC<N>(vE1<N>(), 512), 65536))    \
+  X(N, 2, vE1<N>(), vRowC<N>(vE1<N>(), 256), true)                                        \
+  X(N, 3, vE2<N>(), vRowC<N>(vE2<N>(), 256), vsplit(N, vRowC<N>(vE2<N>(), 256), 65536))
 #define RSIM_COL_VARIANTS(N, X)                                                           \
-  X(N, 0, 16, RSIM_COL_C16(N), true)                                                      \
-  X(N, 1, 16, RSIM_COL_C16(N), ((size_t)N * RSIM_COL_C16(N) * 8 > 163840 - 8192))        \
-  X(N, 2, 32, (512 / (N / 32) < 16 ? 512 / (N / 32) : 16), true)                         \
-  X(N, 3, 16, (RSIM_COL_C16(N) / 2 > 0 ? RSIM_COL_C16(N) / 2 : 1), true)                 \
-  X(N, 4, 32, (1024 / (N / 32) < 16 ? 1024 / (N / 32) : 16), true)
+  X(N, 0, vE1<N>(), vColC<N>(vE1<N>(), 1024), true)                                       \
+  X(N, 1, vE1<N>(), vColC<N>(vE1<N>(), 1024), vsplit(N, vColC<N>(vE1<N>(), 1024), 163840 - 8192)) \
+  X(N, 2, vE2<N>(), vColC<N>(vE2<N>(), 512), true)                                        \
+  X(N, 3, vE1<N>(), vmax1(vColC<N>(vE1<N>(), 1024) / 2), true)                            \
+  X(N, 4, vE2<N>(), vColC<N>(vE2<N>(), 1024), true)
 constexpr int kRowVariants = 4;
 constexpr int kColVariants = 5;
 

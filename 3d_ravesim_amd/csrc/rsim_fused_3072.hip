@@ -1,0 +1,7 @@
+// Fused multislice kernels for length-3072 axes (2^a * 3: radix-8 and prime-factor
+// radix-6 stages, 24 elements per thread).
+#include "rsim_fused.hpp"
+
+namespace rsim {
+RSIM_FUSED_INSTANTIATE(3072)
+}  // namespace rsim

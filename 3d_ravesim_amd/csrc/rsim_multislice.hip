@@ -45,6 +45,8 @@ bool fused_row(int N, int v, int mode, bool lut, const void* in, void* out, int 
     case 2048: return row_go<2048, TW>(v, mode, lut, in, out, ny, tw, ta, s);
     case 4096: return row_go<4096, TW>(v, mode, lut, in, out, ny, tw, ta, s);
     case 8192: return row_go<8192, TW>(v, mode, lut, in, out, ny, tw, ta, s);
+    case 3072: return row_go<3072, TW>(v, mode, lut, in, out, ny, tw, ta, s);
+    case 6144: return row_go<6144, TW>(v, mode, lut, in, out, ny, tw, ta, s);
     case 16384: return row_go<16384, TW>(v, mode, lut, in, out, ny, tw, ta, s);
     default: return false;
   }
@@ -61,6 +63,8 @@ bool fused_col(int N, int v, bool spec_out, const void* in, void* out, void* spe
     case 2048: return col_go<2048, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
     case 4096: return col_go<4096, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
     case 8192: return col_go<8192, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
+    case 3072: return col_go<3072, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
+    case 6144: return col_go<6144, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
     default: return false;
   }
 }
@@ -311,10 +315,12 @@ constexpr int kBigCol = 16384;
 // volumes deeper than this default to compensated FP32 butterflies
 constexpr int kFp32MaxSlices = 256;
 static bool fused_row_len(int n) {
-  return n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096 || n == 8192 || n == 16384;
+  return n == 256 || n == 512 || n == 1024 || n == 2048 || n == 3072 || n == 4096 || n == 6144 ||
+         n == 8192 || n == 16384;
 }
 static bool fused_col_len(int n) {
-  return n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096 || n == 8192 || n == kBigCol;
+  return n == 256 || n == 512 || n == 1024 || n == 2048 || n == 3072 || n == 4096 || n == 6144 ||
+         n == 8192 || n == kBigCol;
 }
 
 bool fused_supported(int nx, int ny, int dtype) {

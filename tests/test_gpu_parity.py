@@ -271,7 +271,7 @@ def test_sample_apply_matches_oracle(engine, aligned):
         assert rel(host(U, n, n), Ue) < tol, (engine, dt)
 
 
-@pytest.mark.parametrize("n", [512, 1024, 2048])
+@pytest.mark.parametrize("n", [512, 1024, 2048, 3072, 6144])
 def test_fused_sizes_match_plain_c128(n):
     """Fused complex64 against the plain complex128 GPU path (itself pinned to the oracle)."""
     rng = np.random.default_rng(n)
@@ -293,11 +293,13 @@ def test_fused_sizes_match_plain_c128(n):
     assert rel(res[np.complex64][1], res[np.complex128][1]) < 2e-6
 
 
-@pytest.mark.parametrize("ny,nx", [(1024, 16384), (16384, 1024), (16384, 16384)])
+@pytest.mark.parametrize("ny,nx", [(1024, 16384), (16384, 1024), (16384, 16384), (3072, 2048),
+                                   (2048, 6144), (6144, 3072)])
 def test_fused_16384_axes_match_plain_c128(ny, nx):
-    """16384-point rows (1024-thread register pass) and 16384-point columns
-    (three-pass 128 x 128 schedule) against the plain complex128 path (whose
-    16384-point axes use the four-step out-of-LDS transform)."""
+    """Mixed and large shapes: 16384-point rows (1024-thread register pass),
+    16384-point columns (three-pass 128 x 128 schedule), 3072/6144-point axes
+    (prime-factor radix-6/12 stages) against the plain complex128 path (whose
+    6144- and 16384-point axes use the four-step out-of-LDS transform)."""
     nz = 3
     rng = np.random.default_rng(ny + 3 * nx)
     vol = rs.DeviceVolume.empty((nz, ny, nx), DB3 * 30)
@@ -412,7 +414,9 @@ def test_phase_stepping_replays_from_snapshot():
 N_ROW_VARIANTS, N_COL_VARIANTS = 4, 5
 VARIANT_CASES = ([(1024, 1024, (r, c)) for r in range(N_ROW_VARIANTS) for c in range(N_COL_VARIANTS)]
                  + [(4096, 4096, (i % N_ROW_VARIANTS, i)) for i in range(N_COL_VARIANTS)]
-                 + [(256, 16384, (r, 0)) for r in range(N_ROW_VARIANTS)])
+                 + [(256, 16384, (r, 0)) for r in range(N_ROW_VARIANTS)]
+                 + [(3072, 6144, (i % N_ROW_VARIANTS, i)) for i in range(N_COL_VARIANTS)]
+                 + [(6144, 3072, (i % N_ROW_VARIANTS, i)) for i in range(N_COL_VARIANTS)])
 
 
 @pytest.mark.parametrize("ny,nx,variant", VARIANT_CASES)
