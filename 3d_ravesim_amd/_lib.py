@@ -42,6 +42,14 @@ _ip = C.POINTER(C.c_int)
 
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_char_p, C.c_int64, C.c_int64, C.c_void_p)
 
+
+class BatchInstance(C.Structure):
+    """rsim_batch_instance of include/ravesim.h."""
+    _fields_ = [("u", _vp), ("U", _vp), ("nx", _i), ("ny", _i), ("dx", _d), ("dy", _d),
+                ("wl", _d), ("cutoff_freq", _d), ("vol", _vp), ("pixel_size_x", _d),
+                ("pixel_size_y", _d), ("pixel_size_z", _d), ("x_pos", _d), ("y_pos", _d)]
+
+
 # name -> argtypes (restype is int unless listed in _RESTYPES)
 SIGNATURES = {
     "rsim_last_error": [],
@@ -88,6 +96,10 @@ SIGNATURES = {
                                       _vp, _i, _vp, _vp, _vp, _vp],
     "rsim_detector_area_v1_adjoint": [_vp, _vp, _i, _i, _i, _d, _d, _i, _i, _d, _d, _d, _vp,
                                       _vp],
+    "rsim_batch_create": [C.POINTER(_vp), C.POINTER(BatchInstance), _i, _i, _i, _i, _vp],
+    "rsim_batch_run": [_vp, _vp],
+    "rsim_batch_info": [_vp, _ip, _ip],
+    "rsim_batch_destroy": [_vp],
 }
 _RESTYPES = {"rsim_last_error": C.c_char_p, "rsim_version": C.c_char_p}
 

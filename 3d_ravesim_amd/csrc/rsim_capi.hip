@@ -39,6 +39,11 @@ int multislice_db_adjoint(void* g, void* U, int nx, int ny, int dtype, double dx
                           double wl, double freq, const float* delta, const float* beta,
                           double pz, int n, const void* ckpt, int every, void* work,
                           float* gdelta, float* gbeta, hipStream_t s);
+int batch_create(rsim_batch** out, const rsim_batch_instance* inst, int count, int first,
+                 int n, int flags, hipStream_t s);
+int batch_run(rsim_batch* b, hipStream_t s);
+int batch_info(const rsim_batch* b, int* groups, int* kernels);
+int batch_destroy(rsim_batch* b);
 int rsim_release_tables();
 int release_tables();
 int release_multislice_tables();
@@ -141,8 +146,13 @@ hipEvent_t kt_event() {
 
 bool ktime_on() { return g_kt_on; }
 
+namespace {
+thread_local bool t_kt_suspended = false;
+}
+void ktime_suspend(bool on) { t_kt_suspended = on; }
+
 void ktime_begin(int kind, hipStream_t s) {
-  if (!g_kt_on) return;
+  if (!g_kt_on || t_kt_suspended) return;
   std::lock_guard<std::mutex> lk(g_kt_mu);
   // sample one launch in g_kt_every of each kind: an event pair costs a few
   // microseconds of dispatch gap, which would skew the run it measures
@@ -154,7 +164,7 @@ void ktime_begin(int kind, hipStream_t s) {
 }
 
 void ktime_end(int kind, hipStream_t s) {
-  if (!g_kt_on) return;
+  if (!g_kt_on || t_kt_suspended) return;
   std::lock_guard<std::mutex> lk(g_kt_mu);
   if (!g_kt_has_open[kind]) return;
   hipEventRecord(g_kt_open[kind].b, s);
@@ -385,7 +395,9 @@ static int volume_alloc(rsim_volume** vol, int nz, int ny_g, int nx_g, const dou
     return fail(RSIM_E_INVALID_ARG, "deltabeta table must hold 1..256 entries (got %d)", n_db);
   if (db[0] != 0.0 || db[1] != 0.0)
     return fail(RSIM_E_INVALID_ARG, "deltabeta entry 0 (vacuum) must be 0");
+  static std::atomic<unsigned long long> serial{0};
   rsim_volume* v = new rsim_volume();
+  v->serial = ++serial;
   v->device = current_device();
   v->nz = nz;
   v->ny_g = ny_g;
@@ -621,6 +633,43 @@ int rsim_multislice_db_adjoint_2d(void* g, void* U, int nx, int ny, int dtype, d
   return multislice_db_adjoint(g, U, nx, ny, dtype, dx, dy, wl, cutoff_freq, delta, beta, pz,
                                n_slices, ckpt, ckpt_every, work, grad_delta, grad_beta,
                                resolve_stream(stream));
+}
+
+int rsim_batch_create(rsim_batch** batch, const rsim_batch_instance* instances, int count,
+                      int first_slice, int n_slices, int flags, void* stream) {
+  if (!batch) return fail(RSIM_E_INVALID_ARG, "batch is NULL");
+  *batch = nullptr;
+  if (!instances || count < 1) return fail(RSIM_E_INVALID_ARG, "a batch needs at least one instance");
+  if (first_slice < 0 || n_slices < 1)
+    return fail(RSIM_E_INVALID_ARG, "first_slice must be >= 0 and n_slices >= 1");
+  for (int i = 0; i < count; ++i) {
+    const rsim_batch_instance& q = instances[i];
+    RSIM_CHECK(check_field(q.u, q.nx, q.ny, RSIM_C64, "instance u"));
+    RSIM_CHECK(check_field(q.U, q.nx, q.ny, RSIM_C64, "instance U"));
+    if (!q.vol) return fail(RSIM_E_INVALID_ARG, "instance %d: vol is NULL", i);
+    if (first_slice + n_slices > q.vol->nz)
+      return fail(RSIM_E_INVALID_ARG, "instance %d: slices [%d, %d) exceed the volume depth %d", i,
+                  first_slice, first_slice + n_slices, q.vol->nz);
+    if (!(q.dx > 0) || !(q.dy > 0) || !(q.wl > 0) || !(q.pixel_size_x > 0) ||
+        !(q.pixel_size_y > 0) || !(q.pixel_size_z > 0))
+      return fail(RSIM_E_INVALID_ARG, "instance %d: spacings and wl must be positive", i);
+  }
+  return batch_create(batch, instances, count, first_slice, n_slices, flags, resolve_stream(stream));
+}
+
+int rsim_batch_run(rsim_batch* batch, void* stream) {
+  if (!batch) return fail(RSIM_E_INVALID_ARG, "batch is NULL");
+  return batch_run(batch, resolve_stream(stream));
+}
+
+int rsim_batch_info(const rsim_batch* batch, int* groups, int* kernels) {
+  if (!batch) return fail(RSIM_E_INVALID_ARG, "batch is NULL");
+  return batch_info(batch, groups, kernels);
+}
+
+int rsim_batch_destroy(rsim_batch* batch) {
+  if (!batch) return RSIM_OK;
+  return batch_destroy(batch);
 }
 
 }  // extern "C"

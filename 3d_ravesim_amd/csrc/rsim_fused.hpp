@@ -91,13 +91,40 @@ __device__ __forceinline__ RowT row_state(const TransArgsF& a, int y) {
 
 enum { ROW_FIRST = 0, ROW_MID = 1, ROW_LAST = 2 };
 
+// Per-instance launch arguments.  A launch covers up to kMaxBatch fields of
+// one shape (grid.y = instance); the descriptors travel by value in the
+// kernel arguments, so a captured graph keeps each slice's pointers.
+constexpr int kMaxBatch = 16;
+struct RowInst {
+  const cpx<float>* in;
+  cpx<float>* out;
+  TransArgsF ta;   // ta.ids: this launch's voxel slice
+};
+struct RowBatch {
+  RowInst inst[kMaxBatch];
+};
+struct ColInst {
+  const cpx<float>* in;
+  cpx<float>* out;
+  cpx<float>* spec;
+  const double2* Bx;
+  const double2* Ay;
+  const int* Rmax;
+};
+struct ColBatch {
+  ColInst inst[kMaxBatch];
+};
+
 template <class F, int C, int PS, bool SPLIT, int MODE, bool LUT, typename TW>
-__global__ void __launch_bounds__(F::TC * C) ms_row_kernel(const cpx<float>* in, cpx<float>* out,
-                                                           int ny, uint32_t nbytes,
-                                                           StageTw<TW> tw, TransArgsF ta) {
+__global__ void __launch_bounds__(F::TC * C) ms_row_kernel(RowBatch rb, int ny, uint32_t nbytes,
+                                                           StageTw<TW> tw) {
   constexpr int N = F::N;
   __shared__ unsigned char lds[LdsSize<F, C, PS>::slots * (SPLIT ? 4 : 8)];
   __shared__ float4 lds_lut[(MODE != ROW_LAST && LUT) ? 256 : 1];
+  const RowInst& I = rb.inst[blockIdx.y];
+  const cpx<float>* in = I.in;
+  cpx<float>* out = I.out;
+  const TransArgsF& ta = I.ta;
   const int tid = threadIdx.x;
   const int c = tid % C;
   const int t = tid / C;
@@ -139,15 +166,17 @@ __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(const cpx<float>* in,
 }
 
 template <class F, int C, int PS, bool SPLIT, int SPEC, typename TW>
-__global__ void __launch_bounds__(F::TC * C) ms_col_kernel(const cpx<float>* in, cpx<float>* out,
-                                                           cpx<float>* spec, int nx,
-                                                           uint32_t nbytes, StageTw<TW> tw,
-                                                           const double2* __restrict__ Bx,
-                                                           const double2* __restrict__ Ay,
-                                                           const int* __restrict__ Rmax,
-                                                           float spec_scale) {
+__global__ void __launch_bounds__(F::TC * C) ms_col_kernel(ColBatch cb, int nx, uint32_t nbytes,
+                                                           StageTw<TW> tw, float spec_scale) {
   constexpr int N = F::N;
   __shared__ unsigned char lds[LdsSize<F, C, PS>::slots * (SPLIT ? 4 : 8)];
+  const ColInst& I = cb.inst[blockIdx.y];
+  const cpx<float>* in = I.in;
+  cpx<float>* out = I.out;
+  cpx<float>* spec = I.spec;
+  const double2* __restrict__ Bx = I.Bx;
+  const double2* __restrict__ Ay = I.Ay;
+  const int* __restrict__ Rmax = I.Rmax;
   const int tid = threadIdx.x;
   const int c = tid % C;
   const int t = tid / C;
@@ -215,40 +244,34 @@ struct Shape {
 
 template <class S, typename TW>
 struct RowLaunch {
-  static void go(int mode, bool lut, const void* in, void* out, int ny, uint32_t nb,
-                 const StageTw<TW>& tw, const TransArgsF& ta, hipStream_t s) {
+  static void go(int mode, bool lut, const RowBatch& b, int count, int ny, uint32_t nb,
+                 const StageTw<TW>& tw, hipStream_t s) {
     using F = typename S::F;
     constexpr int C = S::CC;
-    dim3 grid((ny + C - 1) / C), block(S::threads);
-    const cpx<float>* i = (const cpx<float>*)in;
-    cpx<float>* o = (cpx<float>*)out;
+    dim3 grid((ny + C - 1) / C, count), block(S::threads);
     if (mode == ROW_FIRST) {
-      if (lut) hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_FIRST, true, TW>), grid, block, 0, s, i, o, ny, nb, tw, ta);
-      else hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_FIRST, false, TW>), grid, block, 0, s, i, o, ny, nb, tw, ta);
+      if (lut) hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_FIRST, true, TW>), grid, block, 0, s, b, ny, nb, tw);
+      else hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_FIRST, false, TW>), grid, block, 0, s, b, ny, nb, tw);
     } else if (mode == ROW_MID) {
-      if (lut) hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_MID, true, TW>), grid, block, 0, s, i, o, ny, nb, tw, ta);
-      else hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_MID, false, TW>), grid, block, 0, s, i, o, ny, nb, tw, ta);
+      if (lut) hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_MID, true, TW>), grid, block, 0, s, b, ny, nb, tw);
+      else hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_MID, false, TW>), grid, block, 0, s, b, ny, nb, tw);
     } else {
-      hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_LAST, true, TW>), grid, block, 0, s, i, o, ny, nb, tw, ta);
+      hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_LAST, true, TW>), grid, block, 0, s, b, ny, nb, tw);
     }
   }
 };
 
 template <class S, typename TW>
 struct ColLaunch {
-  static void go(bool spec_out, const void* in, void* out, void* spec, int nx, uint32_t nb,
-                 const StageTw<TW>& tw, const double2* Bx, const double2* Ay, const int* Rmax,
-                 float spec_scale, hipStream_t s) {
+  static void go(bool spec_out, const ColBatch& b, int count, int nx, uint32_t nb,
+                 const StageTw<TW>& tw, float spec_scale, hipStream_t s) {
     using F = typename S::F;
     constexpr int C = S::CC;
-    dim3 grid((nx + C - 1) / C), block(S::threads);
-    const cpx<float>* i = (const cpx<float>*)in;
-    cpx<float>* o = (cpx<float>*)out;
-    cpx<float>* sp = (cpx<float>*)spec;
+    dim3 grid((nx + C - 1) / C, count), block(S::threads);
     if (spec_out)
-      hipLaunchKernelGGL((ms_col_kernel<F, C, S::PS, S::SPLIT, 1, TW>), grid, block, 0, s, i, o, sp, nx, nb, tw, Bx, Ay, Rmax, spec_scale);
+      hipLaunchKernelGGL((ms_col_kernel<F, C, S::PS, S::SPLIT, 1, TW>), grid, block, 0, s, b, nx, nb, tw, spec_scale);
     else
-      hipLaunchKernelGGL((ms_col_kernel<F, C, S::PS, S::SPLIT, 0, TW>), grid, block, 0, s, i, o, sp, nx, nb, tw, Bx, Ay, Rmax, spec_scale);
+      hipLaunchKernelGGL((ms_col_kernel<F, C, S::PS, S::SPLIT, 0, TW>), grid, block, 0, s, b, nx, nb, tw, spec_scale);
   }
 };
 
@@ -278,78 +301,74 @@ template <int N> constexpr int vColC(int E, int t) { return vmin(vmin(vmax1(t / 
 constexpr int kRowVariants = 4;
 constexpr int kColVariants = 5;
 
-// Per-size launchers (explicitly instantiated in rsim_fused_<N>.hip).
+// Per-size launchers (explicitly instantiated in rsim_fused_<N>.hip); count
+// instances (1..kMaxBatch) of shape (N, ny) or (nx, N) in one launch.
 template <int N, typename TW>
-bool row_go(int v, int mode, bool lut, const void* in, void* out, int ny, const StageTw<TW>& tw,
-            const TransArgsF& ta, hipStream_t s);
+bool row_go(int v, int mode, bool lut, const RowBatch& b, int count, int ny,
+            const StageTw<TW>& tw, hipStream_t s);
 template <int N, typename TW>
-bool col_go(int v, bool spec_out, const void* in, void* out, void* spec, int nx,
-            const StageTw<TW>& tw, const double2* Bx, const double2* Ay, const int* Rmax,
+bool col_go(int v, bool spec_out, const ColBatch& b, int count, int nx, const StageTw<TW>& tw,
             float ss, hipStream_t s);
 
-#define RSIM_ROW_CASE(N, V, E, C, SP)                                               \
-  case V:                                                                          \
-    RowLaunch<Shape<N, E, C, SP>, TW>::go(mode, lut, in, out, ny, nb, tw, ta, s);  \
+#define RSIM_ROW_CASE(N, V, E, C, SP)                                          \
+  case V:                                                                     \
+    RowLaunch<Shape<N, E, C, SP>, TW>::go(mode, lut, b, count, ny, nb, tw, s); \
     return true;
-#define RSIM_COL_CASE(N, V, E, C, SP)                                                                 \
-  case V:                                                                                             \
-    ColLaunch<Shape<N, E, C, SP>, TW>::go(spec_out, in, out, spec, nx, nb, tw, Bx, Ay, Rmax, ss, s);  \
+#define RSIM_COL_CASE(N, V, E, C, SP)                                              \
+  case V:                                                                         \
+    ColLaunch<Shape<N, E, C, SP>, TW>::go(spec_out, b, count, nx, nb, tw, ss, s); \
     return true;
 
 #define RSIM_FUSED_INSTANTIATE_ROWS(N)                                                            \
   template <int NN, typename TW>                                                                  \
-  bool row_go_impl(int v, int mode, bool lut, const void* in, void* out, int ny,                  \
-                   const StageTw<TW>& tw, const TransArgsF& ta, hipStream_t s) {                  \
-    const uint32_t nb = (uint32_t)((long long)NN * ny * 8);                                      \
+  bool row_go_impl(int v, int mode, bool lut, const RowBatch& b, int count, int ny,               \
+                   const StageTw<TW>& tw, hipStream_t s) {                                        \
+    const uint32_t nb = (uint32_t)((long long)NN * ny * 8);                                       \
     switch (v) {                                                                                  \
       RSIM_ROW_VARIANTS(NN, RSIM_ROW_CASE)                                                        \
       default: return false;                                                                      \
     }                                                                                             \
   }                                                                                               \
   template <>                                                                                     \
-  bool row_go<N, float>(int v, int mode, bool lut, const void* in, void* out, int ny,             \
-                        const StageTw<float>& tw, const TransArgsF& ta, hipStream_t s) {          \
-    return row_go_impl<N, float>(v, mode, lut, in, out, ny, tw, ta, s);                           \
+  bool row_go<N, float>(int v, int mode, bool lut, const RowBatch& b, int count, int ny,          \
+                        const StageTw<float>& tw, hipStream_t s) {                                \
+    return row_go_impl<N, float>(v, mode, lut, b, count, ny, tw, s);                              \
   }                                                                                               \
   template <>                                                                                     \
-  bool row_go<N, double>(int v, int mode, bool lut, const void* in, void* out, int ny,            \
-                         const StageTw<double>& tw, const TransArgsF& ta, hipStream_t s) {        \
-    return row_go_impl<N, double>(v, mode, lut, in, out, ny, tw, ta, s);                          \
+  bool row_go<N, double>(int v, int mode, bool lut, const RowBatch& b, int count, int ny,         \
+                         const StageTw<double>& tw, hipStream_t s) {                              \
+    return row_go_impl<N, double>(v, mode, lut, b, count, ny, tw, s);                             \
   }                                                                                               \
   template <>                                                                                     \
-  bool row_go<N, twc>(int v, int mode, bool lut, const void* in, void* out, int ny,               \
-                      const StageTw<twc>& tw, const TransArgsF& ta, hipStream_t s) {              \
-    return row_go_impl<N, twc>(v, mode, lut, in, out, ny, tw, ta, s);                             \
+  bool row_go<N, twc>(int v, int mode, bool lut, const RowBatch& b, int count, int ny,            \
+                      const StageTw<twc>& tw, hipStream_t s) {                                    \
+    return row_go_impl<N, twc>(v, mode, lut, b, count, ny, tw, s);                                \
   }
 
 #define RSIM_FUSED_INSTANTIATE_COLS(N)                                                            \
   template <int NN, typename TW>                                                                  \
-  bool col_go_impl(int v, bool spec_out, const void* in, void* out, void* spec, int nx,           \
-                   const StageTw<TW>& tw, const double2* Bx, const double2* Ay, const int* Rmax,  \
-                   float ss, hipStream_t s) {                                                     \
-    const uint32_t nb = (uint32_t)((long long)NN * nx * 8);                                      \
+  bool col_go_impl(int v, bool spec_out, const ColBatch& b, int count, int nx,                    \
+                   const StageTw<TW>& tw, float ss, hipStream_t s) {                              \
+    const uint32_t nb = (uint32_t)((long long)NN * nx * 8);                                       \
     switch (v) {                                                                                  \
       RSIM_COL_VARIANTS(NN, RSIM_COL_CASE)                                                        \
       default: return false;                                                                      \
     }                                                                                             \
   }                                                                                               \
   template <>                                                                                     \
-  bool col_go<N, float>(int v, bool spec_out, const void* in, void* out, void* spec, int nx,      \
-                        const StageTw<float>& tw, const double2* Bx, const double2* Ay,           \
-                        const int* Rmax, float ss, hipStream_t s) {                               \
-    return col_go_impl<N, float>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);        \
+  bool col_go<N, float>(int v, bool spec_out, const ColBatch& b, int count, int nx,               \
+                        const StageTw<float>& tw, float ss, hipStream_t s) {                      \
+    return col_go_impl<N, float>(v, spec_out, b, count, nx, tw, ss, s);                           \
   }                                                                                               \
   template <>                                                                                     \
-  bool col_go<N, double>(int v, bool spec_out, const void* in, void* out, void* spec, int nx,     \
-                         const StageTw<double>& tw, const double2* Bx, const double2* Ay,         \
-                         const int* Rmax, float ss, hipStream_t s) {                              \
-    return col_go_impl<N, double>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);       \
+  bool col_go<N, double>(int v, bool spec_out, const ColBatch& b, int count, int nx,              \
+                         const StageTw<double>& tw, float ss, hipStream_t s) {                    \
+    return col_go_impl<N, double>(v, spec_out, b, count, nx, tw, ss, s);                          \
   }                                                                                               \
   template <>                                                                                     \
-  bool col_go<N, twc>(int v, bool spec_out, const void* in, void* out, void* spec, int nx,        \
-                      const StageTw<twc>& tw, const double2* Bx, const double2* Ay,               \
-                      const int* Rmax, float ss, hipStream_t s) {                                 \
-    return col_go_impl<N, twc>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);          \
+  bool col_go<N, twc>(int v, bool spec_out, const ColBatch& b, int count, int nx,                 \
+                      const StageTw<twc>& tw, float ss, hipStream_t s) {                          \
+    return col_go_impl<N, twc>(v, spec_out, b, count, nx, tw, ss, s);                             \
   }
 
 #define RSIM_FUSED_INSTANTIATE(N) RSIM_FUSED_INSTANTIATE_ROWS(N) RSIM_FUSED_INSTANTIATE_COLS(N)

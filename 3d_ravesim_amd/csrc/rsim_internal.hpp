@@ -75,6 +75,8 @@ enum { KT_ROW = 0, KT_COL = 1, KT_PW = 2 };
 bool ktime_on();
 void ktime_begin(int kind, hipStream_t s);
 void ktime_end(int kind, hipStream_t s);
+// suspend kernel timing on this thread (while a stream is being captured)
+void ktime_suspend(bool on);
 
 // ---- cancel / progress ----
 bool cancel_requested();
@@ -86,6 +88,7 @@ void report_progress(const char* stage, long long completed, long long total);
 }  // namespace rsim
 
 struct rsim_volume {
+  unsigned long long serial;   // unique per created volume (cache key; addresses are reused)
   int device;
   int nz, ny_g, nx_g;
   uint8_t* ids;      // device, nz*ny_g*nx_g

@@ -27,6 +27,15 @@
 #include "rsim_tables.hpp"
 #include "rsim_fused.hpp"
 
+// Captured graph of a batched run.
+struct rsim_batch {
+  int device = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int groups = 0;
+  int kernels = 0;
+};
+
 namespace rsim {
 
 int transmission(void* u, int nx, int ny, int dtype, double dx, double dy, double wl,
@@ -36,37 +45,57 @@ int transmission(void* u, int nx, int ny, int dtype, double dx, double dy, doubl
 namespace {
 
 template <typename TW>
-bool fused_row(int N, int v, int mode, bool lut, const void* in, void* out, int ny,
-               const StageTw<TW>& tw, const TransArgsF& ta, hipStream_t s) {
+bool fused_row_b(int N, int v, int mode, bool lut, const RowBatch& b, int count, int ny,
+                 const StageTw<TW>& tw, hipStream_t s) {
   switch (N) {
-    case 256: return row_go<256, TW>(v, mode, lut, in, out, ny, tw, ta, s);
-    case 512: return row_go<512, TW>(v, mode, lut, in, out, ny, tw, ta, s);
-    case 1024: return row_go<1024, TW>(v, mode, lut, in, out, ny, tw, ta, s);
-    case 2048: return row_go<2048, TW>(v, mode, lut, in, out, ny, tw, ta, s);
-    case 4096: return row_go<4096, TW>(v, mode, lut, in, out, ny, tw, ta, s);
-    case 8192: return row_go<8192, TW>(v, mode, lut, in, out, ny, tw, ta, s);
-    case 3072: return row_go<3072, TW>(v, mode, lut, in, out, ny, tw, ta, s);
-    case 6144: return row_go<6144, TW>(v, mode, lut, in, out, ny, tw, ta, s);
-    case 16384: return row_go<16384, TW>(v, mode, lut, in, out, ny, tw, ta, s);
+    case 256: return row_go<256, TW>(v, mode, lut, b, count, ny, tw, s);
+    case 512: return row_go<512, TW>(v, mode, lut, b, count, ny, tw, s);
+    case 1024: return row_go<1024, TW>(v, mode, lut, b, count, ny, tw, s);
+    case 2048: return row_go<2048, TW>(v, mode, lut, b, count, ny, tw, s);
+    case 3072: return row_go<3072, TW>(v, mode, lut, b, count, ny, tw, s);
+    case 4096: return row_go<4096, TW>(v, mode, lut, b, count, ny, tw, s);
+    case 6144: return row_go<6144, TW>(v, mode, lut, b, count, ny, tw, s);
+    case 8192: return row_go<8192, TW>(v, mode, lut, b, count, ny, tw, s);
+    case 16384: return row_go<16384, TW>(v, mode, lut, b, count, ny, tw, s);
     default: return false;
   }
+}
+
+template <typename TW>
+bool fused_col_b(int N, int v, bool spec_out, const ColBatch& b, int count, int nx,
+                 const StageTw<TW>& tw, float ss, hipStream_t s) {
+  switch (N) {
+    case 256: return col_go<256, TW>(v, spec_out, b, count, nx, tw, ss, s);
+    case 512: return col_go<512, TW>(v, spec_out, b, count, nx, tw, ss, s);
+    case 1024: return col_go<1024, TW>(v, spec_out, b, count, nx, tw, ss, s);
+    case 2048: return col_go<2048, TW>(v, spec_out, b, count, nx, tw, ss, s);
+    case 3072: return col_go<3072, TW>(v, spec_out, b, count, nx, tw, ss, s);
+    case 4096: return col_go<4096, TW>(v, spec_out, b, count, nx, tw, ss, s);
+    case 6144: return col_go<6144, TW>(v, spec_out, b, count, nx, tw, ss, s);
+    case 8192: return col_go<8192, TW>(v, spec_out, b, count, nx, tw, ss, s);
+    default: return false;
+  }
+}
+
+// single-field launches
+template <typename TW>
+bool fused_row(int N, int v, int mode, bool lut, const void* in, void* out, int ny,
+               const StageTw<TW>& tw, const TransArgsF& ta, hipStream_t s) {
+  RowBatch b{};
+  b.inst[0].in = reinterpret_cast<const cpx<float>*>(in);
+  b.inst[0].out = reinterpret_cast<cpx<float>*>(out);
+  b.inst[0].ta = ta;
+  return fused_row_b<TW>(N, v, mode, lut, b, 1, ny, tw, s);
 }
 
 template <typename TW>
 bool fused_col(int N, int v, bool spec_out, const void* in, void* out, void* spec, int nx,
                const StageTw<TW>& tw, const double2* Bx, const double2* Ay, const int* Rmax,
                float ss, hipStream_t s) {
-  switch (N) {
-    case 256: return col_go<256, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
-    case 512: return col_go<512, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
-    case 1024: return col_go<1024, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
-    case 2048: return col_go<2048, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
-    case 4096: return col_go<4096, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
-    case 8192: return col_go<8192, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
-    case 3072: return col_go<3072, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
-    case 6144: return col_go<6144, TW>(v, spec_out, in, out, spec, nx, tw, Bx, Ay, Rmax, ss, s);
-    default: return false;
-  }
+  ColBatch b{};
+  b.inst[0] = ColInst{reinterpret_cast<const cpx<float>*>(in), reinterpret_cast<cpx<float>*>(out),
+                      reinterpret_cast<cpx<float>*>(spec), Bx, Ay, Rmax};
+  return fused_col_b<TW>(N, v, spec_out, b, 1, nx, tw, ss, s);
 }
 
 // FP64 chirp tables of the fused column pass.
@@ -156,12 +185,22 @@ int chirp_tables(int nx, int ny, double dx, double dy, double wl, double dz, dou
 struct MatBuf {
   void* dev = nullptr;
 };
-std::map<int, MatBuf> g_mat;
+// one table per (device, volume, K): runs in flight on other streams, and the
+// instances of a batch, each keep their own
+std::map<std::tuple<int, unsigned long long, uint64_t>, MatBuf> g_mat;
 
 int material_tables(const rsim_volume* vol, double K, const float4** lut, const float2** dbf) {
   std::lock_guard<std::mutex> lk(g_mu);
-  MatBuf& mb = g_mat[current_device()];
-  if (!mb.dev) RSIM_HIP(hipMalloc(&mb.dev, 256 * (16 + 8)));
+  const auto key = std::make_tuple(current_device(), vol->serial, bits(K));
+  auto it = g_mat.find(key);
+  if (it != g_mat.end()) {
+    *lut = (const float4*)it->second.dev;
+    *dbf = (const float2*)((char*)it->second.dev + 256 * 16);
+    return RSIM_OK;
+  }
+  MatBuf mb;
+  RSIM_HIP(hipMalloc(&mb.dev, 256 * (16 + 8)));
+  g_mat[key] = mb;
   std::vector<float4> l(256);
   std::vector<float2> d(256);
   for (int m = 0; m < 256; ++m) {
@@ -341,6 +380,59 @@ int propagate(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
   return propagate_plain(u, U, nx, ny, dtype, dx, dy, wl, dz, freq, skip_fft, s);
 }
 
+// Tables and transmission arguments of one fused run (all cached; built
+// before any launch, and before graph capture in the batched form).
+template <typename TW>
+struct FusedPrep {
+  StageTw<TW> tx{}, ty{};
+  const ChirpTables* ct = nullptr;
+  TransArgsF ta{};
+  bool lut = false;
+  size_t slice_bytes = 0;
+  const uint8_t* ids0 = nullptr;
+};
+
+template <typename TW>
+static int fused_prepare(int nx, int ny, double dx, double dy, double wl, double freq,
+                         const rsim_volume* vol, double px, double py, double pz, double xpos,
+                         double ypos, FusedPrep<TW>* P) {
+  const int tdtype = std::is_same<TW, twc>::value ? kTwComp : sizeof(TW) == 8 ? RSIM_C128 : RSIM_C64;
+  const void* twx[4];
+  const void* twy[4];
+  if (!reg_stage_tables(tdtype, nx, twx) || !reg_stage_tables(tdtype, ny, twy))
+    return fail(RSIM_E_HIP, "missing FFT schedule tables");
+  using TP = typename std::remove_reference<decltype(P->tx.t[0])>::type;
+  for (int i = 0; i < 4; ++i) {
+    P->tx.t[i] = reinterpret_cast<TP>(twx[i]);
+    P->ty.t[i] = reinterpret_cast<TP>(twy[i]);
+  }
+  RSIM_CHECK(chirp_tables(nx, ny, dx, dy, wl, pz, freq, &P->ct));
+  const AxisMap* xm = nullptr;
+  const AxisMap* ym = nullptr;
+  RSIM_CHECK(axis_map(nx, dx, 1.0, xpos, vol->nx_g, px, &xm));
+  RSIM_CHECK(axis_map(ny, dy, 1.0, ypos, vol->ny_g, py, &ym));
+  const double K = ((2.0 * M_PI) * pz) / wl;
+  TransArgsF& ta = P->ta;
+  RSIM_CHECK(material_tables(vol, K, &ta.lut, &ta.dbf));
+  ta.nxg = vol->nx_g;
+  ta.xm = xm->dev;
+  ta.ym = ym->dev;
+  ta.K = (float)K;
+  P->lut = xm->all_integral && ym->all_integral && xm->affine && ym->affine;
+  ta.ax = xm->a;
+  ta.cx = xm->c;
+  ta.xlo = xm->lo;
+  ta.xhi = xm->hi;
+  ta.ay = ym->a;
+  ta.cy = ym->c;
+  ta.ylo = ym->lo;
+  ta.yhi = ym->hi;
+  P->slice_bytes = (size_t)vol->ny_g * vol->nx_g;
+  P->ids0 = vol->ids;
+  ta.ids = vol->ids;
+  return RSIM_OK;
+}
+
 template <typename TW>
 static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, double wl,
                      double freq, const rsim_volume* vol, double px, double py, double pz,
@@ -348,16 +440,14 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
                      hipStream_t s) {
   const int total = count;
   const int tdtype = std::is_same<TW, twc>::value ? kTwComp : sizeof(TW) == 8 ? RSIM_C128 : RSIM_C64;
-  const void* twx[4];
-  const void* twy[4];
-  if (!reg_stage_tables(tdtype, nx, twx) || !reg_stage_tables(tdtype, ny, twy))
-    return fail(RSIM_E_HIP, "missing FFT schedule tables");
-  StageTw<TW> tx, ty;
-  using TP = typename std::remove_reference<decltype(tx.t[0])>::type;
-  for (int i = 0; i < 4; ++i) {
-    tx.t[i] = reinterpret_cast<TP>(twx[i]);
-    ty.t[i] = reinterpret_cast<TP>(twy[i]);
-  }
+  FusedPrep<TW> P;
+  RSIM_CHECK(fused_prepare<TW>(nx, ny, dx, dy, wl, freq, vol, px, py, pz, xpos, ypos, &P));
+  const StageTw<TW>& tx = P.tx;
+  const StageTw<TW>& ty = P.ty;
+  const ChirpTables* ct = P.ct;
+  TransArgsF ta = P.ta;
+  const bool lut = P.lut;
+  const size_t slice_bytes = P.slice_bytes;
   // three-pass columns: 128-point sub-transform tables and the W_ny table
   const bool big = ny == kBigCol;
   StageTw<TW> t128{};
@@ -371,30 +461,7 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
         fft_full_table(std::is_same<TW, double>::value ? RSIM_C128 : kTwComp, ny));
     if (!wN) return fail(RSIM_E_HIP, "missing FFT twiddle table for %d", ny);
   }
-  const ChirpTables* ct = nullptr;
-  RSIM_CHECK(chirp_tables(nx, ny, dx, dy, wl, pz, freq, &ct));
-  const AxisMap* xm = nullptr;
-  const AxisMap* ym = nullptr;
-  RSIM_CHECK(axis_map(nx, dx, 1.0, xpos, vol->nx_g, px, &xm));
-  RSIM_CHECK(axis_map(ny, dy, 1.0, ypos, vol->ny_g, py, &ym));
-  const double K = ((2.0 * M_PI) * pz) / wl;
-  TransArgsF ta;
-  RSIM_CHECK(material_tables(vol, K, &ta.lut, &ta.dbf));
-  ta.nxg = vol->nx_g;
-  ta.xm = xm->dev;
-  ta.ym = ym->dev;
-  ta.K = (float)K;
-  const bool lut = xm->all_integral && ym->all_integral && xm->affine && ym->affine;
-  ta.ax = xm->a;
-  ta.cx = xm->c;
-  ta.xlo = xm->lo;
-  ta.xhi = xm->hi;
-  ta.ay = ym->a;
-  ta.cy = ym->c;
-  ta.ylo = ym->lo;
-  ta.yhi = ym->hi;
   const float spec_scale = (float)((double)nx * (double)ny);
-  const size_t slice_bytes = (size_t)vol->ny_g * vol->nx_g;
   ta.ids = vol->ids + (size_t)first * slice_bytes;
   int rv = 0, cv = 0;
   RSIM_CHECK(fused_schedule<TW>(nx, ny, lut, tx, ty, ta, ct, s, &rv, &cv));
@@ -471,6 +538,170 @@ int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double 
                              count, want_spec, s);
   return run_fused<float>(u, U, nx, ny, dx, dy, wl, freq, vol, px, py, pz, xpos, ypos, first,
                           count, want_spec, s);
+}
+
+// ---------------------------------------------------------------------------
+// Batched fused multislice: instances grouped by shape into launches of up to
+// kMaxBatch fields, the whole slice loop captured as one hipGraph.
+// ---------------------------------------------------------------------------
+template <typename TW>
+static int batch_capture(const rsim_batch_instance* in, int count, int first, int n,
+                         bool want_spec, hipStream_t cs, hipGraph_t* graph, int* ngroups,
+                         int* nkern) {
+  std::vector<FusedPrep<TW>> prep(count);
+  for (int i = 0; i < count; ++i) {
+    const rsim_batch_instance& q = in[i];
+    if (!fused_supported(q.nx, q.ny, RSIM_C64) || q.ny == kBigCol)
+      return fail(RSIM_E_UNSUPPORTED, "instance %d: %dx%d has no batched fused schedule", i, q.nx,
+                  q.ny);
+    RSIM_CHECK(fused_prepare<TW>(q.nx, q.ny, q.dx, q.dy, q.wl, q.cutoff_freq, q.vol,
+                                 q.pixel_size_x, q.pixel_size_y, q.pixel_size_z, q.x_pos,
+                                 q.y_pos, &prep[i]));
+  }
+  struct Group {
+    int nx, ny;
+    bool lut;
+    int rv, cv;
+    std::vector<int> idx;
+  };
+  std::vector<Group> groups;
+  for (int i = 0; i < count; ++i) {
+    int gi = -1;
+    for (size_t g = 0; g < groups.size(); ++g)
+      if (groups[g].nx == in[i].nx && groups[g].ny == in[i].ny && groups[g].lut == prep[i].lut)
+        gi = (int)g;
+    if (gi < 0) {
+      groups.push_back(Group{in[i].nx, in[i].ny, prep[i].lut, 0, 0, {}});
+      gi = (int)groups.size() - 1;
+    }
+    groups[gi].idx.push_back(i);
+  }
+  // launch shapes: autotuned (scratch allocation and timing) before capture
+  for (auto& g : groups) {
+    const FusedPrep<TW>& P = prep[g.idx[0]];
+    RSIM_CHECK(fused_schedule<TW>(g.nx, g.ny, g.lut, P.tx, P.ty, P.ta, P.ct, cs, &g.rv, &g.cv));
+  }
+  RSIM_HIP(hipStreamSynchronize(cs));
+
+  ktime_suspend(true);
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    ktime_suspend(false);
+    return fail(RSIM_E_HIP, "hipStreamBeginCapture failed: %s", hipGetErrorString(e));
+  }
+  int kern = 0;
+  bool ok = true;
+  auto rows = [&](int k, int mode) {
+    for (auto& g : groups)
+      for (size_t c0 = 0; c0 < g.idx.size() && ok; c0 += kMaxBatch) {
+        const int cnt = (int)std::min<size_t>(kMaxBatch, g.idx.size() - c0);
+        RowBatch b{};
+        for (int j = 0; j < cnt; ++j) {
+          const int i = g.idx[c0 + j];
+          const FusedPrep<TW>& P = prep[i];
+          const void* src = mode == ROW_MID ? in[i].U : in[i].u;
+          void* dst = mode == ROW_LAST ? in[i].u : in[i].U;
+          b.inst[j].in = reinterpret_cast<const cpx<float>*>(src);
+          b.inst[j].out = reinterpret_cast<cpx<float>*>(dst);
+          b.inst[j].ta = P.ta;
+          b.inst[j].ta.ids = P.ids0 + (size_t)(first + k) * P.slice_bytes;
+        }
+        ok = fused_row_b<TW>(g.nx, g.rv, mode, g.lut, b, cnt, g.ny, prep[g.idx[c0]].tx, cs) &&
+             hipGetLastError() == hipSuccess;
+        ++kern;
+      }
+  };
+  auto cols = [&](bool last) {
+    for (auto& g : groups)
+      for (size_t c0 = 0; c0 < g.idx.size() && ok; c0 += kMaxBatch) {
+        const int cnt = (int)std::min<size_t>(kMaxBatch, g.idx.size() - c0);
+        ColBatch b{};
+        for (int j = 0; j < cnt; ++j) {
+          const int i = g.idx[c0 + j];
+          const ChirpTables* ct = prep[i].ct;
+          b.inst[j] = ColInst{reinterpret_cast<const cpx<float>*>(in[i].U),
+                              reinterpret_cast<cpx<float>*>(last ? in[i].u : in[i].U),
+                              reinterpret_cast<cpx<float>*>(in[i].U), ct->Bx, ct->Ay, ct->Rmax};
+        }
+        const float spec_scale = (float)((double)g.nx * (double)g.ny);
+        ok = fused_col_b<TW>(g.ny, g.cv, last && want_spec, b, cnt, g.nx, prep[g.idx[c0]].ty,
+                             spec_scale, cs) &&
+             hipGetLastError() == hipSuccess;
+        ++kern;
+      }
+  };
+  for (int k = 0; k < n && ok; ++k) {
+    rows(k, k == 0 ? ROW_FIRST : ROW_MID);
+    cols(k == n - 1);
+  }
+  if (ok) rows(n - 1, ROW_LAST);
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(cs, &g);
+  ktime_suspend(false);
+  if (!ok || e2 != hipSuccess || !g) {
+    if (g) hipGraphDestroy(g);
+    return fail(RSIM_E_HIP, "batch capture failed: %s", hipGetErrorString(ok ? e2 : hipGetLastError()));
+  }
+  *graph = g;
+  *ngroups = (int)groups.size();
+  *nkern = kern;
+  return RSIM_OK;
+}
+
+int batch_create(rsim_batch** out, const rsim_batch_instance* inst, int count, int first, int n,
+                 int flags, hipStream_t s) {
+  if (!(flags & (RSIM_FLAG_MATH_FP32 | RSIM_FLAG_MATH_FP64 | RSIM_FLAG_MATH_COMP32)) &&
+      n > kFp32MaxSlices)
+    flags |= RSIM_FLAG_MATH_COMP32;
+  const bool want_spec = !(flags & RSIM_FLAG_NO_SPECTRUM);
+  // the caller's pending work (uploads) completes before tables and tuning
+  RSIM_HIP(hipStreamSynchronize(s));
+  hipStream_t cs = nullptr;
+  RSIM_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipGraph_t graph = nullptr;
+  int ng = 0, nk = 0;
+  int rc;
+  if (flags & RSIM_FLAG_MATH_COMP32)
+    rc = batch_capture<twc>(inst, count, first, n, want_spec, cs, &graph, &ng, &nk);
+  else if (flags & RSIM_FLAG_MATH_FP64)
+    rc = batch_capture<double>(inst, count, first, n, want_spec, cs, &graph, &ng, &nk);
+  else
+    rc = batch_capture<float>(inst, count, first, n, want_spec, cs, &graph, &ng, &nk);
+  hipStreamDestroy(cs);
+  if (rc != RSIM_OK) return rc;
+  rsim_batch* b = new rsim_batch();
+  b->device = current_device();
+  b->graph = graph;
+  b->groups = ng;
+  b->kernels = nk;
+  const hipError_t e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    hipGraphDestroy(graph);
+    delete b;
+    return fail(RSIM_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+  }
+  *out = b;
+  return RSIM_OK;
+}
+
+int batch_run(rsim_batch* b, hipStream_t s) {
+  if (current_device() != b->device)
+    return fail(RSIM_E_INVALID_ARG, "batch belongs to device %d", b->device);
+  RSIM_HIP(hipGraphLaunch(b->exec, s));
+  return RSIM_OK;
+}
+
+int batch_info(const rsim_batch* b, int* groups, int* kernels) {
+  if (groups) *groups = b->groups;
+  if (kernels) *kernels = b->kernels;
+  return RSIM_OK;
+}
+
+int batch_destroy(rsim_batch* b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  delete b;
+  return RSIM_OK;
 }
 
 }  // namespace rsim

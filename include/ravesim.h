@@ -162,6 +162,37 @@ int rsim_sample_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx,
                          double x_pos, double y_pos, int first_slice, int n_slices, int engine,
                          int flags, void* stream);
 
+/* ---- Batched fused multislice (no counterpart in the reference: the
+ * reference runs each (source, energy) instance as its own process,
+ * fast-wave/Readme.md:27) ----
+ * One instance = the arguments of one RSIM_FUSED complex64
+ * rsim_sample_apply_2d call.  rsim_batch_create groups the instances by
+ * (nx, ny) into launches of up to 16 fields (grid.y = instance), builds every
+ * table and autotunes each shape, then captures the whole slice loop of all
+ * instances (rows, columns and the final row pass of each slice) as one
+ * hipGraph.  rsim_batch_run replays it with a single graph launch: every
+ * instance's u goes through its n_slices slices and, unless
+ * RSIM_FLAG_NO_SPECTRUM, U = FFT2(u) on return.  Fields are read from and
+ * written to the instances' own u / U buffers, which must stay allocated for
+ * the batch's life.  Sizes: the fused engine's (ny <= 8192).  flags: the
+ * arithmetic flags of rsim_sample_apply_2d (default: FP32 up to 256 slices).
+ * rsim_release_caches invalidates existing batches. */
+typedef struct rsim_batch rsim_batch;
+typedef struct {
+  void* u;
+  void* U;
+  int nx, ny;
+  double dx, dy, wl, cutoff_freq;
+  const rsim_volume* vol;
+  double pixel_size_x, pixel_size_y, pixel_size_z, x_pos, y_pos;
+} rsim_batch_instance;
+int rsim_batch_create(rsim_batch** batch, const rsim_batch_instance* instances, int count,
+                      int first_slice, int n_slices, int flags, void* stream);
+int rsim_batch_run(rsim_batch* batch, void* stream);
+/* Number of shape groups and of kernel nodes in the captured graph. */
+int rsim_batch_info(const rsim_batch* batch, int* groups, int* kernels);
+int rsim_batch_destroy(rsim_batch* batch);
+
 /* Launch-shape autotuner of the fused engine (no counterpart in the
  * reference): the first RSIM_FUSED call for an (nx, ny) times each row-pass
  * and column-pass variant on a scratch field and keeps the fastest; the

@@ -40,6 +40,10 @@ def test_invalid_arguments_map_to_value_error_without_a_device():
     assert rc == lib.RSIM_E_INVALID_ARG
     with pytest.raises(ValueError):
         lib.call("rsim_kernel_timing", -1)
+    b = C.c_void_p()
+    with pytest.raises(ValueError):
+        lib.call("rsim_batch_create", C.byref(b), None, 0, 0, 1, 0, None)
+    assert L.rsim_batch_destroy(None) == 0
     assert L.rsim_kernel_timing(16) == 0 and L.rsim_kernel_timing(0) == 0
 
 
