@@ -14,6 +14,7 @@ from .propagation import (
     apply_frequency_cutoff_2d,
     convert_energy_wavelength,
     detector_shape,
+    fresnel_plane_2d,
     grid_density_check,
     grid_density_check_2d,
     propagate_2d,
@@ -31,14 +32,16 @@ from .optical_element import (
 )
 from .source import PointSource
 from .wavesim import run_simulation
+from .multisim import compute_fresnel_cutoff_geometry, fresnel_mode_name, validate_fresnel_mode
 from . import config
 
 __all__ = [
     "Vector", "DeviceVector", "SimParams", "apply_frequency_cutoff_2d", "convert_energy_wavelength",
-    "detector_shape", "grid_density_check", "grid_density_check_2d", "propagate_2d",
+    "detector_shape", "fresnel_plane_2d", "grid_density_check", "grid_density_check_2d", "propagate_2d",
     "propagate_analytically_2d", "require_row_aligned_chunk_size", "square_and_downsample_2d",
     "DeviceVolume", "Material", "OpticalElement", "Sample", "extract_from_deltabeta_table",
-    "material_factor", "PointSource", "run_simulation", "config",
+    "material_factor", "PointSource", "run_simulation", "compute_fresnel_cutoff_geometry",
+    "fresnel_mode_name", "validate_fresnel_mode", "config",
 ]
 
 

@@ -96,6 +96,11 @@ SIGNATURES = {
                                       _vp, _i, _vp, _vp, _vp, _vp],
     "rsim_detector_area_v1_adjoint": [_vp, _vp, _i, _i, _i, _d, _d, _i, _i, _d, _d, _d, _vp,
                                       _vp],
+    "rsim_sample_apply_2d_scaled": [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _vp, _d, _d, _d, _d,
+                                    _d, _i, _i, _dp, _dp, _i, _i, _vp],
+    "rsim_transmission_2d_scaled": [_vp, _i, _i, _i, _d, _d, _d, _vp, _i, _d, _d, _d, _d, _d,
+                                    _d, _vp],
+    "rsim_tilted_plane_2d": [_vp, _i, _i, _i, _d, _d, _d, _d, _d, _d, _vp],
     "rsim_batch_create": [C.POINTER(_vp), C.POINTER(BatchInstance), _i, _i, _i, _i, _vp],
     "rsim_batch_run": [_vp, _vp],
     "rsim_batch_info": [_vp, _ip, _ip],

@@ -16,7 +16,9 @@ int point_source(void* u, int nx, int ny, int dtype, double dx, double dy, doubl
                  double ys, double wl, hipStream_t s);
 int transmission(void* u, int nx, int ny, int dtype, double dx, double dy, double wl,
                  const rsim_volume* vol, int slice, double px, double py, double pz, double xpos,
-                 double ypos, hipStream_t s);
+                 double ypos, double coord_scale, hipStream_t s);
+int tilted_plane(void* u, int nx, int ny, int dtype, double dx, double dy, double wl, double dz,
+                 double xs, double ys, hipStream_t s);
 int detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double dy, int ox,
                      int oy, double dsx, double dsy, double z, double weight, bool accumulate,
                      double* out_dev, hipStream_t s);
@@ -25,8 +27,8 @@ int propagate(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
               double dz, double freq, bool skip_fft, hipStream_t s);
 int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
                  double freq, const rsim_volume* vol, double px, double py, double pz,
-                 double xpos, double ypos, int first, int count, int engine, int flags,
-                 hipStream_t s);
+                 double xpos, double ypos, int first, int count, const double* scales,
+                 const double* dzs, int engine, int flags, hipStream_t s);
 int detector_area_v1_adjoint(const void* v, void* g, int nx, int ny, int dtype, double dx,
                              double dy, int ox, int oy, double dsx, double dsy, double z,
                              const double* gimg_dev, hipStream_t s);
@@ -509,7 +511,28 @@ int rsim_transmission_2d(void* u, int nx, int ny, int dtype, double dx, double d
   RSIM_CHECK(check_sample(vol, slice, 1, pixel_size_x, pixel_size_y, pixel_size_z));
   if (!(wl > 0)) return fail(RSIM_E_INVALID_ARG, "wl must be positive");
   return transmission(u, nx, ny, dtype, dx, dy, wl, vol, slice, pixel_size_x, pixel_size_y,
-                      pixel_size_z, x_pos, y_pos, resolve_stream(stream));
+                      pixel_size_z, x_pos, y_pos, 1.0, resolve_stream(stream));
+}
+
+int rsim_transmission_2d_scaled(void* u, int nx, int ny, int dtype, double dx, double dy,
+                                double wl, const rsim_volume* vol, int slice, double pixel_size_x,
+                                double pixel_size_y, double pixel_size_z, double x_pos,
+                                double y_pos, double coord_scale, void* stream) {
+  RSIM_CHECK(check_field(u, nx, ny, dtype, "u"));
+  RSIM_CHECK(check_sample(vol, slice, 1, pixel_size_x, pixel_size_y, pixel_size_z));
+  if (!(wl > 0)) return fail(RSIM_E_INVALID_ARG, "wl must be positive");
+  if (!(coord_scale > 0) || !std::isfinite(coord_scale))
+    return fail(RSIM_E_INVALID_ARG, "coord_scale must be positive and finite");
+  return transmission(u, nx, ny, dtype, dx, dy, wl, vol, slice, pixel_size_x, pixel_size_y,
+                      pixel_size_z, x_pos, y_pos, coord_scale, resolve_stream(stream));
+}
+
+int rsim_tilted_plane_2d(void* u, int nx, int ny, int dtype, double dx, double dy, double wl,
+                         double dz, double x_source, double y_source, void* stream) {
+  RSIM_CHECK(check_field(u, nx, ny, dtype, "u"));
+  if (!(wl > 0) || !(dz > 0) || !std::isfinite(dz))
+    return fail(RSIM_E_INVALID_ARG, "Fresnel source-to-sample distance must be positive");
+  return tilted_plane(u, nx, ny, dtype, dx, dy, wl, dz, x_source, y_source, resolve_stream(stream));
 }
 
 int rsim_sample_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
@@ -525,8 +548,32 @@ int rsim_sample_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx,
     return fail(RSIM_E_INVALID_ARG, "engine must be RSIM_PLAIN or RSIM_FUSED");
   if (!(wl > 0) || !(dx > 0) || !(dy > 0)) return fail(RSIM_E_INVALID_ARG, "dx, dy and wl must be positive");
   return sample_apply(u, U, nx, ny, dtype, dx, dy, wl, cutoff_freq, vol, pixel_size_x,
-                      pixel_size_y, pixel_size_z, x_pos, y_pos, first_slice, n_slices, engine,
-                      flags, resolve_stream(stream));
+                      pixel_size_y, pixel_size_z, x_pos, y_pos, first_slice, n_slices, nullptr,
+                      nullptr, engine, flags, resolve_stream(stream));
+}
+
+int rsim_sample_apply_2d_scaled(void* u, void* U, int nx, int ny, int dtype, double dx,
+                                double dy, double wl, double cutoff_freq, const rsim_volume* vol,
+                                double pixel_size_x, double pixel_size_y, double pixel_size_z,
+                                double x_pos, double y_pos, int first_slice, int n_slices,
+                                const double* coord_scale, const double* slice_dz, int engine,
+                                int flags, void* stream) {
+  RSIM_CHECK(check_field(u, nx, ny, dtype, "u"));
+  RSIM_CHECK(check_field(U, nx, ny, dtype, "U"));
+  RSIM_CHECK(check_fft_sizes(nx, ny));
+  RSIM_CHECK(check_sample(vol, first_slice, n_slices, pixel_size_x, pixel_size_y, pixel_size_z));
+  if (engine != RSIM_PLAIN && engine != RSIM_FUSED)
+    return fail(RSIM_E_INVALID_ARG, "engine must be RSIM_PLAIN or RSIM_FUSED");
+  if (!(wl > 0) || !(dx > 0) || !(dy > 0)) return fail(RSIM_E_INVALID_ARG, "dx, dy and wl must be positive");
+  for (int k = 0; k < n_slices; ++k) {
+    if (coord_scale && (!(coord_scale[k] > 0) || !std::isfinite(coord_scale[k])))
+      return fail(RSIM_E_INVALID_ARG, "coord_scale[%d] must be positive and finite", k);
+    if (slice_dz && !std::isfinite(slice_dz[k]))
+      return fail(RSIM_E_INVALID_ARG, "slice_dz[%d] must be finite", k);
+  }
+  return sample_apply(u, U, nx, ny, dtype, dx, dy, wl, cutoff_freq, vol, pixel_size_x,
+                      pixel_size_y, pixel_size_z, x_pos, y_pos, first_slice, n_slices,
+                      coord_scale, slice_dz, engine, flags, resolve_stream(stream));
 }
 
 int rsim_fused_schedule(int nx, int ny, int* row_variant, int* col_variant) {

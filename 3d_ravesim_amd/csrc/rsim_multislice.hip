@@ -40,7 +40,7 @@ namespace rsim {
 
 int transmission(void* u, int nx, int ny, int dtype, double dx, double dy, double wl,
                  const rsim_volume* vol, int slice, double px, double py, double pz, double xpos,
-                 double ypos, hipStream_t s);
+                 double ypos, double coord_scale, hipStream_t s);
 
 namespace {
 
@@ -392,10 +392,14 @@ struct FusedPrep {
   const uint8_t* ids0 = nullptr;
 };
 
+// pz: slice thickness of the transmission; prop_dz: propagation distance of
+// the chirp (pz, or its cone-beam effective value); scale: voxel-map scale.
 template <typename TW>
 static int fused_prepare(int nx, int ny, double dx, double dy, double wl, double freq,
                          const rsim_volume* vol, double px, double py, double pz, double xpos,
-                         double ypos, FusedPrep<TW>* P) {
+                         double ypos, FusedPrep<TW>* P, double prop_dz = 0.0,
+                         double scale = 1.0) {
+  if (prop_dz == 0.0) prop_dz = pz;
   const int tdtype = std::is_same<TW, twc>::value ? kTwComp : sizeof(TW) == 8 ? RSIM_C128 : RSIM_C64;
   const void* twx[4];
   const void* twy[4];
@@ -406,11 +410,11 @@ static int fused_prepare(int nx, int ny, double dx, double dy, double wl, double
     P->tx.t[i] = reinterpret_cast<TP>(twx[i]);
     P->ty.t[i] = reinterpret_cast<TP>(twy[i]);
   }
-  RSIM_CHECK(chirp_tables(nx, ny, dx, dy, wl, pz, freq, &P->ct));
+  RSIM_CHECK(chirp_tables(nx, ny, dx, dy, wl, prop_dz, freq, &P->ct));
   const AxisMap* xm = nullptr;
   const AxisMap* ym = nullptr;
-  RSIM_CHECK(axis_map(nx, dx, 1.0, xpos, vol->nx_g, px, &xm));
-  RSIM_CHECK(axis_map(ny, dy, 1.0, ypos, vol->ny_g, py, &ym));
+  RSIM_CHECK(axis_map(nx, dx, scale, xpos, vol->nx_g, px, &xm));
+  RSIM_CHECK(axis_map(ny, dy, scale, ypos, vol->ny_g, py, &ym));
   const double K = ((2.0 * M_PI) * pz) / wl;
   TransArgsF& ta = P->ta;
   RSIM_CHECK(material_tables(vol, K, &ta.lut, &ta.dbf));
@@ -437,7 +441,7 @@ template <typename TW>
 static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, double wl,
                      double freq, const rsim_volume* vol, double px, double py, double pz,
                      double xpos, double ypos, int first, int count, bool want_spec,
-                     hipStream_t s) {
+                     const double* scales, const double* dzs, hipStream_t s) {
   const int total = count;
   const int tdtype = std::is_same<TW, twc>::value ? kTwComp : sizeof(TW) == 8 ? RSIM_C128 : RSIM_C64;
   FusedPrep<TW> P;
@@ -446,8 +450,11 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
   const StageTw<TW>& ty = P.ty;
   const ChirpTables* ct = P.ct;
   TransArgsF ta = P.ta;
-  const bool lut = P.lut;
+  bool lut = P.lut;
   const size_t slice_bytes = P.slice_bytes;
+  // cone-beam runs: the voxel-map scale and the propagation distance change
+  // from slice to slice (tables cached per value)
+  const bool per_slice = scales || dzs;
   // three-pass columns: 128-point sub-transform tables and the W_ny table
   const bool big = ny == kBigCol;
   StageTw<TW> t128{};
@@ -467,6 +474,15 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
   RSIM_CHECK(fused_schedule<TW>(nx, ny, lut, tx, ty, ta, ct, s, &rv, &cv));
   for (int k = 0; k < count; ++k) {
     const int sl = first + k;
+    if (per_slice) {
+      FusedPrep<TW> Pk;
+      RSIM_CHECK(fused_prepare<TW>(nx, ny, dx, dy, wl, freq, vol, px, py, pz, xpos, ypos, &Pk,
+                                   dzs ? dzs[k] : pz, scales ? scales[k] : 1.0));
+      ct = Pk.ct;
+      ta = Pk.ta;
+      lut = Pk.lut;
+      RSIM_CHECK(fused_schedule<TW>(nx, ny, lut, tx, ty, ta, ct, s, &rv, &cv));
+    }
     ta.ids = vol->ids + (size_t)sl * slice_bytes;
     const bool last = k == count - 1;
     ktime_begin(KT_ROW, s);
@@ -506,16 +522,17 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
 
 int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
                  double freq, const rsim_volume* vol, double px, double py, double pz,
-                 double xpos, double ypos, int first, int count, int engine, int flags,
-                 hipStream_t s) {
+                 double xpos, double ypos, int first, int count, const double* scales,
+                 const double* dzs, int engine, int flags, hipStream_t s) {
   const int total = count;
   if (count == 0) return RSIM_OK;
   const bool want_spec = !(flags & RSIM_FLAG_NO_SPECTRUM);
   if (engine == RSIM_PLAIN || !fused_supported(nx, ny, dtype)) {
     for (int k = 0; k < count; ++k) {
       if (cancel_requested()) return fail(RSIM_E_CANCELLED, "multislice cancelled at slice %d", first + k);
-      RSIM_CHECK(transmission(u, nx, ny, dtype, dx, dy, wl, vol, first + k, px, py, pz, xpos, ypos, s));
-      RSIM_CHECK(propagate_plain(u, U, nx, ny, dtype, dx, dy, wl, pz, freq, false, s));
+      RSIM_CHECK(transmission(u, nx, ny, dtype, dx, dy, wl, vol, first + k, px, py, pz, xpos, ypos,
+                              scales ? scales[k] : 1.0, s));
+      RSIM_CHECK(propagate_plain(u, U, nx, ny, dtype, dx, dy, wl, dzs ? dzs[k] : pz, freq, false, s));
       if (g_progress_sync()) RSIM_HIP(hipStreamSynchronize(s));
       report_progress("sample_slice", k + 1, total);
     }
@@ -532,12 +549,12 @@ int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double 
     flags |= RSIM_FLAG_MATH_COMP32;
   if (flags & RSIM_FLAG_MATH_COMP32)
     return run_fused<twc>(u, U, nx, ny, dx, dy, wl, freq, vol, px, py, pz, xpos, ypos, first,
-                          count, want_spec, s);
+                          count, want_spec, scales, dzs, s);
   if (flags & RSIM_FLAG_MATH_FP64)
     return run_fused<double>(u, U, nx, ny, dx, dy, wl, freq, vol, px, py, pz, xpos, ypos, first,
-                             count, want_spec, s);
+                             count, want_spec, scales, dzs, s);
   return run_fused<float>(u, U, nx, ny, dx, dy, wl, freq, vol, px, py, pz, xpos, ypos, first,
-                          count, want_spec, s);
+                          count, want_spec, scales, dzs, s);
 }
 
 // ---------------------------------------------------------------------------

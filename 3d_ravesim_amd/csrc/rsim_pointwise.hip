@@ -91,6 +91,43 @@ __global__ void __launch_bounds__(256) point_source_kernel(cpx<R>* u, int nx, in
   }
 }
 
+// ---------------------------------------------------------------------------
+// Tilted plane wave of the Fresnel-scaled point source (big-wave/source.py:
+// 62-90): u = exp(i phase) / dz, phase = -k (x xs + y ys) / dz, FP64.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void __launch_bounds__(256) tilted_plane_kernel(cpx<R>* u, int nx, int ny, double dx,
+                                                           double dy, double k, double xs,
+                                                           double ys, double dz) {
+  const long long n = (long long)nx * ny;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int iy = (int)(idx / nx);
+    const int ix = (int)(idx - (long long)iy * nx);
+    const double x = ((double)ix - nx / 2.0) * dx;
+    const double y = ((double)iy - ny / 2.0) * dy;
+    const double phase = ((-k) * (x * xs + y * ys)) / dz;
+    double sn, cs;
+    sincos(phase, &sn, &cs);
+    u[idx] = mk((R)(cs / dz), (R)(sn / dz));
+  }
+}
+
+int tilted_plane(void* u, int nx, int ny, int dtype, double dx, double dy, double wl, double dz,
+                 double xs, double ys, hipStream_t s) {
+  const double k = (2.0 * M_PI) / wl;
+  const long long n = (long long)nx * ny;
+  const int blocks = (int)std::min<long long>((n + 255) / 256, 65536);
+  if (dtype == RSIM_C64)
+    hipLaunchKernelGGL(tilted_plane_kernel<float>, dim3(blocks), dim3(256), 0, s, (cpx<float>*)u,
+                       nx, ny, dx, dy, k, xs, ys, dz);
+  else
+    hipLaunchKernelGGL(tilted_plane_kernel<double>, dim3(blocks), dim3(256), 0, s,
+                       (cpx<double>*)u, nx, ny, dx, dy, k, xs, ys, dz);
+  RSIM_LAUNCHED();
+  return RSIM_OK;
+}
+
 int point_source(void* u, int nx, int ny, int dtype, double dx, double dy, double z, double xs,
                  double ys, double wl, hipStream_t s) {
   std::vector<double> hx(nx), hy(ny);
@@ -164,11 +201,11 @@ __global__ void __launch_bounds__(256) transmission_kernel(cpx<R>* u, int nx, in
 
 int transmission(void* u, int nx, int ny, int dtype, double dx, double dy, double wl,
                  const rsim_volume* vol, int slice, double px, double py, double pz, double xpos,
-                 double ypos, hipStream_t s) {
+                 double ypos, double coord_scale, hipStream_t s) {
   const AxisMap* xm = nullptr;
   const AxisMap* ym = nullptr;
-  RSIM_CHECK(axis_map(nx, dx, 1.0, xpos, vol->nx_g, px, &xm));
-  RSIM_CHECK(axis_map(ny, dy, 1.0, ypos, vol->ny_g, py, &ym));
+  RSIM_CHECK(axis_map(nx, dx, coord_scale, xpos, vol->nx_g, px, &xm));
+  RSIM_CHECK(axis_map(ny, dy, coord_scale, ypos, vol->ny_g, py, &ym));
   const double* db = nullptr;
   RSIM_CHECK(volume_db_dev(vol, &db));
   // -2j*pi*thickness/wl (optical_element.py:42)

@@ -231,9 +231,24 @@ class Sample:
         args = (u.ptr, U.ptr, nx, ny, u.code, float(sim_params.dx), float(sim_params.get_dy()),
                 float(sim_params.wl), float(cutoff_freq), vol.handle, float(self.pixel_size_x),
                 float(py), float(self.pixel_size_z), x_pos, y_pos)
+        pz = self.pixel_size_z
+        if sim_params.use_cone_beam_bpm:
+            # moving coordinates: slice k samples the voxels at m(z_k + pz/2) times
+            # the field coordinates and propagates effective_slice_dz (:549-592)
+            zs = self.z_start + np.arange(z_len, dtype=np.float64) * pz
+            scales = np.array([sim_params.fresnel_transverse_scale(z + 0.5 * pz) for z in zs])
+            dzs = np.array([sim_params.effective_slice_dz(z, pz) for z in zs])
+
+            def run(first: int, n: int) -> None:
+                _lib.call("rsim_sample_apply_2d_scaled", *args, first, n,
+                          scales[first:].ctypes.data_as(_lib._dp),
+                          dzs[first:].ctypes.data_as(_lib._dp), engine, 0, None)
+        else:
+            def run(first: int, n: int) -> None:
+                _lib.call("rsim_sample_apply_2d", *args, first, n, engine, 0, None)
         if callback is None:
-            _lib.call("rsim_sample_apply_2d", *args, start, z_len - start, engine, 0, None)
+            run(start, z_len - start)
         else:
             for rowidx in range(start, z_len):
-                _lib.call("rsim_sample_apply_2d", *args, rowidx, 1, engine, 0, None)
-                callback(rowidx + 1, self.z_start + (rowidx + 1) * self.pixel_size_z)
+                run(rowidx, 1)
+                callback(rowidx + 1, self.z_start + (rowidx + 1) * pz)

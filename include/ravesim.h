@@ -162,6 +162,29 @@ int rsim_sample_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx,
                          double x_pos, double y_pos, int first_slice, int n_slices, int engine,
                          int flags, void* stream);
 
+/* Fresnel-scaled / cone-beam BPM forms (big-wave/propagation.py:114-196,
+ * big-wave/optical_element.py:533-592, big-wave/source.py:62-90).
+ * rsim_sample_apply_2d_scaled: slice k samples the voxel grid at field
+ * coordinates times coord_scale[k] (the local cone-beam magnification at the
+ * slice midpoint) and propagates over slice_dz[k] (effective_slice_dz); the
+ * transmission keeps the physical pixel_size_z.  NULL arrays mean 1 and
+ * pixel_size_z.  rsim_transmission_2d_scaled: one slice at coord_scale.
+ * rsim_tilted_plane_2d: the Fresnel-scaled point source, u = exp(-i k (x xs
+ * + y ys) / dz) / dz with k = 2 pi / wl (FFT + cutoff + IFFT follow in the
+ * caller, as in the reference). */
+int rsim_sample_apply_2d_scaled(void* u, void* U, int nx, int ny, int dtype, double dx,
+                                double dy, double wl, double cutoff_freq, const rsim_volume* vol,
+                                double pixel_size_x, double pixel_size_y, double pixel_size_z,
+                                double x_pos, double y_pos, int first_slice, int n_slices,
+                                const double* coord_scale, const double* slice_dz, int engine,
+                                int flags, void* stream);
+int rsim_transmission_2d_scaled(void* u, int nx, int ny, int dtype, double dx, double dy,
+                                double wl, const rsim_volume* vol, int slice, double pixel_size_x,
+                                double pixel_size_y, double pixel_size_z, double x_pos,
+                                double y_pos, double coord_scale, void* stream);
+int rsim_tilted_plane_2d(void* u, int nx, int ny, int dtype, double dx, double dy, double wl,
+                         double dz, double x_source, double y_source, void* stream);
+
 /* ---- Batched fused multislice (no counterpart in the reference: the
  * reference runs each (source, energy) instance as its own process,
  * fast-wave/Readme.md:27) ----

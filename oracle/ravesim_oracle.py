@@ -41,6 +41,13 @@ Pinning (what checks each function against something other than itself):
 * ``multislice_db_gradient``    -- central finite differences of
   ``multislice_db_intensity`` (FP64) and complex128 torch autograd of an
   independent torch forward, on small volumes.
+* ``FresnelGeometry``           -- the reference's geometry known answers
+  (tests/fresnel_scaling/test_cone_beam_geometry.py:32-66: magnification,
+  local scale, effective slice and final distances, partition sum = z_eff).
+* ``fresnel_plane_source_2d``   -- centred source = constant 1/dz, off-axis
+  linear phase (tests/fresnel_scaling/test_big_wave_source.py:46-73).
+* ``sample_apply_2d`` (CB-BPM)  -- sampled voxel coordinates at m = 1.1 and
+  the slice distance 0.2/1.2 (test_big_wave_cone_beam_sample.py:21-76).
 * ``run_point_source_sample``   -- the reference's own end-to-end golden
   fingerprints tests/big_wave_2d/p5_results/{vacuum,sample}_convergence_4096.json
   (inputs in tests/big_wave_2d/p5_benchmark.py:69-185).
@@ -244,26 +251,95 @@ class SampleSpec:
         return self.pixel_size_y if self.pixel_size_y != 0.0 else self.pixel_size_x
 
 
+@dataclass
+class FresnelGeometry:
+    """Fresnel-scaling / cone-beam BPM geometry of SimParams
+    (big-wave/propagation.py:114-196): thin mode (cone_beam False) or CB-BPM."""
+    z_detector: float
+    cone_beam: bool = False
+    source_z: float = 0.0
+    reference_z: float = 0.0
+    magnification: float = 1.0
+    effective_z: float = 0.0
+
+    def configure(self, z_source: float, z_sample: float) -> "FresnelGeometry":
+        """configure_fresnel_detector (propagation.py:114-135)."""
+        a = z_sample - z_source
+        b = self.z_detector - z_sample
+        if a <= 0 or b <= 0:
+            raise ValueError("Fresnel scaling requires z_source < z_sample < z_detector")
+        self.source_z, self.reference_z = z_source, z_sample
+        self.effective_z = a * b / (a + b)
+        self.magnification = (a + b) / a
+        return self
+
+    def transverse_scale(self, z: float) -> float:
+        """m(z) = (z - z_source) / (z_reference - z_source) under CB-BPM, else 1."""
+        if not self.cone_beam:
+            return 1.0
+        return (z - self.source_z) / (self.reference_z - self.source_z)
+
+    def slice_dz(self, z: float, dz: float) -> float:
+        """effective_slice_dz: dz / (m(z) m(z + dz)) under CB-BPM."""
+        if not self.cone_beam:
+            return dz
+        return dz / (self.transverse_scale(z) * self.transverse_scale(z + dz))
+
+    def final_dz(self, current_z: float) -> float:
+        """effective_final_dz (propagation.py:158-170)."""
+        physical = self.z_detector - current_z
+        if self.cone_beam:
+            return physical / (self.transverse_scale(current_z) * self.magnification)
+        return self.effective_z
+
+    def detector(self, ds_x: float, ds_y: float) -> tuple[float, float, float, float]:
+        """effective_detector_geometry at the detector: (ds_x/M, ds_y/M, z_eff, M)."""
+        M = self.magnification
+        return ds_x / M, ds_y / M, self.effective_z, M
+
+
+def fresnel_plane_source_2d(nx: int, ny: int, dx: float, dy: float, wl: float, dz: float,
+                            xs: float, ys: float, cutoff_freq: float,
+                            dtype=np.complex128) -> np.ndarray:
+    """PointSource.propagate_to under Fresnel scaling (source.py:62-90): the
+    equivalent tilted plane wave exp(-i k (x xs + y ys) / dz) / dz, then FFT,
+    cutoff, IFFT."""
+    k = 2.0 * np.pi / wl
+    x = (np.arange(nx, dtype=np.float64) - nx / 2.0) * dx
+    y = (np.arange(ny, dtype=np.float64) - ny / 2.0) * dy
+    phase = -k * (x[None, :] * xs + y[:, None] * ys) / dz
+    u = (np.exp(1j * phase) / dz).astype(dtype)
+    U = apply_frequency_cutoff_2d(fft2(u), cutoff_freq, dx, dy)
+    return ifft2(U)
+
+
 def sample_apply_2d(u2d: np.ndarray, sample: SampleSpec, dx: float, dy: float, wl: float,
                     cutoff_freq: float, first_slice: int = 0, n_slices: int | None = None,
-                    workers: int = 1) -> tuple[np.ndarray, np.ndarray]:
-    """Sample._apply_2d (optical_element.py:502-596) without Fresnel scaling.
+                    workers: int = 1, geometry: FresnelGeometry | None = None
+                    ) -> tuple[np.ndarray, np.ndarray]:
+    """Sample._apply_2d (optical_element.py:502-596).
 
-    Per slice: u *= material_factor(interpolated db, pz, wl); then
-    propagate_2d over pz.  Returns (u, U) with U = FFT2(u) (the apply contract,
-    optical_element.py:97).
+    Per slice: u *= material_factor(interpolated db, pz, wl), the voxel map
+    scaled by m(slice_z + pz/2) under CB-BPM; then propagate_2d over
+    effective_slice_dz(slice_z, pz).  Returns (u, U) with U = FFT2(u) (the
+    apply contract, optical_element.py:97).
     """
     ny, nx = u2d.shape
     nz = sample.grid.shape[0]
     last = nz if n_slices is None else min(nz, first_slice + n_slices)
     u = u2d.copy()
     U = None
+    pz = sample.pixel_size_z
     for rowidx in range(first_slice, last):
+        slice_z = sample.z_start + rowidx * pz
+        scale = geometry.transverse_scale(slice_z + 0.5 * pz) if geometry else 1.0
+        dz = geometry.slice_dz(slice_z, pz) if geometry else pz
         db = sample_deltabeta_slice(sample.grid, sample.db_list, rowidx, nx, ny, dx, dy,
-                                    sample.pixel_size_x, sample.py, sample.x_pos, sample.y_pos)
-        t = material_factor(db, sample.pixel_size_z, wl)
+                                    sample.pixel_size_x, sample.py, sample.x_pos, sample.y_pos,
+                                    scale)
+        t = material_factor(db, pz, wl)
         u = (u * t).astype(u.dtype, copy=False)
-        u, U = propagate_2d(u, dx, dy, wl, sample.pixel_size_z, cutoff_freq, workers=workers)
+        u, U = propagate_2d(u, dx, dy, wl, dz, cutoff_freq, workers=workers)
     return u, U
 
 
@@ -301,15 +377,19 @@ def area_axis_plan(positions: np.ndarray, ds: float, n: int, spacing: float):
 
 
 def area_v1(u2d: np.ndarray, dx: float, dy: float, size_x: float, size_y: float,
-            ds_x: float, ds_y: float, z: float) -> np.ndarray:
+            ds_x: float, ds_y: float, z: float, out_shape=None) -> np.ndarray:
     """Area-weighted |u|^2 detector, FP64 (propagation.py:995-1110).
 
     I[q,p] = sum_cells |u|^2 * overlap_x * overlap_y * z/sqrt(x_p^2+y_q^2+z^2).
     The x overlap is integrated with a row prefix sum; the y overlap with the
-    explicit min/max interval formula, as the reference does.
+    explicit min/max interval formula, as the reference does.  ``out_shape``
+    (out_y, out_x) overrides the shape derived from size / ds: under Fresnel
+    scaling the shape comes from the physical pixels while ds and z are the
+    effective ones (propagation.py:791-794).
     """
     ny, nx = u2d.shape
-    out_y, out_x = detector_shape(size_x, size_y, ds_x, ds_y)
+    out_y, out_x = (out_shape if out_shape is not None
+                    else detector_shape(size_x, size_y, ds_x, ds_y))
     if out_x <= 0 or out_y <= 0:
         raise ValueError("2D detector output must be non-empty")
     x_det = (np.arange(out_x, dtype=np.float64) - out_x // 2) * ds_x
@@ -345,24 +425,38 @@ def area_v1(u2d: np.ndarray, dx: float, dy: float, size_x: float, size_y: float,
 
 def run_point_source_sample(nx, ny, dx, dy, wl, source_xyz, cutoff_freq, z_detector,
                             det_size_x, det_size_y, ds_x, ds_y, sample: SampleSpec | None = None,
-                            dtype=np.complex128, workers=1, return_field=False):
+                            dtype=np.complex128, workers=1, return_field=False,
+                            geometry: FresnelGeometry | None = None):
     """Point source -> [Sample] -> final leg (skip_fft) -> area_v1 detector.
 
     Mirrors wavesim.run_simulation for one 2D Sample and one cutoff frequency
-    (wavesim.py:175-186, 257-260, 279-303, 314-315).
+    (wavesim.py:175-186, 257-260, 279-303, 314-315).  With a configured
+    ``geometry`` (Fresnel scaling, thin or CB-BPM): the tilted-plane source
+    (source.py:62-90), scaled slices, effective final distance and the
+    effective detector geometry at the detector plane (propagation.py:172-196).
     """
     xs, ys, zs = source_xyz
     current_z = sample.z_start if sample is not None else z_detector
-    u = point_source_propagate_to(nx, ny, dx, dy, current_z - zs, xs, ys, wl, cutoff_freq,
-                                  dtype=dtype, workers=workers)
+    if geometry is not None:
+        u = fresnel_plane_source_2d(nx, ny, dx, dy, wl, current_z - zs, xs, ys, cutoff_freq,
+                                    dtype=dtype)
+    else:
+        u = point_source_propagate_to(nx, ny, dx, dy, current_z - zs, xs, ys, wl, cutoff_freq,
+                                      dtype=dtype, workers=workers)
     U = None
     if sample is not None:
-        u, U = sample_apply_2d(u, sample, dx, dy, wl, cutoff_freq, workers=workers)
+        u, U = sample_apply_2d(u, sample, dx, dy, wl, cutoff_freq, workers=workers,
+                               geometry=geometry)
         current_z = sample.z_start + sample.pixel_size_z * sample.grid.shape[0]
     if current_z < z_detector - 1e-8:
-        u, U = propagate_2d(u, dx, dy, wl, z_detector - current_z, cutoff_freq, U2d=U,
-                            workers=workers)
-    det = area_v1(u, dx, dy, det_size_x, det_size_y, ds_x, ds_y, z_detector)
+        leg = geometry.final_dz(current_z) if geometry is not None else z_detector - current_z
+        u, U = propagate_2d(u, dx, dy, wl, leg, cutoff_freq, U2d=U, workers=workers)
+    if geometry is not None:
+        shape = detector_shape(det_size_x, det_size_y, ds_x, ds_y)
+        eds_x, eds_y, ez, _ = geometry.detector(ds_x, ds_y)
+        det = area_v1(u, dx, dy, det_size_x, det_size_y, eds_x, eds_y, ez, out_shape=shape)
+    else:
+        det = area_v1(u, dx, dy, det_size_x, det_size_y, ds_x, ds_y, z_detector)
     return (det, u) if return_field else det
 
 

@@ -1,0 +1,311 @@
+"""Fresnel scaling / cone-beam BPM: geometry known answers and host logic.
+
+The known answers are the ones the reference's own tests hold
+(tests/fresnel_scaling/test_cone_beam_geometry.py:32-105,
+test_cutoff_geometry.py:38-62, test_big_wave_cone_beam_sample.py:21-72,
+test_big_wave_source.py:46-73, test_big_wave_wavesim.py:55-155).  They pin
+the oracle's FresnelGeometry / fresnel_plane_source_2d / scaled voxel map and
+the product's SimParams alike; the driver's distance sequencing and the
+arrays Sample.apply hands to the C ABI are checked with the library calls
+recorded on the CPU (no GPU work).
+"""
+
+import importlib
+
+import numpy as np
+import pytest
+
+import oracle as O
+from _rs import rs
+
+ORACLE_MOD = importlib.import_module("oracle.ravesim_oracle")
+WAVESIM = importlib.import_module("3d_ravesim_amd.wavesim")
+OPTICAL = importlib.import_module("3d_ravesim_amd.optical_element")
+
+
+def make_params(**overrides):
+    values = dict(N=16, ny=4, dx=1e-6, z_detector=3.0, detector_size=16e-6,
+                  detector_pixel_size_x=1e-6, detector_pixel_size_y=1e-6, wl=1e-10,
+                  chunk_size=16)
+    values.update(overrides)
+    return rs.SimParams(**values)
+
+
+# ------------------------------------------------ geometry known answers ----
+
+def test_cb_bpm_enables_fresnel_and_maps_coordinates():
+    p = make_params(use_cone_beam_bpm=True)
+    p.configure_fresnel_detector(z_source=0.0, z_sample=1.0)
+    assert p.use_fresnel_scaling
+    assert p.fresnel_magnification == pytest.approx(3.0, abs=1e-7)
+    assert p.fresnel_transverse_scale(1.5) == pytest.approx(1.5, abs=1e-7)
+    assert p.effective_slice_dz(1.0, 0.5) == pytest.approx(1.0 / 3.0, abs=1e-7)
+    assert p.effective_final_dz(1.5) == pytest.approx(1.0 / 3.0, abs=1e-7)
+    g = O.FresnelGeometry(z_detector=3.0, cone_beam=True).configure(0.0, 1.0)
+    assert g.magnification == pytest.approx(3.0, abs=1e-7)
+    assert g.transverse_scale(1.5) == pytest.approx(1.5, abs=1e-7)
+    assert g.slice_dz(1.0, 0.5) == pytest.approx(1.0 / 3.0, abs=1e-7)
+    assert g.final_dz(1.5) == pytest.approx(1.0 / 3.0, abs=1e-7)
+
+
+def test_thin_fresnel_keeps_single_effective_detector_leg():
+    p = make_params(use_fresnel_scaling=True)
+    p.configure_fresnel_detector(z_source=0.0, z_sample=1.0)
+    assert p.fresnel_transverse_scale(2.0) == 1.0
+    assert p.effective_slice_dz(1.0, 0.5) == 0.5
+    assert p.effective_final_dz(1.25) == pytest.approx(2.0 / 3.0, abs=1e-7)
+    g = O.FresnelGeometry(z_detector=3.0).configure(0.0, 1.0)
+    assert g.transverse_scale(2.0) == 1.0
+    assert g.slice_dz(1.0, 0.5) == 0.5
+    assert g.final_dz(1.25) == pytest.approx(2.0 / 3.0, abs=1e-7)
+
+
+def test_cb_bpm_distance_partition_sums_to_thin_z_eff():
+    p = make_params(use_cone_beam_bpm=True)
+    p.configure_fresnel_detector(z_source=0.0, z_sample=1.0)
+    total = p.effective_slice_dz(1.0, 0.2) + p.effective_slice_dz(1.2, 0.5) + p.effective_final_dz(1.7)
+    assert total == pytest.approx(p.fresnel_effective_z, abs=1e-15)
+    assert p.fresnel_transverse_scale(p.z_detector) == pytest.approx(p.fresnel_magnification, abs=1e-15)
+    g = O.FresnelGeometry(z_detector=3.0, cone_beam=True).configure(0.0, 1.0)
+    total = g.slice_dz(1.0, 0.2) + g.slice_dz(1.2, 0.5) + g.final_dz(1.7)
+    assert total == pytest.approx(g.effective_z, abs=1e-15)
+    assert g.transverse_scale(3.0) == pytest.approx(g.magnification, abs=1e-15)
+
+
+def test_partition_sums_to_z_eff_for_many_slices():
+    # the moving-coordinate distances of any partition of [z_sample, z_det]
+    # integrate dz / m(z)^2 exactly: sum = a b / (a + b) (telescoping)
+    p = make_params(use_cone_beam_bpm=True, z_detector=2.7)
+    p.configure_fresnel_detector(z_source=-0.4, z_sample=0.35)
+    zs = np.concatenate([[0.35], np.sort(np.random.default_rng(1).uniform(0.35, 2.7, 37))])
+    total = sum(p.effective_slice_dz(a, b - a) for a, b in zip(zs[:-1], zs[1:]))
+    total += p.effective_final_dz(zs[-1])
+    assert total == pytest.approx(p.fresnel_effective_z, rel=1e-13)
+
+
+def test_effective_detector_geometry():
+    p = make_params(use_fresnel_scaling=True, detector_pixel_size_x=2e-6, detector_pixel_size_y=3e-6)
+    p.configure_fresnel_detector(z_source=0.0, z_sample=1.0)
+    ds_x, ds_y, z, m = p.effective_detector_geometry(3.0)
+    assert (ds_x, ds_y, z, m) == pytest.approx((2e-6 / 3.0, 1e-6, 2.0 / 3.0, 3.0))
+    assert p.effective_detector_geometry(2.0) == (2e-6, 3e-6, 2.0, 1.0)
+    g = O.FresnelGeometry(z_detector=3.0).configure(0.0, 1.0)
+    assert g.detector(2e-6, 3e-6) == pytest.approx((ds_x, ds_y, z, m))
+    unconfigured = make_params(use_fresnel_scaling=True)
+    with pytest.raises(ValueError, match="has not been configured"):
+        unconfigured.effective_detector_geometry(3.0)
+
+
+def test_configure_rejects_bad_ordering():
+    p = make_params(use_fresnel_scaling=True)
+    with pytest.raises(ValueError, match="z_source < z_sample < z_detector"):
+        p.configure_fresnel_detector(z_source=1.0, z_sample=0.5)
+    with pytest.raises(ValueError):
+        O.FresnelGeometry(z_detector=3.0).configure(0.0, 3.5)
+
+
+def test_config_cb_bpm_boolean_is_strict_and_implies_fresnel():
+    base = dict(N=16, ny=4, dx=1e-6, z_detector=3.0, detector_size=16e-6,
+                detector_pixel_size_x=1e-6, detector_pixel_size_y=1e-6, chunk_size=16,
+                use_cone_beam_bpm="true")
+    parsed = rs.config.parse_sim_params(base)
+    assert parsed.use_cone_beam_bpm and parsed.use_fresnel_scaling
+    base["use_cone_beam_bpm"] = "sometimes"
+    with pytest.raises(ValueError):
+        rs.config.parse_sim_params(base)
+
+
+def test_mode_validation_separates_thin_and_cb_bpm_scope():
+    Sample = type("Sample", (), {})
+    Grating = type("Grating", (), {})
+    thin = make_params(use_fresnel_scaling=True)
+    rs.validate_fresnel_mode(thin, [Sample()], "points")
+    with pytest.raises(ValueError, match="exactly one"):
+        rs.validate_fresnel_mode(thin, [Sample(), Sample()], "points")
+    cb = make_params(use_cone_beam_bpm=True)
+    rs.validate_fresnel_mode(cb, [Sample(), Sample()], "points")
+    with pytest.raises(ValueError, match="Sample/PlasmaSample"):
+        rs.validate_fresnel_mode(cb, [Grating()], "points")
+    with pytest.raises(ValueError, match="point sources"):
+        rs.validate_fresnel_mode(cb, [Sample()], "vectors")
+    assert rs.fresnel_mode_name(make_params()) == "off"
+
+
+def _cutoff_params():
+    return rs.SimParams(N=16 * 8, ny=8, dx=1.0e-6, dy=1.0e-6, z_detector=4.8, detector_size=0.014,
+                        detector_size_x=0.014, detector_size_y=0.010, detector_pixel_size_x=2.0e-6,
+                        detector_pixel_size_y=2.0e-6, wl=1.0e-10, chunk_size=16 * 4,
+                        use_fresnel_scaling=True)
+
+
+def test_cutoff_geometry_known_answers():
+    ax, ay, m, z = rs.compute_fresnel_cutoff_geometry(_cutoff_params(), 0.0, 0.5, (0.0, 0.0), (0.0, 0.0))
+    assert m == pytest.approx(9.6, abs=1e-14)
+    assert z == pytest.approx(0.5 * 4.3 / 4.8, abs=1e-14)
+    assert ax == pytest.approx(np.arctan(0.007 / 4.3), abs=1e-14)
+    assert ay == pytest.approx(np.arctan(0.005 / 4.3), abs=1e-14)
+    off, _, _, _ = rs.compute_fresnel_cutoff_geometry(_cutoff_params(), 0.0, 0.5,
+                                                      (-50.0e-6, 25.0e-6), (0.0, 0.0))
+    assert off > ax
+    assert off == pytest.approx(np.arctan(0.007 / 4.3 + 50.0e-6 / 0.5), abs=1e-14)
+
+
+# ----------------------------------------------------- oracle known answers ----
+
+def test_oracle_cb_sample_samples_scaled_coordinates(monkeypatch):
+    """At the slice midpoint m = 1.1 the central row samples x indices
+    [0.3, 1.4, 2.5, 3.6]; the slice propagates 0.2 / 1.2."""
+    grid = np.broadcast_to(np.arange(5, dtype=np.uint32)[None, None, :], (1, 5, 5)).copy()
+    spec = O.SampleSpec(grid=grid, db_list=np.arange(5, dtype=np.complex128), pixel_size_x=1.0,
+                        pixel_size_y=1.0, pixel_size_z=0.2, x_pos=0.0, y_pos=0.0, z_start=1.0)
+    geom = O.FresnelGeometry(z_detector=3.0, cone_beam=True).configure(0.0, 1.0)
+    distances = []
+
+    def capture(u, dx, dy, wl, dz, cutoff, **kw):
+        distances.append(dz)
+        return u, None
+
+    monkeypatch.setattr(ORACLE_MOD, "material_factor", lambda db, dz, wl: 1.0 + db)
+    monkeypatch.setattr(ORACLE_MOD, "propagate_2d", capture)
+    u, _ = O.sample_apply_2d(np.ones((4, 4), np.complex128), spec, 1.0, 1.0, 1.0, 0.0,
+                             geometry=geom)
+    np.testing.assert_allclose(u[2].real, 1.0 + np.array([0.3, 1.4, 2.5, 3.6]))
+    assert distances == [pytest.approx(0.2 / 1.2)]
+
+
+def _source_params():
+    p = rs.SimParams(N=16 * 8, nx=16, ny=8, dx=2.0e-6, dy=3.0e-6, z_detector=5.0,
+                     detector_size=1.0e-3, detector_size_x=1.0e-3, detector_size_y=1.0e-3,
+                     detector_pixel_size_x=10.0e-6, detector_pixel_size_y=10.0e-6, wl=1.0e-10,
+                     chunk_size=16 * 4, use_fresnel_scaling=True)
+    p.configure_fresnel_detector(z_source=0.0, z_sample=1.0)
+    return p
+
+
+def test_oracle_fresnel_source_centred_and_off_axis():
+    p = _source_params()
+    u = O.fresnel_plane_source_2d(16, 8, p.dx, p.dy, p.wl, 1.0, 0.0, 0.0, 1.0 / p.dx)
+    np.testing.assert_allclose(u, 1.0 + 0.0j, rtol=0.0, atol=1e-13)
+    xs, ys = 25.0e-6, -30.0e-6
+    u = O.fresnel_plane_source_2d(16, 8, p.dx, p.dy, p.wl, 1.0, xs, ys, 1.0 / p.dx)
+    iy, ix = np.indices((8, 16))
+    x = (ix - 8.0) * p.dx
+    y = (iy - 4.0) * p.dy
+    np.testing.assert_allclose(u, np.exp(-2j * np.pi * (x * xs + y * ys) / p.wl), rtol=0.0, atol=2e-12)
+    # the amplitude is 1/dz at any distance
+    u = O.fresnel_plane_source_2d(16, 8, p.dx, p.dy, p.wl, 0.25, 0.0, 0.0, 1.0 / p.dx)
+    np.testing.assert_allclose(u, 4.0 + 0.0j, rtol=0.0, atol=1e-12)
+
+
+def test_point_source_requires_configured_effective_geometry():
+    p = _source_params()
+    p.fresnel_effective_z = 0.0
+    with pytest.raises(ValueError, match="has not been configured"):
+        rs.PointSource(x=0.0, y=0.0, z=0.0).propagate_to(1.0, p, 1.0 / p.dx, None, None, None)
+    p = _source_params()
+    with pytest.raises(ValueError, match="must be positive"):
+        rs.PointSource(x=0.0, y=0.0, z=1.5).propagate_to(1.0, p, 1.0 / p.dx, None, None, None)
+
+
+# ------------------------------------------------ driver distance sequencing ----
+
+class _PlaneSource:
+    z = 0.0
+
+    def propagate_to(self, z_out, params, cutoff_freq, u, U, history):
+        pass
+
+
+class _EmptyElement:
+    z_start = 1.0
+    x_positions = [0.0]
+
+    def __init__(self, thickness=0.0):
+        self.thickness = thickness
+
+    def store_deltabetas(self, table):
+        pass
+
+    def get_thickness(self):
+        return self.thickness
+
+    def apply(self, u, U, params, cutoff_freq, stepping_iteration, history):
+        pass
+
+
+def _run_recording(monkeypatch, params, element):
+    distances = []
+    monkeypatch.setattr(WAVESIM, "propagate_2d", lambda *a, **k: distances.append(a[5]))
+    monkeypatch.setattr(WAVESIM, "square_and_downsample_2d", lambda *a: np.zeros((1, 1)))
+    WAVESIM.run_simulation(params, _PlaneSource(), [element], [0.01, 0.01], None, None, [])
+    return distances
+
+
+def _wavesim_params(**mode):
+    p = rs.SimParams(N=64, nx=8, ny=8, dx=1e-6, dy=1e-6, z_detector=5.0, detector_size=8e-6,
+                     detector_size_x=8e-6, detector_size_y=8e-6, detector_pixel_size_x=1e-6,
+                     detector_pixel_size_y=1e-6, wl=1e-10, chunk_size=32, **mode)
+    p.configure_fresnel_detector(z_source=0.0, z_sample=1.0)
+    return p
+
+
+def test_final_leg_uses_effective_distance(monkeypatch):
+    p = _wavesim_params(use_fresnel_scaling=True)
+    d = _run_recording(monkeypatch, p, _EmptyElement())
+    assert len(d) == 1
+    assert d[0] == pytest.approx(p.fresnel_effective_z, abs=1e-15)
+    assert abs(d[0] - 4.0) > 1e-6
+
+
+def test_cb_bpm_final_leg_starts_at_sample_exit_scale(monkeypatch):
+    p = _wavesim_params(use_cone_beam_bpm=True)
+    d = _run_recording(monkeypatch, p, _EmptyElement(0.5))
+    assert d == [pytest.approx(3.5 / (1.5 * 5.0))]
+
+
+def test_sample_apply_hands_scaled_slices_to_the_library(monkeypatch):
+    """Under CB-BPM Sample.apply passes m(z_k + pz/2) and dz/(m(z_k) m(z_k+pz))
+    per slice to rsim_sample_apply_2d_scaled; outside it the unscaled entry."""
+    nz = 5
+    vol = OPTICAL.DeviceVolume(handle=None, shape=(nz, 8, 8), db_list=np.zeros(2))
+    vol._h = 1
+    calls = []
+
+    def record(name, *args):
+        if name == "rsim_sample_apply_2d_scaled":
+            first, n = args[15], args[16]
+            sc = np.ctypeslib.as_array(args[17], shape=(n,)).copy()
+            dz = np.ctypeslib.as_array(args[18], shape=(n,)).copy()
+            calls.append((name, first, n, sc, dz))
+        else:
+            calls.append((name, args[15], args[16]))
+
+    monkeypatch.setattr(OPTICAL._lib, "call", record)
+    try:
+        for mode in ({"use_cone_beam_bpm": True}, {"use_fresnel_scaling": True}, {}):
+            calls.clear()
+            p = rs.SimParams(N=64, ny=8, dx=1e-6, z_detector=5.0, detector_size=8e-6,
+                             detector_pixel_size_x=1e-6, detector_pixel_size_y=1e-6, wl=1e-10,
+                             chunk_size=8, **mode)
+            p.configure_fresnel_detector(z_source=0.2, z_sample=1.0)
+            s = rs.Sample(z_start=1.0, pixel_size_x=1e-6, pixel_size_z=0.01, grid=vol,
+                          materials=[rs.Material("a", 1.0)], x_positions=np.array([0.0]),
+                          engine="plain")
+            u = rs.DeviceVector.__new__(rs.DeviceVector)
+            u._ptr, u._dtype, u._n = 0x1000, np.dtype(np.complex128), 64
+            u._code = rs._lib.dtype_code(u._dtype)
+            try:
+                s.apply(u, u, p, 1.0, 0, None)
+            finally:
+                u._ptr = None
+            if mode.get("use_cone_beam_bpm"):
+                (name, first, n, sc, dz), = calls
+                assert (name, first, n) == ("rsim_sample_apply_2d_scaled", 0, nz)
+                z = 1.0 + 0.01 * np.arange(nz)
+                np.testing.assert_allclose(sc, (z + 0.005 - 0.2) / 0.8, rtol=1e-15)
+                np.testing.assert_allclose(dz, 0.01 / ((z - 0.2) / 0.8 * (z + 0.01 - 0.2) / 0.8),
+                                           rtol=1e-14)
+            else:
+                assert calls == [("rsim_sample_apply_2d", 0, nz)]
+    finally:
+        vol._h = None
