@@ -30,6 +30,8 @@ from .optical_element import (
     extract_from_deltabeta_table,
     material_factor,
 )
+from .plasma import DevicePlasma, plasma_delta_beta_grid_multi
+from .plasma_sample import PlasmaSample
 from .source import PointSource
 from .wavesim import run_simulation
 from .multisim import compute_fresnel_cutoff_geometry, fresnel_mode_name, validate_fresnel_mode
@@ -40,7 +42,7 @@ __all__ = [
     "detector_shape", "fresnel_plane_2d", "grid_density_check", "grid_density_check_2d", "propagate_2d",
     "propagate_analytically_2d", "require_row_aligned_chunk_size", "square_and_downsample_2d",
     "DeviceVolume", "Material", "OpticalElement", "Sample", "extract_from_deltabeta_table",
-    "material_factor", "PointSource", "run_simulation", "compute_fresnel_cutoff_geometry",
+    "material_factor", "DevicePlasma", "plasma_delta_beta_grid_multi", "PlasmaSample", "PointSource", "run_simulation", "compute_fresnel_cutoff_geometry",
     "fresnel_mode_name", "validate_fresnel_mode", "config",
 ]
 

@@ -101,6 +101,13 @@ SIGNATURES = {
     "rsim_transmission_2d_scaled": [_vp, _i, _i, _i, _d, _d, _d, _vp, _i, _d, _d, _d, _d, _d,
                                     _d, _vp],
     "rsim_tilted_plane_2d": [_vp, _i, _i, _i, _d, _d, _d, _d, _d, _d, _vp],
+    "rsim_plasma_create": [C.POINTER(_vp), _vp, _vp, _vp, _vp, _i, _i, _i, _i],
+    "rsim_plasma_destroy": [_vp],
+    "rsim_plasma_deltabeta": [_vp, _i, _i, _d, _dp, _vp, _vp],
+    "rsim_plasma_transmission_2d": [_vp, _i, _i, _i, _d, _d, _d, _vp, _i, _d, _dp, _d, _d, _d,
+                                    _d, _d, _d, _vp],
+    "rsim_plasma_apply_2d": [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _vp, _d, _dp, _d, _d, _d, _d,
+                             _d, _i, _i, _dp, _dp, _i, _i, _vp],
     "rsim_batch_create": [C.POINTER(_vp), C.POINTER(BatchInstance), _i, _i, _i, _i, _vp],
     "rsim_batch_run": [_vp, _vp],
     "rsim_batch_info": [_vp, _ip, _ip],

@@ -17,8 +17,14 @@
 
 namespace rsim {
 
+// Transmission source of the fused row pass: bilinear over material ids,
+// per-material lookup (integral voxel map) or bilinear over a per-voxel
+// delta-beta field (plasma samples).
+enum { TM_BILINEAR = 0, TM_LUT = 1, TM_FIELD = 2 };
+
 struct TransArgsF {
   const uint8_t* ids;     // slice base (ny_g x nx_g)
+  const float2* dbv;      // per-voxel (delta, beta) slice base, TM_FIELD
   int nxg;
   AxisMapDev xm, ym;
   const float4* lut;      // per material (hi.re, hi.im, lo.re, lo.im) of the FP64 factor
@@ -33,15 +39,16 @@ __device__ __forceinline__ void lut_to_lds(float4* lds_lut, const float4* lut) {
   for (int i = threadIdx.x; i < 256; i += blockDim.x) lds_lut[i] = lut[i];
 }
 
-// Transmission of one element.  LUT: the voxel map is integral, hence affine
-// (x0 = ax*x + cx); the factor is a per-material lookup (LDS) of the FP64
-// value split hi+lo.  Otherwise bilinear delta-beta interpolation from the
-// four corners and an FP32 exponential.  row0 = ids row of this field row.
-template <bool LUT>
+// Transmission of one element.  TM_LUT: the voxel map is integral, hence
+// affine (x0 = ax*x + cx); the factor is a per-material lookup (LDS) of the
+// FP64 value split hi+lo.  Otherwise bilinear delta-beta interpolation from
+// the four corners (material table or per-voxel field) and an FP32
+// exponential.  row0 = ids row of this field row (TM_LUT).
+template <int TM>
 __device__ __forceinline__ cpx<float> transmit(cpx<float> v, const TransArgsF& a, int x, int y0,
                                                float fy, bool iny, const uint8_t* row0,
                                                const float4* lds_lut) {
-  if constexpr (LUT) {
+  if constexpr (TM == TM_LUT) {
     if (!(iny && x >= a.xlo && x < a.xhi)) return v;
     const float4 t = lds_lut[row0[a.ax * x + a.cx]];
     const float rr = v.x * t.x - v.y * t.y + (v.x * t.z - v.y * t.w);
@@ -50,10 +57,23 @@ __device__ __forceinline__ cpx<float> transmit(cpx<float> v, const TransArgsF& a
   } else {
     if (!(iny && a.xm.inside[x])) return v;
     const int x0 = a.xm.i0[x];
-    const uint8_t* r0 = a.ids + (long long)y0 * a.nxg + x0;
-    const uint8_t* r1 = r0 + a.nxg;
     const float fx = a.xm.fracf[x];
-    const float2 d00 = a.dbf[r0[0]], d01 = a.dbf[r0[1]], d10 = a.dbf[r1[0]], d11 = a.dbf[r1[1]];
+    float2 d00, d01, d10, d11;
+    if constexpr (TM == TM_FIELD) {
+      const float2* r0 = a.dbv + (long long)y0 * a.nxg + x0;
+      const float2* r1 = r0 + a.nxg;
+      d00 = r0[0];
+      d01 = r0[1];
+      d10 = r1[0];
+      d11 = r1[1];
+    } else {
+      const uint8_t* r0 = a.ids + (long long)y0 * a.nxg + x0;
+      const uint8_t* r1 = r0 + a.nxg;
+      d00 = a.dbf[r0[0]];
+      d01 = a.dbf[r0[1]];
+      d10 = a.dbf[r1[0]];
+      d11 = a.dbf[r1[1]];
+    }
     const float gx = 1.f - fx, gy = 1.f - fy;
     const float de = (d00.x * gx + d01.x * fx) * gy + (d10.x * gx + d11.x * fx) * fy;
     const float be = (d00.y * gx + d01.y * fx) * gy + (d10.y * gx + d11.y * fx) * fy;
@@ -72,10 +92,10 @@ struct RowT {
   bool iny;
   const uint8_t* row0;
 };
-template <bool LUT>
+template <int TM>
 __device__ __forceinline__ RowT row_state(const TransArgsF& a, int y) {
   RowT r;
-  if constexpr (LUT) {
+  if constexpr (TM == TM_LUT) {
     r.iny = y >= a.ylo && y < a.yhi;
     r.y0 = a.ay * y + a.cy;
     r.fy = 0.f;
@@ -115,9 +135,10 @@ struct ColBatch {
   ColInst inst[kMaxBatch];
 };
 
-template <class F, int C, int PS, bool SPLIT, int MODE, bool LUT, typename TW>
+template <class F, int C, int PS, bool SPLIT, int MODE, int TM, typename TW>
 __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(RowBatch rb, int ny, uint32_t nbytes,
                                                            StageTw<TW> tw) {
+  constexpr bool LUT = TM == TM_LUT;
   constexpr int N = F::N;
   __shared__ unsigned char lds[LdsSize<F, C, PS>::slots * (SPLIT ? 4 : 8)];
   __shared__ float4 lds_lut[(MODE != ROW_LAST && LUT) ? 256 : 1];
@@ -138,21 +159,21 @@ __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(RowBatch rb, int ny, 
 #pragma unroll
   for (int e = 0; e < F::E; ++e) v[e] = gi.ld(vo, first_off<F>(e, 8u));
   if constexpr (MODE == ROW_FIRST) {
-    const RowT r = row_state<LUT>(ta, yy);
+    const RowT r = row_state<TM>(ta, yy);
     if constexpr (LUT) __syncthreads();
 #pragma unroll
     for (int e = 0; e < F::E; ++e) {
-      v[e] = transmit<LUT>(v[e], ta, first_pos<F>(t, e), r.y0, r.fy, r.iny, r.row0, lds_lut);
+      v[e] = transmit<TM>(v[e], ta, first_pos<F>(t, e), r.y0, r.fy, r.iny, r.row0, lds_lut);
       if ((e & 3) == 3) compiler_fence();
     }
     reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
   } else if constexpr (MODE == ROW_MID) {
     reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);  // its barriers publish lds_lut
     compiler_fence();
-    const RowT r = row_state<LUT>(ta, yy);
+    const RowT r = row_state<TM>(ta, yy);
 #pragma unroll
     for (int e = 0; e < F::E; ++e) {
-      v[e] = transmit<LUT>(v[e], ta, last_pos<F>(t, e), r.y0, r.fy, r.iny, r.row0, lds_lut);
+      v[e] = transmit<TM>(v[e], ta, last_pos<F>(t, e), r.y0, r.fy, r.iny, r.row0, lds_lut);
       if ((e & 3) == 3) compiler_fence();
     }
     reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
@@ -244,19 +265,31 @@ struct Shape {
 
 template <class S, typename TW>
 struct RowLaunch {
-  static void go(int mode, bool lut, const RowBatch& b, int count, int ny, uint32_t nb,
-                 const StageTw<TW>& tw, hipStream_t s) {
+  template <int MODE>
+  static void go_tm(int tm, const RowBatch& b, int count, int ny, uint32_t nb,
+                    const StageTw<TW>& tw, hipStream_t s) {
     using F = typename S::F;
     constexpr int C = S::CC;
     dim3 grid((ny + C - 1) / C, count), block(S::threads);
+    if (tm == TM_LUT)
+      hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, MODE, TM_LUT, TW>), grid, block, 0, s, b, ny, nb, tw);
+    else if (tm == TM_FIELD)
+      hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, MODE, TM_FIELD, TW>), grid, block, 0, s, b, ny, nb, tw);
+    else
+      hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, MODE, TM_BILINEAR, TW>), grid, block, 0, s, b, ny, nb, tw);
+  }
+  static void go(int mode, int tm, const RowBatch& b, int count, int ny, uint32_t nb,
+                 const StageTw<TW>& tw, hipStream_t s) {
     if (mode == ROW_FIRST) {
-      if (lut) hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_FIRST, true, TW>), grid, block, 0, s, b, ny, nb, tw);
-      else hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_FIRST, false, TW>), grid, block, 0, s, b, ny, nb, tw);
+      go_tm<ROW_FIRST>(tm, b, count, ny, nb, tw, s);
     } else if (mode == ROW_MID) {
-      if (lut) hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_MID, true, TW>), grid, block, 0, s, b, ny, nb, tw);
-      else hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_MID, false, TW>), grid, block, 0, s, b, ny, nb, tw);
+      go_tm<ROW_MID>(tm, b, count, ny, nb, tw, s);
     } else {
-      hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_LAST, true, TW>), grid, block, 0, s, b, ny, nb, tw);
+      // no transmission in the last pass: one instantiation serves every source
+      using F = typename S::F;
+      constexpr int C = S::CC;
+      dim3 grid((ny + C - 1) / C, count), block(S::threads);
+      hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_LAST, TM_LUT, TW>), grid, block, 0, s, b, ny, nb, tw);
     }
   }
 };
@@ -304,7 +337,7 @@ constexpr int kColVariants = 5;
 // Per-size launchers (explicitly instantiated in rsim_fused_<N>.hip); count
 // instances (1..kMaxBatch) of shape (N, ny) or (nx, N) in one launch.
 template <int N, typename TW>
-bool row_go(int v, int mode, bool lut, const RowBatch& b, int count, int ny,
+bool row_go(int v, int mode, int tm, const RowBatch& b, int count, int ny,
             const StageTw<TW>& tw, hipStream_t s);
 template <int N, typename TW>
 bool col_go(int v, bool spec_out, const ColBatch& b, int count, int nx, const StageTw<TW>& tw,
@@ -312,7 +345,7 @@ bool col_go(int v, bool spec_out, const ColBatch& b, int count, int nx, const St
 
 #define RSIM_ROW_CASE(N, V, E, C, SP)                                          \
   case V:                                                                     \
-    RowLaunch<Shape<N, E, C, SP>, TW>::go(mode, lut, b, count, ny, nb, tw, s); \
+    RowLaunch<Shape<N, E, C, SP>, TW>::go(mode, tm, b, count, ny, nb, tw, s); \
     return true;
 #define RSIM_COL_CASE(N, V, E, C, SP)                                              \
   case V:                                                                         \
@@ -321,7 +354,7 @@ bool col_go(int v, bool spec_out, const ColBatch& b, int count, int nx, const St
 
 #define RSIM_FUSED_INSTANTIATE_ROWS(N)                                                            \
   template <int NN, typename TW>                                                                  \
-  bool row_go_impl(int v, int mode, bool lut, const RowBatch& b, int count, int ny,               \
+  bool row_go_impl(int v, int mode, int tm, const RowBatch& b, int count, int ny,                \
                    const StageTw<TW>& tw, hipStream_t s) {                                        \
     const uint32_t nb = (uint32_t)((long long)NN * ny * 8);                                       \
     switch (v) {                                                                                  \
@@ -330,19 +363,19 @@ bool col_go(int v, bool spec_out, const ColBatch& b, int count, int nx, const St
     }                                                                                             \
   }                                                                                               \
   template <>                                                                                     \
-  bool row_go<N, float>(int v, int mode, bool lut, const RowBatch& b, int count, int ny,          \
+  bool row_go<N, float>(int v, int mode, int tm, const RowBatch& b, int count, int ny,           \
                         const StageTw<float>& tw, hipStream_t s) {                                \
-    return row_go_impl<N, float>(v, mode, lut, b, count, ny, tw, s);                              \
+    return row_go_impl<N, float>(v, mode, tm, b, count, ny, tw, s);                               \
   }                                                                                               \
   template <>                                                                                     \
-  bool row_go<N, double>(int v, int mode, bool lut, const RowBatch& b, int count, int ny,         \
+  bool row_go<N, double>(int v, int mode, int tm, const RowBatch& b, int count, int ny,          \
                          const StageTw<double>& tw, hipStream_t s) {                              \
-    return row_go_impl<N, double>(v, mode, lut, b, count, ny, tw, s);                             \
+    return row_go_impl<N, double>(v, mode, tm, b, count, ny, tw, s);                              \
   }                                                                                               \
   template <>                                                                                     \
-  bool row_go<N, twc>(int v, int mode, bool lut, const RowBatch& b, int count, int ny,            \
+  bool row_go<N, twc>(int v, int mode, int tm, const RowBatch& b, int count, int ny,             \
                       const StageTw<twc>& tw, hipStream_t s) {                                    \
-    return row_go_impl<N, twc>(v, mode, lut, b, count, ny, tw, s);                                \
+    return row_go_impl<N, twc>(v, mode, tm, b, count, ny, tw, s);                                 \
   }
 
 #define RSIM_FUSED_INSTANTIATE_COLS(N)                                                            \

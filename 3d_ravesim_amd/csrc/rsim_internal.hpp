@@ -5,6 +5,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <functional>
 #include <string>
 
 #include "../../include/ravesim.h"
@@ -69,6 +70,29 @@ bool launch_reg_fft_c128(int dir, int N, bool colmode, const void* in, void* out
 // ---- pointwise (rsim_pointwise.hip) ----
 int chirp_cutoff(void* U, int nx, int ny, int dtype, double dx, double dy, double wl, double dz,
                  double freq, bool chirp, hipStream_t s);
+
+// ---- multislice over a voxel source (rsim_multislice.hip) ----
+// Where slice transmissions come from: material ids with a delta-beta table
+// (Sample), or a per-voxel delta-beta field (PlasmaSample).  ``plain`` applies
+// slice k's FP64 transmission to a field (plain engine); ``field`` is the
+// per-voxel (delta, beta) FP32 volume the fused engine interpolates.
+struct VoxSrc {
+  int nz = 0, ny_g = 0, nx_g = 0;
+  const struct ::rsim_volume* vol = nullptr;
+  const float2* field = nullptr;
+  std::function<int(void* u, int dtype, int slice, double scale, hipStream_t s)> plain;
+};
+// Slices [first, first + count) of a sample applied to u (optical_element.py:
+// 502-596): transmission, then propagation over dzs[k] (pz when null) with the
+// voxel map scaled by scales[k] (1 when null).  U = FFT2(u) afterwards unless
+// RSIM_FLAG_NO_SPECTRUM.
+int slices_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
+                 double freq, const VoxSrc& src, double px, double py, double pz, double xpos,
+                 double ypos, int first, int count, const double* scales, const double* dzs,
+                 int engine, int flags, hipStream_t s);
+bool fused_supported(int nx, int ny, int dtype);
+int propagate(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
+              double dz, double freq, bool skip_fft, hipStream_t s);
 
 // ---- kernel timing (kinds: 0 row pass, 1 column pass, 2 pointwise) ----
 enum { KT_ROW = 0, KT_COL = 1, KT_PW = 2 };

@@ -45,18 +45,18 @@ int transmission(void* u, int nx, int ny, int dtype, double dx, double dy, doubl
 namespace {
 
 template <typename TW>
-bool fused_row_b(int N, int v, int mode, bool lut, const RowBatch& b, int count, int ny,
+bool fused_row_b(int N, int v, int mode, int tm, const RowBatch& b, int count, int ny,
                  const StageTw<TW>& tw, hipStream_t s) {
   switch (N) {
-    case 256: return row_go<256, TW>(v, mode, lut, b, count, ny, tw, s);
-    case 512: return row_go<512, TW>(v, mode, lut, b, count, ny, tw, s);
-    case 1024: return row_go<1024, TW>(v, mode, lut, b, count, ny, tw, s);
-    case 2048: return row_go<2048, TW>(v, mode, lut, b, count, ny, tw, s);
-    case 3072: return row_go<3072, TW>(v, mode, lut, b, count, ny, tw, s);
-    case 4096: return row_go<4096, TW>(v, mode, lut, b, count, ny, tw, s);
-    case 6144: return row_go<6144, TW>(v, mode, lut, b, count, ny, tw, s);
-    case 8192: return row_go<8192, TW>(v, mode, lut, b, count, ny, tw, s);
-    case 16384: return row_go<16384, TW>(v, mode, lut, b, count, ny, tw, s);
+    case 256: return row_go<256, TW>(v, mode, tm, b, count, ny, tw, s);
+    case 512: return row_go<512, TW>(v, mode, tm, b, count, ny, tw, s);
+    case 1024: return row_go<1024, TW>(v, mode, tm, b, count, ny, tw, s);
+    case 2048: return row_go<2048, TW>(v, mode, tm, b, count, ny, tw, s);
+    case 3072: return row_go<3072, TW>(v, mode, tm, b, count, ny, tw, s);
+    case 4096: return row_go<4096, TW>(v, mode, tm, b, count, ny, tw, s);
+    case 6144: return row_go<6144, TW>(v, mode, tm, b, count, ny, tw, s);
+    case 8192: return row_go<8192, TW>(v, mode, tm, b, count, ny, tw, s);
+    case 16384: return row_go<16384, TW>(v, mode, tm, b, count, ny, tw, s);
     default: return false;
   }
 }
@@ -79,13 +79,13 @@ bool fused_col_b(int N, int v, bool spec_out, const ColBatch& b, int count, int 
 
 // single-field launches
 template <typename TW>
-bool fused_row(int N, int v, int mode, bool lut, const void* in, void* out, int ny,
+bool fused_row(int N, int v, int mode, int tm, const void* in, void* out, int ny,
                const StageTw<TW>& tw, const TransArgsF& ta, hipStream_t s) {
   RowBatch b{};
   b.inst[0].in = reinterpret_cast<const cpx<float>*>(in);
   b.inst[0].out = reinterpret_cast<cpx<float>*>(out);
   b.inst[0].ta = ta;
-  return fused_row_b<TW>(N, v, mode, lut, b, 1, ny, tw, s);
+  return fused_row_b<TW>(N, v, mode, tm, b, 1, ny, tw, s);
 }
 
 template <typename TW>
@@ -262,7 +262,7 @@ int time_launches(hipStream_t s, const std::function<bool()>& launch, float* ms)
 }  // namespace
 
 template <typename TW>
-int fused_schedule(int nx, int ny, bool lut, const StageTw<TW>& tx, const StageTw<TW>& ty,
+int fused_schedule(int nx, int ny, int tm, const StageTw<TW>& tx, const StageTw<TW>& ty,
                    const TransArgsF& ta, const ChirpTables* ct, hipStream_t s, int* rv, int* cv) {
   if (const char* env = std::getenv("RAVESIM_FUSED_VARIANT")) {
     int r = 0, c = 0;
@@ -275,7 +275,7 @@ int fused_schedule(int nx, int ny, bool lut, const StageTw<TW>& tx, const StageT
   }
   std::lock_guard<std::mutex> lk(g_tune_mu);
   const int math = std::is_same<TW, twc>::value ? 4 : sizeof(TW) == 8 ? 2 : 0;
-  auto key = std::make_tuple(current_device(), nx, ny, (lut ? 1 : 0) | math);
+  auto key = std::make_tuple(current_device(), nx, ny, tm + 4 * math);
   auto it = g_tune.find(key);
   if (it != g_tune.end()) {
     *rv = it->second.first;
@@ -290,7 +290,7 @@ int fused_schedule(int nx, int ny, bool lut, const StageTw<TW>& tx, const StageT
   const bool verbose = std::getenv("RAVESIM_TUNE_VERBOSE") != nullptr;
   for (int v = 0; v < kRowVariants; ++v) {
     float ms = -1.f;
-    int rc = time_launches(s, [&] { return fused_row(nx, v, ROW_MID, lut, tmp, tmp, ny, tx, ta, s); }, &ms);
+    int rc = time_launches(s, [&] { return fused_row(nx, v, ROW_MID, tm, tmp, tmp, ny, tx, ta, s); }, &ms);
     if (rc != RSIM_OK) { hipFree(tmp); return rc; }
     if (ms > 0.f && ms < tr_best) { tr_best = ms; best_r = v; }
     if (verbose)
@@ -387,16 +387,33 @@ struct FusedPrep {
   StageTw<TW> tx{}, ty{};
   const ChirpTables* ct = nullptr;
   TransArgsF ta{};
-  bool lut = false;
-  size_t slice_bytes = 0;
+  int tm = TM_BILINEAR;
+  size_t slice_elems = 0;
   const uint8_t* ids0 = nullptr;
+  const float2* field0 = nullptr;
+  // point the transmission arguments at voxel slice sl
+  void at(TransArgsF& a, int sl) const {
+    if (tm == TM_FIELD)
+      a.dbv = field0 + (size_t)sl * slice_elems;
+    else
+      a.ids = ids0 + (size_t)sl * slice_elems;
+  }
 };
+
+static VoxSrc src_of(const rsim_volume* vol) {
+  VoxSrc v;
+  v.nz = vol->nz;
+  v.ny_g = vol->ny_g;
+  v.nx_g = vol->nx_g;
+  v.vol = vol;
+  return v;
+}
 
 // pz: slice thickness of the transmission; prop_dz: propagation distance of
 // the chirp (pz, or its cone-beam effective value); scale: voxel-map scale.
 template <typename TW>
 static int fused_prepare(int nx, int ny, double dx, double dy, double wl, double freq,
-                         const rsim_volume* vol, double px, double py, double pz, double xpos,
+                         const VoxSrc& src, double px, double py, double pz, double xpos,
                          double ypos, FusedPrep<TW>* P, double prop_dz = 0.0,
                          double scale = 1.0) {
   if (prop_dz == 0.0) prop_dz = pz;
@@ -413,16 +430,26 @@ static int fused_prepare(int nx, int ny, double dx, double dy, double wl, double
   RSIM_CHECK(chirp_tables(nx, ny, dx, dy, wl, prop_dz, freq, &P->ct));
   const AxisMap* xm = nullptr;
   const AxisMap* ym = nullptr;
-  RSIM_CHECK(axis_map(nx, dx, scale, xpos, vol->nx_g, px, &xm));
-  RSIM_CHECK(axis_map(ny, dy, scale, ypos, vol->ny_g, py, &ym));
+  RSIM_CHECK(axis_map(nx, dx, scale, xpos, src.nx_g, px, &xm));
+  RSIM_CHECK(axis_map(ny, dy, scale, ypos, src.ny_g, py, &ym));
   const double K = ((2.0 * M_PI) * pz) / wl;
   TransArgsF& ta = P->ta;
-  RSIM_CHECK(material_tables(vol, K, &ta.lut, &ta.dbf));
-  ta.nxg = vol->nx_g;
+  ta.nxg = src.nx_g;
   ta.xm = xm->dev;
   ta.ym = ym->dev;
   ta.K = (float)K;
-  P->lut = xm->all_integral && ym->all_integral && xm->affine && ym->affine;
+  P->slice_elems = (size_t)src.ny_g * src.nx_g;
+  if (src.vol) {
+    RSIM_CHECK(material_tables(src.vol, K, &ta.lut, &ta.dbf));
+    const bool lut = xm->all_integral && ym->all_integral && xm->affine && ym->affine;
+    P->tm = lut ? TM_LUT : TM_BILINEAR;
+    P->ids0 = src.vol->ids;
+    ta.ids = src.vol->ids;
+  } else {
+    P->tm = TM_FIELD;
+    P->field0 = src.field;
+    ta.dbv = src.field;
+  }
   ta.ax = xm->a;
   ta.cx = xm->c;
   ta.xlo = xm->lo;
@@ -431,27 +458,23 @@ static int fused_prepare(int nx, int ny, double dx, double dy, double wl, double
   ta.cy = ym->c;
   ta.ylo = ym->lo;
   ta.yhi = ym->hi;
-  P->slice_bytes = (size_t)vol->ny_g * vol->nx_g;
-  P->ids0 = vol->ids;
-  ta.ids = vol->ids;
   return RSIM_OK;
 }
 
 template <typename TW>
 static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, double wl,
-                     double freq, const rsim_volume* vol, double px, double py, double pz,
+                     double freq, const VoxSrc& src, double px, double py, double pz,
                      double xpos, double ypos, int first, int count, bool want_spec,
                      const double* scales, const double* dzs, hipStream_t s) {
   const int total = count;
   const int tdtype = std::is_same<TW, twc>::value ? kTwComp : sizeof(TW) == 8 ? RSIM_C128 : RSIM_C64;
   FusedPrep<TW> P;
-  RSIM_CHECK(fused_prepare<TW>(nx, ny, dx, dy, wl, freq, vol, px, py, pz, xpos, ypos, &P));
+  RSIM_CHECK(fused_prepare<TW>(nx, ny, dx, dy, wl, freq, src, px, py, pz, xpos, ypos, &P));
   const StageTw<TW>& tx = P.tx;
   const StageTw<TW>& ty = P.ty;
   const ChirpTables* ct = P.ct;
   TransArgsF ta = P.ta;
-  bool lut = P.lut;
-  const size_t slice_bytes = P.slice_bytes;
+  int tm = P.tm;
   // cone-beam runs: the voxel-map scale and the propagation distance change
   // from slice to slice (tables cached per value)
   const bool per_slice = scales || dzs;
@@ -469,24 +492,24 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
     if (!wN) return fail(RSIM_E_HIP, "missing FFT twiddle table for %d", ny);
   }
   const float spec_scale = (float)((double)nx * (double)ny);
-  ta.ids = vol->ids + (size_t)first * slice_bytes;
+  P.at(ta, first);
   int rv = 0, cv = 0;
-  RSIM_CHECK(fused_schedule<TW>(nx, ny, lut, tx, ty, ta, ct, s, &rv, &cv));
+  RSIM_CHECK(fused_schedule<TW>(nx, ny, tm, tx, ty, ta, ct, s, &rv, &cv));
   for (int k = 0; k < count; ++k) {
     const int sl = first + k;
     if (per_slice) {
       FusedPrep<TW> Pk;
-      RSIM_CHECK(fused_prepare<TW>(nx, ny, dx, dy, wl, freq, vol, px, py, pz, xpos, ypos, &Pk,
+      RSIM_CHECK(fused_prepare<TW>(nx, ny, dx, dy, wl, freq, src, px, py, pz, xpos, ypos, &Pk,
                                    dzs ? dzs[k] : pz, scales ? scales[k] : 1.0));
       ct = Pk.ct;
       ta = Pk.ta;
-      lut = Pk.lut;
-      RSIM_CHECK(fused_schedule<TW>(nx, ny, lut, tx, ty, ta, ct, s, &rv, &cv));
+      tm = Pk.tm;
+      RSIM_CHECK(fused_schedule<TW>(nx, ny, tm, tx, ty, ta, ct, s, &rv, &cv));
     }
-    ta.ids = vol->ids + (size_t)sl * slice_bytes;
+    P.at(ta, sl);
     const bool last = k == count - 1;
     ktime_begin(KT_ROW, s);
-    if (!fused_row(nx, rv, k == 0 ? ROW_FIRST : ROW_MID, lut, k == 0 ? u : U, U, ny, tx, ta, s))
+    if (!fused_row(nx, rv, k == 0 ? ROW_FIRST : ROW_MID, tm, k == 0 ? u : U, U, ny, tx, ta, s))
       return fail(RSIM_E_HIP, "no fused row kernel for nx=%d", nx);
     ktime_end(KT_ROW, s);
     RSIM_LAUNCHED();
@@ -508,7 +531,7 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
       // row IFFT of W; after a cancel this leaves u consistent at a slice
       // boundary (U then does not hold FFT2(u))
       ktime_begin(KT_ROW, s);
-      fused_row(nx, rv, ROW_LAST, lut, (last && !big) ? u : U, u, ny, tx, ta, s);
+      fused_row(nx, rv, ROW_LAST, tm, (last && !big) ? u : U, u, ny, tx, ta, s);
       ktime_end(KT_ROW, s);
       RSIM_LAUNCHED();
       if (big && last && want_spec) RSIM_CHECK(fft2(RSIM_C64, RSIM_FORWARD, u, U, nx, ny, s));
@@ -520,18 +543,17 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
   return RSIM_OK;
 }
 
-int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
-                 double freq, const rsim_volume* vol, double px, double py, double pz,
-                 double xpos, double ypos, int first, int count, const double* scales,
-                 const double* dzs, int engine, int flags, hipStream_t s) {
+int slices_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
+                 double freq, const VoxSrc& src, double px, double py, double pz, double xpos,
+                 double ypos, int first, int count, const double* scales, const double* dzs,
+                 int engine, int flags, hipStream_t s) {
   const int total = count;
   if (count == 0) return RSIM_OK;
   const bool want_spec = !(flags & RSIM_FLAG_NO_SPECTRUM);
   if (engine == RSIM_PLAIN || !fused_supported(nx, ny, dtype)) {
     for (int k = 0; k < count; ++k) {
       if (cancel_requested()) return fail(RSIM_E_CANCELLED, "multislice cancelled at slice %d", first + k);
-      RSIM_CHECK(transmission(u, nx, ny, dtype, dx, dy, wl, vol, first + k, px, py, pz, xpos, ypos,
-                              scales ? scales[k] : 1.0, s));
+      RSIM_CHECK(src.plain(u, dtype, first + k, scales ? scales[k] : 1.0, s));
       RSIM_CHECK(propagate_plain(u, U, nx, ny, dtype, dx, dy, wl, dzs ? dzs[k] : pz, freq, false, s));
       if (g_progress_sync()) RSIM_HIP(hipStreamSynchronize(s));
       report_progress("sample_slice", k + 1, total);
@@ -545,16 +567,28 @@ int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double 
   // (measured on 4096^2 fields: -3.5e-6 after 256 slices, -8e-6 after 512),
   // so deeper volumes default to compensated FP32.
   if (!(flags & (RSIM_FLAG_MATH_FP32 | RSIM_FLAG_MATH_FP64 | RSIM_FLAG_MATH_COMP32)) &&
-      vol->nz > kFp32MaxSlices)
+      src.nz > kFp32MaxSlices)
     flags |= RSIM_FLAG_MATH_COMP32;
   if (flags & RSIM_FLAG_MATH_COMP32)
-    return run_fused<twc>(u, U, nx, ny, dx, dy, wl, freq, vol, px, py, pz, xpos, ypos, first,
+    return run_fused<twc>(u, U, nx, ny, dx, dy, wl, freq, src, px, py, pz, xpos, ypos, first,
                           count, want_spec, scales, dzs, s);
   if (flags & RSIM_FLAG_MATH_FP64)
-    return run_fused<double>(u, U, nx, ny, dx, dy, wl, freq, vol, px, py, pz, xpos, ypos, first,
+    return run_fused<double>(u, U, nx, ny, dx, dy, wl, freq, src, px, py, pz, xpos, ypos, first,
                              count, want_spec, scales, dzs, s);
-  return run_fused<float>(u, U, nx, ny, dx, dy, wl, freq, vol, px, py, pz, xpos, ypos, first,
+  return run_fused<float>(u, U, nx, ny, dx, dy, wl, freq, src, px, py, pz, xpos, ypos, first,
                           count, want_spec, scales, dzs, s);
+}
+
+int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
+                 double freq, const rsim_volume* vol, double px, double py, double pz,
+                 double xpos, double ypos, int first, int count, const double* scales,
+                 const double* dzs, int engine, int flags, hipStream_t s) {
+  VoxSrc src = src_of(vol);
+  src.plain = [&](void* f, int dt, int slice, double scale, hipStream_t st) {
+    return transmission(f, nx, ny, dt, dx, dy, wl, vol, slice, px, py, pz, xpos, ypos, scale, st);
+  };
+  return slices_apply(u, U, nx, ny, dtype, dx, dy, wl, freq, src, px, py, pz, xpos, ypos, first,
+                      count, scales, dzs, engine, flags, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -571,13 +605,13 @@ static int batch_capture(const rsim_batch_instance* in, int count, int first, in
     if (!fused_supported(q.nx, q.ny, RSIM_C64) || q.ny == kBigCol)
       return fail(RSIM_E_UNSUPPORTED, "instance %d: %dx%d has no batched fused schedule", i, q.nx,
                   q.ny);
-    RSIM_CHECK(fused_prepare<TW>(q.nx, q.ny, q.dx, q.dy, q.wl, q.cutoff_freq, q.vol,
+    RSIM_CHECK(fused_prepare<TW>(q.nx, q.ny, q.dx, q.dy, q.wl, q.cutoff_freq, src_of(q.vol),
                                  q.pixel_size_x, q.pixel_size_y, q.pixel_size_z, q.x_pos,
                                  q.y_pos, &prep[i]));
   }
   struct Group {
     int nx, ny;
-    bool lut;
+    int tm;
     int rv, cv;
     std::vector<int> idx;
   };
@@ -585,10 +619,10 @@ static int batch_capture(const rsim_batch_instance* in, int count, int first, in
   for (int i = 0; i < count; ++i) {
     int gi = -1;
     for (size_t g = 0; g < groups.size(); ++g)
-      if (groups[g].nx == in[i].nx && groups[g].ny == in[i].ny && groups[g].lut == prep[i].lut)
+      if (groups[g].nx == in[i].nx && groups[g].ny == in[i].ny && groups[g].tm == prep[i].tm)
         gi = (int)g;
     if (gi < 0) {
-      groups.push_back(Group{in[i].nx, in[i].ny, prep[i].lut, 0, 0, {}});
+      groups.push_back(Group{in[i].nx, in[i].ny, prep[i].tm, 0, 0, {}});
       gi = (int)groups.size() - 1;
     }
     groups[gi].idx.push_back(i);
@@ -596,7 +630,7 @@ static int batch_capture(const rsim_batch_instance* in, int count, int first, in
   // launch shapes: autotuned (scratch allocation and timing) before capture
   for (auto& g : groups) {
     const FusedPrep<TW>& P = prep[g.idx[0]];
-    RSIM_CHECK(fused_schedule<TW>(g.nx, g.ny, g.lut, P.tx, P.ty, P.ta, P.ct, cs, &g.rv, &g.cv));
+    RSIM_CHECK(fused_schedule<TW>(g.nx, g.ny, g.tm, P.tx, P.ty, P.ta, P.ct, cs, &g.rv, &g.cv));
   }
   RSIM_HIP(hipStreamSynchronize(cs));
 
@@ -621,9 +655,9 @@ static int batch_capture(const rsim_batch_instance* in, int count, int first, in
           b.inst[j].in = reinterpret_cast<const cpx<float>*>(src);
           b.inst[j].out = reinterpret_cast<cpx<float>*>(dst);
           b.inst[j].ta = P.ta;
-          b.inst[j].ta.ids = P.ids0 + (size_t)(first + k) * P.slice_bytes;
+          P.at(b.inst[j].ta, first + k);
         }
-        ok = fused_row_b<TW>(g.nx, g.rv, mode, g.lut, b, cnt, g.ny, prep[g.idx[c0]].tx, cs) &&
+        ok = fused_row_b<TW>(g.nx, g.rv, mode, g.tm, b, cnt, g.ny, prep[g.idx[c0]].tx, cs) &&
              hipGetLastError() == hipSuccess;
         ++kern;
       }

@@ -185,6 +185,41 @@ int rsim_transmission_2d_scaled(void* u, int nx, int ny, int dtype, double dx, d
 int rsim_tilted_plane_2d(void* u, int nx, int ny, int dtype, double dx, double dy, double wl,
                          double dz, double x_source, double y_source, void* stream);
 
+/* ---- PlasmaSample (big-wave/plasma_sample.py:126-233, big-wave/plasma.py:380-434) ----
+ * A plasma volume: four (nz, ny_g, nx_g) grids -- electron density n_e and ion
+ * density n_i [cm^-3], electron temperature T_e [eV] and mean ionisation Z* --
+ * of float32 (elem_size 4) or float64 (8), copied to the device.  ``optics``
+ * is the per-atom Chantler quadruple (f1bar, f2eff_bar, zbar, z2bar) of the
+ * element or compound at ``energy`` [eV] (plasma.py:238-262; the
+ * plasma_optics_table of a prepared subconfig).
+ * rsim_plasma_deltabeta: (delta, beta) per voxel of slices [first, first + n)
+ * as complex128 into the device buffer ``out`` (plasma_delta_beta_grid_multi,
+ * plasma.py:380-434, FP64 in its operation order).
+ * rsim_plasma_transmission_2d: one slice's transmission, bilinear in (delta,
+ * beta), vacuum outside the grid (plasma_sample.py:160-218).
+ * rsim_plasma_apply_2d: PlasmaSample._apply_2d over slices [first, first + n)
+ * -- transmission then propagate_2d per slice, optional cone-beam scale and
+ * distance arrays as in rsim_sample_apply_2d_scaled; engine RSIM_PLAIN or
+ * RSIM_FUSED (complex64; the FP32 (delta, beta) volume of each (plasma,
+ * energy, optics) is built once and cached until rsim_plasma_destroy). */
+typedef struct rsim_plasma rsim_plasma;
+int rsim_plasma_create(rsim_plasma** out, const void* ne, const void* ni, const void* te,
+                       const void* zstar, int elem_size, int nz, int ny_g, int nx_g);
+int rsim_plasma_destroy(rsim_plasma* plasma);
+int rsim_plasma_deltabeta(const rsim_plasma* plasma, int first_slice, int n_slices, double energy,
+                          const double* optics, void* out, void* stream);
+int rsim_plasma_transmission_2d(void* u, int nx, int ny, int dtype, double dx, double dy,
+                                double wl, const rsim_plasma* plasma, int slice, double energy,
+                                const double* optics, double pixel_size_x, double pixel_size_y,
+                                double pixel_size_z, double x_pos, double y_pos,
+                                double coord_scale, void* stream);
+int rsim_plasma_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
+                         double wl, double cutoff_freq, const rsim_plasma* plasma, double energy,
+                         const double* optics, double pixel_size_x, double pixel_size_y,
+                         double pixel_size_z, double x_pos, double y_pos, int first_slice,
+                         int n_slices, const double* coord_scale, const double* slice_dz,
+                         int engine, int flags, void* stream);
+
 /* ---- Batched fused multislice (no counterpart in the reference: the
  * reference runs each (source, energy) instance as its own process,
  * fast-wave/Readme.md:27) ----

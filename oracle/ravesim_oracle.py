@@ -48,6 +48,15 @@ Pinning (what checks each function against something other than itself):
   linear phase (tests/fresnel_scaling/test_big_wave_source.py:46-73).
 * ``sample_apply_2d`` (CB-BPM)  -- sampled voxel coordinates at m = 1.1 and
   the slice distance 0.2/1.2 (test_big_wave_cone_beam_sample.py:21-76).
+* ``plasma_delta_beta_grid_multi`` -- delta_free = 1.077221e-5 at n_e = 1e24
+  cm^-3, 8 keV, and the Kramers beta_ff of fully / partially ionised Al
+  (7.61e-12 / 4.68e-12; tests/plasma_sample/TEST_REPORT.md:76, 100-106);
+  scaling laws of beta_ff (n_e n_i Z*^2 T_e^-1/2 E^-4), q clipping, vacuum.
+* ``plasma_sample_apply_2d``     -- uniform-slab phase pi and 2 pi and
+  Beer-Lambert modulus by the reference's gold-standard ratio method
+  (tests/plasma_sample/test_plasma_phase_absorption.py:60-101, 187-250);
+  equal to ``sample_apply_2d`` on a material-id volume with the same
+  per-voxel (delta, beta).
 * ``run_point_source_sample``   -- the reference's own end-to-end golden
   fingerprints tests/big_wave_2d/p5_results/{vacuum,sample}_convergence_4096.json
   (inputs in tests/big_wave_2d/p5_benchmark.py:69-185).
@@ -339,6 +348,116 @@ def sample_apply_2d(u2d: np.ndarray, sample: SampleSpec, dx: float, dy: float, w
                                     scale)
         t = material_factor(db, pz, wl)
         u = (u * t).astype(u.dtype, copy=False)
+        u, U = propagate_2d(u, dx, dy, wl, dz, cutoff_freq, workers=workers)
+    return u, U
+
+
+# ---------------------------------------------------------------------------
+# PlasmaSample: big-wave/plasma.py:380-434, big-wave/plasma_sample.py:126-233
+# ---------------------------------------------------------------------------
+
+# nist_lookup/nist_lookup/physical_constants.py:20-27 under the reference's
+# pinned scipy~=1.10 (requirements.txt:4), i.e. CODATA 2018.
+PLANCK_HC = 1.e10 * 6.62607015e-34 * 299792458.0 / 1.602176634e-19
+R_ELECTRON_CM = 100.0 * 2.8179403262e-15
+
+
+def plasma_delta_beta_grid_multi(ne, ni, te, zstar, f1bar, f2eff_bar, zbar, z2bar, energy):
+    """(delta, beta) per voxel (plasma.py:380-434): free electrons, the bound
+    fraction (1 - q) of the Chantler terms, q = clip(Z*/zbar, 0, 1), and
+    Kramers inverse bremsstrahlung with mean-square charge q^2 z2bar."""
+    ne = np.asarray(ne, dtype=np.float64)
+    ni = np.asarray(ni, dtype=np.float64)
+    te = np.asarray(te, dtype=np.float64)
+    zstar = np.asarray(zstar, dtype=np.float64)
+    if energy <= 0.0:
+        raise ValueError(f"energy must be positive, got {energy}")
+    lamb_cm = 1.0e-8 * PLANCK_HC / energy
+    prefactor = R_ELECTRON_CM * lamb_cm * lamb_cm / (2.0 * np.pi)
+    delta = ne * prefactor
+    beta = np.zeros(ne.shape, dtype=np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        q = np.where(zbar > 0.0, zstar / zbar, 0.0)
+    q = np.clip(q, 0.0, 1.0)
+    fraction_bound = 1.0 - q
+    bound = (fraction_bound > 0.0) & (ni > 0.0)
+    bound_scale = ni * prefactor * fraction_bound
+    delta = delta + np.where(bound, bound_scale * f1bar, 0.0)
+    beta = beta + np.where(bound, bound_scale * f2eff_bar, 0.0)
+    ff = (ne > 0.0) & (ni > 0.0) & (q > 0.0) & (z2bar > 0.0) & (te > 0.0)
+    nu = energy / 4.135667e-15
+    alpha = np.zeros(ne.shape, dtype=np.float64)
+    alpha[ff] = (3.7e8 * ne[ff] * ni[ff] * (np.square(q[ff]) * z2bar)
+                 / (np.sqrt(te[ff] * 11604.5) * nu ** 3))
+    beta = beta + alpha * lamb_cm / (4.0 * np.pi)
+    return delta, beta
+
+
+@dataclass
+class PlasmaSpec:
+    """The fields of big-wave's PlasmaSample (plasma_sample.py:40-66) the 2D path reads."""
+    ne: np.ndarray              # (nz, ny_g, nx_g) [cm^-3]
+    ni: np.ndarray
+    te: np.ndarray              # [eV]
+    zstar: np.ndarray
+    optics: tuple               # (f1bar, f2eff_bar, zbar, z2bar) at the run's energy
+    pixel_size_x: float
+    pixel_size_z: float
+    pixel_size_y: float = 0.0
+    x_pos: float = 0.0
+    y_pos: float = 0.0
+    z_start: float = 0.0
+
+    @property
+    def py(self) -> float:
+        return self.pixel_size_y if self.pixel_size_y != 0.0 else self.pixel_size_x
+
+
+def plasma_deltabeta_slice(spec: PlasmaSpec, rowidx: int, nx: int, ny: int, dx: float, dy: float,
+                           energy: float, scale: float = 1.0) -> np.ndarray:
+    """Bilinear delta+i*beta of one plasma slice on the field grid, 0 outside
+    (plasma_sample.py:160-217): the four corners' plasma optics, then the
+    Sample interpolation."""
+    _, y_len, x_len = spec.ne.shape
+    x0, xf, inside_x = axis_map(nx, dx, scale, spec.x_pos, x_len, spec.pixel_size_x)
+    y0, yf, inside_y = axis_map(ny, dy, scale, spec.y_pos, y_len, spec.py)
+    X0 = x0[None, :]
+    Y0 = y0[:, None]
+
+    def corner(oy, ox):
+        idx = (rowidx, Y0 + oy, X0 + ox)
+        d, b = plasma_delta_beta_grid_multi(spec.ne[idx], spec.ni[idx], spec.te[idx],
+                                            spec.zstar[idx], *spec.optics, energy)
+        return d + 1j * b
+
+    db00, db01, db10, db11 = corner(0, 0), corner(0, 1), corner(1, 0), corner(1, 1)
+    db_y0 = db00 * (1.0 - xf[None, :]) + db01 * xf[None, :]
+    db_y1 = db10 * (1.0 - xf[None, :]) + db11 * xf[None, :]
+    interp = db_y0 * (1.0 - yf[:, None]) + db_y1 * yf[:, None]
+    interp[~(inside_y[:, None] & inside_x[None, :])] = 0.0
+    return interp
+
+
+def plasma_sample_apply_2d(u2d: np.ndarray, spec: PlasmaSpec, dx: float, dy: float, wl: float,
+                           cutoff_freq: float, first_slice: int = 0, n_slices: int | None = None,
+                           workers: int = 1, geometry: FresnelGeometry | None = None
+                           ) -> tuple[np.ndarray, np.ndarray]:
+    """PlasmaSample._apply_2d (plasma_sample.py:126-233): per slice u *=
+    exp(-2 pi i pz/wl conj(db)) with db from the plasma optics at
+    energy = convert_energy_wavelength(wl), then propagate_2d."""
+    energy = convert_energy_wavelength(wl)
+    ny, nx = u2d.shape
+    nz = spec.ne.shape[0]
+    last = nz if n_slices is None else min(nz, first_slice + n_slices)
+    u = u2d.copy()
+    U = None
+    pz = spec.pixel_size_z
+    for rowidx in range(first_slice, last):
+        slice_z = spec.z_start + rowidx * pz
+        scale = geometry.transverse_scale(slice_z + 0.5 * pz) if geometry else 1.0
+        dz = geometry.slice_dz(slice_z, pz) if geometry else pz
+        db = plasma_deltabeta_slice(spec, rowidx, nx, ny, dx, dy, energy, scale)
+        u = (u * material_factor(db, pz, wl)).astype(u.dtype, copy=False)
         u, U = propagate_2d(u, dx, dy, wl, dz, cutoff_freq, workers=workers)
     return u, U
 

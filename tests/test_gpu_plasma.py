@@ -1,0 +1,190 @@
+"""GPU parity of the PlasmaSample path versus the oracle.
+
+Per-voxel plasma optics: FP64 in the reference's operation order, compared
+at 4e-16 relative (a few ulp).  Transmission: complex128 1e-14, complex64
+1e-7.  Slice loop: plain complex128 1e-12, plain and fused complex64 3e-6,
+fused vs plain complex128 over 40 slices 1e-5 (relative L2).
+"""
+
+import importlib
+import sys
+
+import numpy as np
+import pytest
+import yaml
+
+import oracle as O
+from _prepared import write_prepared
+from _rs import lib, rs
+
+pytestmark = pytest.mark.gpu
+MS = importlib.import_module("3d_ravesim_amd.multisim")
+OPTICS = (13.31, 0.187, 13.0, 169.0)
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def dev(x):
+    return rs.DeviceVector.from_numpy(np.ascontiguousarray(x).reshape(-1))
+
+
+def host(v, ny, nx):
+    return v.to_numpy().reshape(ny, nx)
+
+
+def plasma_grids(rng, nz, ny, nx, dtype=np.float64):
+    """Blobs of plasma in vacuum with every branch of the optics: fully and
+    partially ionised, Z* above zbar, cold (T_e = 0) and neutral (Z* = 0)."""
+    zz, yy, xx = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ne = np.zeros((nz, ny, nx))
+    ni = np.zeros_like(ne)
+    te = np.zeros_like(ne)
+    zs = np.zeros_like(ne)
+    for _ in range(6):
+        c = rng.uniform([0, 0, 0], [nz, ny, nx])
+        r = rng.uniform(0.15, 0.35) * min(ny, nx)
+        m = (zz - c[0]) ** 2 * 4 + (yy - c[1]) ** 2 + (xx - c[2]) ** 2 < r * r
+        ne[m] = rng.uniform(1e21, 3e23) * (1 + 0.3 * rng.uniform(-1, 1, m.sum()))
+        zs[m] = rng.choice([0.0, 3.0, 8.0, 13.0, 15.0])
+        ni[m] = ne[m] / max(zs[m].max(), 1.0)
+        te[m] = rng.choice([0.0, 50.0, 300.0]) * (1 + 0.1 * rng.uniform(-1, 1, m.sum()))
+    return [g.astype(dtype) for g in (ne, ni, te, zs)]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_plasma_deltabeta_matches_oracle(dtype):
+    rng = np.random.default_rng(5)
+    g = plasma_grids(rng, 4, 40, 52, dtype)
+    p = rs.DevicePlasma(*g)
+    say("plasma deltabeta", np.dtype(dtype).name)
+    for energy in (8000.0, 2500.0):
+        got = p.deltabeta(energy, OPTICS)
+        d, b = O.plasma_delta_beta_grid_multi(*g, *OPTICS, energy)
+        for a, e in ((got.real, d), (got.imag, b)):
+            scale = np.maximum(np.abs(e), 1e-300)
+            assert np.max(np.abs(a - e) / scale * (e != 0)) < 4e-16
+            assert np.array_equal(a == 0, e == 0)
+    # a tile of the reference's function, through the mirror
+    dl, bt, at = rs.plasma_delta_beta_grid_multi(g[0][1], g[1][1], g[2][1], g[3][1], *OPTICS, 8000.0)
+    d, b = O.plasma_delta_beta_grid_multi(g[0][1], g[1][1], g[2][1], g[3][1], *OPTICS, 8000.0)
+    assert np.allclose(dl, d, rtol=4e-16, atol=0) and np.allclose(bt, b, rtol=4e-16, atol=0)
+    p.destroy()
+
+
+@pytest.mark.parametrize("scale", [1.0, 1.07])
+def test_plasma_transmission_matches_oracle(scale):
+    rng = np.random.default_rng(6)
+    nx, ny, dx, dy = 96, 64, 1e-6, 1.2e-6
+    g = plasma_grids(rng, 2, 50, 70)
+    wl = O.convert_energy_wavelength(7000.0)
+    energy = O.convert_energy_wavelength(wl)
+    spec = O.PlasmaSpec(*g, optics=OPTICS, pixel_size_x=1.37e-6, pixel_size_z=2e-6,
+                        pixel_size_y=1.21e-6, x_pos=0.33e-6, y_pos=-0.71e-6)
+    p = rs.DevicePlasma(*g)
+    u0 = rng.normal(size=(ny, nx)) + 1j * rng.normal(size=(ny, nx))
+    opt = np.array(OPTICS)
+    db = O.plasma_deltabeta_slice(spec, 1, nx, ny, dx, dy, energy, scale)
+    exp = u0 * O.material_factor(db, spec.pixel_size_z, wl)
+    for dt, tol in ((np.complex128, 1e-14), (np.complex64, 1e-7)):
+        u = dev(u0.astype(dt))
+        say("plasma transmission", scale, dt.__name__)
+        lib.call("rsim_plasma_transmission_2d", u.ptr, nx, ny, u.code, dx, dy, wl, p.handle, 1,
+                 energy, opt.ctypes.data_as(lib._dp), spec.pixel_size_x, spec.py,
+                 spec.pixel_size_z, spec.x_pos, spec.y_pos, scale, None)
+        assert O.relative_l2(host(u, ny, nx), exp) < tol
+
+
+def _plasma_case(n, nz, seed, pitch=1.0):
+    rng = np.random.default_rng(seed)
+    dx = 0.5e-6
+    g = plasma_grids(rng, nz, int(n * pitch) + 6, int(n * pitch) + 4)
+    spec = O.PlasmaSpec(*g, optics=OPTICS, pixel_size_x=dx / pitch, pixel_size_z=1e-6,
+                        pixel_size_y=dx / pitch, z_start=1e-4)
+    u0 = np.exp(1j * rng.uniform(0, 0.1, size=(n, n))) * (1 + 0.01 * rng.normal(size=(n, n)))
+    return g, spec, u0, dx
+
+
+def _mirror(spec, g, engine):
+    return rs.PlasmaSample(z_start=spec.z_start, pixel_size_x=spec.pixel_size_x,
+                           pixel_size_z=spec.pixel_size_z, ne_grid=g[0], ni_grid=g[1],
+                           te_grid=g[2], zstar_grid=g[3], Z=13, x_positions=np.array([0.0]),
+                           pixel_size_y=spec.py, optics=OPTICS, engine=engine)
+
+
+@pytest.mark.parametrize("cone", [False, True])
+@pytest.mark.parametrize("engine", ["plain", "fused"])
+def test_plasma_sample_apply_matches_oracle(engine, cone):
+    n, nz = 256, 6
+    g, spec, u0, dx = _plasma_case(n, nz, 21)
+    wl = O.convert_energy_wavelength(6000.0)
+    cut = 0.4 / dx
+    geom = O.FresnelGeometry(z_detector=0.05, cone_beam=True).configure(0.0, 1e-4) if cone else None
+    ue, Ue = O.plasma_sample_apply_2d(u0, spec, dx, dx, wl, cut, geometry=geom)
+    p = rs.SimParams(N=n * n, ny=n, dx=dx, z_detector=0.05, detector_size=n * dx,
+                     detector_pixel_size_x=dx, detector_pixel_size_y=dx, wl=wl, chunk_size=n,
+                     use_cone_beam_bpm=cone)
+    p.configure_fresnel_detector(z_source=0.0, z_sample=1e-4)
+    cases = [(np.complex64, 3e-6)] if engine == "fused" else [(np.complex128, 1e-12),
+                                                              (np.complex64, 3e-6)]
+    s = _mirror(spec, g, engine)
+    for dt, tol in cases:
+        u, U = dev(u0.astype(dt)), rs.DeviceVector(n * n, dt)
+        say("plasma sample_apply", engine, "cone" if cone else "flat", dt.__name__)
+        s.apply(u, U, p, cut, 0, None)
+        assert O.relative_l2(host(u, n, n), ue) < tol, (engine, dt)
+        assert O.relative_l2(host(U, n, n), Ue) < tol, (engine, dt)
+
+
+def test_plasma_fused_deep_matches_plain_c128():
+    n, nz = 1024, 40
+    g, spec, u0, dx = _plasma_case(n, nz, 23, pitch=0.5)
+    wl = O.convert_energy_wavelength(6000.0)
+    p = rs.SimParams(N=n * n, ny=n, dx=dx, z_detector=0.05, detector_size=n * dx,
+                     detector_pixel_size_x=dx, detector_pixel_size_y=dx, wl=wl, chunk_size=n)
+    out = {}
+    for engine, dt in (("plain", np.complex128), ("fused", np.complex64)):
+        s = _mirror(spec, g, engine)
+        u, U = dev(u0.astype(dt)), rs.DeviceVector(n * n, dt)
+        say("plasma deep", engine, dt.__name__)
+        s.apply(u, U, p, 0.4 / dx, 0, None)
+        out[engine] = host(u, n, n)
+    assert O.relative_l2(out["fused"], out["plain"]) < 1e-5
+
+
+def test_runner_with_plasma_sample(tmp_path):
+    """A prepared directory with a plasma_sample element and the subconfig's
+    plasma_optics_table, run like fast-wave, against the oracle."""
+    n = 256
+    rng = np.random.default_rng(29)
+    root, _ = write_prepared(tmp_path, n=n, nz=2, dtype="c16", grid=np.zeros((2, 8, 8), np.uint32))
+    g = plasma_grids(rng, 5, n + 6, n + 4)
+    for name, a in zip(("ne", "ni", "te", "zs"), g):
+        np.save(root / f"{name}.npy", a)
+    cfg = yaml.safe_load(open(root / "config.yaml"))
+    dx = cfg["sim_params"]["dx"]
+    cfg["elements"] = [{"type": "plasma_sample", "z_start": 0.6, "pixel_size_x": dx,
+                        "pixel_size_y": dx, "pixel_size_z": 1e-6, "ne_grid_path": "ne.npy",
+                        "ni_grid_path": "ni.npy", "te_grid_path": "te.npy",
+                        "zstar_grid_path": "zs.npy", "Z": 13, "x_positions": [0.0]}]
+    yaml.safe_dump(cfg, open(root / "config.yaml", "w"))
+    sub = yaml.safe_load(open(root / "00000000" / "subconfig.yaml"))
+    sub["plasma_optics_table"] = [dict(zip(("f1bar", "f2eff_bar", "zbar", "z2bar"), OPTICS))]
+    yaml.safe_dump(sub, open(root / "00000000" / "subconfig.yaml", "w"))
+    say("runner plasma_sample")
+    out = MS.run_single_simulation(root, 0)
+    det = np.load(out / "detected.npy")
+    sp = cfg["sim_params"]
+    wl = O.convert_energy_wavelength(sub["energy"])
+    cut = np.sin(yaml.safe_load(open(root / "computed.yaml"))["cutoff_angles"][0]) / wl
+    src = sub["source"]
+    u = O.point_source_propagate_to(n, n, dx, dx, 0.6 - src["z"], src["x"], src["y"], wl, cut)
+    spec = O.PlasmaSpec(*g, optics=OPTICS, pixel_size_x=dx, pixel_size_z=1e-6, pixel_size_y=dx)
+    u, U = O.plasma_sample_apply_2d(u, spec, dx, dx, wl, cut)
+    u, _ = O.propagate_2d(u, dx, dx, wl, sp["z_detector"] - (0.6 + 1e-6 * 5), cut, U2d=U)
+    exp = O.area_v1(u, dx, dx, sp["detector_size_x"], sp["detector_size_y"],
+                    sp["detector_pixel_size_x"], sp["detector_pixel_size_y"], sp["z_detector"])
+    assert det.shape == (1, *exp.shape)
+    assert O.relative_l2(det[0], exp) < 1e-11
