@@ -50,6 +50,12 @@ class BatchInstance(C.Structure):
                 ("pixel_size_y", _d), ("pixel_size_z", _d), ("x_pos", _d), ("y_pos", _d)]
 
 
+class HistoryFrames(C.Structure):
+    """rsim_history_frames of include/ravesim.h."""
+    _fields_ = [("out_x", _i), ("out_y", _i), ("pixel_size_x", _d), ("pixel_size_y", _d),
+                ("z_start", _d), ("frames", _vp)]
+
+
 # name -> argtypes (restype is int unless listed in _RESTYPES)
 SIGNATURES = {
     "rsim_last_error": [],
@@ -101,6 +107,11 @@ SIGNATURES = {
     "rsim_transmission_2d_scaled": [_vp, _i, _i, _i, _d, _d, _d, _vp, _i, _d, _d, _d, _d, _d,
                                     _d, _vp],
     "rsim_tilted_plane_2d": [_vp, _i, _i, _i, _d, _d, _d, _d, _d, _d, _vp],
+    "rsim_sample_apply_2d_history": [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _vp, _d, _d, _d, _d,
+                                     _d, _i, _i, _dp, _dp, _i, _i, C.POINTER(HistoryFrames), _vp],
+    "rsim_plasma_apply_2d_history": [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _vp, _d, _dp, _d, _d,
+                                     _d, _d, _d, _i, _i, _dp, _dp, _i, _i,
+                                     C.POINTER(HistoryFrames), _vp],
     "rsim_plasma_create": [C.POINTER(_vp), _vp, _vp, _vp, _vp, _i, _i, _i, _i],
     "rsim_plasma_destroy": [_vp],
     "rsim_plasma_deltabeta": [_vp, _i, _i, _d, _dp, _vp, _vp],

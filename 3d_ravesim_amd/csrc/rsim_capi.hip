@@ -28,7 +28,8 @@ int propagate(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
 int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
                  double freq, const rsim_volume* vol, double px, double py, double pz,
                  double xpos, double ypos, int first, int count, const double* scales,
-                 const double* dzs, int engine, int flags, hipStream_t s);
+                 const double* dzs, int engine, int flags, hipStream_t s,
+                 const HistSpec* hist = nullptr);
 int detector_area_v1_adjoint(const void* v, void* g, int nx, int ny, int dtype, double dx,
                              double dy, int ox, int oy, double dsx, double dsy, double z,
                              const double* gimg_dev, hipStream_t s);
@@ -50,6 +51,7 @@ int rsim_release_tables();
 int release_tables();
 int release_multislice_tables();
 int release_tune();
+int release_detector_plans();
 int fused_schedule_lookup(int nx, int ny, int* rv, int* cv);
 
 namespace {
@@ -320,6 +322,7 @@ int rsim_release_caches(void) {
   release_tables();
   release_multislice_tables();
   release_tune();
+  release_detector_plans();
   return RSIM_OK;
 }
 
@@ -552,12 +555,33 @@ int rsim_sample_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx,
                       nullptr, engine, flags, resolve_stream(stream));
 }
 
-int rsim_sample_apply_2d_scaled(void* u, void* U, int nx, int ny, int dtype, double dx,
-                                double dy, double wl, double cutoff_freq, const rsim_volume* vol,
-                                double pixel_size_x, double pixel_size_y, double pixel_size_z,
-                                double x_pos, double y_pos, int first_slice, int n_slices,
-                                const double* coord_scale, const double* slice_dz, int engine,
-                                int flags, void* stream) {
+}  // extern "C"
+
+namespace {
+int history_spec(const rsim_history_frames* h, int n_slices, HistSpec* out) {
+  if (!h) return RSIM_OK;
+  if (!h->frames && n_slices > 0) return fail(RSIM_E_INVALID_ARG, "history frames buffer is NULL");
+  if (h->out_x <= 0 || h->out_y <= 0)
+    return fail(RSIM_E_INVALID_ARG, "2D detector output must be non-empty, got (%d, %d)", h->out_y,
+                h->out_x);
+  if (!(h->pixel_size_x > 0) || !(h->pixel_size_y > 0))
+    return fail(RSIM_E_INVALID_ARG, "2D detector grid and pixel spacings must be positive");
+  if (!std::isfinite(h->z_start)) return fail(RSIM_E_INVALID_ARG, "history z_start must be finite");
+  out->ox = h->out_x;
+  out->oy = h->out_y;
+  out->dsx = h->pixel_size_x;
+  out->dsy = h->pixel_size_y;
+  out->z_start = h->z_start;
+  out->frames = h->frames;
+  return RSIM_OK;
+}
+
+int sample_apply_checked(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
+                         double wl, double cutoff_freq, const rsim_volume* vol,
+                         double pixel_size_x, double pixel_size_y, double pixel_size_z,
+                         double x_pos, double y_pos, int first_slice, int n_slices,
+                         const double* coord_scale, const double* slice_dz, int engine, int flags,
+                         const rsim_history_frames* history, void* stream) {
   RSIM_CHECK(check_field(u, nx, ny, dtype, "u"));
   RSIM_CHECK(check_field(U, nx, ny, dtype, "U"));
   RSIM_CHECK(check_fft_sizes(nx, ny));
@@ -571,9 +595,39 @@ int rsim_sample_apply_2d_scaled(void* u, void* U, int nx, int ny, int dtype, dou
     if (slice_dz && !std::isfinite(slice_dz[k]))
       return fail(RSIM_E_INVALID_ARG, "slice_dz[%d] must be finite", k);
   }
+  HistSpec hs;
+  RSIM_CHECK(history_spec(history, n_slices, &hs));
   return sample_apply(u, U, nx, ny, dtype, dx, dy, wl, cutoff_freq, vol, pixel_size_x,
                       pixel_size_y, pixel_size_z, x_pos, y_pos, first_slice, n_slices,
-                      coord_scale, slice_dz, engine, flags, resolve_stream(stream));
+                      coord_scale, slice_dz, engine, flags, resolve_stream(stream),
+                      history ? &hs : nullptr);
+}
+}  // namespace
+
+extern "C" {
+
+int rsim_sample_apply_2d_scaled(void* u, void* U, int nx, int ny, int dtype, double dx,
+                                double dy, double wl, double cutoff_freq, const rsim_volume* vol,
+                                double pixel_size_x, double pixel_size_y, double pixel_size_z,
+                                double x_pos, double y_pos, int first_slice, int n_slices,
+                                const double* coord_scale, const double* slice_dz, int engine,
+                                int flags, void* stream) {
+  return sample_apply_checked(u, U, nx, ny, dtype, dx, dy, wl, cutoff_freq, vol, pixel_size_x,
+                              pixel_size_y, pixel_size_z, x_pos, y_pos, first_slice, n_slices,
+                              coord_scale, slice_dz, engine, flags, nullptr, stream);
+}
+
+int rsim_sample_apply_2d_history(void* u, void* U, int nx, int ny, int dtype, double dx,
+                                 double dy, double wl, double cutoff_freq,
+                                 const rsim_volume* vol, double pixel_size_x,
+                                 double pixel_size_y, double pixel_size_z, double x_pos,
+                                 double y_pos, int first_slice, int n_slices,
+                                 const double* coord_scale, const double* slice_dz, int engine,
+                                 int flags, const rsim_history_frames* history, void* stream) {
+  if (!history) return fail(RSIM_E_INVALID_ARG, "history is NULL");
+  return sample_apply_checked(u, U, nx, ny, dtype, dx, dy, wl, cutoff_freq, vol, pixel_size_x,
+                              pixel_size_y, pixel_size_z, x_pos, y_pos, first_slice, n_slices,
+                              coord_scale, slice_dz, engine, flags, history, stream);
 }
 
 int rsim_fused_schedule(int nx, int ny, int* row_variant, int* col_variant) {

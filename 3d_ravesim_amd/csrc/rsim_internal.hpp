@@ -82,6 +82,14 @@ struct VoxSrc {
   const float2* field = nullptr;
   std::function<int(void* u, int dtype, int slice, double scale, hipStream_t s)> plain;
 };
+// History frames of a slice loop (optical_element.py:548-551): frame k is the
+// area_v1 image of the field entering slice first + k, taken at
+// z = z_start + (first + k) * pz, into frames + k * oy * ox (device).
+struct HistSpec {
+  int ox = 0, oy = 0;
+  double dsx = 0.0, dsy = 0.0, z_start = 0.0;
+  double* frames = nullptr;
+};
 // Slices [first, first + count) of a sample applied to u (optical_element.py:
 // 502-596): transmission, then propagation over dzs[k] (pz when null) with the
 // voxel map scaled by scales[k] (1 when null).  U = FFT2(u) afterwards unless
@@ -89,7 +97,10 @@ struct VoxSrc {
 int slices_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
                  double freq, const VoxSrc& src, double px, double py, double pz, double xpos,
                  double ypos, int first, int count, const double* scales, const double* dzs,
-                 int engine, int flags, hipStream_t s);
+                 int engine, int flags, hipStream_t s, const HistSpec* hist = nullptr);
+int detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double dy, int ox,
+                     int oy, double dsx, double dsy, double z, double weight, bool accumulate,
+                     double* out_dev, hipStream_t s);
 bool fused_supported(int nx, int ny, int dtype);
 int propagate(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
               double dz, double freq, bool skip_fft, hipStream_t s);

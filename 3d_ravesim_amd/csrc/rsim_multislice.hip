@@ -380,6 +380,14 @@ int propagate(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
   return propagate_plain(u, U, nx, ny, dtype, dx, dy, wl, dz, freq, skip_fft, s);
 }
 
+// History frame k: area_v1 of the field in f at the entry plane of slice sl.
+static int hist_frame(const void* f, int nx, int ny, double dx, double dy, double pz, int sl,
+                      int k, const HistSpec* h, hipStream_t s, int dtype = RSIM_C64) {
+  const double z = h->z_start + sl * pz;   // optical_element.py:548
+  return detector_area_v1(f, nx, ny, dtype, dx, dy, h->ox, h->oy, h->dsx, h->dsy, z, 1.0, false,
+                          h->frames + (size_t)k * h->oy * h->ox, s);
+}
+
 // Tables and transmission arguments of one fused run (all cached; built
 // before any launch, and before graph capture in the batched form).
 template <typename TW>
@@ -465,7 +473,8 @@ template <typename TW>
 static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, double wl,
                      double freq, const VoxSrc& src, double px, double py, double pz,
                      double xpos, double ypos, int first, int count, bool want_spec,
-                     const double* scales, const double* dzs, hipStream_t s) {
+                     const double* scales, const double* dzs, hipStream_t s,
+                     const HistSpec* hist) {
   const int total = count;
   const int tdtype = std::is_same<TW, twc>::value ? kTwComp : sizeof(TW) == 8 ? RSIM_C128 : RSIM_C64;
   FusedPrep<TW> P;
@@ -508,6 +517,15 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
     }
     P.at(ta, sl);
     const bool last = k == count - 1;
+    if (hist) {
+      // the field entering this slice: the input, or the row IFFT of W (U)
+      // materialised in u, which the loop does not read after slice 0
+      if (k > 0) {
+        fused_row(nx, rv, ROW_LAST, tm, U, u, ny, tx, ta, s);
+        RSIM_LAUNCHED();
+      }
+      RSIM_CHECK(hist_frame(u, nx, ny, dx, dy, pz, sl, k, hist, s));
+    }
     ktime_begin(KT_ROW, s);
     if (!fused_row(nx, rv, k == 0 ? ROW_FIRST : ROW_MID, tm, k == 0 ? u : U, U, ny, tx, ta, s))
       return fail(RSIM_E_HIP, "no fused row kernel for nx=%d", nx);
@@ -546,13 +564,14 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
 int slices_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
                  double freq, const VoxSrc& src, double px, double py, double pz, double xpos,
                  double ypos, int first, int count, const double* scales, const double* dzs,
-                 int engine, int flags, hipStream_t s) {
+                 int engine, int flags, hipStream_t s, const HistSpec* hist) {
   const int total = count;
   if (count == 0) return RSIM_OK;
   const bool want_spec = !(flags & RSIM_FLAG_NO_SPECTRUM);
   if (engine == RSIM_PLAIN || !fused_supported(nx, ny, dtype)) {
     for (int k = 0; k < count; ++k) {
       if (cancel_requested()) return fail(RSIM_E_CANCELLED, "multislice cancelled at slice %d", first + k);
+      if (hist) RSIM_CHECK(hist_frame(u, nx, ny, dx, dy, pz, first + k, k, hist, s, dtype));
       RSIM_CHECK(src.plain(u, dtype, first + k, scales ? scales[k] : 1.0, s));
       RSIM_CHECK(propagate_plain(u, U, nx, ny, dtype, dx, dy, wl, dzs ? dzs[k] : pz, freq, false, s));
       if (g_progress_sync()) RSIM_HIP(hipStreamSynchronize(s));
@@ -571,24 +590,24 @@ int slices_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double 
     flags |= RSIM_FLAG_MATH_COMP32;
   if (flags & RSIM_FLAG_MATH_COMP32)
     return run_fused<twc>(u, U, nx, ny, dx, dy, wl, freq, src, px, py, pz, xpos, ypos, first,
-                          count, want_spec, scales, dzs, s);
+                          count, want_spec, scales, dzs, s, hist);
   if (flags & RSIM_FLAG_MATH_FP64)
     return run_fused<double>(u, U, nx, ny, dx, dy, wl, freq, src, px, py, pz, xpos, ypos, first,
-                             count, want_spec, scales, dzs, s);
+                             count, want_spec, scales, dzs, s, hist);
   return run_fused<float>(u, U, nx, ny, dx, dy, wl, freq, src, px, py, pz, xpos, ypos, first,
-                          count, want_spec, scales, dzs, s);
+                          count, want_spec, scales, dzs, s, hist);
 }
 
 int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
                  double freq, const rsim_volume* vol, double px, double py, double pz,
                  double xpos, double ypos, int first, int count, const double* scales,
-                 const double* dzs, int engine, int flags, hipStream_t s) {
+                 const double* dzs, int engine, int flags, hipStream_t s, const HistSpec* hist) {
   VoxSrc src = src_of(vol);
   src.plain = [&](void* f, int dt, int slice, double scale, hipStream_t st) {
     return transmission(f, nx, ny, dt, dx, dy, wl, vol, slice, px, py, pz, xpos, ypos, scale, st);
   };
   return slices_apply(u, U, nx, ny, dtype, dx, dy, wl, freq, src, px, py, pz, xpos, ypos, first,
-                      count, scales, dzs, engine, flags, s);
+                      count, scales, dzs, engine, flags, s, hist);
 }
 
 // ---------------------------------------------------------------------------

@@ -329,12 +329,15 @@ int rsim_plasma_transmission_2d(void* u, int nx, int ny, int dtype, double dx, d
                              pixel_size_z, x_pos, y_pos, coord_scale, resolve_stream(stream));
 }
 
-int rsim_plasma_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
-                         double wl, double cutoff_freq, const rsim_plasma* p, double energy,
-                         const double* optics, double pixel_size_x, double pixel_size_y,
-                         double pixel_size_z, double x_pos, double y_pos, int first_slice,
-                         int n_slices, const double* coord_scale, const double* slice_dz,
-                         int engine, int flags, void* stream) {
+}  // extern "C"
+
+namespace {
+int plasma_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
+                 double cutoff_freq, const rsim_plasma* p, double energy, const double* optics,
+                 double pixel_size_x, double pixel_size_y, double pixel_size_z, double x_pos,
+                 double y_pos, int first_slice, int n_slices, const double* coord_scale,
+                 const double* slice_dz, int engine, int flags, const rsim_history_frames* history,
+                 void* stream) {
   if (!u || !U) return fail(RSIM_E_INVALID_ARG, "u and U must be non-NULL");
   if (!valid_dtype(dtype)) return fail(RSIM_E_INVALID_ARG, "dtype must be RSIM_C64 or RSIM_C128");
   if (nx <= 0 || ny <= 0) return fail(RSIM_E_INVALID_ARG, "nx and ny must be positive");
@@ -354,6 +357,20 @@ int rsim_plasma_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx,
   }
   PlasmaK k;
   RSIM_CHECK(plasma_consts(energy, optics, &k));
+  HistSpec hs;
+  if (history) {
+    if (!history->frames && n_slices > 0)
+      return fail(RSIM_E_INVALID_ARG, "history frames buffer is NULL");
+    if (history->out_x <= 0 || history->out_y <= 0 || !(history->pixel_size_x > 0) ||
+        !(history->pixel_size_y > 0) || !std::isfinite(history->z_start))
+      return fail(RSIM_E_INVALID_ARG, "invalid history detector geometry");
+    hs.ox = history->out_x;
+    hs.oy = history->out_y;
+    hs.dsx = history->pixel_size_x;
+    hs.dsy = history->pixel_size_y;
+    hs.z_start = history->z_start;
+    hs.frames = history->frames;
+  }
   hipStream_t s = resolve_stream(stream);
   VoxSrc src;
   src.nz = p->nz;
@@ -367,7 +384,34 @@ int rsim_plasma_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx,
   };
   return slices_apply(u, U, nx, ny, dtype, dx, dy, wl, cutoff_freq, src, pixel_size_x,
                       pixel_size_y, pixel_size_z, x_pos, y_pos, first_slice, n_slices, coord_scale,
-                      slice_dz, engine, flags, s);
+                      slice_dz, engine, flags, s, history ? &hs : nullptr);
+}
+}  // namespace
+
+extern "C" {
+
+int rsim_plasma_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
+                         double wl, double cutoff_freq, const rsim_plasma* p, double energy,
+                         const double* optics, double pixel_size_x, double pixel_size_y,
+                         double pixel_size_z, double x_pos, double y_pos, int first_slice,
+                         int n_slices, const double* coord_scale, const double* slice_dz,
+                         int engine, int flags, void* stream) {
+  return plasma_apply(u, U, nx, ny, dtype, dx, dy, wl, cutoff_freq, p, energy, optics,
+                      pixel_size_x, pixel_size_y, pixel_size_z, x_pos, y_pos, first_slice,
+                      n_slices, coord_scale, slice_dz, engine, flags, nullptr, stream);
+}
+
+int rsim_plasma_apply_2d_history(void* u, void* U, int nx, int ny, int dtype, double dx,
+                                 double dy, double wl, double cutoff_freq, const rsim_plasma* p,
+                                 double energy, const double* optics, double pixel_size_x,
+                                 double pixel_size_y, double pixel_size_z, double x_pos,
+                                 double y_pos, int first_slice, int n_slices,
+                                 const double* coord_scale, const double* slice_dz, int engine,
+                                 int flags, const rsim_history_frames* history, void* stream) {
+  if (!history) return fail(RSIM_E_INVALID_ARG, "history is NULL");
+  return plasma_apply(u, U, nx, ny, dtype, dx, dy, wl, cutoff_freq, p, energy, optics,
+                      pixel_size_x, pixel_size_y, pixel_size_z, x_pos, y_pos, first_slice,
+                      n_slices, coord_scale, slice_dz, engine, flags, history, stream);
 }
 
 }  // extern "C"

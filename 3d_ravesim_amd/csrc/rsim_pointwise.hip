@@ -4,6 +4,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "rsim_device.hpp"
@@ -357,33 +360,81 @@ int check_detector(int ox, int oy, double z) {
 
 }  // namespace
 
-int detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double dy, int ox,
-                     int oy, double dsx, double dsy, double z, double weight, bool accumulate,
-                     double* out_dev, hipStream_t s) {
-  RSIM_CHECK(check_detector(ox, oy, z));
+namespace {
+
+// Device plans, one per (device, field grid, detector grid): the detector
+// distance only enters the kernel's obliquity factor, so a history of frames
+// at many z shares one plan.  Built synchronously once, kept until released.
+struct DetCached {
+  unsigned char* d = nullptr;
+  DetPlanDev pl{};
+  double gyl = 0.0;
+};
+std::mutex g_det_mu;
+std::map<std::tuple<int, int, int, uint64_t, uint64_t, int, int, uint64_t, uint64_t>, DetCached>
+    g_det;
+
+uint64_t fbits(double v) {
+  uint64_t b;
+  std::memcpy(&b, &v, 8);
+  return b;
+}
+
+int det_plan_dev(int nx, int ny, double dx, double dy, int ox, int oy, double dsx, double dsy,
+                 const DetCached** out) {
+  std::lock_guard<std::mutex> lk(g_det_mu);
+  const auto key = std::make_tuple(current_device(), nx, ny, fbits(dx), fbits(dy), ox, oy,
+                                   fbits(dsx), fbits(dsy));
+  auto it = g_det.find(key);
+  if (it != g_det.end()) {
+    *out = &it->second;
+    return RSIM_OK;
+  }
   DetPlanHost P;
   det_plan(nx, ny, dx, dy, ox, oy, dsx, dsy, &P);
   Blob b = make_blob({&P.xi0, &P.xi1, &P.jlo, &P.jhi},
                      {&P.xf0, &P.xf1, &P.xdet, &P.ylo, &P.yhi, &P.ydet});
-  unsigned char* d = nullptr;
-  RSIM_HIP(hipMallocAsync((void**)&d, b.bytes.size(), s));
-  RSIM_HIP(hipMemcpyAsync(d, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice, s));
-  DetPlanDev pl;
-  pl.xi0 = (const int*)(d + b.ioff[0]); pl.xi1 = (const int*)(d + b.ioff[1]);
-  pl.jlo = (const int*)(d + b.ioff[2]); pl.jhi = (const int*)(d + b.ioff[3]);
-  pl.xf0 = (const double*)(d + b.doff[0]); pl.xf1 = (const double*)(d + b.doff[1]);
-  pl.xdet = (const double*)(d + b.doff[2]); pl.ylo = (const double*)(d + b.doff[3]);
-  pl.yhi = (const double*)(d + b.doff[4]); pl.ydet = (const double*)(d + b.doff[5]);
+  DetCached c;
+  RSIM_HIP(hipMalloc((void**)&c.d, b.bytes.size()));
+  const hipError_t e = hipMemcpy(c.d, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(c.d);
+    return fail(RSIM_E_HIP, "detector plan upload failed: %s", hipGetErrorString(e));
+  }
+  unsigned char* d = c.d;
+  c.pl.xi0 = (const int*)(d + b.ioff[0]); c.pl.xi1 = (const int*)(d + b.ioff[1]);
+  c.pl.jlo = (const int*)(d + b.ioff[2]); c.pl.jhi = (const int*)(d + b.ioff[3]);
+  c.pl.xf0 = (const double*)(d + b.doff[0]); c.pl.xf1 = (const double*)(d + b.doff[1]);
+  c.pl.xdet = (const double*)(d + b.doff[2]); c.pl.ylo = (const double*)(d + b.doff[3]);
+  c.pl.yhi = (const double*)(d + b.doff[4]); c.pl.ydet = (const double*)(d + b.doff[5]);
+  c.gyl = P.gyl;
+  *out = &(g_det[key] = c);
+  return RSIM_OK;
+}
+
+}  // namespace
+
+int release_detector_plans() {
+  std::lock_guard<std::mutex> lk(g_det_mu);
+  for (auto& kv : g_det) (void)hipFree(kv.second.d);
+  g_det.clear();
+  return RSIM_OK;
+}
+
+int detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double dy, int ox,
+                     int oy, double dsx, double dsy, double z, double weight, bool accumulate,
+                     double* out_dev, hipStream_t s) {
+  RSIM_CHECK(check_detector(ox, oy, z));
+  const DetCached* c = nullptr;
+  RSIM_CHECK(det_plan_dev(nx, ny, dx, dy, ox, oy, dsx, dsy, &c));
   dim3 grid((ox + 15) / 16, (oy + 15) / 16);
   if (dtype == RSIM_C64)
     hipLaunchKernelGGL(area_v1_kernel<float>, grid, dim3(256), 0, s, (const cpx<float>*)u, nx, ny,
-                       ox, oy, pl, dx, dy, P.gyl, z, weight, accumulate ? 1 : 0, out_dev);
+                       ox, oy, c->pl, dx, dy, c->gyl, z, weight, accumulate ? 1 : 0, out_dev);
   else
     hipLaunchKernelGGL(area_v1_kernel<double>, grid, dim3(256), 0, s, (const cpx<double>*)u, nx,
-                       ny, ox, oy, pl, dx, dy, P.gyl, z, weight, accumulate ? 1 : 0, out_dev);
+                       ny, ox, oy, c->pl, dx, dy, c->gyl, z, weight, accumulate ? 1 : 0, out_dev);
   RSIM_LAUNCHED();
-  RSIM_HIP(hipFreeAsync(d, s));
-  RSIM_HIP(hipStreamSynchronize(s));   // host plan vectors must outlive the copy
   return RSIM_OK;
 }
 

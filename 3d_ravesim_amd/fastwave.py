@@ -6,7 +6,9 @@ source on the GPU and writes ``<SIM_DIR>/<idx:08>/detected.npy`` of shape
 (phases, ny_pix, nx_pix) plus ``source_geometry.yaml`` for point sources.
 Errors print ``uncaught exception in the fast-wave framework: ...`` and exit
 with status 1.  The detector images are float64 (big-wave's dtype; fast-wave
-writes its field's scalar type).
+writes its field's scalar type).  ``--history_dz`` records history frames
+(history.npy (z, y, x) and history_z.npy, as fast-wave writes them, plus
+big-wave's coordinate and metadata files).
 """
 
 from __future__ import annotations

@@ -212,8 +212,6 @@ class Sample:
 
         After return U == FFT2(u) (OpticalElement.apply contract, :97).
         """
-        if history is not None:
-            raise NotImplementedError("history frames are not produced by the HIP engine")
         if not isinstance(u, DeviceVector) or not isinstance(U, DeviceVector):
             raise TypeError("Sample.apply needs DeviceVector fields")
         nx, ny = sim_params.nx, sim_params.ny
@@ -226,12 +224,12 @@ class Sample:
         start = int(getattr(self, "_checkpoint_start_slice", 0))
         if start < 0 or start > z_len:
             raise ValueError(f"invalid Sample checkpoint slice {start}/{z_len}")
-        callback = getattr(self, "_checkpoint_callback", None)
         engine = self.engine_code(u.dtype)
         args = (u.ptr, U.ptr, nx, ny, u.code, float(sim_params.dx), float(sim_params.get_dy()),
                 float(sim_params.wl), float(cutoff_freq), vol.handle, float(self.pixel_size_x),
                 float(py), float(self.pixel_size_z), x_pos, y_pos)
         pz = self.pixel_size_z
+        scales = dzs = None
         if sim_params.use_cone_beam_bpm:
             # moving coordinates: slice k samples the voxels at m(z_k + pz/2) times
             # the field coordinates and propagates effective_slice_dz (:549-592)
@@ -239,16 +237,18 @@ class Sample:
             scales = np.array([sim_params.fresnel_transverse_scale(z + 0.5 * pz) for z in zs])
             dzs = np.array([sim_params.effective_slice_dz(z, pz) for z in zs])
 
-            def run(first: int, n: int) -> None:
+        def run(first: int, n: int, frames) -> None:
+            if frames is not None:
+                _lib.call("rsim_sample_apply_2d_history", *args, first, n,
+                          scales[first:].ctypes.data_as(_lib._dp) if scales is not None else None,
+                          dzs[first:].ctypes.data_as(_lib._dp) if dzs is not None else None,
+                          engine, 0, C.byref(frames), None)
+            elif scales is not None:
                 _lib.call("rsim_sample_apply_2d_scaled", *args, first, n,
                           scales[first:].ctypes.data_as(_lib._dp),
                           dzs[first:].ctypes.data_as(_lib._dp), engine, 0, None)
-        else:
-            def run(first: int, n: int) -> None:
+            else:
                 _lib.call("rsim_sample_apply_2d", *args, first, n, engine, 0, None)
-        if callback is None:
-            run(start, z_len - start)
-        else:
-            for rowidx in range(start, z_len):
-                run(rowidx, 1)
-                callback(rowidx + 1, self.z_start + (rowidx + 1) * pz)
+
+        from .history import drive_slices
+        drive_slices(run, self, sim_params, history, start, z_len)

@@ -9,6 +9,7 @@ propagate_2d -- plain or fused engine, cone-beam scaling as for Sample.
 
 from __future__ import annotations
 
+import ctypes as C
 import logging
 from dataclasses import dataclass
 from typing import Optional
@@ -91,8 +92,6 @@ class PlasmaSample:
     def apply(self, u: DeviceVector, U: DeviceVector, sim_params: SimParams, cutoff_freq: float,
               stepping_iteration: int, history) -> None:
         """PlasmaSample._apply_2d (plasma_sample.py:126-233); U = FFT2(u) after."""
-        if history is not None:
-            raise NotImplementedError("history frames are not produced by the HIP engine")
         if not (sim_params.is_2d and self.ne_grid.ndim == 3):
             raise NotImplementedError("the HIP engine implements the 2D plasma path (3D grids) only")
         if not isinstance(u, DeviceVector) or not isinstance(U, DeviceVector):
@@ -109,7 +108,6 @@ class PlasmaSample:
         start = int(getattr(self, "_checkpoint_start_slice", 0))
         if start < 0 or start > nz:
             raise ValueError(f"invalid PlasmaSample checkpoint slice {start}/{nz}")
-        callback = getattr(self, "_checkpoint_callback", None)
         pz = self.pixel_size_z
         engine = self.engine_code(u.dtype)
         scales = dzs = None
@@ -118,18 +116,17 @@ class PlasmaSample:
             scales = np.array([sim_params.fresnel_transverse_scale(z + 0.5 * pz) for z in zs])
             dzs = np.array([sim_params.effective_slice_dz(z, pz) for z in zs])
 
-        def run(first: int, n: int) -> None:
+        def run(first: int, n: int, frames) -> None:
             sp = scales[first:].ctypes.data_as(_lib._dp) if scales is not None else None
             dp = dzs[first:].ctypes.data_as(_lib._dp) if dzs is not None else None
-            _lib.call("rsim_plasma_apply_2d", u.ptr, U.ptr, nx, ny, u.code, float(sim_params.dx),
-                      float(sim_params.get_dy()), float(sim_params.wl), float(cutoff_freq),
-                      dev.handle, float(energy), optics.ctypes.data_as(_lib._dp),
-                      float(self.pixel_size_x), float(py), float(pz), x_pos, y_pos, first, n,
-                      sp, dp, engine, 0, None)
+            args = (u.ptr, U.ptr, nx, ny, u.code, float(sim_params.dx), float(sim_params.get_dy()),
+                    float(sim_params.wl), float(cutoff_freq), dev.handle, float(energy),
+                    optics.ctypes.data_as(_lib._dp), float(self.pixel_size_x), float(py),
+                    float(pz), x_pos, y_pos, first, n, sp, dp, engine, 0)
+            if frames is None:
+                _lib.call("rsim_plasma_apply_2d", *args, None)
+            else:
+                _lib.call("rsim_plasma_apply_2d_history", *args, C.byref(frames), None)
 
-        if callback is None:
-            run(start, nz - start)
-        else:
-            for rowidx in range(start, nz):
-                run(rowidx, 1)
-                callback(rowidx + 1, self.z_start + (rowidx + 1) * pz)
+        from .history import drive_slices
+        drive_slices(run, self, sim_params, history, start, nz)

@@ -4,6 +4,8 @@ from __future__ import annotations
 
 from dataclasses import dataclass
 
+import numpy as np
+
 from .propagation import (
     SimParams,
     apply_frequency_cutoff_2d,
@@ -38,8 +40,6 @@ class PointSource:
                 raise ValueError("physical-z source history is not supported with Fresnel scaling")
             fresnel_plane_2d(u, dz, self.x, self.y, sim_params)
         else:
-            if history is not None:
-                raise NotImplementedError("history frames are not produced by the HIP engine")
             grid_density_check_2d(dz, self.x, sim_params.nx, sim_params.dx, self.y,
                                   sim_params.ny, sim_params.get_dy(), sim_params.wl)
             propagate_analytically_2d(u, dz, self.x, self.y, sim_params)
@@ -47,3 +47,8 @@ class PointSource:
         apply_frequency_cutoff_2d(U, cutoff_freq, sim_params.dx, sim_params.get_dy(),
                                   sim_params.nx, sim_params.ny, sim_params.chunk_size)
         U.ifft2(u, sim_params.nx, sim_params.ny)
+        if history is not None:
+            # analytic frames between the source and z_out (source.py:109-113)
+            from .history import generate_analytic_history_2d
+            zs = np.arange(0, z_out, history[1])[1:]
+            generate_analytic_history_2d(history[0], zs, self.x, self.y, sim_params, cutoff_freq)

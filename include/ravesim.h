@@ -185,6 +185,28 @@ int rsim_transmission_2d_scaled(void* u, int nx, int ny, int dtype, double dx, d
 int rsim_tilted_plane_2d(void* u, int nx, int ny, int dtype, double dx, double dy, double wl,
                          double dz, double x_source, double y_source, void* stream);
 
+/* ---- History frames of a slice loop (big-wave/optical_element.py:548-551,
+ * big-wave/plasma_sample.py:172-176; fast-wave keeps its history on the device
+ * too, fast-wave/src/simulation.cpp:43-102) ----
+ * Frame k of a call over slices [first, first + n) is the area_v1 image
+ * (out_y x out_x, float64, physical detector pixels) of the field entering
+ * slice first + k, at z = z_start + (first + k) * pixel_size_z, written to
+ * frames + k * out_y * out_x (a device buffer of n * out_y * out_x doubles).
+ * The frames are stream-ordered with the loop; nothing synchronises. */
+typedef struct rsim_history_frames {
+  int out_x, out_y;
+  double pixel_size_x, pixel_size_y;
+  double z_start;
+  double* frames;
+} rsim_history_frames;
+int rsim_sample_apply_2d_history(void* u, void* U, int nx, int ny, int dtype, double dx,
+                                 double dy, double wl, double cutoff_freq,
+                                 const rsim_volume* vol, double pixel_size_x,
+                                 double pixel_size_y, double pixel_size_z, double x_pos,
+                                 double y_pos, int first_slice, int n_slices,
+                                 const double* coord_scale, const double* slice_dz, int engine,
+                                 int flags, const rsim_history_frames* history, void* stream);
+
 /* ---- PlasmaSample (big-wave/plasma_sample.py:126-233, big-wave/plasma.py:380-434) ----
  * A plasma volume: four (nz, ny_g, nx_g) grids -- electron density n_e and ion
  * density n_i [cm^-3], electron temperature T_e [eV] and mean ionisation Z* --
@@ -219,6 +241,14 @@ int rsim_plasma_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx,
                          double pixel_size_z, double x_pos, double y_pos, int first_slice,
                          int n_slices, const double* coord_scale, const double* slice_dz,
                          int engine, int flags, void* stream);
+/* rsim_plasma_apply_2d with history frames, as rsim_sample_apply_2d_history */
+int rsim_plasma_apply_2d_history(void* u, void* U, int nx, int ny, int dtype, double dx,
+                                 double dy, double wl, double cutoff_freq,
+                                 const rsim_plasma* plasma, double energy, const double* optics,
+                                 double pixel_size_x, double pixel_size_y, double pixel_size_z,
+                                 double x_pos, double y_pos, int first_slice, int n_slices,
+                                 const double* coord_scale, const double* slice_dz, int engine,
+                                 int flags, const rsim_history_frames* history, void* stream);
 
 /* ---- Batched fused multislice (no counterpart in the reference: the
  * reference runs each (source, energy) instance as its own process,

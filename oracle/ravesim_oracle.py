@@ -324,14 +324,18 @@ def fresnel_plane_source_2d(nx: int, ny: int, dx: float, dy: float, wl: float, d
 
 def sample_apply_2d(u2d: np.ndarray, sample: SampleSpec, dx: float, dy: float, wl: float,
                     cutoff_freq: float, first_slice: int = 0, n_slices: int | None = None,
-                    workers: int = 1, geometry: FresnelGeometry | None = None
+                    workers: int = 1, geometry: FresnelGeometry | None = None,
+                    history: list | None = None, frame=None
                     ) -> tuple[np.ndarray, np.ndarray]:
     """Sample._apply_2d (optical_element.py:502-596).
 
     Per slice: u *= material_factor(interpolated db, pz, wl), the voxel map
     scaled by m(slice_z + pz/2) under CB-BPM; then propagate_2d over
     effective_slice_dz(slice_z, pz).  Returns (u, U) with U = FFT2(u) (the
-    apply contract, optical_element.py:97).
+    apply contract, optical_element.py:97).  With ``history`` (a list) and
+    ``frame`` = (size_x, size_y, ds_x, ds_y), the area_v1 image of the field
+    entering each slice at z = slice_z is appended with its z first
+    (optical_element.py:548-551).
     """
     ny, nx = u2d.shape
     nz = sample.grid.shape[0]
@@ -343,6 +347,8 @@ def sample_apply_2d(u2d: np.ndarray, sample: SampleSpec, dx: float, dy: float, w
         slice_z = sample.z_start + rowidx * pz
         scale = geometry.transverse_scale(slice_z + 0.5 * pz) if geometry else 1.0
         dz = geometry.slice_dz(slice_z, pz) if geometry else pz
+        if history is not None:
+            history.append((area_v1(u, dx, dy, *frame, slice_z), slice_z))
         db = sample_deltabeta_slice(sample.grid, sample.db_list, rowidx, nx, ny, dx, dy,
                                     sample.pixel_size_x, sample.py, sample.x_pos, sample.y_pos,
                                     scale)
@@ -440,7 +446,8 @@ def plasma_deltabeta_slice(spec: PlasmaSpec, rowidx: int, nx: int, ny: int, dx: 
 
 def plasma_sample_apply_2d(u2d: np.ndarray, spec: PlasmaSpec, dx: float, dy: float, wl: float,
                            cutoff_freq: float, first_slice: int = 0, n_slices: int | None = None,
-                           workers: int = 1, geometry: FresnelGeometry | None = None
+                           workers: int = 1, geometry: FresnelGeometry | None = None,
+                           history: list | None = None, frame=None
                            ) -> tuple[np.ndarray, np.ndarray]:
     """PlasmaSample._apply_2d (plasma_sample.py:126-233): per slice u *=
     exp(-2 pi i pz/wl conj(db)) with db from the plasma optics at
@@ -456,6 +463,8 @@ def plasma_sample_apply_2d(u2d: np.ndarray, spec: PlasmaSpec, dx: float, dy: flo
         slice_z = spec.z_start + rowidx * pz
         scale = geometry.transverse_scale(slice_z + 0.5 * pz) if geometry else 1.0
         dz = geometry.slice_dz(slice_z, pz) if geometry else pz
+        if history is not None:   # plasma_sample.py:172-176
+            history.append((area_v1(u, dx, dy, *frame, slice_z), slice_z))
         db = plasma_deltabeta_slice(spec, rowidx, nx, ny, dx, dy, energy, scale)
         u = (u * material_factor(db, pz, wl)).astype(u.dtype, copy=False)
         u, U = propagate_2d(u, dx, dy, wl, dz, cutoff_freq, workers=workers)
@@ -536,6 +545,57 @@ def area_v1(u2d: np.ndarray, dx: float, dy: float, size_x: float, size_y: float,
     overlap = np.where(overlap > 0, overlap, 0.0)  # (out_y, ny)
     cosine = z / np.sqrt(np.square(x_det)[None, :] + np.square(y_det)[:, None] + z * z)
     return (overlap @ x_int) * cosine
+
+
+# ---------------------------------------------------------------------------
+# History: big-wave/history.py:348-435
+# ---------------------------------------------------------------------------
+
+def history_z_steps(dz: float, hist_dz: float) -> np.ndarray:
+    """_history_z_steps (history.py:370-378): 0, hist_dz, 2 hist_dz, ... with
+    the last step stretched to dz when less than half a step remains."""
+    zs = np.arange(0, dz, hist_dz)
+    if len(zs) == 0:
+        return np.array([0.0, dz])
+    if dz - zs[-1] < hist_dz / 2.0 and len(zs) > 1:
+        zs[-1] = dz
+    else:
+        zs = np.append(zs, [dz])
+    return zs
+
+
+def propagate_with_history_2d(u2d, dx, dy, wl, dz, cutoff_freq, current_z, hist_dz, frame,
+                              U2d=None, workers=1):
+    """propagate_with_history_2d (history.py:406-435): every frame is a fresh
+    propagation of the start spectrum over zs[i] - zs[0]; returns
+    (u, U, [(frame, z), ...])."""
+    zs = history_z_steps(dz, hist_dz)
+    U0 = fft2(u2d, workers=workers) if U2d is None else U2d
+    out = []
+    u = U = None
+    for i in range(1, len(zs)):
+        step = float(zs[i] - zs[0])
+        u, U = propagate_2d(u2d, dx, dy, wl, step, cutoff_freq, U2d=U0, workers=workers)
+        z = float(zs[i] + current_z)
+        out.append((area_v1(u, dx, dy, *frame, z), z))
+    return u, U, out
+
+
+def analytic_history_2d(zs, x_source, y_source, size_x, size_y, ds_x, ds_y, wl, cutoff_freq):
+    """generate_analytic_history_2d (history.py:348-367): ds_x ds_y z / r^3
+    inside the cutoff cone, 0 outside."""
+    gradient = cutoff_freq * wl
+    nxp, nyp = int(size_x // ds_x), int(size_y // ds_y)
+    x = (np.arange(nxp) - nxp // 2) * ds_x - x_source
+    y = (np.arange(nyp) - nyp // 2) * ds_y - y_source
+    out = []
+    for z in zs:
+        r2 = x[None, :] ** 2 + y[:, None] ** 2 + z * z
+        img = ds_x * ds_y * z / np.power(r2, 1.5)
+        img[:, np.abs(x) > gradient * z] = 0
+        img[np.abs(y) > gradient * z, :] = 0
+        out.append((img, z))
+    return out
 
 
 # ---------------------------------------------------------------------------

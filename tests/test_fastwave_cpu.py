@@ -65,10 +65,12 @@ def test_thin_fresnel_rejects_two_elements_before_gpu_work(tmp_path):
     assert FW.main([str(root), "-s", "0"]) == 1
 
 
-def test_history_is_rejected(tmp_path):
+def test_history_options_are_validated_before_gpu_work(tmp_path):
     root, _ = write_prepared(tmp_path, n=16, nz=2, grid=np.zeros((2, 24, 24), np.uint32))
-    with pytest.raises(NotImplementedError):
-        MS.run_single_simulation(root, 0, history_dz=1e-3)
+    with pytest.raises(ValueError, match="unknown history options"):
+        MS.run_single_simulation(root, 0, history_dz=1e-3, history_options={"bogus": 1})
+    with pytest.raises(ValueError, match="downsample"):
+        MS.run_single_simulation(root, 0, history_dz=1e-3, history_options={"downsample": [0, 1]})
 
 
 def test_prepared_config_parses_like_the_reference(tmp_path):

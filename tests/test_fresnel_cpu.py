@@ -235,7 +235,8 @@ class _EmptyElement:
 
 def _run_recording(monkeypatch, params, element):
     distances = []
-    monkeypatch.setattr(WAVESIM, "propagate_2d", lambda *a, **k: distances.append(a[5]))
+    monkeypatch.setattr(WAVESIM, "propagate_with_history_2d",
+                        lambda u, U, p, dz, *a, **k: distances.append(dz))
     monkeypatch.setattr(WAVESIM, "square_and_downsample_2d", lambda *a: np.zeros((1, 1)))
     WAVESIM.run_simulation(params, _PlaneSource(), [element], [0.01, 0.01], None, None, [])
     return distances

@@ -1,0 +1,223 @@
+"""History frames and runtime checkpoints on the GPU versus the oracle.
+
+Slice-loop frames come from the library (rsim_sample_apply_2d_history,
+rsim_plasma_apply_2d_history); free-space frames from
+propagate_with_history_2d.  complex128 frames 1e-12, complex64 3e-6
+(relative L2 per frame).  A run interrupted after a slice checkpoint and
+resumed gives the same detector image, bit for bit, as the uninterrupted
+checkpointed run.
+"""
+
+import importlib
+import sys
+
+import numpy as np
+import pytest
+import yaml
+
+import oracle as O
+from _prepared import write_prepared
+from _rs import rs
+
+pytestmark = pytest.mark.gpu
+H = importlib.import_module("3d_ravesim_amd.history")
+CK = importlib.import_module("3d_ravesim_amd.checkpoint")
+WS = importlib.import_module("3d_ravesim_amd.wavesim")
+MS = importlib.import_module("3d_ravesim_amd.multisim")
+DB3 = np.array([0, 3.5e-7 + 2e-10j, 1.1e-6 + 3e-9j, 2.6e-7 + 1.5e-10j])
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def dev(x):
+    return rs.DeviceVector.from_numpy(np.ascontiguousarray(x).reshape(-1))
+
+
+def spheres(rng, nz, ny, nx, k):
+    g = np.zeros((nz, ny, nx), np.uint8)
+    zz, yy, xx = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    for _ in range(k):
+        c = rng.uniform(0, 1, 3) * [nz, ny, nx]
+        r = rng.uniform(0.1, 0.3) * min(ny, nx)
+        g[(zz - c[0]) ** 2 * 4 + (yy - c[1]) ** 2 + (xx - c[2]) ** 2 < r * r] = rng.integers(1, 4)
+    return g
+
+
+def params(n, dx, wl, **kw):
+    return rs.SimParams(N=n * n, ny=n, dx=dx, z_detector=0.3, detector_size=n * dx * 0.9,
+                        detector_size_y=n * dx * 0.8, detector_pixel_size_x=2.3 * dx,
+                        detector_pixel_size_y=1.7 * dx, wl=wl, chunk_size=n, **kw)
+
+
+def frame_geom(p):
+    return (p.get_detector_size_x(), p.get_detector_size_y(), p.detector_pixel_size_x,
+            p.detector_pixel_size_y)
+
+
+@pytest.mark.parametrize("engine,dt,tol", [("plain", np.complex128, 1e-12),
+                                           ("fused", np.complex64, 3e-6)])
+def test_sample_slice_frames_match_oracle(engine, dt, tol):
+    rng = np.random.default_rng(41)
+    n, nz, dx, wl = 256, 7, 0.5e-6, 0.5e-10
+    grid = spheres(rng, nz, n + 6, n + 4, 8)
+    u0 = np.exp(1j * rng.uniform(0, 0.2, (n, n)))
+    p = params(n, dx, wl)
+    spec = O.SampleSpec(grid=grid, db_list=DB3 * 30, pixel_size_x=1.1 * dx, pixel_size_y=1.1 * dx,
+                        pixel_size_z=0.5e-6, z_start=0.1)
+    exp_frames = []
+    ue, _ = O.sample_apply_2d(u0, spec, dx, dx, wl, 0.4 / dx, history=exp_frames,
+                              frame=frame_geom(p))
+    s = rs.Sample(z_start=0.1, pixel_size_x=1.1 * dx, pixel_size_z=0.5e-6, grid=grid,
+                  materials=[rs.Material(str(i), 1.0) for i in range(3)],
+                  x_positions=np.array([0.0]), engine=engine)
+    s.db_list = DB3 * 30
+    u, U = dev(u0.astype(dt)), rs.DeviceVector(n * n, dt)
+    hist = H.History()
+    say("sample history", engine, dt.__name__)
+    s.apply(u, U, p, 0.4 / dx, 0, hist)
+    got = hist.get_history()
+    assert got.shape == (nz, *exp_frames[0][0].shape)
+    assert hist.get_z() == [z for _, z in exp_frames]
+    for k in range(nz):
+        assert O.relative_l2(got[k], exp_frames[k][0]) < tol, k
+    assert O.relative_l2(u.to_numpy().reshape(n, n), ue) < tol
+
+
+def test_plasma_slice_frames_match_oracle():
+    rng = np.random.default_rng(43)
+    n, nz, dx = 256, 5, 0.5e-6
+    wl = O.convert_energy_wavelength(6000.0)
+    shape = (nz, n + 6, n + 4)
+    zz, yy, xx = np.meshgrid(*[np.linspace(-1, 1, k) for k in shape], indexing="ij")
+    ne = 2e23 * np.exp(-(xx ** 2 + yy ** 2) / 0.2) * (1 + 0.1 * rng.uniform(size=shape))
+    g = [ne, ne / 8.0, np.full(shape, 200.0), np.full(shape, 8.0)]
+    optics = (13.3, 0.19, 13.0, 169.0)
+    p = params(n, dx, wl)
+    spec = O.PlasmaSpec(*g, optics=optics, pixel_size_x=dx, pixel_size_z=1e-6,
+                        pixel_size_y=dx, z_start=0.1)
+    u0 = np.exp(1j * rng.uniform(0, 0.2, (n, n)))
+    exp_frames = []
+    O.plasma_sample_apply_2d(u0, spec, dx, dx, wl, 0.4 / dx, history=exp_frames,
+                             frame=frame_geom(p))
+    for engine, dt, tol in (("plain", np.complex128, 1e-12), ("fused", np.complex64, 3e-6)):
+        ps = rs.PlasmaSample(z_start=0.1, pixel_size_x=dx, pixel_size_z=1e-6, ne_grid=g[0],
+                             ni_grid=g[1], te_grid=g[2], zstar_grid=g[3], Z=13,
+                             x_positions=np.array([0.0]), pixel_size_y=dx, optics=optics,
+                             engine=engine)
+        u, U = dev(u0.astype(dt)), rs.DeviceVector(n * n, dt)
+        hist = H.History()
+        say("plasma history", engine)
+        ps.apply(u, U, p, 0.4 / dx, 0, hist)
+        got = hist.get_history()
+        for k in range(nz):
+            assert O.relative_l2(got[k], exp_frames[k][0]) < tol, (engine, k)
+
+
+def test_free_space_frames_match_oracle():
+    rng = np.random.default_rng(44)
+    n, dx, wl = 256, 0.5e-6, 0.5e-10
+    u0 = np.exp(1j * rng.uniform(0, 0.5, (n, n)))
+    p = params(n, dx, wl)
+    ue, Ue, exp_frames = O.propagate_with_history_2d(u0, dx, dx, wl, 0.011, 0.4 / dx, 0.2, 0.003,
+                                                     frame_geom(p))
+    u, U = dev(u0), rs.DeviceVector(n * n, np.complex128)
+    hist = H.History()
+    say("free-space history")
+    H.propagate_with_history_2d(u, U, p, 0.011, 0.4 / dx, 0.2, False, (hist, 0.003))
+    assert hist.get_z() == pytest.approx([z for _, z in exp_frames], abs=1e-15)
+    for k, (img, _) in enumerate(exp_frames):
+        assert O.relative_l2(hist.get_history()[k], img) < 1e-12, k
+    assert O.relative_l2(u.to_numpy().reshape(n, n), ue) < 1e-12
+    assert O.relative_l2(U.to_numpy().reshape(n, n), Ue) < 1e-12
+
+
+def test_runner_history_files_match_oracle_sequence(tmp_path):
+    """history_dz through the runner: analytic source frames, the sample's
+    slice frames, the final leg's frames; history*.npy and metadata."""
+    root, grid = write_prepared(tmp_path, n=256, nz=4, dtype="c16", z_start=0.6, z_detector=0.62,
+                                source={"type": "point", "x": 0.0, "y": 0.0, "z": 0.0})
+    say("runner with history_dz")
+    sub = MS.run_single_simulation(root, 0, history_dz=0.004,
+                                   history_options={"storage": "memory"})
+    cfg = yaml.safe_load(open(root / "config.yaml"))
+    sp, el = cfg["sim_params"], cfg["elements"][0]
+    subc = yaml.safe_load(open(sub / "subconfig.yaml"))
+    wl = O.convert_energy_wavelength(subc["energy"])
+    cut = np.sin(yaml.safe_load(open(root / "computed.yaml"))["cutoff_angles"][0]) / wl
+    dx = sp["dx"]
+    fr = (sp["detector_size_x"], sp["detector_size_y"], sp["detector_pixel_size_x"],
+          sp["detector_pixel_size_y"])
+    frames = O.analytic_history_2d(np.arange(0, 0.6, 0.004)[1:], 0.0, 0.0, *fr, wl, cut)
+    u = O.point_source_propagate_to(256, 256, dx, dx, 0.6, 0.0, 0.0, wl, cut)
+    from _prepared import DB
+    spec = O.SampleSpec(grid=grid, db_list=np.array([0] + [complex(d, b) for _, (d, b) in DB]),
+                        pixel_size_x=dx, pixel_size_y=dx, pixel_size_z=el["pixel_size_z"],
+                        z_start=0.6)
+    slice_frames = []
+    u, U = O.sample_apply_2d(u, spec, dx, dx, wl, cut, history=slice_frames, frame=fr)
+    frames += slice_frames
+    z_end = 0.6 + el["pixel_size_z"] * grid.shape[0]
+    u, U, leg = O.propagate_with_history_2d(u, dx, dx, wl, sp["z_detector"] - z_end, cut, z_end,
+                                            0.004, fr, U2d=U)
+    frames += leg
+    got = np.load(sub / "history.npy")
+    zs = np.load(sub / "history_z.npy")
+    assert got.shape == (len(frames), *frames[0][0].shape)
+    np.testing.assert_allclose(zs, [z for _, z in frames], rtol=0, atol=1e-15)
+    for k, (img, _) in enumerate(frames):
+        assert O.relative_l2(got[k], img) < 1e-11, k
+    ox = int(sp["detector_size_x"] // sp["detector_pixel_size_x"])
+    np.testing.assert_allclose(np.load(sub / "history_x.npy"),
+                               (np.arange(ox) - ox // 2) * sp["detector_pixel_size_x"])
+    meta = yaml.safe_load(open(sub / "history_metadata.yaml"))
+    assert meta["axis_order"] == ["z", "y", "x"] and meta["frames"] == len(frames)
+
+
+class _StopAfter(CK.RunCheckpoint):
+    """Checkpoints that end the run right after committing a chosen slice."""
+
+    stop_slice = 5
+
+    def save(self, u, U, **kw):
+        super().save(u, U, **kw)
+        if kw.get("stage") == "element_slice" and kw.get("slice_index") == self.stop_slice:
+            raise RuntimeError("simulated interruption")
+
+
+@pytest.mark.parametrize("engine,dt", [("plain", np.complex128), ("fused", np.complex64)])
+def test_resume_from_slice_checkpoint_is_bit_identical(tmp_path, engine, dt):
+    rng = np.random.default_rng(45)
+    n, nz, dx, wl = 256, 12, 0.5e-6, 0.5e-10
+    grid = spheres(rng, nz, n + 6, n + 4, 10)
+    p = params(n, dx, wl)
+    fp = {"N": n * n, "dtype": np.dtype(dt).str}
+    table = [(rs.Material(str(i), 1.0), DB3[i + 1] * 30) for i in range(3)]
+
+    def sample():
+        return rs.Sample(z_start=0.1, pixel_size_x=dx, pixel_size_z=0.5e-6, grid=grid,
+                         materials=[rs.Material(str(i), 1.0) for i in range(3)],
+                         x_positions=np.array([0.0]), engine=engine)
+
+    src = rs.PointSource(x=0.0, z=-2.0)
+    ang = [float(np.arcsin(0.4 / dx * wl))] * 2
+
+    def run(ck, resume=False):
+        u, U = rs.DeviceVector(n * n, dt), rs.DeviceVector(n * n, dt)
+        out = WS.run_simulation(p, src, [sample()], ang, u, U, table, checkpoint=ck,
+                                resume_checkpoint=resume)
+        return out[0]
+
+    say("checkpointed run", engine)
+    full = run(CK.RunCheckpoint(tmp_path / "a", fp))
+    say("interrupted run", engine)
+    with pytest.raises(RuntimeError, match="simulated"):
+        run(_StopAfter(tmp_path / "b", fp))
+    st = CK.RunCheckpoint(tmp_path / "b", fp)
+    say("resumed run", engine)
+    resumed = run(st, resume=True)
+    assert np.array_equal(resumed, full)
+    plain = run(None)
+    assert O.relative_l2(full, plain) < (1e-12 if dt == np.complex128 else 3e-6)
