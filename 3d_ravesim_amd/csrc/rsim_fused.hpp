@@ -34,9 +34,11 @@ struct TransArgsF {
   int ax, cx, xlo, xhi, ay, cy, ylo, yhi;
 };
 
-// Material LUT staged in LDS (256 entries); filled by the whole workgroup.
-__device__ __forceinline__ void lut_to_lds(float4* lds_lut, const float4* lut) {
-  for (int i = threadIdx.x; i < 256; i += blockDim.x) lds_lut[i] = lut[i];
+// Material tables staged in LDS (256 entries); filled by the whole workgroup:
+// the LUT factors (TM_LUT) or the (delta, beta) pairs (TM_BILINEAR).
+template <typename T>
+__device__ __forceinline__ void table_to_lds(T* lds_tab, const T* tab) {
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) lds_tab[i] = tab[i];
 }
 
 // Transmission of one element.  TM_LUT: the voxel map is integral, hence
@@ -47,7 +49,7 @@ __device__ __forceinline__ void lut_to_lds(float4* lds_lut, const float4* lut) {
 template <int TM>
 __device__ __forceinline__ cpx<float> transmit(cpx<float> v, const TransArgsF& a, int x, int y0,
                                                float fy, bool iny, const uint8_t* row0,
-                                               const float4* lds_lut) {
+                                               const float4* lds_lut, const float2* lds_dbf) {
   if constexpr (TM == TM_LUT) {
     if (!(iny && x >= a.xlo && x < a.xhi)) return v;
     const float4 t = lds_lut[row0[a.ax * x + a.cx]];
@@ -69,19 +71,16 @@ __device__ __forceinline__ cpx<float> transmit(cpx<float> v, const TransArgsF& a
     } else {
       const uint8_t* r0 = a.ids + (long long)y0 * a.nxg + x0;
       const uint8_t* r1 = r0 + a.nxg;
-      d00 = a.dbf[r0[0]];
-      d01 = a.dbf[r0[1]];
-      d10 = a.dbf[r1[0]];
-      d11 = a.dbf[r1[1]];
+      d00 = lds_dbf[r0[0]];
+      d01 = lds_dbf[r0[1]];
+      d10 = lds_dbf[r1[0]];
+      d11 = lds_dbf[r1[1]];
     }
     const float gx = 1.f - fx, gy = 1.f - fy;
     const float de = (d00.x * gx + d01.x * fx) * gy + (d10.x * gx + d11.x * fx) * fy;
     const float be = (d00.y * gx + d01.y * fx) * gy + (d10.y * gx + d11.y * fx) * fy;
-    const float e = expf(-a.K * be);
-    float s, c;
-    sincosf(a.K * de, &s, &c);
-    const float tr = e * c, ti = -e * s;
-    return mk(v.x * tr - v.y * ti, v.x * ti + v.y * tr);
+    const cpx<float> t = voxel_factor(a.K * de, a.K * be);
+    return mk(v.x * t.x - v.y * t.y, v.x * t.y + v.y * t.x);
   }
 }
 
@@ -142,6 +141,7 @@ __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(RowBatch rb, int ny, 
   constexpr int N = F::N;
   __shared__ unsigned char lds[LdsSize<F, C, PS>::slots * (SPLIT ? 4 : 8)];
   __shared__ float4 lds_lut[(MODE != ROW_LAST && LUT) ? 256 : 1];
+  __shared__ float2 lds_dbf[(MODE != ROW_LAST && TM == TM_BILINEAR) ? 256 : 1];
   const RowInst& I = rb.inst[blockIdx.y];
   const cpx<float>* in = I.in;
   cpx<float>* out = I.out;
@@ -154,26 +154,27 @@ __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(RowBatch rb, int ny, 
   const int yy = live ? y : 0;
   const Gio<float> gi(in, nbytes), go(out, nbytes);
   const uint32_t vo = ((uint32_t)yy * N + t) * 8u;
-  if constexpr (MODE != ROW_LAST && LUT) lut_to_lds(lds_lut, ta.lut);
+  if constexpr (MODE != ROW_LAST && LUT) table_to_lds(lds_lut, ta.lut);
+  if constexpr (MODE != ROW_LAST && TM == TM_BILINEAR) table_to_lds(lds_dbf, ta.dbf);
   cpx<float> v[F::E];
 #pragma unroll
   for (int e = 0; e < F::E; ++e) v[e] = gi.ld(vo, first_off<F>(e, 8u));
   if constexpr (MODE == ROW_FIRST) {
     const RowT r = row_state<TM>(ta, yy);
-    if constexpr (LUT) __syncthreads();
+    if constexpr (LUT || TM == TM_BILINEAR) __syncthreads();
 #pragma unroll
     for (int e = 0; e < F::E; ++e) {
-      v[e] = transmit<TM>(v[e], ta, first_pos<F>(t, e), r.y0, r.fy, r.iny, r.row0, lds_lut);
+      v[e] = transmit<TM>(v[e], ta, first_pos<F>(t, e), r.y0, r.fy, r.iny, r.row0, lds_lut, lds_dbf);
       if ((e & 3) == 3) compiler_fence();
     }
     reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
   } else if constexpr (MODE == ROW_MID) {
-    reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);  // its barriers publish lds_lut
+    reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);  // its barriers publish the LDS tables
     compiler_fence();
     const RowT r = row_state<TM>(ta, yy);
 #pragma unroll
     for (int e = 0; e < F::E; ++e) {
-      v[e] = transmit<TM>(v[e], ta, last_pos<F>(t, e), r.y0, r.fy, r.iny, r.row0, lds_lut);
+      v[e] = transmit<TM>(v[e], ta, last_pos<F>(t, e), r.y0, r.fy, r.iny, r.row0, lds_lut, lds_dbf);
       if ((e & 3) == 3) compiler_fence();
     }
     reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
