@@ -74,3 +74,34 @@ def test_energy_wavelength_matches_reference_constants():
     wl = rs.convert_energy_wavelength(24796.8)
     assert abs(wl - 0.5e-10) / 0.5e-10 < 1e-4
     assert rs.convert_energy_wavelength(rs.convert_energy_wavelength(8000.0)) == pytest.approx(8000.0, rel=1e-14)
+
+
+def test_new_entry_points_reject_bad_arguments_without_a_device():
+    """Argument checks of the Fresnel, plasma and history entry points run
+    before any device call (no GPU here)."""
+    L = lib.load()
+    f = C.c_void_p(16)
+    # tilted plane: non-positive distance
+    assert L.rsim_tilted_plane_2d(f, 4, 4, lib.RSIM_C64, 1e-6, 1e-6, 1e-10, 0.0, 0.0, 0.0,
+                                  None) == lib.RSIM_E_INVALID_ARG
+    assert b"distance" in L.rsim_last_error()
+    # plasma: grid pointers, element size, dimensions
+    p = C.c_void_p()
+    g = np.zeros(8)
+    ptr = g.ctypes.data
+    assert L.rsim_plasma_create(C.byref(p), None, ptr, ptr, ptr, 8, 1, 2, 4) == lib.RSIM_E_INVALID_ARG
+    assert L.rsim_plasma_create(C.byref(p), ptr, ptr, ptr, ptr, 2, 1, 2, 4) == lib.RSIM_E_INVALID_ARG
+    assert b"elem_size" in L.rsim_last_error()
+    assert L.rsim_plasma_create(C.byref(p), ptr, ptr, ptr, ptr, 8, 0, 2, 4) == lib.RSIM_E_INVALID_ARG
+    assert L.rsim_plasma_destroy(None) == 0
+    opt = np.array([1.0, 0.0, 1.0, 1.0])
+    assert L.rsim_plasma_deltabeta(None, 0, 1, 8000.0, opt.ctypes.data_as(lib._dp), f,
+                                   None) == lib.RSIM_E_INVALID_ARG
+    assert L.rsim_plasma_apply_2d(f, f, 4, 4, lib.RSIM_C64, 1e-6, 1e-6, 1e-10, 1e5, None, 8000.0,
+                                  opt.ctypes.data_as(lib._dp), 1e-6, 1e-6, 1e-6, 0.0, 0.0, 0, 1,
+                                  None, None, lib.RSIM_FUSED, 0, None) == lib.RSIM_E_INVALID_ARG
+    # history: a NULL spec is refused
+    assert L.rsim_sample_apply_2d_history(f, f, 4, 4, lib.RSIM_C64, 1e-6, 1e-6, 1e-10, 1e5, None,
+                                          1e-6, 1e-6, 1e-6, 0.0, 0.0, 0, 1, None, None,
+                                          lib.RSIM_FUSED, 0, None, None) == lib.RSIM_E_INVALID_ARG
+    assert b"history" in L.rsim_last_error()
