@@ -190,10 +190,19 @@ int plasma_transmission(void* u, int nx, int ny, int dtype, double dx, double dy
 }
 
 // FP32 (delta, beta) volumes of the fused engine, one per (plasma, device,
-// energy, optics); freed with the plasma.
+// energy, optics), at most kFieldsPerPlasma per plasma (a multi-energy loop
+// would otherwise keep one 8-byte-per-voxel volume per energy); freed with the
+// plasma.  Evicting synchronises the device first, so no queued kernel still
+// reads the volume.
+constexpr int kFieldsPerPlasma = 2;
+struct FieldEntry {
+  void* dev = nullptr;
+  unsigned long long stamp = 0;
+};
 std::mutex g_pmu;
+unsigned long long g_field_clock = 0;
 std::map<std::tuple<unsigned long long, int, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t>,
-         void*>
+         FieldEntry>
     g_fields;
 
 uint64_t dbits(double v) {
@@ -209,8 +218,23 @@ int plasma_field(const rsim_plasma* p, double energy, const double* optics, cons
                                    dbits(optics[1]), dbits(optics[2]), dbits(optics[3]));
   auto it = g_fields.find(key);
   if (it != g_fields.end()) {
-    *out = (const float2*)it->second;
+    it->second.stamp = ++g_field_clock;
+    *out = (const float2*)it->second.dev;
     return RSIM_OK;
+  }
+  // make room: drop this plasma's least recently used volumes
+  for (;;) {
+    int mine = 0;
+    auto oldest = g_fields.end();
+    for (auto e = g_fields.begin(); e != g_fields.end(); ++e) {
+      if (std::get<0>(e->first) != p->serial) continue;
+      ++mine;
+      if (oldest == g_fields.end() || e->second.stamp < oldest->second.stamp) oldest = e;
+    }
+    if (mine < kFieldsPerPlasma) break;
+    RSIM_HIP(hipDeviceSynchronize());
+    (void)hipFree(oldest->second.dev);
+    g_fields.erase(oldest);
   }
   const long long n = (long long)p->nz * p->ny_g * p->nx_g;
   void* f = nullptr;
@@ -220,7 +244,7 @@ int plasma_field(const rsim_plasma* p, double energy, const double* optics, cons
     (void)hipFree(f);
     return rc;
   }
-  g_fields[key] = f;
+  g_fields[key] = FieldEntry{f, ++g_field_clock};
   *out = (const float2*)f;
   return RSIM_OK;
 }
@@ -229,7 +253,7 @@ void plasma_release(unsigned long long serial) {
   std::lock_guard<std::mutex> lk(g_pmu);
   for (auto it = g_fields.begin(); it != g_fields.end();) {
     if (std::get<0>(it->first) == serial) {
-      (void)hipFree(it->second);
+      (void)hipFree(it->second.dev);
       it = g_fields.erase(it);
     } else {
       ++it;

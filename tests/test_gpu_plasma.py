@@ -188,3 +188,34 @@ def test_runner_with_plasma_sample(tmp_path):
                     sp["detector_pixel_size_x"], sp["detector_pixel_size_y"], sp["z_detector"])
     assert det.shape == (1, *exp.shape)
     assert O.relative_l2(det[0], exp) < 1e-11
+
+
+def test_fused_field_cache_across_energies_matches_fresh_runs():
+    """Three energies in turn, then the first again: the per-plasma cache keeps
+    two (delta, beta) volumes and evicts the oldest; every run must equal a
+    run on a freshly created plasma."""
+    n, nz = 256, 3
+    g, spec, u0, dx = _plasma_case(n, nz, 25)
+    p = rs.DevicePlasma(*g)
+    opt = np.array(OPTICS)
+    res = {}
+
+    def run(plasma, energy):
+        wl = O.convert_energy_wavelength(energy)
+        u, U = dev(u0.astype(np.complex64)), rs.DeviceVector(n * n, np.complex64)
+        lib.call("rsim_plasma_apply_2d", u.ptr, U.ptr, n, n, u.code, dx, dx, wl, 0.4 / dx,
+                 plasma.handle, O.convert_energy_wavelength(wl), opt.ctypes.data_as(lib._dp),
+                 spec.pixel_size_x, spec.py, spec.pixel_size_z, 0.0, 0.0, 0, nz, None, None,
+                 lib.RSIM_FUSED, 0, None)
+        return host(u, n, n)
+
+    for e in (6000.0, 7000.0, 9000.0, 6000.0):
+        say("plasma energy", e)
+        got = run(p, e)
+        fresh_p = rs.DevicePlasma(*g)
+        fresh = run(fresh_p, e)
+        fresh_p.destroy()
+        assert np.array_equal(got, fresh), e
+        res.setdefault(e, got)
+    assert np.array_equal(res[6000.0], run(p, 6000.0))
+    p.destroy()
