@@ -221,3 +221,48 @@ def test_resume_from_slice_checkpoint_is_bit_identical(tmp_path, engine, dt):
     assert np.array_equal(resumed, full)
     plain = run(None)
     assert O.relative_l2(full, plain) < (1e-12 if dt == np.complex128 else 3e-6)
+
+
+@pytest.mark.parametrize("engine,dt,tol", [("plain", np.complex128, 1e-12),
+                                           ("fused", np.complex64, 3e-6)])
+def test_cb_bpm_resume_from_slice_checkpoint(tmp_path, engine, dt, tol):
+    """Cone-beam BPM with a slice checkpoint: the resumed run equals the
+    uninterrupted checkpointed run bit for bit, and the oracle's CB-BPM
+    pipeline within the dtype's bar."""
+    rng = np.random.default_rng(46)
+    n, nz, dx, wl = 256, 12, 0.5e-6, 0.5e-10
+    grid = spheres(rng, nz, n + 6, n + 4, 10)
+    zs = -1.9
+    p = rs.SimParams(N=n * n, ny=n, dx=dx, z_detector=0.3, detector_size=n * dx * 0.9,
+                     detector_size_y=n * dx * 0.8, detector_pixel_size_x=2.3 * dx,
+                     detector_pixel_size_y=1.7 * dx, wl=wl, chunk_size=n, use_cone_beam_bpm=True)
+    p.configure_fresnel_detector(z_source=zs, z_sample=0.1)
+    table = [(rs.Material(str(i), 1.0), DB3[i + 1] * 30) for i in range(3)]
+    fp = {"N": n * n, "dtype": np.dtype(dt).str}
+    ang = [float(np.arcsin(0.4 / dx * wl))] * 2
+    src = rs.PointSource(x=0.2e-6, z=zs)
+
+    def sample():
+        return rs.Sample(z_start=0.1, pixel_size_x=dx, pixel_size_z=0.5e-6, grid=grid,
+                         materials=[rs.Material(str(i), 1.0) for i in range(3)],
+                         x_positions=np.array([0.0]), engine=engine)
+
+    def run(ck, resume=False):
+        u, U = rs.DeviceVector(n * n, dt), rs.DeviceVector(n * n, dt)
+        return WS.run_simulation(p, src, [sample()], ang, u, U, table, checkpoint=ck,
+                                 resume_checkpoint=resume)[0]
+
+    say("cb-bpm + checkpoint", engine)
+    full = run(CK.RunCheckpoint(tmp_path / "a", fp))
+    with pytest.raises(RuntimeError, match="simulated"):
+        run(_StopAfter(tmp_path / "b", fp))
+    resumed = run(CK.RunCheckpoint(tmp_path / "b", fp), resume=True)
+    assert np.array_equal(resumed, full)
+    geom = O.FresnelGeometry(z_detector=0.3, cone_beam=True).configure(zs, 0.1)
+    spec = O.SampleSpec(grid=grid, db_list=DB3 * 30, pixel_size_x=dx, pixel_size_y=dx,
+                        pixel_size_z=0.5e-6, z_start=0.1)
+    exp = O.run_point_source_sample(n, n, dx, dx, wl, (0.2e-6, 0.0, zs), np.sin(ang[0]) / wl, 0.3,
+                                    p.get_detector_size_x(), p.get_detector_size_y(),
+                                    p.detector_pixel_size_x, p.detector_pixel_size_y,
+                                    sample=spec, geometry=geom)
+    assert O.relative_l2(full, exp) < (1e-11 if dt == np.complex128 else 2e-5)
