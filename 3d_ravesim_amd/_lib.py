@@ -96,10 +96,10 @@ SIGNATURES = {
                                          _vp],
     "rsim_transmission_db_2d": [_vp, _i, _i, _i, _d, _d, _vp, _vp, _vp],
     "rsim_multislice_db_2d": [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _vp, _vp, _d, _i, _vp, _i,
-                              _vp],
+                              _i, _i, _vp],
     "rsim_multislice_adjoint_workspace": [_i, _i, _i, _i, _i, C.POINTER(_sz), C.POINTER(_sz)],
     "rsim_multislice_db_adjoint_2d": [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _vp, _vp, _d, _i,
-                                      _vp, _i, _vp, _vp, _vp, _vp],
+                                      _vp, _i, _vp, _vp, _vp, _i, _i, _vp],
     "rsim_detector_area_v1_adjoint": [_vp, _vp, _i, _i, _i, _d, _d, _i, _i, _d, _d, _d, _vp,
                                       _vp],
     "rsim_sample_apply_2d_scaled": [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _vp, _d, _d, _d, _d,

@@ -79,10 +79,26 @@ class MultisliceGeometry:
     z_det: float              # current_z of area_v1's obliquity factor
     dy: float = 0.0           # 0: dy = dx
     ckpt_every: int = 0       # 0: ceil(sqrt(n_slices))
+    engine: str = "fused"     # "fused" (complex64 fused chain where the shape allows) or "plain"
+    math: str = "fp32"        # fused butterflies: "fp32", "comp32" (compensated) or "fp64"
 
     @property
     def dyy(self) -> float:
         return self.dy if self.dy != 0.0 else self.dx
+
+    @property
+    def engine_code(self) -> int:
+        if self.engine not in ("fused", "plain"):
+            raise ValueError(f"engine must be 'fused' or 'plain', got {self.engine!r}")
+        return lib.RSIM_FUSED if self.engine == "fused" else lib.RSIM_PLAIN
+
+    @property
+    def math_flags(self) -> int:
+        flags = {"fp32": lib.RSIM_FLAG_MATH_FP32, "comp32": lib.RSIM_FLAG_MATH_COMP32,
+                 "fp64": lib.RSIM_FLAG_MATH_FP64}
+        if self.math not in flags:
+            raise ValueError(f"math must be one of {sorted(flags)}, got {self.math!r}")
+        return flags[self.math]
 
 
 def _check_volume(name: str, t: torch.Tensor) -> None:
@@ -112,7 +128,8 @@ class MultisliceIntensity(torch.autograd.Function):
         img = torch.zeros((geom.out_y, geom.out_x), dtype=torch.float64, device=dev)
         with _OnStream() as s:
             lib.call("rsim_multislice_db_2d", _p(u), _p(U), nx, ny, code, geom.dx, geom.dyy,
-                     geom.wl, geom.cutoff, _p(delta), _p(beta), geom.pz, n, _p(ckpt), every, s)
+                     geom.wl, geom.cutoff, _p(delta), _p(beta), geom.pz, n, _p(ckpt), every,
+                     geom.engine_code, geom.math_flags, s)
             lib.call("rsim_propagate_2d", _p(u), _p(U), nx, ny, code, geom.dx, geom.dyy, geom.wl,
                      geom.z_leg, geom.cutoff, 0, s)
             lib.call("rsim_detector_area_v1_accumulate", _p(u), nx, ny, code, geom.dx, geom.dyy,
@@ -142,7 +159,7 @@ class MultisliceIntensity(torch.autograd.Function):
                      -geom.z_leg, geom.cutoff, 0, s)
             lib.call("rsim_multislice_db_adjoint_2d", _p(g), _p(U), nx, ny, code, geom.dx,
                      geom.dyy, geom.wl, geom.cutoff, _p(delta), _p(beta), geom.pz, n, _p(ckpt),
-                     every, _p(work), _p(gd), _p(gb), s)
+                     every, _p(work), _p(gd), _p(gb), geom.engine_code, geom.math_flags, s)
         return gd, gb, None, None, None
 
 

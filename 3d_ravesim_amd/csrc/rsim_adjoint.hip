@@ -15,6 +15,8 @@
 //   g  <- conj(T_s) h  (dL/du_s*)
 // Forward fields are rebuilt from checkpoints stored every `every` slices
 // (sqrt(n) checkpointing: ceil(n/every) stored fields plus every-1 rebuilt).
+// This file is the plain engine (any size, complex64 or complex128); the
+// fused complex64 engine's form of both passes is in rsim_multislice.hip.
 #include <algorithm>
 #include <cmath>
 
@@ -107,7 +109,10 @@ int transmission_db_2d(void* u, int nx, int ny, int dtype, double wl, double pz,
 
 int multislice_db(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
                   double freq, const float* delta, const float* beta, double pz, int n,
-                  void* ckpt, int every, hipStream_t s) {
+                  void* ckpt, int every, int engine, int flags, hipStream_t s) {
+  if (engine == RSIM_FUSED && fused_db_supported(nx, ny, dtype))
+    return fused_multislice_db(u, U, nx, ny, dx, dy, wl, freq, delta, beta, pz, n, ckpt, every,
+                               flags, s);
   const size_t fb = (size_t)nx * ny * elem_bytes(dtype);
   const size_t vs = (size_t)nx * ny;
   const double K = ((2.0 * M_PI) * pz) / wl;
@@ -127,7 +132,10 @@ int multislice_db(void* u, void* U, int nx, int ny, int dtype, double dx, double
 int multislice_db_adjoint(void* g, void* U, int nx, int ny, int dtype, double dx, double dy,
                           double wl, double freq, const float* delta, const float* beta,
                           double pz, int n, const void* ckpt, int every, void* work,
-                          float* gdelta, float* gbeta, hipStream_t s) {
+                          float* gdelta, float* gbeta, int engine, int flags, hipStream_t s) {
+  if (engine == RSIM_FUSED && fused_db_supported(nx, ny, dtype))
+    return fused_multislice_db_adjoint(g, U, nx, ny, dx, dy, wl, freq, delta, beta, pz, n, ckpt,
+                                       every, work, gdelta, gbeta, flags, s);
   const size_t fb = (size_t)nx * ny * elem_bytes(dtype);
   const size_t vs = (size_t)nx * ny;
   const long long np = (long long)vs;

@@ -37,11 +37,11 @@ int transmission_db_2d(void* u, int nx, int ny, int dtype, double wl, double pz,
                        const float* delta, const float* beta, hipStream_t s);
 int multislice_db(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
                   double freq, const float* delta, const float* beta, double pz, int n,
-                  void* ckpt, int every, hipStream_t s);
+                  void* ckpt, int every, int engine, int flags, hipStream_t s);
 int multislice_db_adjoint(void* g, void* U, int nx, int ny, int dtype, double dx, double dy,
                           double wl, double freq, const float* delta, const float* beta,
                           double pz, int n, const void* ckpt, int every, void* work,
-                          float* gdelta, float* gbeta, hipStream_t s);
+                          float* gdelta, float* gbeta, int engine, int flags, hipStream_t s);
 int batch_create(rsim_batch** out, const rsim_batch_instance* inst, int count, int first,
                  int n, int flags, hipStream_t s);
 int batch_run(rsim_batch* b, hipStream_t s);
@@ -678,6 +678,21 @@ int rsim_detector_area_v1_adjoint(const void* v, void* g, int nx, int ny, int dt
                                   grad_img, resolve_stream(stream));
 }
 
+static int check_engine(int engine) {
+  if (engine != RSIM_PLAIN && engine != RSIM_FUSED)
+    return fail(RSIM_E_INVALID_ARG, "engine must be RSIM_PLAIN or RSIM_FUSED");
+  return RSIM_OK;
+}
+
+// butterfly arithmetic of the fused differentiable chain: at most one mode
+static int check_db_flags(int flags) {
+  const int math = RSIM_FLAG_MATH_FP32 | RSIM_FLAG_MATH_FP64 | RSIM_FLAG_MATH_COMP32;
+  if (flags & ~math) return fail(RSIM_E_INVALID_ARG, "flags may only select the butterfly arithmetic");
+  const int m = flags & math;
+  if (m & (m - 1)) return fail(RSIM_E_INVALID_ARG, "at most one RSIM_FLAG_MATH_* flag");
+  return RSIM_OK;
+}
+
 static int check_db_args(int nx, int ny, double dx, double dy, double wl, double pz,
                          const float* delta, const float* beta, int n, int every) {
   RSIM_CHECK(check_fft_sizes(nx, ny));
@@ -699,12 +714,15 @@ int rsim_transmission_db_2d(void* u, int nx, int ny, int dtype, double wl, doubl
 
 int rsim_multislice_db_2d(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
                           double wl, double cutoff_freq, const float* delta, const float* beta,
-                          double pz, int n_slices, void* ckpt, int ckpt_every, void* stream) {
+                          double pz, int n_slices, void* ckpt, int ckpt_every, int engine,
+                          int flags, void* stream) {
   RSIM_CHECK(check_field(u, nx, ny, dtype, "u"));
   RSIM_CHECK(check_field(U, nx, ny, dtype, "U"));
   RSIM_CHECK(check_db_args(nx, ny, dx, dy, wl, pz, delta, beta, n_slices, ckpt ? ckpt_every : 1));
+  RSIM_CHECK(check_engine(engine));
+  RSIM_CHECK(check_db_flags(flags));
   return multislice_db(u, U, nx, ny, dtype, dx, dy, wl, cutoff_freq, delta, beta, pz, n_slices,
-                       ckpt, ckpt ? ckpt_every : 1, resolve_stream(stream));
+                       ckpt, ckpt ? ckpt_every : 1, engine, flags, resolve_stream(stream));
 }
 
 int rsim_multislice_adjoint_workspace(int nx, int ny, int dtype, int n_slices, int ckpt_every,
@@ -723,7 +741,7 @@ int rsim_multislice_db_adjoint_2d(void* g, void* U, int nx, int ny, int dtype, d
                                   double dy, double wl, double cutoff_freq, const float* delta,
                                   const float* beta, double pz, int n_slices, const void* ckpt,
                                   int ckpt_every, void* work, float* grad_delta,
-                                  float* grad_beta, void* stream) {
+                                  float* grad_beta, int engine, int flags, void* stream) {
   RSIM_CHECK(check_field(g, nx, ny, dtype, "g"));
   RSIM_CHECK(check_field(U, nx, ny, dtype, "U"));
   RSIM_CHECK(check_db_args(nx, ny, dx, dy, wl, pz, delta, beta, n_slices, ckpt_every));
@@ -731,9 +749,11 @@ int rsim_multislice_db_adjoint_2d(void* g, void* U, int nx, int ny, int dtype, d
   if (!grad_delta || !grad_beta) return fail(RSIM_E_INVALID_ARG, "gradient outputs must not be NULL");
   if (!work && ckpt_every > 1 && n_slices > 1)
     return fail(RSIM_E_INVALID_ARG, "work is NULL but ckpt_every > 1");
+  RSIM_CHECK(check_engine(engine));
+  RSIM_CHECK(check_db_flags(flags));
   return multislice_db_adjoint(g, U, nx, ny, dtype, dx, dy, wl, cutoff_freq, delta, beta, pz,
-                               n_slices, ckpt, ckpt_every, work, grad_delta, grad_beta,
-                               resolve_stream(stream));
+                               n_slices, ckpt, ckpt_every, work, grad_delta, grad_beta, engine,
+                               flags, resolve_stream(stream));
 }
 
 int rsim_batch_create(rsim_batch** batch, const rsim_batch_instance* instances, int count,

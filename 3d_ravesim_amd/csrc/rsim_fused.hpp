@@ -4,6 +4,9 @@
 //   FIRST  V = FFT_x(T_s * u)
 //   MID    V = FFT_x(T_s * IFFT_x(W))      (IFFT unnormalised; 1/N is in M)
 //   LAST   u = IFFT_x(W)
+//   ADJ    V = FFT_x(conj(T_s) * h), h = IFFT_x(W), with the slice's
+//          gradients from h and the stored field u_s (adjoint chain)
+//   ADJ_LAST  g = conj(T_s) * IFFT_x(W), with the gradients
 // Column pass (C columns x per workgroup, positions y):
 //   W = IFFT_y(M * FFT_y(V)),  M = exp(i(theta_g+theta_x+theta_y)) * cutoff / (nx*ny)
 // optionally storing the spectrum M*FFT2 (= FFT2 of the new field) on the
@@ -18,13 +21,16 @@
 namespace rsim {
 
 // Transmission source of the fused row pass: bilinear over material ids,
-// per-material lookup (integral voxel map) or bilinear over a per-voxel
-// delta-beta field (plasma samples).
-enum { TM_BILINEAR = 0, TM_LUT = 1, TM_FIELD = 2 };
+// per-material lookup (integral voxel map), bilinear over a per-voxel
+// delta-beta field (plasma samples), separate delta and beta planes on the
+// field grid itself (differentiable path), or none (a plain row FFT).
+enum { TM_BILINEAR = 0, TM_LUT = 1, TM_FIELD = 2, TM_PLANAR = 3, TM_NONE = 4 };
 
 struct TransArgsF {
   const uint8_t* ids;     // slice base (ny_g x nx_g)
   const float2* dbv;      // per-voxel (delta, beta) slice base, TM_FIELD
+  const float* dpl;       // delta and beta slice bases, TM_PLANAR (row stride nxg = nx)
+  const float* bpl;
   int nxg;
   AxisMapDev xm, ym;
   const float4* lut;      // per material (hi.re, hi.im, lo.re, lo.im) of the FP64 factor
@@ -50,7 +56,13 @@ template <int TM>
 __device__ __forceinline__ cpx<float> transmit(cpx<float> v, const TransArgsF& a, int x, int y0,
                                                float fy, bool iny, const uint8_t* row0,
                                                const float4* lds_lut, const float2* lds_dbf) {
-  if constexpr (TM == TM_LUT) {
+  if constexpr (TM == TM_NONE) {
+    return v;
+  } else if constexpr (TM == TM_PLANAR) {
+    const long long i = (long long)y0 * a.nxg + x;
+    const cpx<float> t = voxel_factor(a.K * a.dpl[i], a.K * a.bpl[i]);
+    return mk(v.x * t.x - v.y * t.y, v.x * t.y + v.y * t.x);
+  } else if constexpr (TM == TM_LUT) {
     if (!(iny && x >= a.xlo && x < a.xhi)) return v;
     const float4 t = lds_lut[row0[a.ax * x + a.cx]];
     const float rr = v.x * t.x - v.y * t.y + (v.x * t.z - v.y * t.w);
@@ -99,6 +111,11 @@ __device__ __forceinline__ RowT row_state(const TransArgsF& a, int y) {
     r.y0 = a.ay * y + a.cy;
     r.fy = 0.f;
     r.row0 = a.ids + (long long)(r.iny ? r.y0 : 0) * a.nxg;
+  } else if constexpr (TM == TM_PLANAR || TM == TM_NONE) {
+    r.y0 = y;
+    r.fy = 0.f;
+    r.iny = true;
+    r.row0 = a.ids;
   } else {
     r.y0 = a.ym.i0[y];
     r.fy = a.ym.fracf[y];
@@ -108,7 +125,19 @@ __device__ __forceinline__ RowT row_state(const TransArgsF& a, int y) {
   return r;
 }
 
-enum { ROW_FIRST = 0, ROW_MID = 1, ROW_LAST = 2 };
+enum { ROW_FIRST = 0, ROW_MID = 1, ROW_LAST = 2, ROW_ADJ = 3, ROW_ADJ_LAST = 4 };
+
+// Extra buffers of the differentiable path (one field per launch).
+//   ROW_MID, TM_PLANAR: fld_out (optional) receives the field IFFT_x(W)
+//     entering the slice (checkpoints and rebuilt fields);
+//   ROW_ADJ / ROW_ADJ_LAST: fld_in = the field u_s entering slice s, and
+//     gd / gb (ny x nx) receive dL/d delta_s and dL/d beta_s.
+struct AdjArgs {
+  const cpx<float>* fld_in;
+  cpx<float>* fld_out;
+  float* gd;
+  float* gb;
+};
 
 // Per-instance launch arguments.  A launch covers up to kMaxBatch fields of
 // one shape (grid.y = instance); the descriptors travel by value in the
@@ -136,7 +165,7 @@ struct ColBatch {
 
 template <class F, int C, int PS, bool SPLIT, int MODE, int TM, typename TW>
 __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(RowBatch rb, int ny, uint32_t nbytes,
-                                                           StageTw<TW> tw) {
+                                                           StageTw<TW> tw, AdjArgs aj) {
   constexpr bool LUT = TM == TM_LUT;
   constexpr int N = F::N;
   __shared__ unsigned char lds[LdsSize<F, C, PS>::slots * (SPLIT ? 4 : 8)];
@@ -171,6 +200,13 @@ __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(RowBatch rb, int ny, 
   } else if constexpr (MODE == ROW_MID) {
     reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);  // its barriers publish the LDS tables
     compiler_fence();
+    if constexpr (TM == TM_PLANAR) {
+      if (aj.fld_out && live) {
+        const Gio<float> gf(aj.fld_out, nbytes);
+#pragma unroll
+        for (int e = 0; e < F::E; ++e) gf.st(v[e], vo, last_off<F>(e, 8u));
+      }
+    }
     const RowT r = row_state<TM>(ta, yy);
 #pragma unroll
     for (int e = 0; e < F::E; ++e) {
@@ -178,6 +214,32 @@ __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(RowBatch rb, int ny, 
       if ((e & 3) == 3) compiler_fence();
     }
     reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
+  } else if constexpr (MODE == ROW_ADJ || MODE == ROW_ADJ_LAST) {
+    // h = IFFT_x(W) is dL/du* after the transmission of slice s; with
+    // w = T_s u_s: dL/d delta = 2K Im(conj(h) w), dL/d beta = -2K Re(conj(h) w),
+    // and dL/du_s* = conj(T_s) h
+    reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);
+    compiler_fence();
+    const Gio<float> gu(aj.fld_in, nbytes);
+    const long long row = (long long)yy * ta.nxg;
+    const float k2 = 2.f * ta.K;
+#pragma unroll
+    for (int e = 0; e < F::E; ++e) {
+      const long long i = row + last_pos<F>(t, e);
+      const cpx<float> tt = voxel_factor(ta.K * ta.dpl[i], ta.K * ta.bpl[i]);
+      const cpx<float> us = gu.ld(vo, last_off<F>(e, 8u));
+      const cpx<float> h = v[e];
+      const float wr = us.x * tt.x - us.y * tt.y, wi = us.x * tt.y + us.y * tt.x;
+      const float cr = h.x * wr + h.y * wi;
+      const float ci = h.x * wi - h.y * wr;
+      if (live) {
+        aj.gd[i] = k2 * ci;
+        aj.gb[i] = -(k2 * cr);
+      }
+      v[e] = mk(tt.x * h.x + tt.y * h.y, tt.x * h.y - tt.y * h.x);
+      if ((e & 3) == 3) compiler_fence();
+    }
+    if constexpr (MODE == ROW_ADJ) reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
   } else {
     reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);
   }
@@ -266,31 +328,40 @@ struct Shape {
 
 template <class S, typename TW>
 struct RowLaunch {
-  template <int MODE>
-  static void go_tm(int tm, const RowBatch& b, int count, int ny, uint32_t nb,
-                    const StageTw<TW>& tw, hipStream_t s) {
+  template <int MODE, int TM>
+  static bool launch(const RowBatch& b, int count, int ny, uint32_t nb, const StageTw<TW>& tw,
+                     hipStream_t s, const AdjArgs& aj) {
     using F = typename S::F;
     constexpr int C = S::CC;
     dim3 grid((ny + C - 1) / C, count), block(S::threads);
-    if (tm == TM_LUT)
-      hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, MODE, TM_LUT, TW>), grid, block, 0, s, b, ny, nb, tw);
-    else if (tm == TM_FIELD)
-      hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, MODE, TM_FIELD, TW>), grid, block, 0, s, b, ny, nb, tw);
-    else
-      hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, MODE, TM_BILINEAR, TW>), grid, block, 0, s, b, ny, nb, tw);
+    hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, MODE, TM, TW>), grid, block, 0, s, b, ny, nb, tw, aj);
+    return true;
   }
-  static void go(int mode, int tm, const RowBatch& b, int count, int ny, uint32_t nb,
-                 const StageTw<TW>& tw, hipStream_t s) {
-    if (mode == ROW_FIRST) {
-      go_tm<ROW_FIRST>(tm, b, count, ny, nb, tw, s);
-    } else if (mode == ROW_MID) {
-      go_tm<ROW_MID>(tm, b, count, ny, nb, tw, s);
-    } else {
+  template <int MODE>
+  static bool go_tm(int tm, const RowBatch& b, int count, int ny, uint32_t nb,
+                    const StageTw<TW>& tw, hipStream_t s, const AdjArgs& aj) {
+    switch (tm) {
+      case TM_LUT: return launch<MODE, TM_LUT>(b, count, ny, nb, tw, s, aj);
+      case TM_FIELD: return launch<MODE, TM_FIELD>(b, count, ny, nb, tw, s, aj);
+      case TM_BILINEAR: return launch<MODE, TM_BILINEAR>(b, count, ny, nb, tw, s, aj);
+      case TM_PLANAR: return launch<MODE, TM_PLANAR>(b, count, ny, nb, tw, s, aj);
+      default: return false;
+    }
+  }
+  static bool go(int mode, int tm, const RowBatch& b, int count, int ny, uint32_t nb,
+                 const StageTw<TW>& tw, hipStream_t s, const AdjArgs& aj) {
+    switch (mode) {
+      case ROW_FIRST:
+        if (tm == TM_NONE) return launch<ROW_FIRST, TM_NONE>(b, count, ny, nb, tw, s, aj);
+        return go_tm<ROW_FIRST>(tm, b, count, ny, nb, tw, s, aj);
+      case ROW_MID: return go_tm<ROW_MID>(tm, b, count, ny, nb, tw, s, aj);
       // no transmission in the last pass: one instantiation serves every source
-      using F = typename S::F;
-      constexpr int C = S::CC;
-      dim3 grid((ny + C - 1) / C, count), block(S::threads);
-      hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, ROW_LAST, TM_LUT, TW>), grid, block, 0, s, b, ny, nb, tw);
+      case ROW_LAST: return launch<ROW_LAST, TM_LUT>(b, count, ny, nb, tw, s, aj);
+      case ROW_ADJ:
+        return tm == TM_PLANAR && launch<ROW_ADJ, TM_PLANAR>(b, count, ny, nb, tw, s, aj);
+      case ROW_ADJ_LAST:
+        return tm == TM_PLANAR && launch<ROW_ADJ_LAST, TM_PLANAR>(b, count, ny, nb, tw, s, aj);
+      default: return false;
     }
   }
 };
@@ -339,44 +410,43 @@ constexpr int kColVariants = 5;
 // instances (1..kMaxBatch) of shape (N, ny) or (nx, N) in one launch.
 template <int N, typename TW>
 bool row_go(int v, int mode, int tm, const RowBatch& b, int count, int ny,
-            const StageTw<TW>& tw, hipStream_t s);
+            const StageTw<TW>& tw, hipStream_t s, const AdjArgs& aj = AdjArgs{});
 template <int N, typename TW>
 bool col_go(int v, bool spec_out, const ColBatch& b, int count, int nx, const StageTw<TW>& tw,
             float ss, hipStream_t s);
 
-#define RSIM_ROW_CASE(N, V, E, C, SP)                                          \
-  case V:                                                                     \
-    RowLaunch<Shape<N, E, C, SP>, TW>::go(mode, tm, b, count, ny, nb, tw, s); \
-    return true;
+#define RSIM_ROW_CASE(N, V, E, C, SP) \
+  case V:                            \
+    return RowLaunch<Shape<N, E, C, SP>, TW>::go(mode, tm, b, count, ny, nb, tw, s, aj);
 #define RSIM_COL_CASE(N, V, E, C, SP)                                              \
   case V:                                                                         \
     ColLaunch<Shape<N, E, C, SP>, TW>::go(spec_out, b, count, nx, nb, tw, ss, s); \
     return true;
 
-#define RSIM_FUSED_INSTANTIATE_ROWS(N)                                                            \
-  template <int NN, typename TW>                                                                  \
-  bool row_go_impl(int v, int mode, int tm, const RowBatch& b, int count, int ny,                \
-                   const StageTw<TW>& tw, hipStream_t s) {                                        \
-    const uint32_t nb = (uint32_t)((long long)NN * ny * 8);                                       \
-    switch (v) {                                                                                  \
-      RSIM_ROW_VARIANTS(NN, RSIM_ROW_CASE)                                                        \
-      default: return false;                                                                      \
-    }                                                                                             \
-  }                                                                                               \
-  template <>                                                                                     \
-  bool row_go<N, float>(int v, int mode, int tm, const RowBatch& b, int count, int ny,           \
-                        const StageTw<float>& tw, hipStream_t s) {                                \
-    return row_go_impl<N, float>(v, mode, tm, b, count, ny, tw, s);                               \
-  }                                                                                               \
-  template <>                                                                                     \
-  bool row_go<N, double>(int v, int mode, int tm, const RowBatch& b, int count, int ny,          \
-                         const StageTw<double>& tw, hipStream_t s) {                              \
-    return row_go_impl<N, double>(v, mode, tm, b, count, ny, tw, s);                              \
-  }                                                                                               \
-  template <>                                                                                     \
-  bool row_go<N, twc>(int v, int mode, int tm, const RowBatch& b, int count, int ny,             \
-                      const StageTw<twc>& tw, hipStream_t s) {                                    \
-    return row_go_impl<N, twc>(v, mode, tm, b, count, ny, tw, s);                                 \
+#define RSIM_FUSED_INSTANTIATE_ROWS(N)                                                  \
+  template <int NN, typename TW>                                                        \
+  bool row_go_impl(int v, int mode, int tm, const RowBatch& b, int count, int ny,       \
+                   const StageTw<TW>& tw, hipStream_t s, const AdjArgs& aj) {           \
+    const uint32_t nb = (uint32_t)((long long)NN * ny * 8);                             \
+    switch (v) {                                                                        \
+      RSIM_ROW_VARIANTS(NN, RSIM_ROW_CASE)                                              \
+      default: return false;                                                            \
+    }                                                                                   \
+  }                                                                                     \
+  template <>                                                                           \
+  bool row_go<N, float>(int v, int mode, int tm, const RowBatch& b, int count, int ny,  \
+                        const StageTw<float>& tw, hipStream_t s, const AdjArgs& aj) {   \
+    return row_go_impl<N, float>(v, mode, tm, b, count, ny, tw, s, aj);                 \
+  }                                                                                     \
+  template <>                                                                           \
+  bool row_go<N, double>(int v, int mode, int tm, const RowBatch& b, int count, int ny, \
+                         const StageTw<double>& tw, hipStream_t s, const AdjArgs& aj) { \
+    return row_go_impl<N, double>(v, mode, tm, b, count, ny, tw, s, aj);                \
+  }                                                                                     \
+  template <>                                                                           \
+  bool row_go<N, twc>(int v, int mode, int tm, const RowBatch& b, int count, int ny,    \
+                      const StageTw<twc>& tw, hipStream_t s, const AdjArgs& aj) {       \
+    return row_go_impl<N, twc>(v, mode, tm, b, count, ny, tw, s, aj);                   \
   }
 
 #define RSIM_FUSED_INSTANTIATE_COLS(N)                                                            \

@@ -102,6 +102,15 @@ int detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double
                      int oy, double dsx, double dsy, double z, double weight, bool accumulate,
                      double* out_dev, hipStream_t s);
 bool fused_supported(int nx, int ny, int dtype);
+// per-voxel delta/beta multislice and its adjoint on the fused engine
+bool fused_db_supported(int nx, int ny, int dtype);
+int fused_multislice_db(void* u, void* U, int nx, int ny, double dx, double dy, double wl,
+                        double freq, const float* delta, const float* beta, double pz, int n,
+                        void* ckpt, int every, int flags, hipStream_t s);
+int fused_multislice_db_adjoint(void* g, void* U, int nx, int ny, double dx, double dy,
+                                double wl, double freq, const float* delta, const float* beta,
+                                double pz, int n, const void* ckpt, int every, void* work,
+                                float* gdelta, float* gbeta, int flags, hipStream_t s);
 int propagate(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
               double dz, double freq, bool skip_fft, hipStream_t s);
 

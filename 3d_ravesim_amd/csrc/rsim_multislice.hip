@@ -46,17 +46,17 @@ namespace {
 
 template <typename TW>
 bool fused_row_b(int N, int v, int mode, int tm, const RowBatch& b, int count, int ny,
-                 const StageTw<TW>& tw, hipStream_t s) {
+                 const StageTw<TW>& tw, hipStream_t s, const AdjArgs& aj = AdjArgs{}) {
   switch (N) {
-    case 256: return row_go<256, TW>(v, mode, tm, b, count, ny, tw, s);
-    case 512: return row_go<512, TW>(v, mode, tm, b, count, ny, tw, s);
-    case 1024: return row_go<1024, TW>(v, mode, tm, b, count, ny, tw, s);
-    case 2048: return row_go<2048, TW>(v, mode, tm, b, count, ny, tw, s);
-    case 3072: return row_go<3072, TW>(v, mode, tm, b, count, ny, tw, s);
-    case 4096: return row_go<4096, TW>(v, mode, tm, b, count, ny, tw, s);
-    case 6144: return row_go<6144, TW>(v, mode, tm, b, count, ny, tw, s);
-    case 8192: return row_go<8192, TW>(v, mode, tm, b, count, ny, tw, s);
-    case 16384: return row_go<16384, TW>(v, mode, tm, b, count, ny, tw, s);
+    case 256: return row_go<256, TW>(v, mode, tm, b, count, ny, tw, s, aj);
+    case 512: return row_go<512, TW>(v, mode, tm, b, count, ny, tw, s, aj);
+    case 1024: return row_go<1024, TW>(v, mode, tm, b, count, ny, tw, s, aj);
+    case 2048: return row_go<2048, TW>(v, mode, tm, b, count, ny, tw, s, aj);
+    case 3072: return row_go<3072, TW>(v, mode, tm, b, count, ny, tw, s, aj);
+    case 4096: return row_go<4096, TW>(v, mode, tm, b, count, ny, tw, s, aj);
+    case 6144: return row_go<6144, TW>(v, mode, tm, b, count, ny, tw, s, aj);
+    case 8192: return row_go<8192, TW>(v, mode, tm, b, count, ny, tw, s, aj);
+    case 16384: return row_go<16384, TW>(v, mode, tm, b, count, ny, tw, s, aj);
     default: return false;
   }
 }
@@ -80,12 +80,13 @@ bool fused_col_b(int N, int v, bool spec_out, const ColBatch& b, int count, int 
 // single-field launches
 template <typename TW>
 bool fused_row(int N, int v, int mode, int tm, const void* in, void* out, int ny,
-               const StageTw<TW>& tw, const TransArgsF& ta, hipStream_t s) {
+               const StageTw<TW>& tw, const TransArgsF& ta, hipStream_t s,
+               const AdjArgs& aj = AdjArgs{}) {
   RowBatch b{};
   b.inst[0].in = reinterpret_cast<const cpx<float>*>(in);
   b.inst[0].out = reinterpret_cast<cpx<float>*>(out);
   b.inst[0].ta = ta;
-  return fused_row_b<TW>(N, v, mode, tm, b, 1, ny, tw, s);
+  return fused_row_b<TW>(N, v, mode, tm, b, 1, ny, tw, s, aj);
 }
 
 template <typename TW>
@@ -596,6 +597,190 @@ int slices_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double 
                              count, want_spec, scales, dzs, s, hist);
   return run_fused<float>(u, U, nx, ny, dx, dy, wl, freq, src, px, py, pz, xpos, ypos, first,
                           count, want_spec, scales, dzs, s, hist);
+}
+
+// ---------------------------------------------------------------------------
+// Differentiable path on the fused engine: per-voxel delta/beta planes on the
+// field grid (TM_PLANAR), single-pass column lengths, butterfly arithmetic as
+// in the voxel engine (FP32, compensated FP32 or FP64 twiddles).
+// ---------------------------------------------------------------------------
+bool fused_db_supported(int nx, int ny, int dtype) {
+  return fused_supported(nx, ny, dtype) && ny != kBigCol;
+}
+
+namespace {
+template <typename TW>
+struct DbPrep {
+  StageTw<TW> tx{}, ty{};
+  const ChirpTables* fwd = nullptr;   // chirp over +pz
+  const ChirpTables* adj = nullptr;   // chirp over -pz (conjugate, same mask)
+  TransArgsF ta{};
+  int rv = 0, cv = 0;
+  size_t vs = 0;
+  const float* d0 = nullptr;
+  const float* b0 = nullptr;
+  void at(int sl) {
+    ta.dpl = d0 + (size_t)sl * vs;
+    ta.bpl = b0 + (size_t)sl * vs;
+  }
+};
+
+template <typename TW>
+int db_prepare(int nx, int ny, double dx, double dy, double wl, double freq, const float* delta,
+               const float* beta, double pz, bool adjoint, DbPrep<TW>* P, hipStream_t s) {
+  const int tdtype = std::is_same<TW, twc>::value ? kTwComp : sizeof(TW) == 8 ? RSIM_C128 : RSIM_C64;
+  const void* twx[4];
+  const void* twy[4];
+  if (!reg_stage_tables(tdtype, nx, twx) || !reg_stage_tables(tdtype, ny, twy))
+    return fail(RSIM_E_HIP, "missing FFT schedule tables");
+  using TP = typename std::remove_reference<decltype(P->tx.t[0])>::type;
+  for (int i = 0; i < 4; ++i) {
+    P->tx.t[i] = reinterpret_cast<TP>(twx[i]);
+    P->ty.t[i] = reinterpret_cast<TP>(twy[i]);
+  }
+  RSIM_CHECK(chirp_tables(nx, ny, dx, dy, wl, pz, freq, &P->fwd));
+  if (adjoint) RSIM_CHECK(chirp_tables(nx, ny, dx, dy, wl, -pz, freq, &P->adj));
+  P->vs = (size_t)nx * ny;
+  P->d0 = delta;
+  P->b0 = beta;
+  P->ta.nxg = nx;
+  P->ta.K = (float)(((2.0 * M_PI) * pz) / wl);
+  P->at(0);
+  return fused_schedule<TW>(nx, ny, TM_PLANAR, P->tx, P->ty, P->ta, P->fwd, s, &P->rv, &P->cv);
+}
+
+template <typename TW>
+bool db_row(const DbPrep<TW>& P, int nx, int ny, int mode, int tm, const void* in, void* out,
+            hipStream_t s, const AdjArgs& aj = AdjArgs{}) {
+  ktime_begin(KT_ROW, s);
+  const bool ok = fused_row<TW>(nx, P.rv, mode, tm, in, out, ny, P.tx, P.ta, s, aj);
+  ktime_end(KT_ROW, s);
+  return ok;
+}
+
+template <typename TW>
+bool db_col(const DbPrep<TW>& P, int nx, int ny, const ChirpTables* ct, const void* in, void* out,
+            void* spec, hipStream_t s) {
+  ktime_begin(KT_COL, s);
+  const bool ok = fused_col<TW>(ny, P.cv, spec != nullptr, in, out, spec, nx, P.ty, ct->Bx,
+                                   ct->Ay, ct->Rmax, (float)((double)nx * (double)ny), s);
+  ktime_end(KT_COL, s);
+  return ok;
+}
+// Forward: the fused slice chain FIRST, col, (MID, col)..., LAST.  The field
+// entering slice k (k % every == 0) is stored by that slice's MID pass (the
+// input itself for k = 0); the last column pass also leaves FFT2(u) in U.
+template <typename TW>
+int db_forward(void* u, void* U, int nx, int ny, double dx, double dy, double wl, double freq,
+               const float* delta, const float* beta, double pz, int n, void* ckpt, int every,
+               hipStream_t s) {
+  DbPrep<TW> P;
+  RSIM_CHECK(db_prepare(nx, ny, dx, dy, wl, freq, delta, beta, pz, false, &P, s));
+  const size_t fb = (size_t)nx * ny * 8;
+  if (ckpt) RSIM_HIP(hipMemcpyAsync(ckpt, u, fb, hipMemcpyDeviceToDevice, s));
+  for (int k = 0; k < n; ++k) {
+    if (cancel_requested()) return fail(RSIM_E_CANCELLED, "multislice cancelled at slice %d", k);
+    P.at(k);
+    AdjArgs aj{};
+    if (ckpt && k > 0 && k % every == 0)
+      aj.fld_out = reinterpret_cast<cpx<float>*>((char*)ckpt + (size_t)(k / every) * fb);
+    const bool last = k == n - 1;
+    if (!db_row(P, nx, ny, k == 0 ? ROW_FIRST : ROW_MID, TM_PLANAR, k == 0 ? u : U, U, s, aj))
+      return fail(RSIM_E_HIP, "no fused row kernel for nx=%d", nx);
+    RSIM_LAUNCHED();
+    if (!db_col(P, nx, ny, P.fwd, U, last ? u : U, last ? U : nullptr, s))
+      return fail(RSIM_E_HIP, "no fused column kernel for ny=%d", ny);
+    RSIM_LAUNCHED();
+    if (g_progress_sync()) RSIM_HIP(hipStreamSynchronize(s));
+    report_progress("multislice", k + 1, n);
+  }
+  db_row(P, nx, ny, ROW_LAST, TM_PLANAR, u, u, s);
+  RSIM_LAUNCHED();
+  return RSIM_OK;
+}
+
+// Adjoint: one chain over all slices in the spectral row layout,
+//   V = FFT_x(g);  per slice s from the last: W = IFFT_y(conj(M) FFT_y(V)),
+//   then ADJ (or ADJ_LAST for s = 0) with the stored field u_s,
+// with the fields of each checkpoint segment rebuilt by a forward chain of
+// their own in the work slots (the last slot doubles as that chain's buffer).
+template <typename TW>
+int db_adjoint(void* g, void* U, int nx, int ny, double dx, double dy, double wl, double freq,
+               const float* delta, const float* beta, double pz, int n, const void* ckpt,
+               int every, void* work, float* gdelta, float* gbeta, hipStream_t s) {
+  DbPrep<TW> P;
+  RSIM_CHECK(db_prepare(nx, ny, dx, dy, wl, freq, delta, beta, pz, true, &P, s));
+  const size_t fb = (size_t)nx * ny * 8;
+  const int nseg = (n + every - 1) / every;
+  if (!db_row(P, nx, ny, ROW_FIRST, TM_NONE, g, U, s))
+    return fail(RSIM_E_HIP, "no fused row kernel for nx=%d", nx);
+  RSIM_LAUNCHED();
+  long long done = 0;
+  for (int seg = nseg - 1; seg >= 0; --seg) {
+    const int s0 = seg * every;
+    const int s1 = std::min(n, s0 + every);
+    const void* ck = (const char*)ckpt + (size_t)seg * fb;
+    auto field = [&](int k) -> void* {
+      return k == s0 ? const_cast<void*>(ck) : (void*)((char*)work + (size_t)(k - s0 - 1) * fb);
+    };
+    if (s1 - s0 >= 2) {
+      void* R = field(s1 - 1);
+      P.at(s0);
+      db_row(P, nx, ny, ROW_FIRST, TM_PLANAR, ck, R, s);
+      db_col(P, nx, ny, P.fwd, R, R, nullptr, s);
+      for (int k = s0 + 1; k < s1 - 1; ++k) {
+        P.at(k);
+        AdjArgs aj{};
+        aj.fld_out = reinterpret_cast<cpx<float>*>(field(k));
+        db_row(P, nx, ny, ROW_MID, TM_PLANAR, R, R, s, aj);
+        db_col(P, nx, ny, P.fwd, R, R, nullptr, s);
+      }
+      db_row(P, nx, ny, ROW_LAST, TM_PLANAR, R, R, s);
+      RSIM_LAUNCHED();
+    }
+    for (int k = s1 - 1; k >= s0; --k) {
+      if (cancel_requested()) return fail(RSIM_E_CANCELLED, "adjoint cancelled at slice %d", k);
+      if (!db_col(P, nx, ny, P.adj, U, U, nullptr, s))
+        return fail(RSIM_E_HIP, "no fused column kernel for ny=%d", ny);
+      P.at(k);
+      AdjArgs aj{};
+      aj.fld_in = reinterpret_cast<const cpx<float>*>(field(k));
+      aj.gd = gdelta + (size_t)k * P.vs;
+      aj.gb = gbeta + (size_t)k * P.vs;
+      if (!db_row(P, nx, ny, k == 0 ? ROW_ADJ_LAST : ROW_ADJ, TM_PLANAR, U, k == 0 ? g : U, s, aj))
+        return fail(RSIM_E_HIP, "no fused adjoint row kernel for nx=%d", nx);
+      RSIM_LAUNCHED();
+      ++done;
+      if (g_progress_sync()) RSIM_HIP(hipStreamSynchronize(s));
+      report_progress("multislice_adjoint", done, n);
+    }
+  }
+  return RSIM_OK;
+}
+}  // namespace
+
+int fused_multislice_db(void* u, void* U, int nx, int ny, double dx, double dy, double wl,
+                        double freq, const float* delta, const float* beta, double pz, int n,
+                        void* ckpt, int every, int flags, hipStream_t s) {
+  if (flags & RSIM_FLAG_MATH_COMP32)
+    return db_forward<twc>(u, U, nx, ny, dx, dy, wl, freq, delta, beta, pz, n, ckpt, every, s);
+  if (flags & RSIM_FLAG_MATH_FP64)
+    return db_forward<double>(u, U, nx, ny, dx, dy, wl, freq, delta, beta, pz, n, ckpt, every, s);
+  return db_forward<float>(u, U, nx, ny, dx, dy, wl, freq, delta, beta, pz, n, ckpt, every, s);
+}
+
+int fused_multislice_db_adjoint(void* g, void* U, int nx, int ny, double dx, double dy,
+                                double wl, double freq, const float* delta, const float* beta,
+                                double pz, int n, const void* ckpt, int every, void* work,
+                                float* gdelta, float* gbeta, int flags, hipStream_t s) {
+  if (flags & RSIM_FLAG_MATH_COMP32)
+    return db_adjoint<twc>(g, U, nx, ny, dx, dy, wl, freq, delta, beta, pz, n, ckpt, every, work,
+                           gdelta, gbeta, s);
+  if (flags & RSIM_FLAG_MATH_FP64)
+    return db_adjoint<double>(g, U, nx, ny, dx, dy, wl, freq, delta, beta, pz, n, ckpt, every,
+                              work, gdelta, gbeta, s);
+  return db_adjoint<float>(g, U, nx, ny, dx, dy, wl, freq, delta, beta, pz, n, ckpt, every, work,
+                           gdelta, gbeta, s);
 }
 
 int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,

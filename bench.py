@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import argparse
 import ctypes as C
+import dataclasses
 import importlib
 import json
 import math
@@ -373,9 +374,12 @@ def adjoint_main(args, rs):
     voxels) seeded normal noise (seed 4); pz = 0.5 um, lambda = 0.5 A, 0.2 m leg,
     area_v1 4x4.  Target: the image of a volume perturbed by +5 % of a second
     smooth field (seed 5).  Loss ||I - I_t||^2.  Times forward and forward +
-    backward (complex64, sqrt(128) checkpointing), checks the gradient against
+    backward (complex64, sqrt(128) checkpointing) on the plain engine and on
+    the fused chain in each butterfly arithmetic, checks the gradient against
     FP64 (complex128) central differences at --fd-voxels random voxels (seed 6)
-    and against the complex128 adjoint.
+    and against the complex128 adjoint, both from each run's own dL/dI and from
+    the complex128 dL/dI (the accuracy of the adjoint chain on its own: the
+    loss amplifies the image error by |I| / |I - I_t|).
     """
     import torch
     AG = importlib.import_module("3d_ravesim_amd.autograd")
@@ -396,37 +400,65 @@ def adjoint_main(args, rs):
                                              dtype=torch.complex128)
         del pert
 
-        def fwd_bwd(dt):
+        def fwd_bwd(dt, geom=G):
             d = delta.clone().requires_grad_(True)
             b = beta.clone().requires_grad_(True)
-            loss = ((AG.multislice_intensity(d, b, G, dtype=dt) - target) ** 2).sum()
+            I = AG.multislice_intensity(d, b, geom, dtype=dt)
+            loss = ((I - target) ** 2).sum()
             loss.backward()
-            return d.grad, b.grad, float(loss.detach())
+            return d.grad, b.grad, float(loss.detach()), I.detach()
 
-        fwd_bwd(torch.complex64)
-        torch.cuda.synchronize()
         reps = max(1, args.steps // 5)
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            with torch.no_grad():
-                AG.multislice_intensity(delta, beta, G, dtype=torch.complex64)
+
+        def timed(geom):
+            fwd_bwd(torch.complex64, geom)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                with torch.no_grad():
+                    AG.multislice_intensity(delta, beta, geom, dtype=torch.complex64)
+            torch.cuda.synchronize()
+            tf = (time.perf_counter() - t0) / reps
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                out = fwd_bwd(torch.complex64, geom)
+            torch.cuda.synchronize()
+            return tf, (time.perf_counter() - t0) / reps, out
+
         torch.cuda.synchronize()
-        t_fwd = (time.perf_counter() - t0) / reps
         t0 = time.perf_counter()
-        for _ in range(reps):
-            gd, gb, loss = fwd_bwd(torch.complex64)
-        torch.cuda.synchronize()
-        t_fb = (time.perf_counter() - t0) / reps
-        log(f"adjoint 2048^2 x 128 c64: forward {t_fwd * 1e3:.1f} ms, forward+backward {t_fb * 1e3:.1f} ms")
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        gd64, gb64, _ = fwd_bwd(torch.complex128)
+        gd64, gb64, _, I64 = fwd_bwd(torch.complex128)
         torch.cuda.synchronize()
         t_fb128 = time.perf_counter() - t0
+        gI128 = (2 * (I64 - target)).contiguous()
         log(f"complex128 forward+backward {t_fb128 * 1e3:.1f} ms")
         rel = lambda a, b: float(torch.linalg.vector_norm((a - b).double()) / torch.linalg.vector_norm(b.double()))  # noqa: E731
-        e_d, e_b = rel(gd, gd64), rel(gb, gb64)
-        log(f"c64 vs c128 adjoint gradient: delta {e_d:.3e}, beta {e_b:.3e}")
+        runs = {}
+        for name, geom in (("plain", dataclasses.replace(G, engine="plain")),
+                           ("fused_fp32", dataclasses.replace(G, math="fp32")),
+                           ("fused_comp32", dataclasses.replace(G, math="comp32")),
+                           ("fused_fp64", dataclasses.replace(G, math="fp64"))):
+            tf, tfb, (gd_, gb_, loss_, I_) = timed(geom)
+            # the adjoint chain on its own: backward from the complex128 dL/dI
+            d_ = delta.clone().requires_grad_(True)
+            b_ = beta.clone().requires_grad_(True)
+            ga_d, ga_b = torch.autograd.grad(
+                AG.multislice_intensity(d_, b_, geom, dtype=torch.complex64), (d_, b_),
+                grad_outputs=gI128)
+            runs[name] = {"forward_ms": round(tf * 1e3, 2), "forward_backward_ms": round(tfb * 1e3, 2),
+                          "rel_l2_image_vs_c128": rel(I_, I64),
+                          "rel_l2_grad_delta_vs_c128": rel(gd_, gd64),
+                          "rel_l2_grad_beta_vs_c128": rel(gb_, gb64),
+                          "rel_l2_adjoint_delta_at_c128_dLdI": rel(ga_d, gd64),
+                          "rel_l2_adjoint_beta_at_c128_dLdI": rel(ga_b, gb64), "loss": loss_}
+            r = runs[name]
+            log(f"adjoint 2048^2 x 128 c64 {name}: forward {tf * 1e3:.1f} ms, forward+backward "
+                f"{tfb * 1e3:.1f} ms; vs c128: image {r['rel_l2_image_vs_c128']:.2e}, "
+                f"grad delta {r['rel_l2_grad_delta_vs_c128']:.2e}, beta {r['rel_l2_grad_beta_vs_c128']:.2e}; "
+                f"adjoint at the c128 dL/dI: delta {r['rel_l2_adjoint_delta_at_c128_dLdI']:.2e}, "
+                f"beta {r['rel_l2_adjoint_beta_at_c128_dLdI']:.2e}")
+            del gd_, gb_, I_, d_, b_, ga_d, ga_b
+        main_run = runs["fused_" + G.math]
 
         def L(dd, bb):
             with torch.no_grad():
@@ -455,13 +487,14 @@ def adjoint_main(args, rs):
                  C.byref(ckb), C.byref(wkb))
     print(json.dumps({
         "metric": "differentiable multislice forward+backward ms (2048^2 x 128, c64)",
-        "value": round(t_fb * 1e3, 2), "unit": "ms", "n_gpus": 1, "steps": reps,
+        "value": main_run["forward_backward_ms"], "unit": "ms", "n_gpus": 1, "steps": reps,
         "higher_is_better": False, "dtype": "c64", "data": "synthetic",
-        "forward_ms": round(t_fwd * 1e3, 2), "c128_forward_backward_ms": round(t_fb128 * 1e3, 2),
-        "slice_steps_per_s_forward_backward": round(nz / t_fb, 1),
+        "engine": "fused", "math": G.math, "forward_ms": main_run["forward_ms"],
+        "c128_forward_backward_ms": round(t_fb128 * 1e3, 2),
+        "slice_steps_per_s_forward_backward": round(nz * 1e3 / main_run["forward_backward_ms"], 1),
         "checkpoint_every": every, "checkpoint_MiB": ckb.value >> 20, "work_MiB": wkb.value >> 20,
-        "rel_l2_grad_delta_c64_vs_c128": e_d, "rel_l2_grad_beta_c64_vs_c128": e_b,
-        "fd_voxels": args.fd_voxels, "fd_max_err_over_max_grad": worst, "loss": loss,
+        "runs": runs,
+        "fd_voxels": args.fd_voxels, "fd_max_err_over_max_grad": worst,
         "config": {"workload": "2048x2048x128 float32 delta/beta (smooth seeded noise, seed 4), "
                                "lambda 0.5A, pz 0.5um, 0.2 m leg, area_v1 4x4, target +5% (seed 5)"},
     }), flush=True)

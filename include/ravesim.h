@@ -317,10 +317,15 @@ int rsim_transmission_db_2d(void* u, int nx, int ny, int dtype, double wl, doubl
                             const float* delta, const float* beta, void* stream);
 /* Forward multislice over n_slices.  U is scratch (holds FFT2(u) on return).
  * ckpt (optional): receives u before slice k for every k % ckpt_every == 0,
- * ceil(n_slices / ckpt_every) fields. */
+ * ceil(n_slices / ckpt_every) fields.  engine RSIM_PLAIN, or RSIM_FUSED: the
+ * fused complex64 chain (row lengths 256..16384, column lengths 256..8192 of
+ * the fused set; other shapes and complex128 run plain), whose butterfly
+ * arithmetic flags selects (one RSIM_FLAG_MATH_*; none = FP32).  The adjoint
+ * must use the engine and flags that wrote the checkpoints. */
 int rsim_multislice_db_2d(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
                           double wl, double cutoff_freq, const float* delta, const float* beta,
-                          double pz, int n_slices, void* ckpt, int ckpt_every, void* stream);
+                          double pz, int n_slices, void* ckpt, int ckpt_every, int engine,
+                          int flags, void* stream);
 /* Bytes of the checkpoint and work buffers for (n_slices, ckpt_every);
  * ckpt_every = ceil(sqrt(n_slices)) minimises their sum. */
 int rsim_multislice_adjoint_workspace(int nx, int ny, int dtype, int n_slices, int ckpt_every,
@@ -328,12 +333,13 @@ int rsim_multislice_adjoint_workspace(int nx, int ny, int dtype, int n_slices, i
 /* Adjoint multislice.  On entry g = dL/du* after the last slice, on return
  * dL/du* before the first.  grad_delta / grad_beta (n_slices, ny, nx) receive
  * dL/d delta and dL/d beta.  ckpt as written by rsim_multislice_db_2d with the
- * same ckpt_every; work holds the rebuilt fields of one segment; U is scratch. */
+ * same ckpt_every, engine and flags; work holds the rebuilt fields of one
+ * segment; U is scratch. */
 int rsim_multislice_db_adjoint_2d(void* g, void* U, int nx, int ny, int dtype, double dx,
                                   double dy, double wl, double cutoff_freq, const float* delta,
                                   const float* beta, double pz, int n_slices, const void* ckpt,
                                   int ckpt_every, void* work, float* grad_delta,
-                                  float* grad_beta, void* stream);
+                                  float* grad_beta, int engine, int flags, void* stream);
 /* Adjoint of rsim_detector_area_v1 (big-wave/propagation.py:995-1130): given
  * grad_img = dL/dI (out_y, out_x) float64 on the device, writes
  * g = dL/dv* = v * A^T grad_img, where I = A |v|^2. */

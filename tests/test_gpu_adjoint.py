@@ -109,15 +109,17 @@ def test_multislice_db_forward_and_checkpoints_match_oracle(dt, tol):
     ck = torch.empty((2, 128, 128), dtype=u.dtype, device=u.device)
     d, b = vol(delta), vol(beta)
     lib.call("rsim_multislice_db_2d", P(u), P(U), 128, 128, code(dt), geo["dx"], geo["dy"],
-             geo["wl"], geo["cutoff_freq"], P(d), P(b), geo["pz"], 6, P(ck), every, S())
+             geo["wl"], geo["cutoff_freq"], P(d), P(b), geo["pz"], 6, P(ck), every, lib.RSIM_PLAIN,
+             0, S())
     assert rel(u.cpu().numpy(), exp) < tol
     assert rel(ck[0].cpu().numpy(), before[0]) < tol
     assert rel(ck[1].cpu().numpy(), before[4]) < tol
     assert rel(U.cpu().numpy(), O.fft2(exp)) < tol
 
 
-def _run_adjoint(delta, beta, u0, geo, gI, dt, every):
+def _run_adjoint(delta, beta, u0, geo, gI, dt, every, engine=None, flags=0):
     """HIP forward (with checkpoints), leg, detector adjoint, leg adjoint, slice adjoint."""
+    engine = lib.RSIM_PLAIN if engine is None else engine
     nz, ny, nx = delta.shape
     c = code(dt)
     d, b = vol(delta), vol(beta)
@@ -130,7 +132,7 @@ def _run_adjoint(delta, beta, u0, geo, gI, dt, every):
     work = torch.empty(max(wkb.value // es, 1), dtype=u.dtype, device=u.device)
     s = S()
     lib.call("rsim_multislice_db_2d", P(u), P(U), nx, ny, c, geo["dx"], geo["dy"], geo["wl"],
-             geo["cutoff_freq"], P(d), P(b), geo["pz"], nz, P(ck), every, s)
+             geo["cutoff_freq"], P(d), P(b), geo["pz"], nz, P(ck), every, engine, flags, s)
     lib.call("rsim_propagate_2d", P(u), P(U), nx, ny, c, geo["dx"], geo["dy"], geo["wl"],
              geo["z_leg"], geo["cutoff_freq"], 0, s)
     oy, ox = gI.shape
@@ -143,7 +145,7 @@ def _run_adjoint(delta, beta, u0, geo, gI, dt, every):
     gd, gb = torch.empty_like(d), torch.empty_like(b)
     lib.call("rsim_multislice_db_adjoint_2d", P(g), P(U), nx, ny, c, geo["dx"], geo["dy"],
              geo["wl"], geo["cutoff_freq"], P(d), P(b), geo["pz"], nz, P(ck), every, P(work),
-             P(gd), P(gb), s)
+             P(gd), P(gb), engine, flags, s)
     torch.cuda.synchronize()
     return gd.cpu().numpy(), gb.cpu().numpy()
 
@@ -168,6 +170,74 @@ def test_adjoint_gradient_matches_oracle(every, dt, tol):
     assert rel(gb, eb) < tol
 
 
+# ---- fused complex64 chain (FP32 butterflies; 256^2 fields) ----
+
+@pytest.mark.parametrize("nz,every", [(6, 1), (6, 4), (6, 6), (1, 1)])
+def test_fused_multislice_db_forward_and_checkpoints_match_oracle(nz, every):
+    n = 256
+    delta, beta, u0, geo = case(n=n, nz=nz, seed=21)
+    exp, before = O.multislice_db_2d(u0, delta, beta, geo["dx"], geo["dy"], geo["wl"], geo["pz"],
+                                     geo["cutoff_freq"], keep=True)
+    say("fused multislice_db forward", nz, every)
+    u = field(u0, np.complex64)
+    U = torch.empty_like(u)
+    nck = -(-nz // every)
+    ck = torch.empty((nck, n, n), dtype=u.dtype, device=u.device)
+    d, b = vol(delta), vol(beta)
+    lib.call("rsim_multislice_db_2d", P(u), P(U), n, n, lib.RSIM_C64, geo["dx"], geo["dy"],
+             geo["wl"], geo["cutoff_freq"], P(d), P(b), geo["pz"], nz, P(ck), every,
+             lib.RSIM_FUSED, 0, S())
+    assert rel(u.cpu().numpy(), exp) < 3e-6
+    for j in range(nck):
+        assert rel(ck[j].cpu().numpy(), before[j * every]) < 3e-6, j
+    assert rel(U.cpu().numpy(), O.fft2(exp)) < 3e-6
+
+
+@pytest.mark.parametrize("nz,every,math", [(7, 1, "fp32"), (7, 3, "fp32"), (7, 7, "fp32"),
+                                           (1, 1, "fp32"), (7, 3, "comp32"), (7, 3, "fp64")])
+def test_fused_adjoint_gradient_matches_oracle(nz, every, math):
+    delta, beta, u0, geo = case(n=256, nz=nz, seed=22)
+    target = _target(delta, beta, u0, geo)
+    _, gI = loss_and_grad_img(delta, beta, u0, geo, target)
+    ed, eb = oracle_gradient(delta, beta, u0, geo, gI)
+    say("fused adjoint gradient", nz, every, math)
+    flags = {"fp32": lib.RSIM_FLAG_MATH_FP32, "comp32": lib.RSIM_FLAG_MATH_COMP32,
+             "fp64": lib.RSIM_FLAG_MATH_FP64}[math]
+    gd, gb = _run_adjoint(delta, beta, u0, geo, gI, np.complex64, every, lib.RSIM_FUSED, flags)
+    assert rel(gd, ed) < 1e-4
+    assert rel(gb, eb) < 1e-4
+
+
+def test_db_entry_points_reject_bad_engine_and_flags():
+    delta, beta, u0, geo = case(n=256, nz=2, seed=24)
+    say("db argument checks")
+    u = field(u0, np.complex64)
+    U = torch.empty_like(u)
+    d, b = vol(delta), vol(beta)
+    for engine, flags in ((2, 0), (lib.RSIM_FUSED, 1),
+                          (lib.RSIM_FUSED, lib.RSIM_FLAG_MATH_FP32 | lib.RSIM_FLAG_MATH_FP64)):
+        with pytest.raises(ValueError, match="engine|flags|RSIM_FLAG"):
+            lib.call("rsim_multislice_db_2d", P(u), P(U), 256, 256, lib.RSIM_C64, geo["dx"],
+                     geo["dy"], geo["wl"], geo["cutoff_freq"], P(d), P(b), geo["pz"], 2, None, 1,
+                     engine, flags, S())
+
+
+def test_fused_adjoint_checkpoint_interval_and_plain_engine_agree():
+    """Rebuilt fields equal the stored ones up to FP32 rounding: the gradients
+    hardly move with the checkpoint interval, and the fused chain agrees with
+    the plain complex64 engine."""
+    delta, beta, u0, geo = case(n=256, nz=9, seed=23)
+    target = _target(delta, beta, u0, geo)
+    _, gI = loss_and_grad_img(delta, beta, u0, geo, target)
+    say("fused adjoint: checkpoint intervals and plain engine")
+    ref = _run_adjoint(delta, beta, u0, geo, gI, np.complex64, 1, lib.RSIM_FUSED)
+    for every in (2, 3, 9, 20):
+        got = _run_adjoint(delta, beta, u0, geo, gI, np.complex64, every, lib.RSIM_FUSED)
+        assert rel(got[0], ref[0]) < 1e-6 and rel(got[1], ref[1]) < 1e-6, every
+    plain = _run_adjoint(delta, beta, u0, geo, gI, np.complex64, 3, lib.RSIM_PLAIN)
+    assert rel(plain[0], ref[0]) < 2e-5 and rel(plain[1], ref[1]) < 2e-5
+
+
 def test_checkpoint_interval_does_not_change_gradients():
     delta, beta, u0, geo = case(n=64, nz=9, seed=15)
     target = _target(delta, beta, u0, geo)
@@ -179,23 +249,26 @@ def test_checkpoint_interval_does_not_change_gradients():
         assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), every
 
 
-def _geometry(geo, n, every=0):
+def _geometry(geo, n, every=0, engine="fused"):
     oy, ox = O.detector_shape(geo["size_x"], geo["size_y"], geo["ds_x"], geo["ds_y"])
     return AG.MultisliceGeometry(dx=geo["dx"], dy=geo["dy"], wl=geo["wl"], pz=geo["pz"],
                                  cutoff=geo["cutoff_freq"], z_leg=geo["z_leg"], out_x=ox, out_y=oy,
                                  ds_x=geo["ds_x"], ds_y=geo["ds_y"], z_det=geo["z_det"],
-                                 ckpt_every=every)
+                                 ckpt_every=every, engine=engine)
 
 
-@pytest.mark.parametrize("dt,tol", [(np.complex128, 1e-9), (np.complex64, 1e-4)])
-def test_autograd_function_matches_cpu_complex128_autograd(dt, tol):
-    delta, beta, u0, geo = case(n=64, nz=6, seed=16)
+# n = 64 runs the plain engine (no fused chain at that length); n = 256 the
+# fused complex64 chain
+@pytest.mark.parametrize("dt,tol,n", [(np.complex128, 1e-9, 64), (np.complex64, 1e-4, 64),
+                                      (np.complex64, 1e-4, 256)])
+def test_autograd_function_matches_cpu_complex128_autograd(dt, tol, n):
+    delta, beta, u0, geo = case(n=n, nz=6, seed=16)
     target = _target(delta, beta, u0, geo)
     td, tb, tI = torch_gradient(delta, beta, u0, geo, target)
-    say("autograd.Function", dt.__name__)
+    say("autograd.Function", dt.__name__, n)
     d = vol(delta).requires_grad_(True)
     b = vol(beta).requires_grad_(True)
-    I = AG.multislice_intensity(d, b, _geometry(geo, 64), u0=field(u0, dt), dtype=TORCH_C[dt])
+    I = AG.multislice_intensity(d, b, _geometry(geo, n), u0=field(u0, dt), dtype=TORCH_C[dt])
     loss = ((I - torch.from_numpy(target).cuda()) ** 2).sum()
     loss.backward()
     assert rel(I.detach().cpu().numpy(), tI) < tol
