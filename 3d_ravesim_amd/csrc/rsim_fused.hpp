@@ -96,6 +96,19 @@ __device__ __forceinline__ cpx<float> transmit(cpx<float> v, const TransArgsF& a
   }
 }
 
+// A copy of a lane index that the compiler cannot prove equal to the
+// original (LD), or the index itself.  With LD the second transform of a
+// kernel recomputes its LDS exchange and twiddle addresses instead of keeping
+// the first transform's live through the pointwise step: fewer registers
+// (no spills in the 2048, 3072, 6144, 8192 and 16384 shapes), a few more
+// VALU instructions.  Both forms are launch-shape variants.  The empty asm
+// statement only hides the value from the optimiser; it emits no instruction.
+template <bool LD>
+__device__ __forceinline__ int opaque_index(int x) {
+  if constexpr (LD) asm volatile("" : "+v"(x));
+  return x;
+}
+
 // Per-row transmission state.
 struct RowT {
   int y0;
@@ -163,7 +176,7 @@ struct ColBatch {
   ColInst inst[kMaxBatch];
 };
 
-template <class F, int C, int PS, bool SPLIT, int MODE, int TM, typename TW>
+template <class F, int C, int PS, bool SPLIT, bool LD, int MODE, int TM, typename TW>
 __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(RowBatch rb, int ny, uint32_t nbytes,
                                                            StageTw<TW> tw, AdjArgs aj) {
   constexpr bool LUT = TM == TM_LUT;
@@ -213,7 +226,7 @@ __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(RowBatch rb, int ny, 
       v[e] = transmit<TM>(v[e], ta, last_pos<F>(t, e), r.y0, r.fy, r.iny, r.row0, lds_lut, lds_dbf);
       if ((e & 3) == 3) compiler_fence();
     }
-    reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
+    reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, opaque_index<LD>(t), opaque_index<LD>(c));
   } else if constexpr (MODE == ROW_ADJ || MODE == ROW_ADJ_LAST) {
     // h = IFFT_x(W) is dL/du* after the transmission of slice s; with
     // w = T_s u_s: dL/d delta = 2K Im(conj(h) w), dL/d beta = -2K Re(conj(h) w),
@@ -239,7 +252,8 @@ __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(RowBatch rb, int ny, 
       v[e] = mk(tt.x * h.x + tt.y * h.y, tt.x * h.y - tt.y * h.x);
       if ((e & 3) == 3) compiler_fence();
     }
-    if constexpr (MODE == ROW_ADJ) reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
+    if constexpr (MODE == ROW_ADJ)
+      reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, opaque_index<LD>(t), opaque_index<LD>(c));
   } else {
     reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);
   }
@@ -249,7 +263,7 @@ __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(RowBatch rb, int ny, 
   }
 }
 
-template <class F, int C, int PS, bool SPLIT, int SPEC, typename TW>
+template <class F, int C, int PS, bool SPLIT, bool LD, int SPEC, typename TW>
 __global__ void __launch_bounds__(F::TC * C) ms_col_kernel(ColBatch cb, int nx, uint32_t nbytes,
                                                            StageTw<TW> tw, float spec_scale) {
   constexpr int N = F::N;
@@ -296,7 +310,7 @@ __global__ void __launch_bounds__(F::TC * C) ms_col_kernel(ColBatch cb, int nx, 
     }
     if ((e & 3) == 3) compiler_fence();
   }
-  reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);
+  reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, opaque_index<LD>(t), opaque_index<LD>(c));
   if (live) {
 #pragma unroll
     for (int e = 0; e < F::E; ++e) go.st(v[e], vo, last_off<F>(e, row_b));
@@ -314,8 +328,9 @@ struct FacE {
   using F = RegFac<N, E, B::R0, B::R1, B::R2, B::R3>;
 };
 
-template <int N, int E, int C, bool SPLIT_ = ((size_t)N * C * 8 > 65536)>
+template <int N, int E, int C, bool SPLIT_ = ((size_t)N * C * 8 > 65536), bool LD_ = false>
 struct Shape {
+  static constexpr bool LD = LD_;
   using F = typename FacE<N, E>::F;
   static constexpr int CC = C;
   static constexpr bool SPLIT = SPLIT_;
@@ -334,7 +349,7 @@ struct RowLaunch {
     using F = typename S::F;
     constexpr int C = S::CC;
     dim3 grid((ny + C - 1) / C, count), block(S::threads);
-    hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, MODE, TM, TW>), grid, block, 0, s, b, ny, nb, tw, aj);
+    hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, S::LD, MODE, TM, TW>), grid, block, 0, s, b, ny, nb, tw, aj);
     return true;
   }
   template <int MODE>
@@ -374,9 +389,9 @@ struct ColLaunch {
     constexpr int C = S::CC;
     dim3 grid((nx + C - 1) / C, count), block(S::threads);
     if (spec_out)
-      hipLaunchKernelGGL((ms_col_kernel<F, C, S::PS, S::SPLIT, 1, TW>), grid, block, 0, s, b, nx, nb, tw, spec_scale);
+      hipLaunchKernelGGL((ms_col_kernel<F, C, S::PS, S::SPLIT, S::LD, 1, TW>), grid, block, 0, s, b, nx, nb, tw, spec_scale);
     else
-      hipLaunchKernelGGL((ms_col_kernel<F, C, S::PS, S::SPLIT, 0, TW>), grid, block, 0, s, b, nx, nb, tw, spec_scale);
+      hipLaunchKernelGGL((ms_col_kernel<F, C, S::PS, S::SPLIT, S::LD, 0, TW>), grid, block, 0, s, b, nx, nb, tw, spec_scale);
   }
 };
 
@@ -392,90 +407,125 @@ constexpr int vpow2floor(int v) { return v >= 16 ? 16 : v >= 8 ? 8 : v >= 4 ? 4 
 constexpr int vLdsC(int n) { return vpow2floor(vmax1((163840 - 8192) / (n * 4))); }
 template <int N> constexpr int vRowC(int E, int t) { return vmin(vmin(vmax1(t / (N / E)), 16), vLdsC(N)); }
 template <int N> constexpr int vColC(int E, int t) { return vmin(vmin(vmax1(t / (N / E)), 16), vLdsC(N)); }
-#define RSIM_ROW_VARIANTS(N, X)                                                           \
-  X(N, 0, vE1<N>(), vRowC<N>(vE1<N>(), 256), vsplit(N, vRowC<N>(vE1<N>(), 256), 65536))    \
-  X(N, 1, vE1<N>(), vRowC<N>(vE1<N>(), 512), vsplit(N, vRowC<N>(vE1<N>(), 512), 65536))    \
-  X(N, 2, vE1<N>(), vRowC<N>(vE1<N>(), 256), true)                                        \
-  X(N, 3, vE2<N>(), vRowC<N>(vE2<N>(), 256), vsplit(N, vRowC<N>(vE2<N>(), 256), 65536))
-#define RSIM_COL_VARIANTS(N, X)                                                           \
-  X(N, 0, vE1<N>(), vColC<N>(vE1<N>(), 1024), true)                                       \
-  X(N, 1, vE1<N>(), vColC<N>(vE1<N>(), 1024), vsplit(N, vColC<N>(vE1<N>(), 1024), 163840 - 8192)) \
-  X(N, 2, vE2<N>(), vColC<N>(vE2<N>(), 512), true)                                        \
-  X(N, 3, vE1<N>(), vmax1(vColC<N>(vE1<N>(), 1024) / 2), true)                            \
-  X(N, 4, vE2<N>(), vColC<N>(vE2<N>(), 1024), true)
-constexpr int kRowVariants = 4;
-constexpr int kColVariants = 5;
+// X(N, id, E, C, SPLIT, LD).  The base lists keep the index of the second
+// transform of a kernel as it is (LD false); the LD lists are copies of some
+// base shapes whose second transform recomputes its addresses
+// (opaque_index).  The two sets are compiled in separate translation units
+// (rsim_fused_<N>.hip and rsim_fused_<N>_ld.hip): kernels of the other kind
+// in the same module change the code generated for the base kernels.
+// Variant ids: base 0..kRowBase-1, then the LD list.
+#define RSIM_ROW_VARIANTS(N, X)                                                                 \
+  X(N, 0, vE1<N>(), vRowC<N>(vE1<N>(), 256), vsplit(N, vRowC<N>(vE1<N>(), 256), 65536), false)   \
+  X(N, 1, vE1<N>(), vRowC<N>(vE1<N>(), 512), vsplit(N, vRowC<N>(vE1<N>(), 512), 65536), false)   \
+  X(N, 2, vE1<N>(), vRowC<N>(vE1<N>(), 256), true, false)                                       \
+  X(N, 3, vE2<N>(), vRowC<N>(vE2<N>(), 256), vsplit(N, vRowC<N>(vE2<N>(), 256), 65536), false)
+#define RSIM_ROW_VARIANTS_LD(N, X)                                                              \
+  X(N, 0, vE1<N>(), vRowC<N>(vE1<N>(), 256), vsplit(N, vRowC<N>(vE1<N>(), 256), 65536), true)    \
+  X(N, 1, vE1<N>(), vRowC<N>(vE1<N>(), 512), vsplit(N, vRowC<N>(vE1<N>(), 512), 65536), true)    \
+  X(N, 2, vE2<N>(), vRowC<N>(vE2<N>(), 256), vsplit(N, vRowC<N>(vE2<N>(), 256), 65536), true)
+#define RSIM_COL_VARIANTS(N, X)                                                                 \
+  X(N, 0, vE1<N>(), vColC<N>(vE1<N>(), 1024), true, false)                                      \
+  X(N, 1, vE1<N>(), vColC<N>(vE1<N>(), 1024), vsplit(N, vColC<N>(vE1<N>(), 1024), 163840 - 8192), false) \
+  X(N, 2, vE2<N>(), vColC<N>(vE2<N>(), 512), true, false)                                       \
+  X(N, 3, vE1<N>(), vmax1(vColC<N>(vE1<N>(), 1024) / 2), true, false)                           \
+  X(N, 4, vE2<N>(), vColC<N>(vE2<N>(), 1024), true, false)
+#define RSIM_COL_VARIANTS_LD(N, X)                                                              \
+  X(N, 0, vE1<N>(), vColC<N>(vE1<N>(), 1024), true, true)                                       \
+  X(N, 1, vE1<N>(), vColC<N>(vE1<N>(), 1024), vsplit(N, vColC<N>(vE1<N>(), 1024), 163840 - 8192), true) \
+  X(N, 2, vE2<N>(), vColC<N>(vE2<N>(), 512), true, true)
+constexpr int kRowBase = 4;
+constexpr int kColBase = 5;
+constexpr int kRowVariants = kRowBase + 3;
+constexpr int kColVariants = kColBase + 3;
 
-// Per-size launchers (explicitly instantiated in rsim_fused_<N>.hip); count
-// instances (1..kMaxBatch) of shape (N, ny) or (nx, N) in one launch.
+// Per-size launchers (explicitly instantiated in rsim_fused_<N>.hip and
+// rsim_fused_<N>_ld.hip); count instances (1..kMaxBatch) of shape (N, ny) or
+// (nx, N) in one launch.
 template <int N, typename TW>
 bool row_go(int v, int mode, int tm, const RowBatch& b, int count, int ny,
             const StageTw<TW>& tw, hipStream_t s, const AdjArgs& aj = AdjArgs{});
 template <int N, typename TW>
 bool col_go(int v, bool spec_out, const ColBatch& b, int count, int nx, const StageTw<TW>& tw,
             float ss, hipStream_t s);
+template <int N, typename TW>
+bool row_go_ld(int v, int mode, int tm, const RowBatch& b, int count, int ny,
+               const StageTw<TW>& tw, hipStream_t s, const AdjArgs& aj);
+template <int N, typename TW>
+bool col_go_ld(int v, bool spec_out, const ColBatch& b, int count, int nx, const StageTw<TW>& tw,
+               float ss, hipStream_t s);
 
-#define RSIM_ROW_CASE(N, V, E, C, SP) \
-  case V:                            \
-    return RowLaunch<Shape<N, E, C, SP>, TW>::go(mode, tm, b, count, ny, nb, tw, s, aj);
-#define RSIM_COL_CASE(N, V, E, C, SP)                                              \
-  case V:                                                                         \
-    ColLaunch<Shape<N, E, C, SP>, TW>::go(spec_out, b, count, nx, nb, tw, ss, s); \
+#define RSIM_ROW_CASE(N, V, E, C, SP, LD) \
+  case V:                                \
+    return RowLaunch<Shape<N, E, C, SP, LD>, TW>::go(mode, tm, b, count, ny, nb, tw, s, aj);
+#define RSIM_COL_CASE(N, V, E, C, SP, LD)                                              \
+  case V:                                                                             \
+    ColLaunch<Shape<N, E, C, SP, LD>, TW>::go(spec_out, b, count, nx, nb, tw, ss, s); \
     return true;
 
-#define RSIM_FUSED_INSTANTIATE_ROWS(N)                                                  \
-  template <int NN, typename TW>                                                        \
-  bool row_go_impl(int v, int mode, int tm, const RowBatch& b, int count, int ny,       \
-                   const StageTw<TW>& tw, hipStream_t s, const AdjArgs& aj) {           \
-    const uint32_t nb = (uint32_t)((long long)NN * ny * 8);                             \
-    switch (v) {                                                                        \
-      RSIM_ROW_VARIANTS(NN, RSIM_ROW_CASE)                                              \
-      default: return false;                                                            \
-    }                                                                                   \
-  }                                                                                     \
-  template <>                                                                           \
-  bool row_go<N, float>(int v, int mode, int tm, const RowBatch& b, int count, int ny,  \
-                        const StageTw<float>& tw, hipStream_t s, const AdjArgs& aj) {   \
-    return row_go_impl<N, float>(v, mode, tm, b, count, ny, tw, s, aj);                 \
-  }                                                                                     \
-  template <>                                                                           \
-  bool row_go<N, double>(int v, int mode, int tm, const RowBatch& b, int count, int ny, \
-                         const StageTw<double>& tw, hipStream_t s, const AdjArgs& aj) { \
-    return row_go_impl<N, double>(v, mode, tm, b, count, ny, tw, s, aj);                \
-  }                                                                                     \
-  template <>                                                                           \
-  bool row_go<N, twc>(int v, int mode, int tm, const RowBatch& b, int count, int ny,    \
-                      const StageTw<twc>& tw, hipStream_t s, const AdjArgs& aj) {       \
-    return row_go_impl<N, twc>(v, mode, tm, b, count, ny, tw, s, aj);                   \
+// Launcher over one variant list (NAME: row_go or row_go_ld); the base one
+// hands the LD ids to row_go_ld (when HAS_LD).
+#define RSIM_ROW_LAUNCHER(NAME, N, LIST, HAS_LD)                                             \
+  template <int NN, typename TW>                                                            \
+  bool NAME##_impl(int v, int mode, int tm, const RowBatch& b, int count, int ny,           \
+                   const StageTw<TW>& tw, hipStream_t s, const AdjArgs& aj) {               \
+    if (HAS_LD && v >= kRowBase)                                                            \
+      return row_go_ld<NN, TW>(v - kRowBase, mode, tm, b, count, ny, tw, s, aj);            \
+    const uint32_t nb = (uint32_t)((long long)NN * ny * 8);                                 \
+    switch (v) {                                                                            \
+      LIST(NN, RSIM_ROW_CASE)                                                               \
+      default: return false;                                                                \
+    }                                                                                       \
+  }                                                                                         \
+  template <>                                                                               \
+  bool NAME<N, float>(int v, int mode, int tm, const RowBatch& b, int count, int ny,        \
+                      const StageTw<float>& tw, hipStream_t s, const AdjArgs& aj) {         \
+    return NAME##_impl<N, float>(v, mode, tm, b, count, ny, tw, s, aj);                     \
+  }                                                                                         \
+  template <>                                                                               \
+  bool NAME<N, double>(int v, int mode, int tm, const RowBatch& b, int count, int ny,       \
+                       const StageTw<double>& tw, hipStream_t s, const AdjArgs& aj) {       \
+    return NAME##_impl<N, double>(v, mode, tm, b, count, ny, tw, s, aj);                    \
+  }                                                                                         \
+  template <>                                                                               \
+  bool NAME<N, twc>(int v, int mode, int tm, const RowBatch& b, int count, int ny,          \
+                    const StageTw<twc>& tw, hipStream_t s, const AdjArgs& aj) {             \
+    return NAME##_impl<N, twc>(v, mode, tm, b, count, ny, tw, s, aj);                       \
   }
 
-#define RSIM_FUSED_INSTANTIATE_COLS(N)                                                            \
-  template <int NN, typename TW>                                                                  \
-  bool col_go_impl(int v, bool spec_out, const ColBatch& b, int count, int nx,                    \
-                   const StageTw<TW>& tw, float ss, hipStream_t s) {                              \
-    const uint32_t nb = (uint32_t)((long long)NN * nx * 8);                                       \
-    switch (v) {                                                                                  \
-      RSIM_COL_VARIANTS(NN, RSIM_COL_CASE)                                                        \
-      default: return false;                                                                      \
-    }                                                                                             \
-  }                                                                                               \
-  template <>                                                                                     \
-  bool col_go<N, float>(int v, bool spec_out, const ColBatch& b, int count, int nx,               \
-                        const StageTw<float>& tw, float ss, hipStream_t s) {                      \
-    return col_go_impl<N, float>(v, spec_out, b, count, nx, tw, ss, s);                           \
-  }                                                                                               \
-  template <>                                                                                     \
-  bool col_go<N, double>(int v, bool spec_out, const ColBatch& b, int count, int nx,              \
-                         const StageTw<double>& tw, float ss, hipStream_t s) {                    \
-    return col_go_impl<N, double>(v, spec_out, b, count, nx, tw, ss, s);                          \
-  }                                                                                               \
-  template <>                                                                                     \
-  bool col_go<N, twc>(int v, bool spec_out, const ColBatch& b, int count, int nx,                 \
-                      const StageTw<twc>& tw, float ss, hipStream_t s) {                          \
-    return col_go_impl<N, twc>(v, spec_out, b, count, nx, tw, ss, s);                             \
+#define RSIM_COL_LAUNCHER(NAME, N, LIST, HAS_LD)                                             \
+  template <int NN, typename TW>                                                            \
+  bool NAME##_impl(int v, bool spec_out, const ColBatch& b, int count, int nx,              \
+                   const StageTw<TW>& tw, float ss, hipStream_t s) {                        \
+    if (HAS_LD && v >= kColBase)                                                            \
+      return col_go_ld<NN, TW>(v - kColBase, spec_out, b, count, nx, tw, ss, s);            \
+    const uint32_t nb = (uint32_t)((long long)NN * nx * 8);                                 \
+    switch (v) {                                                                            \
+      LIST(NN, RSIM_COL_CASE)                                                               \
+      default: return false;                                                                \
+    }                                                                                       \
+  }                                                                                         \
+  template <>                                                                               \
+  bool NAME<N, float>(int v, bool spec_out, const ColBatch& b, int count, int nx,           \
+                      const StageTw<float>& tw, float ss, hipStream_t s) {                  \
+    return NAME##_impl<N, float>(v, spec_out, b, count, nx, tw, ss, s);                     \
+  }                                                                                         \
+  template <>                                                                               \
+  bool NAME<N, double>(int v, bool spec_out, const ColBatch& b, int count, int nx,          \
+                       const StageTw<double>& tw, float ss, hipStream_t s) {                \
+    return NAME##_impl<N, double>(v, spec_out, b, count, nx, tw, ss, s);                    \
+  }                                                                                         \
+  template <>                                                                               \
+  bool NAME<N, twc>(int v, bool spec_out, const ColBatch& b, int count, int nx,             \
+                    const StageTw<twc>& tw, float ss, hipStream_t s) {                      \
+    return NAME##_impl<N, twc>(v, spec_out, b, count, nx, tw, ss, s);                       \
   }
 
+#define RSIM_FUSED_INSTANTIATE_ROWS(N) RSIM_ROW_LAUNCHER(row_go, N, RSIM_ROW_VARIANTS, true)
+#define RSIM_FUSED_INSTANTIATE_COLS(N) RSIM_COL_LAUNCHER(col_go, N, RSIM_COL_VARIANTS, true)
 #define RSIM_FUSED_INSTANTIATE(N) RSIM_FUSED_INSTANTIATE_ROWS(N) RSIM_FUSED_INSTANTIATE_COLS(N)
+#define RSIM_FUSED_INSTANTIATE_ROWS_LD(N) RSIM_ROW_LAUNCHER(row_go_ld, N, RSIM_ROW_VARIANTS_LD, false)
+#define RSIM_FUSED_INSTANTIATE_COLS_LD(N) RSIM_COL_LAUNCHER(col_go_ld, N, RSIM_COL_VARIANTS_LD, false)
+#define RSIM_FUSED_INSTANTIATE_LD(N) RSIM_FUSED_INSTANTIATE_ROWS_LD(N) RSIM_FUSED_INSTANTIATE_COLS_LD(N)
 
 // ---------------------------------------------------------------------------
 // Three-pass column schedule for ny = N1 * N2 beyond one LDS pass

@@ -411,8 +411,10 @@ def test_phase_stepping_replays_from_snapshot():
         assert rel(outs[k], exp) < 1e-11
 
 
-N_ROW_VARIANTS, N_COL_VARIANTS = 4, 5
-VARIANT_CASES = ([(1024, 1024, (r, c)) for r in range(N_ROW_VARIANTS) for c in range(N_COL_VARIANTS)]
+N_ROW_VARIANTS, N_COL_VARIANTS = 7, 8
+VARIANT_CASES = ([(1024, 1024, (r, 0)) for r in range(N_ROW_VARIANTS)]
+                 + [(1024, 1024, (r, c)) for c in range(1, N_COL_VARIANTS)
+                    for r in (0, N_ROW_VARIANTS - 1)]
                  + [(4096, 4096, (i % N_ROW_VARIANTS, i)) for i in range(N_COL_VARIANTS)]
                  + [(256, 16384, (r, 0)) for r in range(N_ROW_VARIANTS)]
                  + [(3072, 6144, (i % N_ROW_VARIANTS, i)) for i in range(N_COL_VARIANTS)]
