@@ -176,9 +176,18 @@ struct ColBatch {
   ColInst inst[kMaxBatch];
 };
 
-template <class F, int C, int PS, bool SPLIT, bool LD, int MODE, int TM, typename TW>
-__global__ void __launch_bounds__(F::TC * C) ms_row_kernel(RowBatch rb, int ny, uint32_t nbytes,
-                                                           StageTw<TW> tw, AdjArgs aj) {
+// Occupancy request of a row launch shape: LD = 2 asks the compiler for three
+// workgroups of up to 512 threads per CU (waves per SIMD = 3 x the
+// workgroup's share), i.e. a register budget that lets them in.
+template <class F, int C, int LD>
+constexpr int row_waves_per_eu() {
+  return (LD == 2 && F::TC * C <= 512) ? 3 * ((F::TC * C + 255) / 256) : 1;
+}
+
+template <class F, int C, int PS, bool SPLIT, int LD, int MODE, int TM, typename TW>
+__global__ void __launch_bounds__(F::TC * C)
+    __attribute__((amdgpu_waves_per_eu(row_waves_per_eu<F, C, LD>())))
+    ms_row_kernel(RowBatch rb, int ny, uint32_t nbytes, StageTw<TW> tw, AdjArgs aj) {
   constexpr bool LUT = TM == TM_LUT;
   constexpr int N = F::N;
   __shared__ unsigned char lds[LdsSize<F, C, PS>::slots * (SPLIT ? 4 : 8)];
@@ -226,7 +235,7 @@ __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(RowBatch rb, int ny, 
       v[e] = transmit<TM>(v[e], ta, last_pos<F>(t, e), r.y0, r.fy, r.iny, r.row0, lds_lut, lds_dbf);
       if ((e & 3) == 3) compiler_fence();
     }
-    reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, opaque_index<LD>(t), opaque_index<LD>(c));
+    reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, opaque_index<(LD != 0)>(t), opaque_index<(LD != 0)>(c));
   } else if constexpr (MODE == ROW_ADJ || MODE == ROW_ADJ_LAST) {
     // h = IFFT_x(W) is dL/du* after the transmission of slice s; with
     // w = T_s u_s: dL/d delta = 2K Im(conj(h) w), dL/d beta = -2K Re(conj(h) w),
@@ -253,7 +262,7 @@ __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(RowBatch rb, int ny, 
       if ((e & 3) == 3) compiler_fence();
     }
     if constexpr (MODE == ROW_ADJ)
-      reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, opaque_index<LD>(t), opaque_index<LD>(c));
+      reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, opaque_index<(LD != 0)>(t), opaque_index<(LD != 0)>(c));
   } else {
     reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);
   }
@@ -263,7 +272,7 @@ __global__ void __launch_bounds__(F::TC * C) ms_row_kernel(RowBatch rb, int ny, 
   }
 }
 
-template <class F, int C, int PS, bool SPLIT, bool LD, int SPEC, typename TW>
+template <class F, int C, int PS, bool SPLIT, int LD, int SPEC, typename TW>
 __global__ void __launch_bounds__(F::TC * C) ms_col_kernel(ColBatch cb, int nx, uint32_t nbytes,
                                                            StageTw<TW> tw, float spec_scale) {
   constexpr int N = F::N;
@@ -310,7 +319,7 @@ __global__ void __launch_bounds__(F::TC * C) ms_col_kernel(ColBatch cb, int nx, 
     }
     if ((e & 3) == 3) compiler_fence();
   }
-  reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, opaque_index<LD>(t), opaque_index<LD>(c));
+  reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, opaque_index<(LD != 0)>(t), opaque_index<(LD != 0)>(c));
   if (live) {
 #pragma unroll
     for (int e = 0; e < F::E; ++e) go.st(v[e], vo, last_off<F>(e, row_b));
@@ -328,9 +337,9 @@ struct FacE {
   using F = RegFac<N, E, B::R0, B::R1, B::R2, B::R3>;
 };
 
-template <int N, int E, int C, bool SPLIT_ = ((size_t)N * C * 8 > 65536), bool LD_ = false>
+template <int N, int E, int C, bool SPLIT_ = ((size_t)N * C * 8 > 65536), int LD_ = 0>
 struct Shape {
-  static constexpr bool LD = LD_;
+  static constexpr int LD = LD_;
   using F = typename FacE<N, E>::F;
   static constexpr int CC = C;
   static constexpr bool SPLIT = SPLIT_;
@@ -408,34 +417,36 @@ constexpr int vLdsC(int n) { return vpow2floor(vmax1((163840 - 8192) / (n * 4)))
 template <int N> constexpr int vRowC(int E, int t) { return vmin(vmin(vmax1(t / (N / E)), 16), vLdsC(N)); }
 template <int N> constexpr int vColC(int E, int t) { return vmin(vmin(vmax1(t / (N / E)), 16), vLdsC(N)); }
 // X(N, id, E, C, SPLIT, LD).  The base lists keep the index of the second
-// transform of a kernel as it is (LD false); the LD lists are copies of some
+// transform of a kernel as it is (LD 0); the LD lists are copies of some
 // base shapes whose second transform recomputes its addresses
-// (opaque_index).  The two sets are compiled in separate translation units
+// (opaque_index; LD 1, or LD 2 with the three-workgroup register budget of
+// row_waves_per_eu).  The two sets are compiled in separate translation units
 // (rsim_fused_<N>.hip and rsim_fused_<N>_ld.hip): kernels of the other kind
 // in the same module change the code generated for the base kernels.
 // Variant ids: base 0..kRowBase-1, then the LD list.
 #define RSIM_ROW_VARIANTS(N, X)                                                                 \
-  X(N, 0, vE1<N>(), vRowC<N>(vE1<N>(), 256), vsplit(N, vRowC<N>(vE1<N>(), 256), 65536), false)   \
-  X(N, 1, vE1<N>(), vRowC<N>(vE1<N>(), 512), vsplit(N, vRowC<N>(vE1<N>(), 512), 65536), false)   \
-  X(N, 2, vE1<N>(), vRowC<N>(vE1<N>(), 256), true, false)                                       \
-  X(N, 3, vE2<N>(), vRowC<N>(vE2<N>(), 256), vsplit(N, vRowC<N>(vE2<N>(), 256), 65536), false)
+  X(N, 0, vE1<N>(), vRowC<N>(vE1<N>(), 256), vsplit(N, vRowC<N>(vE1<N>(), 256), 65536), 0)   \
+  X(N, 1, vE1<N>(), vRowC<N>(vE1<N>(), 512), vsplit(N, vRowC<N>(vE1<N>(), 512), 65536), 0)   \
+  X(N, 2, vE1<N>(), vRowC<N>(vE1<N>(), 256), true, 0)                                       \
+  X(N, 3, vE2<N>(), vRowC<N>(vE2<N>(), 256), vsplit(N, vRowC<N>(vE2<N>(), 256), 65536), 0)
 #define RSIM_ROW_VARIANTS_LD(N, X)                                                              \
-  X(N, 0, vE1<N>(), vRowC<N>(vE1<N>(), 256), vsplit(N, vRowC<N>(vE1<N>(), 256), 65536), true)    \
-  X(N, 1, vE1<N>(), vRowC<N>(vE1<N>(), 512), vsplit(N, vRowC<N>(vE1<N>(), 512), 65536), true)    \
-  X(N, 2, vE2<N>(), vRowC<N>(vE2<N>(), 256), vsplit(N, vRowC<N>(vE2<N>(), 256), 65536), true)
+  X(N, 0, vE1<N>(), vRowC<N>(vE1<N>(), 256), vsplit(N, vRowC<N>(vE1<N>(), 256), 65536), 1)    \
+  X(N, 1, vE1<N>(), vRowC<N>(vE1<N>(), 512), vsplit(N, vRowC<N>(vE1<N>(), 512), 65536), 1)    \
+  X(N, 2, vE2<N>(), vRowC<N>(vE2<N>(), 256), vsplit(N, vRowC<N>(vE2<N>(), 256), 65536), 1)    \
+  X(N, 3, vE1<N>(), vRowC<N>(vE1<N>(), 512), true, 2)
 #define RSIM_COL_VARIANTS(N, X)                                                                 \
-  X(N, 0, vE1<N>(), vColC<N>(vE1<N>(), 1024), true, false)                                      \
-  X(N, 1, vE1<N>(), vColC<N>(vE1<N>(), 1024), vsplit(N, vColC<N>(vE1<N>(), 1024), 163840 - 8192), false) \
-  X(N, 2, vE2<N>(), vColC<N>(vE2<N>(), 512), true, false)                                       \
-  X(N, 3, vE1<N>(), vmax1(vColC<N>(vE1<N>(), 1024) / 2), true, false)                           \
-  X(N, 4, vE2<N>(), vColC<N>(vE2<N>(), 1024), true, false)
+  X(N, 0, vE1<N>(), vColC<N>(vE1<N>(), 1024), true, 0)                                      \
+  X(N, 1, vE1<N>(), vColC<N>(vE1<N>(), 1024), vsplit(N, vColC<N>(vE1<N>(), 1024), 163840 - 8192), 0) \
+  X(N, 2, vE2<N>(), vColC<N>(vE2<N>(), 512), true, 0)                                       \
+  X(N, 3, vE1<N>(), vmax1(vColC<N>(vE1<N>(), 1024) / 2), true, 0)                           \
+  X(N, 4, vE2<N>(), vColC<N>(vE2<N>(), 1024), true, 0)
 #define RSIM_COL_VARIANTS_LD(N, X)                                                              \
-  X(N, 0, vE1<N>(), vColC<N>(vE1<N>(), 1024), true, true)                                       \
-  X(N, 1, vE1<N>(), vColC<N>(vE1<N>(), 1024), vsplit(N, vColC<N>(vE1<N>(), 1024), 163840 - 8192), true) \
-  X(N, 2, vE2<N>(), vColC<N>(vE2<N>(), 512), true, true)
+  X(N, 0, vE1<N>(), vColC<N>(vE1<N>(), 1024), true, 1)                                       \
+  X(N, 1, vE1<N>(), vColC<N>(vE1<N>(), 1024), vsplit(N, vColC<N>(vE1<N>(), 1024), 163840 - 8192), 1) \
+  X(N, 2, vE2<N>(), vColC<N>(vE2<N>(), 512), true, 1)
 constexpr int kRowBase = 4;
 constexpr int kColBase = 5;
-constexpr int kRowVariants = kRowBase + 3;
+constexpr int kRowVariants = kRowBase + 4;
 constexpr int kColVariants = kColBase + 3;
 
 // Per-size launchers (explicitly instantiated in rsim_fused_<N>.hip and
