@@ -11,13 +11,15 @@ import torch
 import oracle as O
 
 
-def case(n=16, nz=3, seed=6):
+def case(n=16, nz=3, seed=6, ny=None, dy=0.5e-6):
+    """Seeded (nz, ny, n) volumes and an (ny, n) field; ny defaults to n."""
+    ny = n if ny is None else ny
     rng = np.random.default_rng(seed)
-    delta = rng.uniform(0, 1e-6, (nz, n, n)).astype(np.float32)
-    beta = rng.uniform(0, 1e-9, (nz, n, n)).astype(np.float32)
-    u0 = np.exp(1j * rng.uniform(0, 0.3, (n, n))) * (1 + 0.05 * rng.normal(size=(n, n)))
-    geo = dict(dx=0.5e-6, dy=0.5e-6, wl=0.5e-10, pz=2e-6, cutoff_freq=0.4 / 0.5e-6, z_leg=2e-4,
-               size_x=n * 0.5e-6, size_y=n * 0.5e-6, ds_x=2 * 0.5e-6, ds_y=2 * 0.5e-6, z_det=0.2)
+    delta = rng.uniform(0, 1e-6, (nz, ny, n)).astype(np.float32)
+    beta = rng.uniform(0, 1e-9, (nz, ny, n)).astype(np.float32)
+    u0 = np.exp(1j * rng.uniform(0, 0.3, (ny, n))) * (1 + 0.05 * rng.normal(size=(ny, n)))
+    geo = dict(dx=0.5e-6, dy=dy, wl=0.5e-10, pz=2e-6, cutoff_freq=0.4 / 0.5e-6, z_leg=2e-4,
+               size_x=n * 0.5e-6, size_y=ny * dy, ds_x=2 * 0.5e-6, ds_y=2 * dy, z_det=0.2)
     return delta, beta, u0, geo
 
 

@@ -208,6 +208,29 @@ def test_fused_adjoint_gradient_matches_oracle(nz, every, math):
     assert rel(gb, eb) < 1e-4
 
 
+@pytest.mark.parametrize("nx,ny", [(512, 256), (256, 1024)])
+def test_fused_adjoint_non_square_matches_oracle(nx, ny):
+    """Rectangular fields with dy != dx: the row (x) and column (y) passes of the
+    fused chain and its adjoint use their own lengths and pitches."""
+    delta, beta, u0, geo = case(n=nx, ny=ny, dy=0.6e-6, nz=5, seed=25)
+    exp = O.multislice_db_2d(u0, delta, beta, geo["dx"], geo["dy"], geo["wl"], geo["pz"],
+                             geo["cutoff_freq"])
+    say("fused non-square", nx, ny)
+    u = field(u0, np.complex64)
+    U = torch.empty_like(u)
+    d, b = vol(delta), vol(beta)
+    lib.call("rsim_multislice_db_2d", P(u), P(U), nx, ny, lib.RSIM_C64, geo["dx"], geo["dy"],
+             geo["wl"], geo["cutoff_freq"], P(d), P(b), geo["pz"], 5, None, 1, lib.RSIM_FUSED, 0,
+             S())
+    assert rel(u.cpu().numpy(), exp) < 3e-6
+    target = _target(delta, beta, u0, geo)
+    _, gI = loss_and_grad_img(delta, beta, u0, geo, target)
+    ed, eb = oracle_gradient(delta, beta, u0, geo, gI)
+    gd, gb = _run_adjoint(delta, beta, u0, geo, gI, np.complex64, 2, lib.RSIM_FUSED)
+    assert rel(gd, ed) < 1e-4
+    assert rel(gb, eb) < 1e-4
+
+
 def test_db_entry_points_reject_bad_engine_and_flags():
     delta, beta, u0, geo = case(n=256, nz=2, seed=24)
     say("db argument checks")
