@@ -762,7 +762,8 @@ int db_forward(void* u, void* U, int nx, int ny, double dx, double dy, double wl
     if (g_progress_sync()) RSIM_HIP(hipStreamSynchronize(s));
     report_progress("multislice", k + 1, n);
   }
-  db_row(P, nx, ny, ROW_LAST, TM_PLANAR, u, u, s);
+  if (!db_row(P, nx, ny, ROW_LAST, TM_PLANAR, u, u, s))
+    return fail(RSIM_E_HIP, "no fused row kernel for nx=%d", nx);
   RSIM_LAUNCHED();
   return RSIM_OK;
 }
@@ -794,16 +795,17 @@ int db_adjoint(void* g, void* U, int nx, int ny, double dx, double dy, double wl
     if (s1 - s0 >= 2) {
       void* R = field(s1 - 1);
       P.at(s0);
-      db_row(P, nx, ny, ROW_FIRST, TM_PLANAR, ck, R, s);
-      db_col(P, nx, ny, P.fwd, R, R, nullptr, s);
-      for (int k = s0 + 1; k < s1 - 1; ++k) {
+      bool ok = db_row(P, nx, ny, ROW_FIRST, TM_PLANAR, ck, R, s) &&
+                db_col(P, nx, ny, P.fwd, R, R, nullptr, s);
+      for (int k = s0 + 1; ok && k < s1 - 1; ++k) {
         P.at(k);
         AdjArgs aj{};
         aj.fld_out = reinterpret_cast<cpx<float>*>(field(k));
-        db_row(P, nx, ny, ROW_MID, TM_PLANAR, R, R, s, aj);
-        db_col(P, nx, ny, P.fwd, R, R, nullptr, s);
+        ok = db_row(P, nx, ny, ROW_MID, TM_PLANAR, R, R, s, aj) &&
+             db_col(P, nx, ny, P.fwd, R, R, nullptr, s);
       }
-      db_row(P, nx, ny, ROW_LAST, TM_PLANAR, R, R, s);
+      ok = ok && db_row(P, nx, ny, ROW_LAST, TM_PLANAR, R, R, s);
+      if (!ok) return fail(RSIM_E_HIP, "no fused kernels for %dx%d", nx, ny);
       RSIM_LAUNCHED();
     }
     for (int k = s1 - 1; k >= s0; --k) {
