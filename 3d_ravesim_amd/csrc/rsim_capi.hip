@@ -58,6 +58,7 @@ namespace {
 thread_local std::string t_err;
 std::mutex g_mu;
 std::map<int, hipStream_t> g_streams;
+std::map<int, hipStream_t> g_side_streams;
 std::atomic<int> g_cancel{0};
 rsim_progress_fn g_progress = nullptr;
 void* g_progress_user = nullptr;
@@ -85,6 +86,19 @@ void release_streams() {
   std::lock_guard<std::mutex> lk(g_mu);
   for (auto& kv : g_streams) hipStreamDestroy(kv.second);
   g_streams.clear();
+  for (auto& kv : g_side_streams) hipStreamDestroy(kv.second);
+  g_side_streams.clear();
+}
+
+hipStream_t side_stream() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  const int dev = current_device();
+  auto it = g_side_streams.find(dev);
+  if (it != g_side_streams.end()) return it->second;
+  hipStream_t st = nullptr;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  g_side_streams[dev] = st;
+  return st;
 }
 
 int current_device() {
@@ -732,8 +746,12 @@ int rsim_multislice_adjoint_workspace(int nx, int ny, int dtype, int n_slices, i
     return fail(RSIM_E_INVALID_ARG, "sizes, n_slices and ckpt_every must be >= 1");
   const size_t fb = (size_t)nx * ny * elem_bytes(dtype);
   const int seg = ckpt_every < n_slices ? ckpt_every : n_slices;
-  if (ckpt_bytes) *ckpt_bytes = (size_t)((n_slices + ckpt_every - 1) / ckpt_every) * fb;
-  if (work_bytes) *work_bytes = (size_t)(seg - 1) * fb;
+  const int nseg = (n_slices + ckpt_every - 1) / ckpt_every;
+  // complex64: the fused engine rebuilds the next segment while the adjoint
+  // runs through the current one (two slot sets)
+  const size_t sets = (dtype == RSIM_C64 && nseg >= 2) ? 2 : 1;
+  if (ckpt_bytes) *ckpt_bytes = (size_t)nseg * fb;
+  if (work_bytes) *work_bytes = sets * (size_t)(seg - 1) * fb;
   return RSIM_OK;
 }
 

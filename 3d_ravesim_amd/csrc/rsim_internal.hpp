@@ -38,6 +38,9 @@ void clear_error();
   } while (0)
 
 hipStream_t resolve_stream(void* s);
+// a second library stream per device (default priority) for work that runs
+// beside the caller's stream, ordered with it by events
+hipStream_t side_stream();
 int current_device();
 size_t max_lds_per_block();
 

@@ -772,7 +772,11 @@ int db_forward(void* u, void* U, int nx, int ny, double dx, double dy, double wl
 //   V = FFT_x(g);  per slice s from the last: W = IFFT_y(conj(M) FFT_y(V)),
 //   then ADJ (or ADJ_LAST for s = 0) with the stored field u_s,
 // with the fields of each checkpoint segment rebuilt by a forward chain of
-// their own in the work slots (the last slot doubles as that chain's buffer).
+// their own in a set of work slots (the last slot doubles as that chain's
+// buffer).  The rebuilds run on a second stream, one segment ahead of the
+// adjoint chain, in two alternating slot sets: events order a rebuild after
+// the adjoint of the segment that last used its set, and the adjoint of a
+// segment after its rebuild.
 template <typename TW>
 int db_adjoint(void* g, void* U, int nx, int ny, double dx, double dy, double wl, double freq,
                const float* delta, const float* beta, double pz, int n, const void* ckpt,
@@ -781,51 +785,97 @@ int db_adjoint(void* g, void* U, int nx, int ny, double dx, double dy, double wl
   RSIM_CHECK(db_prepare(nx, ny, dx, dy, wl, freq, delta, beta, pz, true, &P, s));
   const size_t fb = (size_t)nx * ny * 8;
   const int nseg = (n + every - 1) / every;
-  if (!db_row(P, nx, ny, ROW_FIRST, TM_NONE, g, U, s))
-    return fail(RSIM_E_HIP, "no fused row kernel for nx=%d", nx);
-  RSIM_LAUNCHED();
-  long long done = 0;
-  for (int seg = nseg - 1; seg >= 0; --seg) {
+  const size_t set_b = (size_t)(std::min(every, n) - 1) * fb;
+  hipStream_t s2 = side_stream();
+  if (!s2) return fail(RSIM_E_HIP, "could not create the rebuild stream");
+  // events: [0] the caller's prior work, then rebuilt[seg], consumed[seg]
+  std::vector<hipEvent_t> ev(1 + 2 * (size_t)nseg, nullptr);
+  auto cleanup = [&]() {
+    for (hipEvent_t e : ev)
+      if (e) hipEventDestroy(e);
+  };
+  for (hipEvent_t& e : ev) {
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      cleanup();
+      return fail(RSIM_E_HIP, "hipEventCreateWithFlags failed");
+    }
+  }
+  hipEvent_t* rebuilt = ev.data() + 1;
+  hipEvent_t* consumed = ev.data() + 1 + nseg;
+  auto slots = [&](int seg) { return (char*)work + (size_t)((nseg - 1 - seg) & 1) * set_b; };
+  auto field = [&](int seg, int k) -> void* {
+    const int s0 = seg * every;
+    return k == s0 ? (void*)((const char*)ckpt + (size_t)seg * fb)
+                   : (void*)(slots(seg) + (size_t)(k - s0 - 1) * fb);
+  };
+  // rebuild the fields s0+1 .. s1-1 of segment seg on s2
+  auto rebuild = [&](int seg) -> int {
     const int s0 = seg * every;
     const int s1 = std::min(n, s0 + every);
-    const void* ck = (const char*)ckpt + (size_t)seg * fb;
-    auto field = [&](int k) -> void* {
-      return k == s0 ? const_cast<void*>(ck) : (void*)((char*)work + (size_t)(k - s0 - 1) * fb);
-    };
+    if (seg + 2 < nseg) RSIM_HIP(hipStreamWaitEvent(s2, consumed[seg + 2], 0));
     if (s1 - s0 >= 2) {
-      void* R = field(s1 - 1);
+      void* R = field(seg, s1 - 1);
+      const void* ck = field(seg, s0);
       P.at(s0);
-      bool ok = db_row(P, nx, ny, ROW_FIRST, TM_PLANAR, ck, R, s) &&
-                db_col(P, nx, ny, P.fwd, R, R, nullptr, s);
+      bool ok = db_row(P, nx, ny, ROW_FIRST, TM_PLANAR, ck, R, s2) &&
+                db_col(P, nx, ny, P.fwd, R, R, nullptr, s2);
       for (int k = s0 + 1; ok && k < s1 - 1; ++k) {
         P.at(k);
         AdjArgs aj{};
-        aj.fld_out = reinterpret_cast<cpx<float>*>(field(k));
-        ok = db_row(P, nx, ny, ROW_MID, TM_PLANAR, R, R, s, aj) &&
-             db_col(P, nx, ny, P.fwd, R, R, nullptr, s);
+        aj.fld_out = reinterpret_cast<cpx<float>*>(field(seg, k));
+        ok = db_row(P, nx, ny, ROW_MID, TM_PLANAR, R, R, s2, aj) &&
+             db_col(P, nx, ny, P.fwd, R, R, nullptr, s2);
       }
-      ok = ok && db_row(P, nx, ny, ROW_LAST, TM_PLANAR, R, R, s);
+      ok = ok && db_row(P, nx, ny, ROW_LAST, TM_PLANAR, R, R, s2);
       if (!ok) return fail(RSIM_E_HIP, "no fused kernels for %dx%d", nx, ny);
       RSIM_LAUNCHED();
     }
-    for (int k = s1 - 1; k >= s0; --k) {
-      if (cancel_requested()) return fail(RSIM_E_CANCELLED, "adjoint cancelled at slice %d", k);
-      if (!db_col(P, nx, ny, P.adj, U, U, nullptr, s))
-        return fail(RSIM_E_HIP, "no fused column kernel for ny=%d", ny);
-      P.at(k);
-      AdjArgs aj{};
-      aj.fld_in = reinterpret_cast<const cpx<float>*>(field(k));
-      aj.gd = gdelta + (size_t)k * P.vs;
-      aj.gb = gbeta + (size_t)k * P.vs;
-      if (!db_row(P, nx, ny, k == 0 ? ROW_ADJ_LAST : ROW_ADJ, TM_PLANAR, U, k == 0 ? g : U, s, aj))
-        return fail(RSIM_E_HIP, "no fused adjoint row kernel for nx=%d", nx);
-      RSIM_LAUNCHED();
-      ++done;
-      if (g_progress_sync()) RSIM_HIP(hipStreamSynchronize(s));
-      report_progress("multislice_adjoint", done, n);
+    RSIM_HIP(hipEventRecord(rebuilt[seg], s2));
+    return RSIM_OK;
+  };
+  auto body = [&]() -> int {
+    // the rebuilds read checkpoints written by work already on s
+    RSIM_HIP(hipEventRecord(ev[0], s));
+    RSIM_HIP(hipStreamWaitEvent(s2, ev[0], 0));
+    if (!db_row(P, nx, ny, ROW_FIRST, TM_NONE, g, U, s))
+      return fail(RSIM_E_HIP, "no fused row kernel for nx=%d", nx);
+    RSIM_LAUNCHED();
+    RSIM_CHECK(rebuild(nseg - 1));
+    long long done = 0;
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+      if (seg > 0) RSIM_CHECK(rebuild(seg - 1));
+      RSIM_HIP(hipStreamWaitEvent(s, rebuilt[seg], 0));
+      const int s0 = seg * every;
+      const int s1 = std::min(n, s0 + every);
+      for (int k = s1 - 1; k >= s0; --k) {
+        if (cancel_requested()) return fail(RSIM_E_CANCELLED, "adjoint cancelled at slice %d", k);
+        if (!db_col(P, nx, ny, P.adj, U, U, nullptr, s))
+          return fail(RSIM_E_HIP, "no fused column kernel for ny=%d", ny);
+        P.at(k);
+        AdjArgs aj{};
+        aj.fld_in = reinterpret_cast<const cpx<float>*>(field(seg, k));
+        aj.gd = gdelta + (size_t)k * P.vs;
+        aj.gb = gbeta + (size_t)k * P.vs;
+        if (!db_row(P, nx, ny, k == 0 ? ROW_ADJ_LAST : ROW_ADJ, TM_PLANAR, U, k == 0 ? g : U, s,
+                    aj))
+          return fail(RSIM_E_HIP, "no fused adjoint row kernel for nx=%d", nx);
+        RSIM_LAUNCHED();
+        ++done;
+        if (g_progress_sync()) RSIM_HIP(hipStreamSynchronize(s));
+        report_progress("multislice_adjoint", done, n);
+      }
+      RSIM_HIP(hipEventRecord(consumed[seg], s));
     }
+    return RSIM_OK;
+  };
+  int rc = body();
+  if (rc != RSIM_OK) {
+    // leave no rebuild running into buffers the caller may free
+    hipStreamSynchronize(s2);
+    hipStreamSynchronize(s);
   }
-  return RSIM_OK;
+  cleanup();
+  return rc;
 }
 }  // namespace
 

@@ -327,14 +327,19 @@ int rsim_multislice_db_2d(void* u, void* U, int nx, int ny, int dtype, double dx
                           double pz, int n_slices, void* ckpt, int ckpt_every, int engine,
                           int flags, void* stream);
 /* Bytes of the checkpoint and work buffers for (n_slices, ckpt_every);
- * ckpt_every = ceil(sqrt(n_slices)) minimises their sum. */
+ * ckpt_every = ceil(sqrt(n_slices)) minimises their sum.  The work buffer
+ * holds the rebuilt fields of one segment (ckpt_every - 1 fields), two for
+ * complex64 with more than one segment: the fused engine rebuilds the next
+ * segment while the adjoint runs through the current one. */
 int rsim_multislice_adjoint_workspace(int nx, int ny, int dtype, int n_slices, int ckpt_every,
                                       size_t* ckpt_bytes, size_t* work_bytes);
 /* Adjoint multislice.  On entry g = dL/du* after the last slice, on return
  * dL/du* before the first.  grad_delta / grad_beta (n_slices, ny, nx) receive
  * dL/d delta and dL/d beta.  ckpt as written by rsim_multislice_db_2d with the
- * same ckpt_every, engine and flags; work holds the rebuilt fields of one
- * segment; U is scratch. */
+ * same ckpt_every, engine and flags; work as sized by
+ * rsim_multislice_adjoint_workspace; U is scratch.  The fused engine runs the
+ * segment rebuilds on a second library stream ordered with `stream` by
+ * events; all of its work is ordered before later work on `stream`. */
 int rsim_multislice_db_adjoint_2d(void* g, void* U, int nx, int ny, int dtype, double dx,
                                   double dy, double wl, double cutoff_freq, const float* delta,
                                   const float* beta, double pz, int n_slices, const void* ckpt,
