@@ -159,23 +159,32 @@ int chirp_tables(int nx, int ny, double dx, double dy, double wl, double dz, dou
     ay[2 * y + 1] = std::sin(th);
   }
   const double fsq = 2.0 * (freq * freq);
-  // ky2 by |index|: index m and -m have equal squares; -ny/2 exists for even ny
+  // ky2 by |index|: index m and -m have equal squares; -ny/2 exists for even
+  // ny.  ky2[m] is non-decreasing in m for m <= ny/2, and so is the rounded
+  // sum with kx2[x]: the first m that leaves the disc is a binary search.
+  const int mmax = ny / 2;
   for (int x = 0; x < nx; ++x) {
-    int r = -1;
-    for (int m = 0; m <= ny / 2; ++m) {
-      const int yi = m;  // +m (or the Nyquist bin -ny/2 at index ny/2 for even ny)
-      if (ky2[yi] + kx2[x] > fsq) break;
-      r = m;
+    int lo = 0, hi = mmax + 1;   // first m in [0, mmax + 1) with ky2[m] + kx2[x] > fsq
+    while (lo < hi) {
+      const int mid = (lo + hi) / 2;
+      if (ky2[mid] + kx2[x] > fsq) hi = mid;
+      else lo = mid + 1;
     }
-    rmax[x] = r;
+    rmax[x] = lo - 1;
   }
+  // one allocation and one copy: Bx (16 nx), Ay (16 ny), Rmax (4 nx)
+  const size_t nb = 16 * (size_t)nx + 16 * (size_t)ny + 4 * (size_t)nx;
+  std::vector<unsigned char> host(nb);
+  std::memcpy(host.data(), bx.data(), 16 * (size_t)nx);
+  std::memcpy(host.data() + 16 * (size_t)nx, ay.data(), 16 * (size_t)ny);
+  std::memcpy(host.data() + 16 * (size_t)(nx + ny), rmax.data(), 4 * (size_t)nx);
   ChirpTables t;
-  RSIM_HIP(hipMalloc((void**)&t.Bx, 16 * (size_t)nx));
-  RSIM_HIP(hipMalloc((void**)&t.Ay, 16 * (size_t)ny));
-  RSIM_HIP(hipMalloc((void**)&t.Rmax, 4 * (size_t)nx));
-  RSIM_HIP(hipMemcpy(t.Bx, bx.data(), 16 * (size_t)nx, hipMemcpyHostToDevice));
-  RSIM_HIP(hipMemcpy(t.Ay, ay.data(), 16 * (size_t)ny, hipMemcpyHostToDevice));
-  RSIM_HIP(hipMemcpy(t.Rmax, rmax.data(), 4 * (size_t)nx, hipMemcpyHostToDevice));
+  unsigned char* dev = nullptr;
+  RSIM_HIP(hipMalloc((void**)&dev, nb));
+  RSIM_HIP(hipMemcpy(dev, host.data(), nb, hipMemcpyHostToDevice));
+  t.Bx = reinterpret_cast<double2*>(dev);
+  t.Ay = reinterpret_cast<double2*>(dev + 16 * (size_t)nx);
+  t.Rmax = reinterpret_cast<int*>(dev + 16 * (size_t)(nx + ny));
   auto r = g_chirp.emplace(key, t);
   *out = &r.first->second;
   return RSIM_OK;
@@ -406,11 +415,7 @@ int release_tune() {
 
 int release_multislice_tables() {
   std::lock_guard<std::mutex> lk(g_mu);
-  for (auto& kv : g_chirp) {
-    hipFree(kv.second.Bx);
-    hipFree(kv.second.Ay);
-    hipFree(kv.second.Rmax);
-  }
+  for (auto& kv : g_chirp) hipFree(kv.second.Bx);   // one allocation per table set
   g_chirp.clear();
   for (auto& kv : g_mat) hipFree(kv.second.dev);
   g_mat.clear();

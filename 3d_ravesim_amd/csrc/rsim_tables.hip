@@ -120,22 +120,25 @@ int axis_map(int n, double spacing, double scale, double pos, int grid_len, doub
     m.lo = lo;
     m.hi = hi;
   }
+  // one allocation and one copy: frac (8 n), i0 (4 n), fracf (4 n), inside (n)
   const size_t bytes = n * (4 + 8 + 4 + 1) + 64;
+  std::vector<unsigned char> host(bytes);
+  size_t o = 0;
+  std::memcpy(host.data() + o, fr.data(), 8 * (size_t)n);
+  const size_t o_i0 = (o += 8 * (size_t)n);
+  std::memcpy(host.data() + o, i0.data(), 4 * (size_t)n);
+  const size_t o_ff = (o += 4 * (size_t)n);
+  std::memcpy(host.data() + o, frf.data(), 4 * (size_t)n);
+  const size_t o_in = (o += 4 * (size_t)n);
+  std::memcpy(host.data() + o, ins.data(), (size_t)n);
   unsigned char* d = nullptr;
   RSIM_HIP(hipMalloc((void**)&d, bytes));
   g_allocs.push_back(d);
-  size_t o = 0;
-  RSIM_HIP(hipMemcpy(d + o, fr.data(), 8 * n, hipMemcpyHostToDevice));
-  m.dev.frac = (const double*)(d + o);
-  o += 8 * (size_t)n;
-  RSIM_HIP(hipMemcpy(d + o, i0.data(), 4 * n, hipMemcpyHostToDevice));
-  m.dev.i0 = (const int*)(d + o);
-  o += 4 * (size_t)n;
-  RSIM_HIP(hipMemcpy(d + o, frf.data(), 4 * n, hipMemcpyHostToDevice));
-  m.dev.fracf = (const float*)(d + o);
-  o += 4 * (size_t)n;
-  RSIM_HIP(hipMemcpy(d + o, ins.data(), n, hipMemcpyHostToDevice));
-  m.dev.inside = (const uint8_t*)(d + o);
+  RSIM_HIP(hipMemcpy(d, host.data(), bytes, hipMemcpyHostToDevice));
+  m.dev.frac = (const double*)d;
+  m.dev.i0 = (const int*)(d + o_i0);
+  m.dev.fracf = (const float*)(d + o_ff);
+  m.dev.inside = (const uint8_t*)(d + o_in);
   auto r = g_maps.emplace(key, m);
   *out = &r.first->second;
   return RSIM_OK;
