@@ -140,16 +140,22 @@ __device__ __forceinline__ RowT row_state(const TransArgsF& a, int y) {
 
 enum { ROW_FIRST = 0, ROW_MID = 1, ROW_LAST = 2, ROW_ADJ = 3, ROW_ADJ_LAST = 4 };
 
-// Extra buffers of the differentiable path (one field per launch).
-//   ROW_MID, TM_PLANAR: fld_out (optional) receives the field IFFT_x(W)
-//     entering the slice (checkpoints and rebuilt fields);
+// Extra buffers of single-field launches.
+//   ROW_MID: fld_out (optional) receives the field IFFT_x(W) entering the
+//     slice (history frames; the differentiable path's checkpoints and
+//     rebuilt fields);
 //   ROW_ADJ / ROW_ADJ_LAST: fld_in = the field u_s entering slice s, and
 //     gd / gb (ny x nx) receive dL/d delta_s and dL/d beta_s.
+//   ROW_MID: xb.xs (optional) receives the x-integrals of |IFFT_x(W)|^2 over
+//     the area_v1 detector columns (history frames), rows [j0, j0 + nj) of
+//     the field to xs[(y - j0) * ox + p]: detector column p covers cells
+//     [xi0, xi1] with fractions xf0 / xf1 taken off / added at the ends.
 struct AdjArgs {
   const cpx<float>* fld_in;
   cpx<float>* fld_out;
   float* gd;
   float* gb;
+  XBinArgs xb;
 };
 
 // Per-instance launch arguments.  A launch covers up to kMaxBatch fields of
@@ -222,12 +228,35 @@ __global__ void __launch_bounds__(F::TC * C)
   } else if constexpr (MODE == ROW_MID) {
     reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);  // its barriers publish the LDS tables
     compiler_fence();
-    if constexpr (TM == TM_PLANAR) {
-      if (aj.fld_out && live) {
-        const Gio<float> gf(aj.fld_out, nbytes);
+    if (aj.fld_out && live) {   // uniform: one field per launch
+      const Gio<float> gf(aj.fld_out, nbytes);
 #pragma unroll
-        for (int e = 0; e < F::E; ++e) gf.st(v[e], vo, last_off<F>(e, 8u));
+      for (int e = 0; e < F::E; ++e) gf.st(v[e], vo, last_off<F>(e, 8u));
+    }
+    if (aj.xb.xs) {   // uniform
+      // |u|^2 of the block's rows through LDS (free between the two FFTs),
+      // then each detector column's x-integral in the cells' order
+      float* e2 = reinterpret_cast<float*>(lds);
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < F::E; ++e)
+        e2[c * N + last_pos<F>(t, e)] = v[e].x * v[e].x + v[e].y * v[e].y;
+      __syncthreads();
+      const int jj = y - aj.xb.j0;
+      if (live && jj >= 0 && jj < aj.xb.nj) {
+        const float* r = e2 + c * N;
+        const double dx = aj.xb.dx;
+        for (int p = t; p < aj.xb.ox; p += F::TC) {
+          const int i0 = aj.xb.xi0[p], i1 = aj.xb.xi1[p];
+          double sum = 0.0;
+          for (int i = i0; i < i1; ++i) sum += (double)r[i];
+          double xi = sum * dx;
+          if (i1 < N) xi += aj.xb.xf1[p] * dx * (double)r[i1];
+          if (i0 < N) xi -= aj.xb.xf0[p] * dx * (double)r[i0];
+          aj.xb.xs[(long long)jj * aj.xb.ox + p] = xi;
+        }
       }
+      __syncthreads();
     }
     const RowT r = row_state<TM>(ta, yy);
 #pragma unroll

@@ -104,6 +104,28 @@ int slices_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double 
 int detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double dy, int ox,
                      int oy, double dsx, double dsy, double z, double weight, bool accumulate,
                      double* out_dev, hipStream_t s);
+// area_v1 split at the field rows.  Stage 1 (inside a row pass): field row j
+// in [j0, j0 + nj) and detector column p give the x-integral
+//   xs[(j - j0) * ox + p] = dx * (sum_{xi0 <= i < xi1} |u|^2
+//                                 + xf1 |u[xi1]|^2 - xf0 |u[xi0]|^2)
+// (end terms only for cells inside the field).  Stage 2 (detector_ybin)
+// sums them over each detector row's overlapping field rows.
+struct XBinArgs {
+  const int* xi0;
+  const int* xi1;
+  const double* xf0;
+  const double* xf1;
+  double* xs;   // nj x ox, set by the caller
+  int ox, j0, nj;
+  double dx;
+};
+// The cached plan's stage-1 arguments; *ok false when the detector reads no
+// field row or its column ranges are not monotone (then use detector_area_v1).
+int detector_xbin_plan(int nx, int ny, double dx, double dy, int ox, int oy, double dsx,
+                       double dsy, XBinArgs* xb, bool* ok);
+int detector_ybin(const double* xs, int nx, int ny, double dx, double dy, int ox, int oy,
+                  double dsx, double dsy, double z, double weight, bool accumulate,
+                  double* out_dev, hipStream_t s);
 bool fused_supported(int nx, int ny, int dtype);
 // per-voxel delta/beta multislice and its adjoint on the fused engine
 bool fused_db_supported(int nx, int ny, int dtype);

@@ -578,6 +578,25 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
   P.at(ta, first);
   int rv = 0, cv = 0;
   RSIM_CHECK(fused_schedule<TW>(nx, ny, tm, tx, ty, ta, ct, s, &rv, &cv));
+  // History frames after slice 0: the MID pass bins its field in x into xs
+  // (nj x ox doubles, stream-ordered for this run); the y stage follows it.
+  // Without a separable plan the pass stores the field into u instead.
+  XBinArgs xb{};
+  bool xbin = false;
+  struct XsBuf {
+    double* p = nullptr;
+    hipStream_t s = nullptr;
+    ~XsBuf() { if (p) (void)hipFreeAsync(p, s); }
+  } xsb;
+  if (hist && count > 1) {
+    RSIM_CHECK(detector_xbin_plan(nx, ny, dx, dy, hist->ox, hist->oy, hist->dsx, hist->dsy, &xb,
+                                  &xbin));
+    if (xbin) {
+      xsb.s = s;
+      RSIM_HIP(hipMallocAsync((void**)&xsb.p, (size_t)xb.nj * xb.ox * sizeof(double), s));
+      xb.xs = xsb.p;
+    }
+  }
   for (int k = 0; k < count; ++k) {
     const int sl = first + k;
     if (per_slice) {
@@ -591,20 +610,30 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
     }
     P.at(ta, sl);
     const bool last = k == count - 1;
-    if (hist) {
-      // the field entering this slice: the input, or the row IFFT of W (U)
-      // materialised in u, which the loop does not read after slice 0
-      if (k > 0) {
-        fused_row(nx, rv, ROW_LAST, tm, U, u, ny, tx, ta, s);
-        RSIM_LAUNCHED();
-      }
-      RSIM_CHECK(hist_frame(u, nx, ny, dx, dy, pz, sl, k, hist, s));
+    // History frame of the field entering this slice: the input for slice 0;
+    // otherwise the row IFFT of W that the MID pass holds in registers,
+    // binned in x by that pass (or stored into u, which the loop does not
+    // read after slice 0)
+    if (hist && k == 0) RSIM_CHECK(hist_frame(u, nx, ny, dx, dy, pz, sl, k, hist, s));
+    AdjArgs store{};
+    if (hist && k > 0) {
+      if (xbin) store.xb = xb;
+      else store.fld_out = reinterpret_cast<cpx<float>*>(u);
     }
     ktime_begin(KT_ROW, s);
-    if (!fused_row(nx, rv, k == 0 ? ROW_FIRST : ROW_MID, tm, k == 0 ? u : U, U, ny, tx, ta, s))
+    if (!fused_row(nx, rv, k == 0 ? ROW_FIRST : ROW_MID, tm, k == 0 ? u : U, U, ny, tx, ta, s,
+                   store))
       return fail(RSIM_E_HIP, "no fused row kernel for nx=%d", nx);
     ktime_end(KT_ROW, s);
     RSIM_LAUNCHED();
+    if (hist && k > 0) {
+      if (xbin)
+        RSIM_CHECK(detector_ybin(xb.xs, nx, ny, dx, dy, hist->ox, hist->oy, hist->dsx, hist->dsy,
+                                 hist->z_start + sl * pz, 1.0, false,
+                                 hist->frames + (size_t)k * hist->oy * hist->ox, s));
+      else
+        RSIM_CHECK(hist_frame(u, nx, ny, dx, dy, pz, sl, k, hist, s));
+    }
     ktime_begin(KT_COL, s);
     if (big) {
       // in place in U; the spectrum of the final field is formed after the loop

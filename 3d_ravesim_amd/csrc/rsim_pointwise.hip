@@ -286,6 +286,93 @@ __global__ void __launch_bounds__(256) area_v1_kernel(const cpx<R>* __restrict__
   else out[o] = acc;
 }
 
+// Separable form of area_v1_kernel, same arithmetic in the same order.
+// Stage 1: per field row j and detector column p, the x-integral
+//   xs[j - j0][p] = dx * (sum_{i0 <= i < i1} |u|^2 + f1 |u[i1]|^2 - f0 |u[i0]|^2).
+// xi0/xi1 are non-decreasing in p, so the 256 pixels of a block read one
+// contiguous span of the row: it is loaded once, coalesced, as |u|^2 into LDS
+// and the per-pixel sums run from there (directly from the row when the span
+// exceeds kDetSpan cells).
+constexpr int kDetSpan = 4096;
+template <typename R>
+__global__ void __launch_bounds__(256) area_v1_rows_kernel(const cpx<R>* __restrict__ u, int nx,
+                                                           int ox, int j0, int nj, DetPlanDev pl,
+                                                           double dx, double* __restrict__ xs) {
+  __shared__ R e2[kDetSpan];
+  const int pb = blockIdx.x * 256;
+  const int jj = blockIdx.y;
+  if (pb >= ox || jj >= nj) return;
+  const int p = pb + threadIdx.x;
+  const int pe = min(pb + 256, ox) - 1;
+  const cpx<R>* row = u + (long long)(j0 + jj) * nx;
+  const int lo = min(pl.xi0[pb], nx);
+  const int hi = min(pl.xi1[pe] + 1, nx);
+  if (hi - lo <= kDetSpan) {
+    // four loads in flight per thread (clamped indexes stay in the span)
+    for (int k0 = lo + (int)threadIdx.x; k0 < hi; k0 += 1024) {
+      cpx<R> q[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) q[k] = row[min(k0 + k * 256, hi - 1)];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k0 + k * 256 < hi) e2[k0 + k * 256 - lo] = q[k].x * q[k].x + q[k].y * q[k].y;
+    }
+    __syncthreads();
+    if (p >= ox) return;
+    const int i0 = pl.xi0[p], i1 = pl.xi1[p];
+    double s = 0.0;
+    for (int i = i0; i < i1; ++i) s += (double)e2[i - lo];
+    double xi = s * dx;
+    if (i1 < nx) xi += pl.xf1[p] * dx * (double)e2[i1 - lo];
+    if (i0 < nx) xi -= pl.xf0[p] * dx * (double)e2[i0 - lo];
+    xs[(long long)jj * ox + p] = xi;
+    return;
+  }
+  if (p >= ox) return;
+  const int i0 = pl.xi0[p], i1 = pl.xi1[p];
+  double s = 0.0;
+  for (int i = i0; i < i1; ++i) {
+    const cpx<R> v = row[i];
+    s += (double)(v.x * v.x + v.y * v.y);
+  }
+  double xi = s * dx;
+  if (i1 < nx) {
+    const cpx<R> v = row[i1];
+    xi += pl.xf1[p] * dx * (double)(v.x * v.x + v.y * v.y);
+  }
+  if (i0 < nx) {
+    const cpx<R> v = row[i0];
+    xi -= pl.xf0[p] * dx * (double)(v.x * v.x + v.y * v.y);
+  }
+  xs[(long long)jj * ox + p] = xi;
+}
+
+// Stage 2: per detector pixel, the y-overlap sum of the stage-1 rows times
+// the obliquity factor.
+__global__ void __launch_bounds__(256) area_v1_cols_kernel(const double* __restrict__ xs, int ox,
+                                                           int oy, int j0, DetPlanDev pl,
+                                                           double dy, double gyl, double z,
+                                                           double weight, int accumulate,
+                                                           double* out) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  const int q = blockIdx.y;
+  if (p >= ox || q >= oy) return;
+  const double ylo = pl.ylo[q], yhi = pl.yhi[q];
+  const double xd = pl.xdet[p], yd = pl.ydet[q];
+  const double cosine = z / sqrt(xd * xd + yd * yd + z * z);
+  double acc = 0.0;
+  for (int j = pl.jlo[q]; j < pl.jhi[q]; ++j) {
+    const double cell_lo = gyl + j * dy;
+    const double cell_hi = cell_lo + dy;
+    const double ov = fmin(cell_hi, yhi) - fmax(cell_lo, ylo);
+    if (!(ov > 0.0)) continue;
+    acc += xs[(long long)(j - j0) * ox + p] * ov * cosine;
+  }
+  const long long o = (long long)q * ox + p;
+  if (accumulate) out[o] += weight * acc;
+  else out[o] = acc;
+}
+
 namespace {
 
 // Host plan of the per-axis overlaps (propagation.py:949-977).
@@ -369,6 +456,8 @@ struct DetCached {
   unsigned char* d = nullptr;
   DetPlanDev pl{};
   double gyl = 0.0;
+  int j0 = 0, nj = 0;   // field rows any detector row reads: [j0, j0 + nj)
+  bool mono = false;    // xi0, xi1 non-decreasing in p (the separable form needs it)
 };
 std::mutex g_det_mu;
 std::map<std::tuple<int, int, int, uint64_t, uint64_t, int, int, uint64_t, uint64_t>, DetCached>
@@ -408,6 +497,17 @@ int det_plan_dev(int nx, int ny, double dx, double dy, int ox, int oy, double ds
   c.pl.xdet = (const double*)(d + b.doff[2]); c.pl.ylo = (const double*)(d + b.doff[3]);
   c.pl.yhi = (const double*)(d + b.doff[4]); c.pl.ydet = (const double*)(d + b.doff[5]);
   c.gyl = P.gyl;
+  int jmin = ny, jmax = 0;
+  for (int q = 0; q < oy; ++q)
+    if (P.jhi[q] > P.jlo[q]) {
+      jmin = std::min(jmin, P.jlo[q]);
+      jmax = std::max(jmax, P.jhi[q]);
+    }
+  c.j0 = jmax > jmin ? jmin : 0;
+  c.nj = jmax > jmin ? jmax - jmin : 0;
+  c.mono = true;
+  for (int p = 1; p < ox; ++p)
+    if (P.xi0[p] < P.xi0[p - 1] || P.xi1[p] < P.xi1[p - 1]) c.mono = false;
   *out = &(g_det[key] = c);
   return RSIM_OK;
 }
@@ -427,6 +527,30 @@ int detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double
   RSIM_CHECK(check_detector(ox, oy, z));
   const DetCached* c = nullptr;
   RSIM_CHECK(det_plan_dev(nx, ny, dx, dy, ox, oy, dsx, dsy, &c));
+  // separable two-stage form through a stream-ordered scratch of the
+  // x-integrals (nj x ox doubles); the one-pass kernel when there is none
+  double* xs = nullptr;
+  const size_t xs_bytes = (size_t)c->nj * ox * 8;
+  if (c->mono && c->nj > 0 && c->nj <= 65535 && oy <= 65535 &&
+      hipMallocAsync((void**)&xs, xs_bytes, s) != hipSuccess) {
+    (void)hipGetLastError();
+    xs = nullptr;
+  }
+  if (xs) {
+    const dim3 g1((ox + 255) / 256, c->nj), g2((ox + 255) / 256, oy);
+    if (dtype == RSIM_C64)
+      hipLaunchKernelGGL(area_v1_rows_kernel<float>, g1, dim3(256), 0, s, (const cpx<float>*)u,
+                         nx, ox, c->j0, c->nj, c->pl, dx, xs);
+    else
+      hipLaunchKernelGGL(area_v1_rows_kernel<double>, g1, dim3(256), 0, s,
+                         (const cpx<double>*)u, nx, ox, c->j0, c->nj, c->pl, dx, xs);
+    hipLaunchKernelGGL(area_v1_cols_kernel, g2, dim3(256), 0, s, xs, ox, oy, c->j0, c->pl, dy,
+                       c->gyl, z, weight, accumulate ? 1 : 0, out_dev);
+    const hipError_t e = hipGetLastError();
+    (void)hipFreeAsync(xs, s);
+    if (e != hipSuccess) return fail(RSIM_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+    return RSIM_OK;
+  }
   dim3 grid((ox + 15) / 16, (oy + 15) / 16);
   if (dtype == RSIM_C64)
     hipLaunchKernelGGL(area_v1_kernel<float>, grid, dim3(256), 0, s, (const cpx<float>*)u, nx, ny,
@@ -434,6 +558,30 @@ int detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double
   else
     hipLaunchKernelGGL(area_v1_kernel<double>, grid, dim3(256), 0, s, (const cpx<double>*)u, nx,
                        ny, ox, oy, c->pl, dx, dy, c->gyl, z, weight, accumulate ? 1 : 0, out_dev);
+  RSIM_LAUNCHED();
+  return RSIM_OK;
+}
+
+int detector_xbin_plan(int nx, int ny, double dx, double dy, int ox, int oy, double dsx,
+                       double dsy, XBinArgs* xb, bool* ok) {
+  *ok = false;
+  const DetCached* c = nullptr;
+  RSIM_CHECK(det_plan_dev(nx, ny, dx, dy, ox, oy, dsx, dsy, &c));
+  if (!c->mono || c->nj <= 0 || oy > 65535) return RSIM_OK;
+  *xb = XBinArgs{c->pl.xi0, c->pl.xi1, c->pl.xf0, c->pl.xf1, nullptr, ox, c->j0, c->nj, dx};
+  *ok = true;
+  return RSIM_OK;
+}
+
+int detector_ybin(const double* xs, int nx, int ny, double dx, double dy, int ox, int oy,
+                  double dsx, double dsy, double z, double weight, bool accumulate,
+                  double* out_dev, hipStream_t s) {
+  RSIM_CHECK(check_detector(ox, oy, z));
+  const DetCached* c = nullptr;
+  RSIM_CHECK(det_plan_dev(nx, ny, dx, dy, ox, oy, dsx, dsy, &c));
+  if (oy > 65535) return fail(RSIM_E_INVALID_ARG, "detector rows %d exceed 65535", oy);
+  hipLaunchKernelGGL(area_v1_cols_kernel, dim3((ox + 255) / 256, oy), dim3(256), 0, s, xs, ox,
+                     oy, c->j0, c->pl, dy, c->gyl, z, weight, accumulate ? 1 : 0, out_dev);
   RSIM_LAUNCHED();
   return RSIM_OK;
 }
