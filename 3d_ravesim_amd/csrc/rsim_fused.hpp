@@ -236,24 +236,53 @@ __global__ void __launch_bounds__(F::TC * C)
     if (aj.xb.xs) {   // uniform
       // |u|^2 of the block's rows through LDS (free between the two FFTs),
       // then each detector column's x-integral in the cells' order
+      // four detector columns per thread and round; their plan entries are
+      // loaded (clamped to the last column) before the LDS phase
       float* e2 = reinterpret_cast<float*>(lds);
+      const int ox = aj.xb.ox;
+      const int jj = y - aj.xb.j0;
+      const bool rowin = live && jj >= 0 && jj < aj.xb.nj;
+      int p0 = t;
+      int i0[4], i1[4];
+      double f0[4], f1[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int pc = min(p0 + k * F::TC, ox - 1);
+        i0[k] = aj.xb.xi0[pc];
+        i1[k] = aj.xb.xi1[pc];
+        f0[k] = aj.xb.xf0[pc];
+        f1[k] = aj.xb.xf1[pc];
+      }
       __syncthreads();
 #pragma unroll
       for (int e = 0; e < F::E; ++e)
         e2[c * N + last_pos<F>(t, e)] = v[e].x * v[e].x + v[e].y * v[e].y;
       __syncthreads();
-      const int jj = y - aj.xb.j0;
-      if (live && jj >= 0 && jj < aj.xb.nj) {
-        const float* r = e2 + c * N;
-        const double dx = aj.xb.dx;
-        for (int p = t; p < aj.xb.ox; p += F::TC) {
-          const int i0 = aj.xb.xi0[p], i1 = aj.xb.xi1[p];
-          double sum = 0.0;
-          for (int i = i0; i < i1; ++i) sum += (double)r[i];
-          double xi = sum * dx;
-          if (i1 < N) xi += aj.xb.xf1[p] * dx * (double)r[i1];
-          if (i0 < N) xi -= aj.xb.xf0[p] * dx * (double)r[i0];
-          aj.xb.xs[(long long)jj * aj.xb.ox + p] = xi;
+      const float* r = e2 + c * N;
+      const double dx = aj.xb.dx;
+      while (rowin && p0 < ox) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int p = p0 + k * F::TC;
+          if (p < ox) {
+            double sum = 0.0;
+            for (int i = i0[k]; i < i1[k]; ++i) sum += (double)r[i];
+            double xi = sum * dx;
+            if (i1[k] < N) xi += f1[k] * dx * (double)r[i1[k]];
+            if (i0[k] < N) xi -= f0[k] * dx * (double)r[i0[k]];
+            aj.xb.xs[(long long)jj * ox + p] = xi;
+          }
+        }
+        p0 += 4 * F::TC;
+        if (p0 < ox) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int pc = min(p0 + k * F::TC, ox - 1);
+            i0[k] = aj.xb.xi0[pc];
+            i1[k] = aj.xb.xi1[pc];
+            f0[k] = aj.xb.xf0[pc];
+            f1[k] = aj.xb.xf1[pc];
+          }
         }
       }
       __syncthreads();

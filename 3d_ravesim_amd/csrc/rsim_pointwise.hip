@@ -361,12 +361,15 @@ __global__ void __launch_bounds__(256) area_v1_cols_kernel(const double* __restr
   const double xd = pl.xdet[p], yd = pl.ydet[q];
   const double cosine = z / sqrt(xd * xd + yd * yd + z * z);
   double acc = 0.0;
+  // unconditional loads (rows [jlo, jhi) all lie in the scratch) so that
+  // the unrolled iterations overlap them
+#pragma unroll 4
   for (int j = pl.jlo[q]; j < pl.jhi[q]; ++j) {
     const double cell_lo = gyl + j * dy;
     const double cell_hi = cell_lo + dy;
     const double ov = fmin(cell_hi, yhi) - fmax(cell_lo, ylo);
-    if (!(ov > 0.0)) continue;
-    acc += xs[(long long)(j - j0) * ox + p] * ov * cosine;
+    const double x = xs[(long long)(j - j0) * ox + p];
+    if (ov > 0.0) acc += x * ov * cosine;
   }
   const long long o = (long long)q * ox + p;
   if (accumulate) out[o] += weight * acc;
