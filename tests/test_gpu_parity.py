@@ -352,6 +352,30 @@ def test_area_v1_binning_matches_oracle(dt):
     np.testing.assert_allclose(got, exp, rtol=1e-12 if dt == np.complex128 else 1e-6, atol=0)
 
 
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+@pytest.mark.parametrize("ny,nx,pix,span", [
+    (96, 512, (37.3, 5.1), (1.5, 1.3)),   # detector overhangs the field on all sides
+    (64, 8192, (20.0, 3.0), (1.0, 0.9)),  # 256 detector columns span > 4096 field cells
+])
+def test_area_v1_overhang_and_wide_spans(dt, ny, nx, pix, span):
+    """Detector pixels outside the field, partial edge pixels and x spans wider
+    than the row stage's LDS span (read straight from the field)."""
+    rng = np.random.default_rng(29)
+    f = (rng.normal(size=(ny, nx)) + 1j * rng.normal(size=(ny, nx))).astype(dt)
+    dx, dy = 0.8e-6, 1.3e-6
+    sx, sy = span[0] * nx * dx, span[1] * ny * dy
+    psx, psy = pix[0] * dx, pix[1] * dy
+    p = rs.SimParams(N=nx * ny, nx=nx, ny=ny, dx=dx, dy=dy, z_detector=0.4, detector_size=sx,
+                     detector_size_x=sx, detector_size_y=sy, detector_pixel_size_x=psx,
+                     detector_pixel_size_y=psy, wl=1e-10, chunk_size=nx)
+    exp = O.area_v1(f, dx, dy, sx, sy, psx, psy, 0.4)
+    say("area_v1 overhang/wide", ny, nx, np.dtype(dt).name)
+    got = rs.square_and_downsample_2d(dev(f), p, 0.4)
+    assert got.shape == exp.shape
+    np.testing.assert_allclose(got, exp, rtol=1e-12 if dt == np.complex128 else 1e-6,
+                               atol=1e-12 * np.abs(exp).max())
+
+
 # ----------------------------------------------------- end to end (P5) ----
 
 @pytest.mark.parametrize("mode", ["vacuum", "sample"])
