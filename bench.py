@@ -102,7 +102,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--workload", choices=["multislice", "adjoint", "large", "batch", "plasma",
-                                           "cbbpm", "history"],
+                                           "cbbpm", "history", "grating"],
                     default="multislice",
                     help="multislice: the headline 4096^2 fused run; adjoint: forward + "
                          "backward of the differentiable path at 2048^2 x 128; large: "
@@ -131,6 +131,8 @@ def main():
         return cbbpm_main(args, rs)
     if args.workload == "history":
         return history_main(args, rs)
+    if args.workload == "grating":
+        return grating_main(args, rs)
     if args.workload == "large":
         return large_main(args, rs)
     if args.workload == "plasma":
@@ -996,6 +998,75 @@ def history_main(args, rs):
         "frames": nz, "frame_shape": [oy, ox],
         "config": {"workload": f"config-2 volume ({nz} slices, seed 2), area_v1 4x4 frame at every slice"},
     }), flush=True)
+
+
+def grating_main(args, rs):
+    """The reference's own CPU case at 1024^2: point source 1 m upstream
+    (analytic field), a binary phase grating (period 4.2 um, duty 0.5,
+    delta 2.3e-6, beta 1e-9, 5 um) as one Sample slice on the integral voxel
+    map, 0.1 m of free space, area_v1 with 4 um pixels (256^2).  Whole
+    pipelines per second (source to detector image on the host) for complex128
+    on the plain engine (the case's precision) and complex64 on the fused one;
+    the FP64 CPU oracle runs the same pipeline once."""
+    n, dx, wl = 1024, 1e-6, 0.5e-10
+    z_src, thick, leg = 1.0, 5e-6, 0.1
+    cut = 0.4 / dx
+    db = np.array([0, 2.3e-6 + 1e-9j])
+    x = (np.arange(n + 2) - 1) * dx          # field column i is grid column i + 1
+    grid = np.zeros((1, n + 2, n + 2), np.uint8)
+    grid[0][:, np.mod(x / 4.2e-6 + 0.25, 1) < 0.5] = 1
+    p = rs.SimParams(N=n * n, nx=n, ny=n, dx=dx, dy=dx, z_detector=z_src + thick + leg,
+                     detector_size=n * dx, detector_pixel_size_x=4 * dx,
+                     detector_pixel_size_y=4 * dx, wl=wl, chunk_size=n)
+    vol = rs.DeviceVolume.from_numpy(grid, db)
+
+    def pipeline(dt, engine, u, U):
+        rs.propagate_analytically_2d(u, z_src, 0.0, 0.0, p)
+        s = rs.Sample(z_start=z_src, pixel_size_x=dx, pixel_size_z=thick, grid=vol,
+                      materials=[rs.Material("grating", 1.0)], x_positions=np.array([0.0]),
+                      engine=engine)
+        s.apply(u, U, p, cut, 0, None)
+        rs.propagate_2d(u, U, dx, dx, wl, leg, n, cut, n, n, skip_fft=True)
+        return rs.square_and_downsample_2d(u, p, z_src + thick + leg)
+
+    runs = {}
+    images = {}
+    for dt, engine in ((np.complex128, "plain"), (np.complex64, "fused")):
+        u, U = rs.DeviceVector(n * n, dt), rs.DeviceVector(n * n, dt)
+        images[engine] = pipeline(dt, engine, u, U)
+        reps = max(5, args.steps * 5)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            pipeline(dt, engine, u, U)
+        runs[engine] = reps / (time.perf_counter() - t0)
+        log(f"grating 1024^2 {np.dtype(dt).name} {engine}: {runs[engine]:.1f} pipelines/s")
+    result = {
+        "metric": "source-to-image pipelines/s, 1024^2 grating case (c128 plain)",
+        "value": round(runs["plain"], 2), "unit": "pipelines/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "c128", "data": "synthetic",
+        "fused_c64_pipelines_per_s": round(runs["fused"], 2),
+        "config": {"workload": "1024^2, point source 1 m, binary phase grating 4.2 um as one "
+                               "5 um slice, 0.1 m, area_v1 4x4 (256^2)"},
+    }
+    if not args.no_cpu:
+        import oracle as O
+        t0 = time.perf_counter()
+        u0 = O.propagate_analytically_2d(n, n, dx, dx, z_src, 0.0, 0.0, wl)
+        spec = O.SampleSpec(grid=grid, db_list=db, pixel_size_x=dx, pixel_size_z=thick,
+                            z_start=z_src)
+        us, Us = O.sample_apply_2d(u0, spec, dx, dx, wl, cut)
+        u1, _ = O.propagate_2d(us, dx, dx, wl, leg, cut, U2d=Us)
+        img = O.area_v1(u1, dx, dx, n * dx, n * dx, 4 * dx, 4 * dx, z_src + thick + leg)
+        t_cpu = time.perf_counter() - t0
+        result["cpu_baseline"] = {
+            "value": round(1.0 / t_cpu, 3), "unit": "pipelines/s", "cores": 1, "kind": "port",
+            "sample": "FP64 numpy/scipy oracle (scipy.fft workers=1), one whole pipeline"}
+        for engine in ("plain", "fused"):
+            result[f"rel_l2_image_{engine}_vs_oracle"] = O.relative_l2(images[engine], img)
+        log(f"cpu oracle: {1.0 / t_cpu:.2f} pipelines/s; image vs oracle: plain "
+            f"{result['rel_l2_image_plain_vs_oracle']:.3e}, fused "
+            f"{result['rel_l2_image_fused_vs_oracle']:.3e}")
+    print(json.dumps(result), flush=True)
 
 
 def plasma_main(args, rs):
