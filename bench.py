@@ -108,7 +108,10 @@ def main():
                          "backward of the differentiable path at 2048^2 x 128; large: "
                          "16384^2 through 512 slices (three-pass column schedule); batch: "
                          "64 Talbot-Lau instances of uneven sizes in captured batches; plasma: "
-                         "a 4096^2 field through a 64-slice laser-plasma volume")
+                         "a 4096^2 field through a 64-slice laser-plasma volume; cbbpm: the "
+                         "headline volume under cone-beam Fresnel scaling; history: the "
+                         "headline run with a detector frame at every slice; grating: the "
+                         "1024^2 point-source + phase-grating case as whole pipelines")
     ap.add_argument("--fd-voxels", type=int, default=64)
     ap.add_argument("--large-slices", type=int, default=512)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
