@@ -138,7 +138,10 @@ __device__ __forceinline__ RowT row_state(const TransArgsF& a, int y) {
   return r;
 }
 
-enum { ROW_FIRST = 0, ROW_MID = 1, ROW_LAST = 2, ROW_ADJ = 3, ROW_ADJ_LAST = 4 };
+// ROW_MID_OUT is ROW_MID with its optional outputs (AdjArgs fld_out, xb);
+// the launcher picks it when a launch asks for one, so the plain MID kernel
+// carries no code for them.
+enum { ROW_FIRST = 0, ROW_MID = 1, ROW_LAST = 2, ROW_ADJ = 3, ROW_ADJ_LAST = 4, ROW_MID_OUT = 5 };
 
 // Extra buffers of single-field launches.
 //   ROW_MID: fld_out (optional) receives the field IFFT_x(W) entering the
@@ -225,67 +228,69 @@ __global__ void __launch_bounds__(F::TC * C)
       if ((e & 3) == 3) compiler_fence();
     }
     reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
-  } else if constexpr (MODE == ROW_MID) {
+  } else if constexpr (MODE == ROW_MID || MODE == ROW_MID_OUT) {
     reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, t, c);  // its barriers publish the LDS tables
     compiler_fence();
-    if (aj.fld_out && live) {   // uniform: one field per launch
-      const Gio<float> gf(aj.fld_out, nbytes);
+    if constexpr (MODE == ROW_MID_OUT) {
+      if (aj.fld_out && live) {   // uniform: one field per launch
+        const Gio<float> gf(aj.fld_out, nbytes);
 #pragma unroll
-      for (int e = 0; e < F::E; ++e) gf.st(v[e], vo, last_off<F>(e, 8u));
-    }
-    if (aj.xb.xs) {   // uniform
-      // |u|^2 of the block's rows through LDS (free between the two FFTs),
-      // then each detector column's x-integral in the cells' order
-      // four detector columns per thread and round; their plan entries are
-      // loaded (clamped to the last column) before the LDS phase
-      float* e2 = reinterpret_cast<float*>(lds);
-      const int ox = aj.xb.ox;
-      const int jj = y - aj.xb.j0;
-      const bool rowin = live && jj >= 0 && jj < aj.xb.nj;
-      int p0 = t;
-      int i0[4], i1[4];
-      double f0[4], f1[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int pc = min(p0 + k * F::TC, ox - 1);
-        i0[k] = aj.xb.xi0[pc];
-        i1[k] = aj.xb.xi1[pc];
-        f0[k] = aj.xb.xf0[pc];
-        f1[k] = aj.xb.xf1[pc];
+        for (int e = 0; e < F::E; ++e) gf.st(v[e], vo, last_off<F>(e, 8u));
       }
-      __syncthreads();
-#pragma unroll
-      for (int e = 0; e < F::E; ++e)
-        e2[c * N + last_pos<F>(t, e)] = v[e].x * v[e].x + v[e].y * v[e].y;
-      __syncthreads();
-      const float* r = e2 + c * N;
-      const double dx = aj.xb.dx;
-      while (rowin && p0 < ox) {
+      if (aj.xb.xs) {   // uniform
+        // |u|^2 of the block's rows through LDS (free between the two FFTs),
+        // then each detector column's x-integral in the cells' order
+        // four detector columns per thread and round; their plan entries are
+        // loaded (clamped to the last column) before the LDS phase
+        float* e2 = reinterpret_cast<float*>(lds);
+        const int ox = aj.xb.ox;
+        const int jj = y - aj.xb.j0;
+        const bool rowin = live && jj >= 0 && jj < aj.xb.nj;
+        int p0 = t;
+        int i0[4], i1[4];
+        double f0[4], f1[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          const int p = p0 + k * F::TC;
-          if (p < ox) {
-            double sum = 0.0;
-            for (int i = i0[k]; i < i1[k]; ++i) sum += (double)r[i];
-            double xi = sum * dx;
-            if (i1[k] < N) xi += f1[k] * dx * (double)r[i1[k]];
-            if (i0[k] < N) xi -= f0[k] * dx * (double)r[i0[k]];
-            aj.xb.xs[(long long)jj * ox + p] = xi;
-          }
+          const int pc = min(p0 + k * F::TC, ox - 1);
+          i0[k] = aj.xb.xi0[pc];
+          i1[k] = aj.xb.xi1[pc];
+          f0[k] = aj.xb.xf0[pc];
+          f1[k] = aj.xb.xf1[pc];
         }
-        p0 += 4 * F::TC;
-        if (p0 < ox) {
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < F::E; ++e)
+          e2[c * N + last_pos<F>(t, e)] = v[e].x * v[e].x + v[e].y * v[e].y;
+        __syncthreads();
+        const float* r = e2 + c * N;
+        const double dx = aj.xb.dx;
+        while (rowin && p0 < ox) {
 #pragma unroll
           for (int k = 0; k < 4; ++k) {
-            const int pc = min(p0 + k * F::TC, ox - 1);
-            i0[k] = aj.xb.xi0[pc];
-            i1[k] = aj.xb.xi1[pc];
-            f0[k] = aj.xb.xf0[pc];
-            f1[k] = aj.xb.xf1[pc];
+            const int p = p0 + k * F::TC;
+            if (p < ox) {
+              double sum = 0.0;
+              for (int i = i0[k]; i < i1[k]; ++i) sum += (double)r[i];
+              double xi = sum * dx;
+              if (i1[k] < N) xi += f1[k] * dx * (double)r[i1[k]];
+              if (i0[k] < N) xi -= f0[k] * dx * (double)r[i0[k]];
+              aj.xb.xs[(long long)jj * ox + p] = xi;
+            }
+          }
+          p0 += 4 * F::TC;
+          if (p0 < ox) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              const int pc = min(p0 + k * F::TC, ox - 1);
+              i0[k] = aj.xb.xi0[pc];
+              i1[k] = aj.xb.xi1[pc];
+              f0[k] = aj.xb.xf0[pc];
+              f1[k] = aj.xb.xf1[pc];
+            }
           }
         }
+        __syncthreads();
       }
-      __syncthreads();
     }
     const RowT r = row_state<TM>(ta, yy);
 #pragma unroll
@@ -436,7 +441,9 @@ struct RowLaunch {
       case ROW_FIRST:
         if (tm == TM_NONE) return launch<ROW_FIRST, TM_NONE>(b, count, ny, nb, tw, s, aj);
         return go_tm<ROW_FIRST>(tm, b, count, ny, nb, tw, s, aj);
-      case ROW_MID: return go_tm<ROW_MID>(tm, b, count, ny, nb, tw, s, aj);
+      case ROW_MID:
+        if (aj.fld_out || aj.xb.xs) return go_tm<ROW_MID_OUT>(tm, b, count, ny, nb, tw, s, aj);
+        return go_tm<ROW_MID>(tm, b, count, ny, nb, tw, s, aj);
       // no transmission in the last pass: one instantiation serves every source
       case ROW_LAST: return launch<ROW_LAST, TM_LUT>(b, count, ny, nb, tw, s, aj);
       case ROW_ADJ:
