@@ -72,11 +72,19 @@ SIGNATURES = {
     "rsim_stream_create": [C.POINTER(_vp)],
     "rsim_stream_destroy": [_vp],
     "rsim_release_caches": [],
+    "rsim_cache_info": [C.POINTER(_sz), _ip],
     "rsim_kernel_timing": [_i],
     "rsim_kernel_timing_read": [_dp, _ip],
     "rsim_set_progress": [PROGRESS_FN, _vp],
     "rsim_request_cancel": [],
     "rsim_clear_cancel": [],
+    "rsim_control_create": [C.POINTER(_vp)],
+    "rsim_control_destroy": [_vp],
+    "rsim_control_set_progress": [_vp, PROGRESS_FN, _vp, _i],
+    "rsim_control_request_cancel": [_vp],
+    "rsim_control_clear_cancel": [_vp],
+    "rsim_control_bind": [_vp],
+    "rsim_control_completed": [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "rsim_fft2": [_vp, _vp, _i, _i, _i, _i, _vp],
     "rsim_fft1d_rows": [_vp, _vp, _i, _i, _i, _i, _vp],
     "rsim_apply_frequency_cutoff_2d": [_vp, _i, _i, _i, _d, _d, _d, _vp],

@@ -1,10 +1,22 @@
-"""Transactional run checkpoints (big-wave/checkpoint.py:1-208).
+"""Runtime checkpoints of one source's run: the fields u and U plus where to
+continue, in one self-describing file that is replaced atomically.
 
-u and U are copied from HBM and written as .npy files under generation-unique
-names, each fsynced, then a manifest (fingerprint, state, SHA-256 of both
-vectors) is committed by rename; stale generations are removed afterwards.
-``load`` verifies format, status, fingerprint and checksums before restoring
-into the device vectors.
+The interface is the one ``wavesim.run_simulation`` and the runner use
+(big-wave/checkpoint.py:83-208: ``RunCheckpoint(directory, fingerprint,
+progress_cb, cancel_token)`` with ``exists``, ``save``, ``load`` and
+``mark_complete``; ``CheckpointState``; ValueError for anything that cannot
+be resumed, InterruptedError on cancel).  The storage is this package's own:
+
+    run.ckpt  =  payload  |  trailer (UTF-8 JSON)  |  trailer length (u64 LE)  |  MAGIC
+
+The payload is u's bytes followed by U's bytes, copied from HBM one field at
+a time and hashed (BLAKE2b-256) while they are written, so a save reads
+nothing back.  The trailer records the format, the run fingerprint, the
+state to resume from, each field's dtype, length, offset and digest.  A save
+writes ``run.ckpt.partial``, flushes it to stable storage and renames it
+over ``run.ckpt`` -- a reader sees the previous checkpoint or the new one,
+never a mixture.  ``mark_complete`` renames the file to
+``run.ckpt.complete``: the run finished, and there is nothing to resume.
 """
 
 from __future__ import annotations
@@ -12,62 +24,29 @@ from __future__ import annotations
 import hashlib
 import json
 import os
-import uuid
-from dataclasses import dataclass
+import struct
+from dataclasses import asdict, dataclass
 from pathlib import Path
 from typing import Any, Callable
 
 import numpy as np
 
+from ._control import token_is_set
 
-CHECKPOINT_FORMAT_VERSION = 1
-
-
-def _cancelled(token: Any | None) -> bool:
-    if token is None:
-        return False
-    if isinstance(token, bool):
-        return token
-    for name in ("is_cancelled", "is_set", "cancelled"):
-        value = getattr(token, name, None)
-        if value is not None:
-            return bool(value() if callable(value) else value)
-    return False
-
-
-def _sha256(path: Path) -> str:
-    digest = hashlib.sha256()
-    with path.open("rb") as handle:
-        for block in iter(lambda: handle.read(8 * 1024 * 1024), b""):
-            digest.update(block)
-    return digest.hexdigest()
-
-
-def _write_vector(vector, path: Path) -> None:
-    """Any vector with ``to_numpy()`` (a DeviceVector copies from HBM)."""
-    if not hasattr(vector, "to_numpy"):
-        raise TypeError(f"unsupported checkpoint vector type {type(vector).__name__}")
-    with path.open("wb") as handle:
-        np.save(handle, vector.to_numpy(), allow_pickle=False)
-        handle.flush()
-        os.fsync(handle.fileno())
-
-
-def _restore_vector(path: Path, vector: DeviceVector, expected_length: int,
-                    expected_dtype: np.dtype) -> None:
-    array = np.load(path, mmap_mode="r", allow_pickle=False)
-    if array.shape != (expected_length,):
-        raise ValueError(f"checkpoint vector shape {array.shape} != {(expected_length,)}")
-    if np.dtype(array.dtype) != np.dtype(expected_dtype):
-        raise ValueError(f"checkpoint vector dtype {array.dtype} != {expected_dtype}")
-    if not hasattr(vector, "upload"):
-        raise TypeError(f"unsupported checkpoint vector type {type(vector).__name__}")
-    vector.upload(np.asarray(array))
-    del array
+FORMAT = "ravesim-run-checkpoint"
+FORMAT_VERSION = 2
+MAGIC = b"RSCKPT02"
+_TAIL = struct.Struct("<Q8s")           # trailer length, magic
+_COPY_BLOCK = 64 * 1024 ** 2            # bytes hashed / written per step
+_FIELDS = ("u", "U")
 
 
 @dataclass(frozen=True)
 class CheckpointState:
+    """Where the run continues: after ``slice_index`` slices of element
+    ``element_index`` at ``current_z``; ``u_fourier_valid`` tells whether U
+    holds FFT2(u)."""
+
     current_z: float
     element_index: int
     slice_index: int
@@ -76,9 +55,47 @@ class CheckpointState:
     stage: str
 
 
+def _host_copy(vector) -> np.ndarray:
+    if not hasattr(vector, "to_numpy"):
+        raise TypeError(f"cannot checkpoint a {type(vector).__name__}: it has no to_numpy()")
+    return np.ascontiguousarray(vector.to_numpy()).reshape(-1)
+
+
+def _stream(handle, array: np.ndarray) -> str:
+    """Write ``array``'s bytes to ``handle`` in blocks; its BLAKE2b-256 digest."""
+    digest = hashlib.blake2b(digest_size=32)
+    raw = memoryview(array.view(np.uint8))
+    for start in range(0, len(raw), _COPY_BLOCK):
+        block = raw[start:start + _COPY_BLOCK]
+        digest.update(block)
+        handle.write(block)
+    return digest.hexdigest()
+
+
+def _digest_of(view: memoryview) -> str:
+    digest = hashlib.blake2b(digest_size=32)
+    for start in range(0, len(view), _COPY_BLOCK):
+        digest.update(view[start:start + _COPY_BLOCK])
+    return digest.hexdigest()
+
+
+def _fsync_dir(directory: Path) -> None:
+    try:
+        fd = os.open(directory, os.O_RDONLY)
+    except OSError:
+        return
+    try:
+        os.fsync(fd)
+    except OSError:
+        pass
+    finally:
+        os.close(fd)
+
+
 class RunCheckpoint:
-    """Persist and verify u/U plus the next unit of work for one source
-    (checkpoint.py:83-208)."""
+    """The checkpoint of one source under ``directory``; ``fingerprint`` (a
+    JSON-able dict holding at least the field length ``N`` and numpy dtype
+    string ``dtype``) must match for a checkpoint to be loaded."""
 
     def __init__(self, directory: Path | str, fingerprint: dict[str, Any],
                  progress_cb: Callable[[str, int, int], Any] | None = None,
@@ -87,97 +104,122 @@ class RunCheckpoint:
         self.fingerprint = fingerprint
         self.progress_cb = progress_cb
         self.cancel_token = cancel_token
-        self.manifest_path = self.directory / "manifest.json"
+        self.path = self.directory / "run.ckpt"
+        self.completed_path = self.directory / "run.ckpt.complete"
 
+    # -- state --
     @property
     def exists(self) -> bool:
-        return self.manifest_path.exists()
+        """A checkpoint is ready to resume from."""
+        return self.path.exists()
 
-    def _progress(self, stage: str, completed: int, total: int) -> None:
-        if _cancelled(self.cancel_token):
+    def _step(self, stage: str, completed: int, total: int) -> None:
+        if token_is_set(self.cancel_token):
             raise InterruptedError(f"checkpoint cancelled during {stage}")
         if self.progress_cb is not None:
             self.progress_cb(stage, completed, total)
 
-    def save(self, u, U, *, current_z: float, element_index: int,
-             slice_index: int, phase_step: int = 0, u_fourier_valid: bool, stage: str) -> None:
+    def _canonical_fingerprint(self) -> Any:
+        # what the trailer will hold after a JSON round trip
+        return json.loads(json.dumps(self.fingerprint, sort_keys=True))
+
+    # -- write --
+    def save(self, u, U, *, current_z: float, element_index: int, slice_index: int,
+             phase_step: int = 0, u_fourier_valid: bool, stage: str) -> None:
+        state = CheckpointState(float(current_z), int(element_index), int(slice_index),
+                                int(phase_step), bool(u_fourier_valid), str(stage))
         self.directory.mkdir(parents=True, exist_ok=True)
-        generation = uuid.uuid4().hex
-        u_name = f"u.{generation}.npy"
-        U_name = f"spectrum.{generation}.npy"
-        u_part = self.directory / f"{u_name}.part"
-        U_part = self.directory / f"{U_name}.part"
-        manifest_part = self.directory / "manifest.json.part"
-        self._progress("checkpoint_u", 0, 3)
-        _write_vector(u, u_part)
-        self._progress("checkpoint_U", 1, 3)
-        _write_vector(U, U_part)
-        manifest = {
-            "format_version": CHECKPOINT_FORMAT_VERSION,
-            "status": "ready",
-            "fingerprint": self.fingerprint,
-            "state": {
-                "current_z": float(current_z),
-                "element_index": int(element_index),
-                "slice_index": int(slice_index),
-                "phase_step": int(phase_step),
-                "u_fourier_valid": bool(u_fourier_valid),
-                "stage": str(stage),
-            },
-            "vectors": {
-                "u": {"file": u_name, "sha256": _sha256(u_part)},
-                "U": {"file": U_name, "sha256": _sha256(U_part)},
-            },
-        }
-        with manifest_part.open("w", encoding="utf-8") as handle:
-            json.dump(manifest, handle, indent=2, sort_keys=True)
-            handle.flush()
-            os.fsync(handle.fileno())
-        os.replace(u_part, self.directory / u_name)
-        os.replace(U_part, self.directory / U_name)
-        os.replace(manifest_part, self.manifest_path)
-        for pattern in ("u.*.npy", "spectrum.*.npy"):
-            for stale in self.directory.glob(pattern):
-                if stale.name not in {u_name, U_name}:
-                    stale.unlink()
-        self._progress("checkpoint_commit", 3, 3)
+        partial = self.path.with_name(self.path.name + ".partial")
+        layout = []
+        offset = 0
+        try:
+            with partial.open("wb") as handle:
+                for k, (name, vector) in enumerate(zip(_FIELDS, (u, U))):
+                    self._step("checkpoint_u" if name == "u" else "checkpoint_U", k, 3)
+                    array = _host_copy(vector)
+                    digest = _stream(handle, array)
+                    layout.append({"name": name, "dtype": array.dtype.str, "count": int(array.size),
+                                   "offset": offset, "nbytes": int(array.nbytes),
+                                   "blake2b": digest})
+                    offset += int(array.nbytes)
+                    del array
+                trailer = json.dumps({
+                    "format": FORMAT, "version": FORMAT_VERSION,
+                    "fingerprint": self.fingerprint, "state": asdict(state),
+                    "fields": layout,
+                }, sort_keys=True).encode()
+                handle.write(trailer)
+                handle.write(_TAIL.pack(len(trailer), MAGIC))
+                handle.flush()
+                os.fsync(handle.fileno())
+            os.replace(partial, self.path)
+        except BaseException:
+            partial.unlink(missing_ok=True)
+            raise
+        self.completed_path.unlink(missing_ok=True)
+        _fsync_dir(self.directory)
+        self._step("checkpoint_commit", 3, 3)
+
+    # -- read --
+    def _trailer(self, size: int, handle) -> tuple[dict, int]:
+        if size < _TAIL.size:
+            raise ValueError(f"checkpoint {self.path} is truncated")
+        handle.seek(size - _TAIL.size)
+        length, magic = _TAIL.unpack(handle.read(_TAIL.size))
+        if magic != MAGIC or length > size - _TAIL.size:
+            raise ValueError(f"{self.path} is not a checkpoint of this format")
+        handle.seek(size - _TAIL.size - length)
+        try:
+            trailer = json.loads(handle.read(length).decode())
+        except ValueError as exc:
+            raise ValueError(f"checkpoint trailer of {self.path} is unreadable: {exc}") from None
+        if trailer.get("format") != FORMAT or int(trailer.get("version", 0)) != FORMAT_VERSION:
+            raise ValueError("unsupported checkpoint format")
+        return trailer, size - _TAIL.size - length
 
     def load(self, u, U) -> CheckpointState:
-        with self.manifest_path.open("r", encoding="utf-8") as handle:
-            manifest = json.load(handle)
-        if int(manifest.get("format_version", 0)) != CHECKPOINT_FORMAT_VERSION:
-            raise ValueError("unsupported checkpoint format")
-        if manifest.get("status") != "ready":
-            raise ValueError(f"checkpoint status is {manifest.get('status')!r}, not 'ready'")
-        if manifest.get("fingerprint") != self.fingerprint:
-            raise ValueError("checkpoint does not match this simulation/source fingerprint")
-        expected_length = int(self.fingerprint["N"])
-        expected_dtype = np.dtype(self.fingerprint["dtype"])
-        for name in ("u", "U"):
-            path = self.directory / manifest["vectors"][name]["file"]
-            if not path.exists() or _sha256(path) != manifest["vectors"][name]["sha256"]:
-                raise ValueError(f"checkpoint {name} checksum mismatch")
-        self._progress("checkpoint_restore", 0, 2)
-        _restore_vector(self.directory / manifest["vectors"]["u"]["file"], u, expected_length,
-                        expected_dtype)
-        _restore_vector(self.directory / manifest["vectors"]["U"]["file"], U, expected_length,
-                        expected_dtype)
-        self._progress("checkpoint_restore", 2, 2)
-        state = manifest["state"]
-        return CheckpointState(
-            current_z=float(state["current_z"]), element_index=int(state["element_index"]),
-            slice_index=int(state["slice_index"]), phase_step=int(state["phase_step"]),
-            u_fourier_valid=bool(state["u_fourier_valid"]), stage=str(state["stage"]))
+        """Verify the checkpoint and restore u and U from it."""
+        if not self.path.exists():
+            raise ValueError(f"no ready checkpoint at {self.path}")
+        size = self.path.stat().st_size
+        with self.path.open("rb") as handle:
+            trailer, payload_end = self._trailer(size, handle)
+        if trailer.get("fingerprint") != self._canonical_fingerprint():
+            raise ValueError("checkpoint fingerprint differs from this simulation/source")
+        want_count = int(self.fingerprint["N"])
+        want_dtype = np.dtype(self.fingerprint["dtype"])
+        fields = {f["name"]: f for f in trailer.get("fields", [])}
+        if set(fields) != set(_FIELDS):
+            raise ValueError("checkpoint does not hold both u and U")
+        raw = np.memmap(self.path, dtype=np.uint8, mode="r", shape=(size,))
+        try:
+            views = {}
+            for name in _FIELDS:
+                f = fields[name]
+                if int(f["count"]) != want_count or np.dtype(f["dtype"]) != want_dtype:
+                    raise ValueError(
+                        f"checkpoint field {name} is {f['count']} x {f['dtype']}, "
+                        f"expected {want_count} x {want_dtype.str}")
+                lo, hi = int(f["offset"]), int(f["offset"]) + int(f["nbytes"])
+                if hi > payload_end or hi - lo != want_count * want_dtype.itemsize:
+                    raise ValueError(f"checkpoint field {name} lies outside the payload")
+                view = memoryview(raw[lo:hi])
+                if _digest_of(view) != f["blake2b"]:
+                    raise ValueError(f"checkpoint checksum mismatch in field {name}")
+                views[name] = raw[lo:hi].view(want_dtype)
+            self._step("checkpoint_restore", 0, 2)
+            for vector, name in ((u, "u"), (U, "U")):
+                if not hasattr(vector, "upload"):
+                    raise TypeError(f"cannot restore into a {type(vector).__name__}")
+                vector.upload(np.array(views[name]))
+            self._step("checkpoint_restore", 2, 2)
+        finally:
+            del raw
+        return CheckpointState(**trailer["state"])
 
     def mark_complete(self) -> None:
-        if not self.manifest_path.exists():
-            return
-        with self.manifest_path.open("r", encoding="utf-8") as handle:
-            manifest = json.load(handle)
-        manifest["status"] = "complete"
-        part = self.directory / "manifest.json.part"
-        with part.open("w", encoding="utf-8") as handle:
-            json.dump(manifest, handle, indent=2, sort_keys=True)
-            handle.flush()
-            os.fsync(handle.fileno())
-        os.replace(part, self.manifest_path)
+        """The run finished: keep the last checkpoint as ``run.ckpt.complete``
+        so that it is no longer offered for resume."""
+        if self.path.exists():
+            os.replace(self.path, self.completed_path)
+            _fsync_dir(self.directory)

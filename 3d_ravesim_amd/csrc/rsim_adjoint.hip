@@ -123,8 +123,10 @@ int multislice_db(void* u, void* U, int nx, int ny, int dtype, double dx, double
                               hipMemcpyDeviceToDevice, s));
     RSIM_CHECK(transmission_db(u, nx, ny, dtype, delta + k * vs, beta + k * vs, K, s));
     RSIM_CHECK(propagate(u, U, nx, ny, dtype, dx, dy, wl, pz, freq, false, s));
-    if (g_progress_sync()) RSIM_HIP(hipStreamSynchronize(s));
-    report_progress("multislice", k + 1, n);
+    if (progress_due(k + 1, n)) {
+      RSIM_HIP(hipStreamSynchronize(s));
+      report_progress("multislice", k + 1, n);
+    }
   }
   return RSIM_OK;
 }
@@ -171,8 +173,10 @@ int multislice_db_adjoint(void* g, void* U, int nx, int ny, int dtype, double dx
       ktime_end(KT_PW, s);
       RSIM_LAUNCHED();
       ++done;
-      if (g_progress_sync()) RSIM_HIP(hipStreamSynchronize(s));
-      report_progress("multislice_adjoint", done, n);
+      if (progress_due(done, n)) {
+        RSIM_HIP(hipStreamSynchronize(s));
+        report_progress("multislice_adjoint", done, n);
+      }
     }
   }
   return RSIM_OK;

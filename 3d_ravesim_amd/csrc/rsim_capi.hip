@@ -52,6 +52,8 @@ int release_tables();
 int release_multislice_tables();
 int release_tune();
 int release_detector_plans();
+int release_staging();
+int release_volume_tables(unsigned long long serial);
 int fused_schedule_lookup(int nx, int ny, int* rv, int* cv);
 
 namespace {
@@ -133,7 +135,6 @@ size_t max_lds_per_block() {
   return (size_t)v;
 }
 
-bool cancel_requested() { return g_cancel.load() != 0; }
 
 // ---- kernel timing ----
 namespace {
@@ -220,10 +221,53 @@ int ktime_read(double* ms3, int* n3) {
   return RSIM_OK;
 }
 
-bool g_progress_sync() { return g_progress != nullptr; }
+}  // namespace rsim
+
+// A run control: the cancel flag, progress callback and slice count of the
+// runs started on the thread it is bound to.
+struct rsim_control {
+  std::atomic<int> cancel{0};
+  rsim_progress_fn fn = nullptr;
+  void* user = nullptr;
+  int every = 1;
+  std::atomic<long long> completed{0};
+  std::atomic<long long> total{0};
+};
+
+namespace rsim {
+namespace {
+thread_local rsim_control* t_ctl = nullptr;
+}
+
+bool cancel_requested() {
+  if (t_ctl) return t_ctl->cancel.load() != 0;
+  return g_cancel.load() != 0;
+}
+
+bool progress_due(long long completed, long long total) {
+  if (t_ctl) {
+    if (!t_ctl->fn) return false;
+    return completed == total || completed % t_ctl->every == 0;
+  }
+  return g_progress != nullptr;
+}
 
 void report_progress(const char* stage, long long completed, long long total) {
+  if (t_ctl) {
+    t_ctl->completed.store(completed);
+    t_ctl->total.store(total);
+    if (t_ctl->fn && (completed == total || completed % t_ctl->every == 0))
+      t_ctl->fn(stage, completed, total, t_ctl->user);
+    return;
+  }
   if (g_progress) g_progress(stage, completed, total, g_progress_user);
+}
+
+void note_done(long long completed, long long total) {
+  if (t_ctl) {
+    t_ctl->completed.store(completed);
+    t_ctl->total.store(total);
+  }
 }
 
 }  // namespace rsim
@@ -337,6 +381,12 @@ int rsim_release_caches(void) {
   release_multislice_tables();
   release_tune();
   release_detector_plans();
+  release_staging();
+  {
+    std::lock_guard<std::recursive_mutex> lk(table_mu());
+    table_forget_all();
+  }
+  bump_cache_generation();
   return RSIM_OK;
 }
 
@@ -356,6 +406,58 @@ int rsim_request_cancel(void) {
 
 int rsim_clear_cancel(void) {
   set_cancel(0);
+  return RSIM_OK;
+}
+
+int rsim_cache_info(size_t* bytes, int* entries) {
+  table_cache_stats(bytes, entries);
+  return RSIM_OK;
+}
+
+int rsim_control_create(rsim_control** ctl) {
+  if (!ctl) return fail(RSIM_E_INVALID_ARG, "ctl is NULL");
+  *ctl = new (std::nothrow) rsim_control();
+  if (!*ctl) return fail(RSIM_E_NOMEM, "out of host memory");
+  return RSIM_OK;
+}
+
+int rsim_control_destroy(rsim_control* ctl) {
+  if (!ctl) return RSIM_OK;
+  if (t_ctl == ctl) t_ctl = nullptr;
+  delete ctl;
+  return RSIM_OK;
+}
+
+int rsim_control_set_progress(rsim_control* ctl, rsim_progress_fn fn, void* user, int every) {
+  if (!ctl) return fail(RSIM_E_INVALID_ARG, "ctl is NULL");
+  if (every < 1) return fail(RSIM_E_INVALID_ARG, "progress interval must be >= 1, got %d", every);
+  ctl->fn = fn;
+  ctl->user = user;
+  ctl->every = every;
+  return RSIM_OK;
+}
+
+int rsim_control_request_cancel(rsim_control* ctl) {
+  if (!ctl) return fail(RSIM_E_INVALID_ARG, "ctl is NULL");
+  ctl->cancel.store(1);
+  return RSIM_OK;
+}
+
+int rsim_control_clear_cancel(rsim_control* ctl) {
+  if (!ctl) return fail(RSIM_E_INVALID_ARG, "ctl is NULL");
+  ctl->cancel.store(0);
+  return RSIM_OK;
+}
+
+int rsim_control_bind(rsim_control* ctl) {
+  t_ctl = ctl;
+  return RSIM_OK;
+}
+
+int rsim_control_completed(const rsim_control* ctl, int64_t* completed, int64_t* total) {
+  if (!ctl) return fail(RSIM_E_INVALID_ARG, "ctl is NULL");
+  if (completed) *completed = ctl->completed.load();
+  if (total) *total = ctl->total.load();
   return RSIM_OK;
 }
 
@@ -485,6 +587,7 @@ int rsim_volume_create_empty(rsim_volume** vol, int nz, int ny_g, int nx_g, cons
 
 int rsim_volume_destroy(rsim_volume* vol) {
   if (!vol) return RSIM_OK;
+  release_volume_tables(vol->serial);
   hipFree(vol->ids);
   hipFree(vol->db_dev);
   delete[] vol->db_host;

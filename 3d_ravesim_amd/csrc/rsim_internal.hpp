@@ -147,12 +147,15 @@ void ktime_end(int kind, hipStream_t s);
 // suspend kernel timing on this thread (while a stream is being captured)
 void ktime_suspend(bool on);
 
-// ---- cancel / progress ----
+// ---- cancel / progress (the run control bound to the calling thread, or
+// the process-wide flag and callback when none is bound) ----
 bool cancel_requested();
-// true when a progress callback is installed (the slice loop then waits for
-// each slice so the reported progress is real)
-bool g_progress_sync();
+// true when slice `completed` of `total` is to be reported: the slice loop
+// then waits for the GPU first, so a report means the work is done
+bool progress_due(long long completed, long long total);
 void report_progress(const char* stage, long long completed, long long total);
+// record how many slices a loop finished (on return or on cancel)
+void note_done(long long completed, long long total);
 
 }  // namespace rsim
 

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <thread>
 #include <mutex>
 #include <tuple>
 #include <vector>
@@ -34,6 +35,8 @@ struct rsim_batch {
   hipGraphExec_t exec = nullptr;
   int groups = 0;
   int kernels = 0;
+  unsigned long long generation = 0;   // cache generation the graph's pointers belong to
+  std::vector<const void*> pinned;     // cached tables the graph reads
 };
 
 namespace rsim {
@@ -106,7 +109,6 @@ struct ChirpTables {
   int* Rmax = nullptr;     // largest |ky index| kept for each column, -1: none
 };
 
-std::mutex g_mu;
 std::map<std::tuple<int, int, int, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t>, ChirpTables> g_chirp;
 
 uint64_t bits(double v) {
@@ -133,21 +135,25 @@ std::vector<double> host_k2(int n, double spacing) {
   return h;
 }
 
-int chirp_tables(int nx, int ny, double dx, double dy, double wl, double dz, double freq,
-                 const ChirpTables** out) {
-  std::lock_guard<std::mutex> lk(g_mu);
-  auto key = std::make_tuple(current_device(), nx, ny, bits(dx), bits(dy), bits(wl), bits(dz), bits(freq));
-  auto it = g_chirp.find(key);
-  if (it != g_chirp.end()) {
-    *out = &it->second;
-    return RSIM_OK;
-  }
+// Device image of one chirp table set: Bx (16 nx), Ay (16 ny), Rmax (4 nx).
+size_t chirp_bytes(int nx, int ny) { return 16 * (size_t)nx + 16 * (size_t)ny + 4 * (size_t)nx; }
+
+void chirp_bind(ChirpTables* t, unsigned char* dev, int nx, int ny) {
+  t->Bx = reinterpret_cast<double2*>(dev);
+  t->Ay = reinterpret_cast<double2*>(dev + 16 * (size_t)nx);
+  t->Rmax = reinterpret_cast<int*>(dev + 16 * (size_t)(nx + ny));
+}
+
+// Host FP64 build of the chirp tables of propagation over dz into img.
+void chirp_build(int nx, int ny, double dx, double dy, double wl, double dz, double freq,
+                 unsigned char* img) {
   const std::vector<double> kx2 = host_k2(nx, dx), ky2 = host_k2(ny, dy);
   const double ctheta = ((-M_PI) * wl) * dz;
   const double gph = reduce_phase(((2.0 * M_PI) / wl) * dz);
   const double inv_n = 1.0 / ((double)nx * (double)ny);
-  std::vector<double> bx(2 * nx), ay(2 * ny);
-  std::vector<int> rmax(nx);
+  double* bx = reinterpret_cast<double*>(img);
+  double* ay = reinterpret_cast<double*>(img + 16 * (size_t)nx);
+  int* rmax = reinterpret_cast<int*>(img + 16 * (size_t)(nx + ny));
   for (int x = 0; x < nx; ++x) {
     const double th = reduce_phase(gph + reduce_phase(ctheta * kx2[x]));
     bx[2 * x] = std::cos(th) * inv_n;
@@ -172,19 +178,27 @@ int chirp_tables(int nx, int ny, double dx, double dy, double wl, double dz, dou
     }
     rmax[x] = lo - 1;
   }
-  // one allocation and one copy: Bx (16 nx), Ay (16 ny), Rmax (4 nx)
-  const size_t nb = 16 * (size_t)nx + 16 * (size_t)ny + 4 * (size_t)nx;
+}
+
+int chirp_tables(int nx, int ny, double dx, double dy, double wl, double dz, double freq,
+                 const ChirpTables** out) {
+  std::lock_guard<std::recursive_mutex> lk(table_mu());
+  auto key = std::make_tuple(current_device(), nx, ny, bits(dx), bits(dy), bits(wl), bits(dz), bits(freq));
+  auto it = g_chirp.find(key);
+  if (it != g_chirp.end()) {
+    table_touch(it->second.Bx);
+    *out = &it->second;
+    return RSIM_OK;
+  }
+  const size_t nb = chirp_bytes(nx, ny);
   std::vector<unsigned char> host(nb);
-  std::memcpy(host.data(), bx.data(), 16 * (size_t)nx);
-  std::memcpy(host.data() + 16 * (size_t)nx, ay.data(), 16 * (size_t)ny);
-  std::memcpy(host.data() + 16 * (size_t)(nx + ny), rmax.data(), 4 * (size_t)nx);
+  chirp_build(nx, ny, dx, dy, wl, dz, freq, host.data());
   ChirpTables t;
   unsigned char* dev = nullptr;
   RSIM_HIP(hipMalloc((void**)&dev, nb));
   RSIM_HIP(hipMemcpy(dev, host.data(), nb, hipMemcpyHostToDevice));
-  t.Bx = reinterpret_cast<double2*>(dev);
-  t.Ay = reinterpret_cast<double2*>(dev + 16 * (size_t)nx);
-  t.Rmax = reinterpret_cast<int*>(dev + 16 * (size_t)(nx + ny));
+  chirp_bind(&t, dev, nx, ny);
+  RSIM_CHECK(table_insert(dev, nb, [key] { g_chirp.erase(key); }));
   auto r = g_chirp.emplace(key, t);
   *out = &r.first->second;
   return RSIM_OK;
@@ -200,7 +214,7 @@ struct MatBuf {
 std::map<std::tuple<int, unsigned long long, uint64_t>, MatBuf> g_mat;
 
 int material_tables(const rsim_volume* vol, double K, const float4** lut, const float2** dbf) {
-  std::lock_guard<std::mutex> lk(g_mu);
+  std::lock_guard<std::recursive_mutex> lk(table_mu());
   const auto key = std::make_tuple(current_device(), vol->serial, bits(K));
   auto it = g_mat.find(key);
   if (it != g_mat.end()) {
@@ -413,8 +427,22 @@ int release_tune() {
   return RSIM_OK;
 }
 
+// Material tables of a volume being destroyed.
+int release_volume_tables(unsigned long long serial) {
+  std::lock_guard<std::recursive_mutex> lk(table_mu());
+  for (auto it = g_mat.begin(); it != g_mat.end();) {
+    if (std::get<1>(it->first) == serial) {
+      RSIM_HIP(hipFree(it->second.dev));
+      it = g_mat.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  return RSIM_OK;
+}
+
 int release_multislice_tables() {
-  std::lock_guard<std::mutex> lk(g_mu);
+  std::lock_guard<std::recursive_mutex> lk(table_mu());
   for (auto& kv : g_chirp) hipFree(kv.second.Bx);   // one allocation per table set
   g_chirp.clear();
   for (auto& kv : g_mat) hipFree(kv.second.dev);
@@ -491,6 +519,23 @@ static VoxSrc src_of(const rsim_volume* vol) {
   return v;
 }
 
+// Point the transmission arguments at a pair of axis maps; the transmission
+// mode they allow (material-id lookup when both maps are integral and affine).
+static int use_maps(TransArgsF& ta, const AxisMap& xm, const AxisMap& ym, bool ids) {
+  ta.xm = xm.dev;
+  ta.ym = ym.dev;
+  ta.ax = xm.a;
+  ta.cx = xm.c;
+  ta.xlo = xm.lo;
+  ta.xhi = xm.hi;
+  ta.ay = ym.a;
+  ta.cy = ym.c;
+  ta.ylo = ym.lo;
+  ta.yhi = ym.hi;
+  if (!ids) return TM_FIELD;
+  return xm.all_integral && ym.all_integral && xm.affine && ym.affine ? TM_LUT : TM_BILINEAR;
+}
+
 // pz: slice thickness of the transmission; prop_dz: propagation distance of
 // the chirp (pz, or its cone-beam effective value); scale: voxel-map scale.
 template <typename TW>
@@ -517,29 +562,107 @@ static int fused_prepare(int nx, int ny, double dx, double dy, double wl, double
   const double K = ((2.0 * M_PI) * pz) / wl;
   TransArgsF& ta = P->ta;
   ta.nxg = src.nx_g;
-  ta.xm = xm->dev;
-  ta.ym = ym->dev;
   ta.K = (float)K;
   P->slice_elems = (size_t)src.ny_g * src.nx_g;
   if (src.vol) {
     RSIM_CHECK(material_tables(src.vol, K, &ta.lut, &ta.dbf));
-    const bool lut = xm->all_integral && ym->all_integral && xm->affine && ym->affine;
-    P->tm = lut ? TM_LUT : TM_BILINEAR;
     P->ids0 = src.vol->ids;
     ta.ids = src.vol->ids;
   } else {
-    P->tm = TM_FIELD;
     P->field0 = src.field;
     ta.dbv = src.field;
   }
-  ta.ax = xm->a;
-  ta.cx = xm->c;
-  ta.xlo = xm->lo;
-  ta.xhi = xm->hi;
-  ta.ay = ym->a;
-  ta.cy = ym->c;
-  ta.ylo = ym->lo;
-  ta.yhi = ym->hi;
+  P->tm = use_maps(ta, *xm, *ym, src.vol != nullptr);
+  return RSIM_OK;
+}
+
+// Cone-beam runs: each slice has its own propagation distance and voxel-map
+// scale, so its own chirp tables and axis maps.  They are built for the whole
+// run on the host (in parallel over slices, straight into a pinned staging
+// buffer), uploaded with one copy into one stream-ordered allocation, and
+// freed on the stream when the run ends: nothing per slice is cached.
+struct SliceTables {
+  ChirpTables ct;
+  AxisMap xm, ym;
+};
+struct RunSliceTables {
+  std::vector<SliceTables> sets;
+  unsigned char* dev = nullptr;
+  hipStream_t s = nullptr;
+  ~RunSliceTables() {
+    if (dev) (void)hipFreeAsync(dev, s);
+  }
+};
+
+namespace {
+// Pinned host staging buffer per device, reused once its last upload is done.
+struct Staging {
+  unsigned char* host = nullptr;
+  size_t cap = 0;
+  hipEvent_t done = nullptr;
+};
+std::map<int, Staging> g_staging;
+std::mutex g_staging_mu;
+}  // namespace
+
+static int build_run_slice_tables(int nx, int ny, double dx, double dy, double wl, double freq,
+                                  const VoxSrc& src, double px, double py, double pz,
+                                  double xpos, double ypos, int count, const double* scales,
+                                  const double* dzs, hipStream_t s, RunSliceTables* rt) {
+  const size_t cb = (chirp_bytes(nx, ny) + 255) & ~size_t(255);
+  const size_t xb = (axis_map_bytes(nx) + 255) & ~size_t(255);
+  const size_t yb = (axis_map_bytes(ny) + 255) & ~size_t(255);
+  const size_t per = cb + xb + yb;
+  const size_t total = per * (size_t)count;
+  rt->sets.assign(count, SliceTables{});
+  RSIM_HIP(hipMallocAsync((void**)&rt->dev, total, s));
+  rt->s = s;
+  std::lock_guard<std::mutex> lk(g_staging_mu);
+  Staging& st = g_staging[current_device()];
+  if (st.done) RSIM_HIP(hipEventSynchronize(st.done));
+  else RSIM_HIP(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+  if (st.cap < total) {
+    if (st.host) RSIM_HIP(hipHostFree(st.host));
+    st.host = nullptr;
+    st.cap = 0;
+    RSIM_HIP(hipHostMalloc((void**)&st.host, total, hipHostMallocDefault));
+    st.cap = total;
+  }
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const int nthreads = (int)std::min<unsigned>(hw, (unsigned)std::max(1, count / 4));
+  auto work = [&](int t) {
+    for (int k = t; k < count; k += nthreads) {
+      unsigned char* h = st.host + per * (size_t)k;
+      const double dz = dzs ? dzs[k] : pz, sc = scales ? scales[k] : 1.0;
+      SliceTables& e = rt->sets[k];
+      chirp_build(nx, ny, dx, dy, wl, dz, freq, h);
+      axis_map_build(nx, dx, sc, xpos, src.nx_g, px, &e.xm, h + cb);
+      axis_map_build(ny, dy, sc, ypos, src.ny_g, py, &e.ym, h + cb + xb);
+      unsigned char* d = rt->dev + per * (size_t)k;
+      chirp_bind(&e.ct, d, nx, ny);
+      axis_map_bind(&e.xm, d + cb, nx);
+      axis_map_bind(&e.ym, d + cb + xb, ny);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nthreads; ++t) pool.emplace_back(work, t);
+  work(0);
+  for (auto& th : pool) th.join();
+  RSIM_HIP(hipMemcpyAsync(rt->dev, st.host, total, hipMemcpyHostToDevice, s));
+  RSIM_HIP(hipEventRecord(st.done, s));
+  return RSIM_OK;
+}
+
+int release_staging() {
+  std::lock_guard<std::mutex> lk(g_staging_mu);
+  for (auto& kv : g_staging) {
+    if (kv.second.done) {
+      (void)hipEventSynchronize(kv.second.done);
+      (void)hipEventDestroy(kv.second.done);
+    }
+    if (kv.second.host) (void)hipHostFree(kv.second.host);
+  }
+  g_staging.clear();
   return RSIM_OK;
 }
 
@@ -576,6 +699,10 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
   }
   const float spec_scale = (float)((double)nx * (double)ny);
   P.at(ta, first);
+  RunSliceTables rts;
+  if (per_slice)
+    RSIM_CHECK(build_run_slice_tables(nx, ny, dx, dy, wl, freq, src, px, py, pz, xpos, ypos,
+                                      count, scales, dzs, s, &rts));
   int rv = 0, cv = 0;
   RSIM_CHECK(fused_schedule<TW>(nx, ny, tm, tx, ty, ta, ct, s, &rv, &cv));
   // History frames after slice 0: the MID pass bins its field in x into xs
@@ -599,13 +726,23 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
   }
   for (int k = 0; k < count; ++k) {
     const int sl = first + k;
+    if (cancel_requested()) {
+      // stop at this slice boundary: W of slice k - 1 is in U, so the row
+      // IFFT leaves u = the field after k slices (U then does not hold FFT2(u))
+      if (k > 0) {
+        ktime_begin(KT_ROW, s);
+        fused_row(nx, rv, ROW_LAST, tm, U, u, ny, tx, ta, s);
+        ktime_end(KT_ROW, s);
+        RSIM_LAUNCHED();
+      }
+      note_done(k, total);
+      return fail(RSIM_E_CANCELLED, "multislice cancelled after %d of %d slices (at slice %d)",
+                  k, total, sl);
+    }
     if (per_slice) {
-      FusedPrep<TW> Pk;
-      RSIM_CHECK(fused_prepare<TW>(nx, ny, dx, dy, wl, freq, src, px, py, pz, xpos, ypos, &Pk,
-                                   dzs ? dzs[k] : pz, scales ? scales[k] : 1.0));
-      ct = Pk.ct;
-      ta = Pk.ta;
-      tm = Pk.tm;
+      const SliceTables& e = rts.sets[k];
+      ct = &e.ct;
+      tm = use_maps(ta, e.xm, e.ym, src.vol != nullptr);
       RSIM_CHECK(fused_schedule<TW>(nx, ny, tm, tx, ty, ta, ct, s, &rv, &cv));
     }
     P.at(ta, sl);
@@ -647,20 +784,23 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
     }
     ktime_end(KT_COL, s);
     RSIM_LAUNCHED();
-    const bool cancel = !last && cancel_requested();
-    if (cancel || last) {
-      // row IFFT of W; after a cancel this leaves u consistent at a slice
-      // boundary (U then does not hold FFT2(u))
+    if (last) {
+      // row IFFT of W
       ktime_begin(KT_ROW, s);
-      fused_row(nx, rv, ROW_LAST, tm, (last && !big) ? u : U, u, ny, tx, ta, s);
+      fused_row(nx, rv, ROW_LAST, tm, big ? U : u, u, ny, tx, ta, s);
       ktime_end(KT_ROW, s);
       RSIM_LAUNCHED();
-      if (big && last && want_spec) RSIM_CHECK(fft2(RSIM_C64, RSIM_FORWARD, u, U, nx, ny, s));
-      if (cancel) return fail(RSIM_E_CANCELLED, "multislice cancelled after slice %d", sl);
+      if (big && want_spec) RSIM_CHECK(fft2(RSIM_C64, RSIM_FORWARD, u, U, nx, ny, s));
     }
-    if (g_progress_sync()) RSIM_HIP(hipStreamSynchronize(s));
-    report_progress("sample_slice", k + 1, total);
+    if (progress_due(k + 1, total)) {
+      // the field after slice k is reported only once it exists: for a
+      // slice other than the last it is W in U, still one row IFFT away,
+      // which a cancel at the next slice boundary performs
+      RSIM_HIP(hipStreamSynchronize(s));
+      report_progress("sample_slice", k + 1, total);
+    }
   }
+  note_done(count, total);
   return RSIM_OK;
 }
 
@@ -673,13 +813,20 @@ int slices_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double 
   const bool want_spec = !(flags & RSIM_FLAG_NO_SPECTRUM);
   if (engine == RSIM_PLAIN || !fused_supported(nx, ny, dtype)) {
     for (int k = 0; k < count; ++k) {
-      if (cancel_requested()) return fail(RSIM_E_CANCELLED, "multislice cancelled at slice %d", first + k);
+      if (cancel_requested()) {
+        note_done(k, total);
+        return fail(RSIM_E_CANCELLED, "multislice cancelled after %d of %d slices (at slice %d)",
+                    k, total, first + k);
+      }
       if (hist) RSIM_CHECK(hist_frame(u, nx, ny, dx, dy, pz, first + k, k, hist, s, dtype));
       RSIM_CHECK(src.plain(u, dtype, first + k, scales ? scales[k] : 1.0, s));
       RSIM_CHECK(propagate_plain(u, U, nx, ny, dtype, dx, dy, wl, dzs ? dzs[k] : pz, freq, false, s));
-      if (g_progress_sync()) RSIM_HIP(hipStreamSynchronize(s));
-      report_progress("sample_slice", k + 1, total);
+      if (progress_due(k + 1, total)) {
+        RSIM_HIP(hipStreamSynchronize(s));
+        report_progress("sample_slice", k + 1, total);
+      }
     }
+    note_done(count, total);
     return RSIM_OK;
   }
   // ---- fused complex64 engine ----
@@ -793,8 +940,10 @@ int db_forward(void* u, void* U, int nx, int ny, double dx, double dy, double wl
     if (!db_col(P, nx, ny, P.fwd, U, last ? u : U, last ? U : nullptr, s))
       return fail(RSIM_E_HIP, "no fused column kernel for ny=%d", ny);
     RSIM_LAUNCHED();
-    if (g_progress_sync()) RSIM_HIP(hipStreamSynchronize(s));
-    report_progress("multislice", k + 1, n);
+    if (progress_due(k + 1, n)) {
+      RSIM_HIP(hipStreamSynchronize(s));
+      report_progress("multislice", k + 1, n);
+    }
   }
   if (!db_row(P, nx, ny, ROW_LAST, TM_PLANAR, u, u, s))
     return fail(RSIM_E_HIP, "no fused row kernel for nx=%d", nx);
@@ -895,8 +1044,10 @@ int db_adjoint(void* g, void* U, int nx, int ny, double dx, double dy, double wl
           return fail(RSIM_E_HIP, "no fused adjoint row kernel for nx=%d", nx);
         RSIM_LAUNCHED();
         ++done;
-        if (g_progress_sync()) RSIM_HIP(hipStreamSynchronize(s));
-        report_progress("multislice_adjoint", done, n);
+        if (progress_due(done, n)) {
+          RSIM_HIP(hipStreamSynchronize(s));
+          report_progress("multislice_adjoint", done, n);
+        }
       }
       RSIM_HIP(hipEventRecord(consumed[seg], s));
     }
@@ -1070,22 +1221,30 @@ int batch_create(rsim_batch** out, const rsim_batch_instance* inst, int count, i
   hipGraph_t graph = nullptr;
   int ng = 0, nk = 0;
   int rc;
+  // every cached table the capture looks up is pinned for the batch's life
+  std::vector<const void*> used;
+  table_use_log(&used);
   if (flags & RSIM_FLAG_MATH_COMP32)
     rc = batch_capture<twc>(inst, count, first, n, want_spec, cs, &graph, &ng, &nk);
   else if (flags & RSIM_FLAG_MATH_FP64)
     rc = batch_capture<double>(inst, count, first, n, want_spec, cs, &graph, &ng, &nk);
   else
     rc = batch_capture<float>(inst, count, first, n, want_spec, cs, &graph, &ng, &nk);
+  table_use_log(nullptr);
   hipStreamDestroy(cs);
   if (rc != RSIM_OK) return rc;
   rsim_batch* b = new rsim_batch();
   b->device = current_device();
+  b->generation = cache_generation();
+  for (const void* p : used)
+    if (table_pin(p, +1)) b->pinned.push_back(p);
   b->graph = graph;
   b->groups = ng;
   b->kernels = nk;
   const hipError_t e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
   if (e != hipSuccess) {
     hipGraphDestroy(graph);
+    for (const void* p : b->pinned) table_pin(p, -1);
     delete b;
     return fail(RSIM_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
   }
@@ -1096,6 +1255,9 @@ int batch_create(rsim_batch** out, const rsim_batch_instance* inst, int count, i
 int batch_run(rsim_batch* b, hipStream_t s) {
   if (current_device() != b->device)
     return fail(RSIM_E_INVALID_ARG, "batch belongs to device %d", b->device);
+  if (b->generation != cache_generation())
+    return fail(RSIM_E_INVALID_ARG,
+                "batch was captured before rsim_release_caches freed its tables; create it again");
   RSIM_HIP(hipGraphLaunch(b->exec, s));
   return RSIM_OK;
 }
@@ -1107,6 +1269,8 @@ int batch_info(const rsim_batch* b, int* groups, int* kernels) {
 }
 
 int batch_destroy(rsim_batch* b) {
+  if (b->generation == cache_generation())
+    for (const void* p : b->pinned) table_pin(p, -1);
   if (b->exec) hipGraphExecDestroy(b->exec);
   if (b->graph) hipGraphDestroy(b->graph);
   delete b;

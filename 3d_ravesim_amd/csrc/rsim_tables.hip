@@ -1,5 +1,7 @@
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -22,15 +24,92 @@ uint64_t bits(double v) {
   return b;
 }
 
-std::mutex g_mu;
 std::map<std::tuple<int, int, uint64_t>, AxisK2> g_k2;
 std::map<std::tuple<int, int, uint64_t, uint64_t, uint64_t, int, uint64_t>, AxisMap> g_maps;
-std::vector<void*> g_allocs;
+std::vector<void*> g_allocs;   // axis_k2 tables (a few per grid; freed on release)
+
+// Registry of the evictable device tables of every cache: bytes, graph pins,
+// last use, and how to drop the entry from its owning map.
+struct Slot {
+  int device;
+  size_t bytes;
+  int pins;
+  unsigned long long tick;
+  std::function<void()> drop;   // erase from the owning map (hipFree follows)
+};
+std::map<const void*, Slot> g_slots;
+unsigned long long g_tick = 0;
+std::atomic<unsigned long long> g_generation{1};
 
 }  // namespace
 
+std::recursive_mutex& table_mu() {
+  static std::recursive_mutex mu;
+  return mu;
+}
+
+unsigned long long cache_generation() { return g_generation.load(); }
+void bump_cache_generation() { g_generation.fetch_add(1); }
+
+namespace {
+thread_local std::vector<const void*>* t_use_log = nullptr;
+}
+
+void table_use_log(std::vector<const void*>* log) { t_use_log = log; }
+
+void table_touch(const void* dev) {
+  auto it = g_slots.find(dev);
+  if (it != g_slots.end()) it->second.tick = ++g_tick;
+  if (t_use_log) t_use_log->push_back(dev);
+}
+
+int table_insert(void* dev, size_t bytes, std::function<void()> drop) {
+  const int device = current_device();
+  size_t held = 0;
+  for (auto& kv : g_slots)
+    if (kv.second.device == device) held += kv.second.bytes;
+  // evict least recently used, unpinned entries of this device until it fits
+  while (held + bytes > kTableCacheBytes) {
+    auto victim = g_slots.end();
+    for (auto it = g_slots.begin(); it != g_slots.end(); ++it)
+      if (it->second.device == device && it->second.pins == 0 &&
+          (victim == g_slots.end() || it->second.tick < victim->second.tick))
+        victim = it;
+    if (victim == g_slots.end()) break;   // everything left is pinned
+    void* p = const_cast<void*>(victim->first);
+    held -= victim->second.bytes;
+    victim->second.drop();
+    g_slots.erase(victim);
+    RSIM_HIP(hipFree(p));
+  }
+  g_slots[dev] = Slot{device, bytes, 0, ++g_tick, std::move(drop)};
+  if (t_use_log) t_use_log->push_back(dev);
+  return RSIM_OK;
+}
+
+bool table_pin(const void* dev, int delta) {
+  std::lock_guard<std::recursive_mutex> lk(table_mu());
+  auto it = g_slots.find(dev);
+  if (it == g_slots.end()) return false;
+  it->second.pins += delta;
+  if (it->second.pins < 0) it->second.pins = 0;
+  return true;
+}
+
+void table_cache_stats(size_t* bytes, int* entries) {
+  std::lock_guard<std::recursive_mutex> lk(table_mu());
+  size_t b = 0;
+  for (auto& kv : g_slots) b += kv.second.bytes;
+  if (bytes) *bytes = b;
+  if (entries) *entries = (int)g_slots.size();
+}
+
+void table_forget_all() {
+  g_slots.clear();
+}
+
 int axis_k2(int n, double spacing, const AxisK2** out) {
-  std::lock_guard<std::mutex> lk(g_mu);
+  std::lock_guard<std::recursive_mutex> lk(table_mu());
   auto key = std::make_tuple(current_device(), n, bits(spacing));
   auto it = g_k2.find(key);
   if (it != g_k2.end()) {
@@ -54,20 +133,23 @@ int axis_k2(int n, double spacing, const AxisK2** out) {
   return RSIM_OK;
 }
 
-int axis_map(int n, double spacing, double scale, double pos, int grid_len, double pixel,
-             const AxisMap** out) {
-  std::lock_guard<std::mutex> lk(g_mu);
-  auto key = std::make_tuple(current_device(), n, bits(spacing), bits(scale), bits(pos), grid_len,
-                             bits(pixel));
-  auto it = g_maps.find(key);
-  if (it != g_maps.end()) {
-    *out = &it->second;
-    return RSIM_OK;
-  }
-  std::vector<int> i0(n), inear(n);
-  std::vector<double> fr(n);
-  std::vector<float> frf(n);
-  std::vector<uint8_t> ins(n);
+// device image: frac (8 n), i0 (4 n), fracf (4 n), inside (n), padding
+size_t axis_map_bytes(int n) { return (size_t)n * (8 + 4 + 4 + 1) + 64; }
+
+void axis_map_bind(AxisMap* m, const unsigned char* d, int n) {
+  m->dev.frac = (const double*)d;
+  m->dev.i0 = (const int*)(d + 8 * (size_t)n);
+  m->dev.fracf = (const float*)(d + 12 * (size_t)n);
+  m->dev.inside = (const uint8_t*)(d + 16 * (size_t)n);
+}
+
+void axis_map_build(int n, double spacing, double scale, double pos, int grid_len, double pixel,
+                    AxisMap* mp, unsigned char* img) {
+  double* fr = (double*)img;
+  int* i0 = (int*)(img + 8 * (size_t)n);
+  float* frf = (float*)(img + 12 * (size_t)n);
+  uint8_t* ins = img + 16 * (size_t)n;
+  std::vector<int> inear(n);
   bool integral = true, any = false;
   for (int i = 0; i < n; ++i) {
     // sim = (i - n/2.0)*spacing; idx = (sim*scale + pos + len*pixel*0.5)/pixel
@@ -93,7 +175,7 @@ int axis_map(int n, double spacing, double scale, double pos, int grid_len, doub
       if (std::fabs(idx - r) > kSnap) integral = false;
     }
   }
-  AxisMap m;
+  AxisMap& m = *mp;
   m.all_integral = integral;
   m.any_inside = any;
   m.affine = false;
@@ -120,25 +202,28 @@ int axis_map(int n, double spacing, double scale, double pos, int grid_len, doub
     m.lo = lo;
     m.hi = hi;
   }
-  // one allocation and one copy: frac (8 n), i0 (4 n), fracf (4 n), inside (n)
-  const size_t bytes = n * (4 + 8 + 4 + 1) + 64;
-  std::vector<unsigned char> host(bytes);
-  size_t o = 0;
-  std::memcpy(host.data() + o, fr.data(), 8 * (size_t)n);
-  const size_t o_i0 = (o += 8 * (size_t)n);
-  std::memcpy(host.data() + o, i0.data(), 4 * (size_t)n);
-  const size_t o_ff = (o += 4 * (size_t)n);
-  std::memcpy(host.data() + o, frf.data(), 4 * (size_t)n);
-  const size_t o_in = (o += 4 * (size_t)n);
-  std::memcpy(host.data() + o, ins.data(), (size_t)n);
+}
+
+int axis_map(int n, double spacing, double scale, double pos, int grid_len, double pixel,
+             const AxisMap** out) {
+  std::lock_guard<std::recursive_mutex> lk(table_mu());
+  auto key = std::make_tuple(current_device(), n, bits(spacing), bits(scale), bits(pos), grid_len,
+                             bits(pixel));
+  auto it = g_maps.find(key);
+  if (it != g_maps.end()) {
+    table_touch(it->second.dev.frac);
+    *out = &it->second;
+    return RSIM_OK;
+  }
+  const size_t bytes = axis_map_bytes(n);
+  std::vector<unsigned char> host(bytes, 0);
+  AxisMap m;
+  axis_map_build(n, spacing, scale, pos, grid_len, pixel, &m, host.data());
   unsigned char* d = nullptr;
   RSIM_HIP(hipMalloc((void**)&d, bytes));
-  g_allocs.push_back(d);
   RSIM_HIP(hipMemcpy(d, host.data(), bytes, hipMemcpyHostToDevice));
-  m.dev.frac = (const double*)d;
-  m.dev.i0 = (const int*)(d + o_i0);
-  m.dev.fracf = (const float*)(d + o_ff);
-  m.dev.inside = (const uint8_t*)(d + o_in);
+  axis_map_bind(&m, d, n);
+  RSIM_CHECK(table_insert(d, bytes, [key] { g_maps.erase(key); }));
   auto r = g_maps.emplace(key, m);
   *out = &r.first->second;
   return RSIM_OK;
@@ -150,10 +235,11 @@ int volume_db_dev(const rsim_volume* vol, const double** out) {
 }
 
 int release_tables() {
-  std::lock_guard<std::mutex> lk(g_mu);
+  std::lock_guard<std::recursive_mutex> lk(table_mu());
   for (void* p : g_allocs) hipFree(p);
   g_allocs.clear();
   g_k2.clear();
+  for (auto& kv : g_maps) hipFree(const_cast<double*>(kv.second.dev.frac));
   g_maps.clear();
   return RSIM_OK;
 }

@@ -1,17 +1,30 @@
-"""Simulation histories of the 2D path (big-wave/history.py:52-435).
+"""Histories of the 2D path: detector frames recorded along z.
 
-``History`` collects detector frames with the reference's ROI, downsampling,
-frame limit, cancellation and metadata; frames are kept in memory (HDF5
-streaming needs h5py, which this image does not ship: a history that must
-stream raises NotImplementedError).  The frames themselves are area_v1
-images computed on the GPU: inside a sample's slice loop by the library
-(rsim_sample_apply_2d_history / rsim_plasma_apply_2d_history), in free space
-by ``propagate_with_history_2d``.
+What a history is follows big-wave (big-wave/history.py:52-435): frames of
+the area_v1 detector image at planes along the beam, kept in canonical
+(z, y, x) order, cut to a region of interest and strided before they are
+counted, under an optional frame limit and cancel token; free-space legs get
+a frame every ``history_dz``, each propagated from the leg's start plane;
+the leg from the point source gets analytic 1/r^2 frames.
+
+How it is kept is this package's own:
+- ``FrameWindow`` crops and strides an incoming frame;
+- frames go to a ``_MemoryFrames`` store, an array that grows by doubling;
+- once the memory budget would be exceeded they move to a ``_SpillFrames``
+  store: an append-only raw file next to ``storage_path`` with a small JSON
+  index, read back through a memory map (big-wave streams to HDF5 at this
+  point; h5py is not part of this image, and the raw store needs nothing
+  beyond numpy).  A resumed run reopens an existing spill store and appends.
+- inside a sample the frames are computed on the GPU by the slice loop
+  itself (``rsim_sample_apply_2d_history``) into a device buffer that is
+  copied out once per chunk of slices (``slice_loop_with_history``).
 """
 
 from __future__ import annotations
 
 import ctypes as C
+import json
+import os
 from dataclasses import dataclass
 from pathlib import Path
 from typing import Any, Callable, Optional, Tuple
@@ -19,35 +32,151 @@ from typing import Any, Callable, Optional, Tuple
 import numpy as np
 
 from . import _lib
+from ._control import token_is_set
 from .propagation import SimParams, detector_shape, propagate_2d, square_and_downsample_2d
 from .vector import DeviceVector
 
 HISTORY_FORMAT_VERSION = 2
 HISTORY_AXIS_ORDER_2D = ("z", "y", "x")
+HISTORY_AXIS_ORDER_1D = ("x", "z")
 
 
-def _cancelled(token: Any | None) -> bool:
-    if token is None:
-        return False
-    if isinstance(token, bool):
-        return token
-    for name in ("is_cancelled", "is_set", "cancelled"):
-        value = getattr(token, name, None)
-        if value is not None:
-            return bool(value() if callable(value) else value)
-    return False
+class FrameWindow:
+    """Region of interest (y0, y1, x0, x1) and (sy, sx) stride of 2D frames."""
+
+    def __init__(self, roi, downsample):
+        if isinstance(downsample, int):
+            downsample = (downsample, downsample)
+        stride = tuple(downsample)
+        if len(stride) != 2 or not all(isinstance(s, int) and s > 0 for s in stride):
+            raise ValueError("history downsample must contain two positive integers")
+        if roi is not None:
+            roi = tuple(roi)
+            if len(roi) != 4 or not all(isinstance(v, int) for v in roi):
+                raise ValueError("history roi must be (y_start, y_stop, x_start, x_stop)")
+            y0, y1, x0, x1 = roi
+            if min(y0, x0) < 0 or y1 <= y0 or x1 <= x0:
+                raise ValueError("history roi bounds must be non-negative and increasing")
+        self.roi = roi
+        self.stride = (int(stride[0]), int(stride[1]))
+
+    def apply(self, frame) -> np.ndarray:
+        a = np.asanyarray(frame)
+        if a.ndim == 1:
+            return a
+        if a.ndim != 2:
+            raise ValueError(f"history frame must be 1D or 2D, got shape {a.shape}")
+        if self.roi is not None:
+            y0, y1, x0, x1 = self.roi
+            if y1 > a.shape[0] or x1 > a.shape[1]:
+                raise ValueError(f"history roi {self.roi} exceeds frame shape {a.shape}")
+            a = a[y0:y1, x0:x1]
+        return a[::self.stride[0], ::self.stride[1]]
 
 
-def _pair(value, name: str) -> tuple[int, int]:
-    pair = (value, value) if isinstance(value, int) else tuple(value)
-    if len(pair) != 2 or any(not isinstance(item, int) or item <= 0 for item in pair):
-        raise ValueError(f"{name} must contain two positive integers")
-    return int(pair[0]), int(pair[1])
+class _MemoryFrames:
+    storage = "npy"
+
+    def __init__(self):
+        self._buf: Optional[np.ndarray] = None
+        self.count = 0
+
+    def nbytes(self) -> int:
+        return 0 if self._buf is None else self.count * self._buf[0].nbytes
+
+    def append(self, frame: np.ndarray) -> None:
+        if self._buf is None:
+            self._buf = np.empty((4, *frame.shape), dtype=frame.dtype)
+        elif self.count == len(self._buf):
+            grown = np.empty((2 * len(self._buf), *frame.shape), dtype=self._buf.dtype)
+            grown[:self.count] = self._buf
+            self._buf = grown
+        self._buf[self.count] = frame
+        self.count += 1
+
+    def frames(self) -> np.ndarray:
+        return self._buf[:self.count].copy()
+
+    def close(self) -> None:
+        pass
+
+
+class _SpillFrames:
+    """Frames appended to ``<path>.frames`` (raw, C order) with an index
+    ``<path>.frames.json`` holding dtype, frame shape, count and the z of
+    every frame."""
+
+    storage = "spill"
+
+    def __init__(self, path: Path, shape: tuple, dtype):
+        self.data_path = path.with_name(path.name + ".frames")
+        self.index_path = path.with_name(path.name + ".frames.json")
+        self.shape, self.dtype = tuple(shape), np.dtype(dtype)
+        self.count = 0
+        self.zs: list[float] = []
+        self.data_path.parent.mkdir(parents=True, exist_ok=True)
+        self._fh = self.data_path.open("wb")
+        self._write_index()
+
+    @classmethod
+    def reopen(cls, path: Path) -> "_SpillFrames":
+        index_path = path.with_name(path.name + ".frames.json")
+        info = json.loads(index_path.read_text())
+        if int(info.get("format_version", 0)) != HISTORY_FORMAT_VERSION:
+            raise ValueError("cannot resume a history spill of another format")
+        self = cls.__new__(cls)
+        self.data_path = path.with_name(path.name + ".frames")
+        self.index_path = index_path
+        self.shape, self.dtype = tuple(info["frame_shape"]), np.dtype(info["dtype"])
+        self.count = int(info["frames"])
+        self.zs = [float(z) for z in info["z"]]
+        self._fh = self.data_path.open("r+b")
+        self._fh.truncate(self.count * self._frame_bytes())
+        self._fh.seek(0, os.SEEK_END)
+        return self
+
+    def _frame_bytes(self) -> int:
+        return int(np.prod(self.shape)) * self.dtype.itemsize
+
+    def _write_index(self) -> None:
+        tmp = self.index_path.with_name(self.index_path.name + ".tmp")
+        tmp.write_text(json.dumps({"format_version": HISTORY_FORMAT_VERSION,
+                                   "dtype": self.dtype.str, "frame_shape": list(self.shape),
+                                   "frames": self.count, "z": self.zs}))
+        os.replace(tmp, self.index_path)
+
+    def nbytes(self) -> int:
+        return 0
+
+    def append(self, frame: np.ndarray, z: float) -> None:
+        self._fh.write(np.ascontiguousarray(frame, dtype=self.dtype).tobytes())
+        self._fh.flush()
+        self.count += 1
+        self.zs.append(float(z))
+        self._write_index()
+
+    def frames(self) -> np.ndarray:
+        if self.count == 0:
+            return np.empty((0, *self.shape), dtype=self.dtype)
+        self._fh.flush()
+        return np.memmap(self.data_path, dtype=self.dtype, mode="r",
+                         shape=(self.count, *self.shape))
+
+    def close(self) -> None:
+        if self._fh is not None and not self._fh.closed:
+            self._fh.close()
 
 
 class History:
-    """Frames (z, y, x) with ROI / downsampling applied before accounting
-    (history.py:52-253)."""
+    """Detector frames along z (big-wave/history.py:52-253 interface).
+
+    ``memory_budget_bytes`` bounds the frames held in memory once a
+    ``storage_path`` is given: 0 spills from the first 2D frame, a positive
+    budget spills when the next frame would exceed it.  ``max_frames`` is a
+    hard limit (RuntimeError); ``cancel_token`` stops recording
+    (InterruptedError); ``progress_cb("history_frame", n, max_frames or 0)``
+    follows every frame.
+    """
 
     def __init__(self, storage_path: Path | str | None = None,
                  memory_budget_bytes: int | None = None, max_frames: int | None = None,
@@ -58,99 +187,121 @@ class History:
             raise ValueError("history memory_budget_bytes must be non-negative")
         if max_frames is not None and max_frames <= 0:
             raise ValueError("history max_frames must be positive")
-        if roi is not None:
-            if len(roi) != 4 or any(not isinstance(item, int) for item in roi):
-                raise ValueError("history roi must be (y_start, y_stop, x_start, x_stop)")
-            if roi[0] < 0 or roi[2] < 0 or roi[1] <= roi[0] or roi[3] <= roi[2]:
-                raise ValueError("history roi bounds must be non-negative and increasing")
-        if resume and storage_path is not None and Path(storage_path).exists():
-            raise NotImplementedError("resuming an HDF5 history needs h5py")
-        self.entries: list[np.ndarray] = []
-        self.zs: list[float] = []
+        self.window = FrameWindow(roi, downsample)
         self.storage_path = Path(storage_path) if storage_path is not None else None
         self.memory_budget_bytes = memory_budget_bytes
         self.max_frames = max_frames
-        self.downsample = _pair(downsample, "history downsample")
-        self.roi = roi
         self.progress_cb = progress_cb
         self.cancel_token = cancel_token
-        self.frame_ndim: int | None = None
-        self.frame_shape: tuple[int, ...] | None = None
+        self._store: Any = _MemoryFrames()
+        self._zs: list[float] = []
+        self._shape: Optional[tuple] = None
+        if resume and self.storage_path is not None and \
+                self.storage_path.with_name(self.storage_path.name + ".frames.json").exists():
+            self._store = _SpillFrames.reopen(self.storage_path)
+            self._zs = list(self._store.zs)
+            self._shape = self._store.shape
+
+    # -- configuration as big-wave exposes it --
+    @property
+    def roi(self):
+        return self.window.roi
+
+    @property
+    def downsample(self) -> tuple[int, int]:
+        return self.window.stride
+
+    @property
+    def frame_ndim(self) -> Optional[int]:
+        return None if self._shape is None else len(self._shape)
+
+    @property
+    def frame_shape(self) -> Optional[tuple]:
+        return self._shape
 
     @property
     def is_streaming(self) -> bool:
-        return False
+        return isinstance(self._store, _SpillFrames)
 
-    def _prepare_frame(self, frame: np.ndarray) -> np.ndarray:
-        array = np.asanyarray(frame)
-        if array.ndim not in (1, 2):
-            raise ValueError(f"history frame must be 1D or 2D, got shape {array.shape}")
-        if array.ndim == 2:
-            if self.roi is not None:
-                y0, y1, x0, x1 = self.roi
-                if y1 > array.shape[0] or x1 > array.shape[1]:
-                    raise ValueError(f"history roi {self.roi} exceeds frame shape {array.shape}")
-                array = array[y0:y1, x0:x1]
-            sy, sx = self.downsample
-            array = array[::sy, ::sx]
-        return array
+    # -- recording --
+    def _spill_needed(self, frame: np.ndarray) -> bool:
+        if frame.ndim != 2 or self.storage_path is None:
+            return False
+        if self.is_streaming or self.memory_budget_bytes == 0:
+            return True
+        return (self.memory_budget_bytes is not None
+                and self._store.nbytes() + frame.nbytes > self.memory_budget_bytes)
+
+    def _start_spill(self, frame: np.ndarray) -> None:
+        spill = _SpillFrames(self.storage_path, frame.shape, frame.dtype)
+        if self._store.count:
+            for old, z in zip(self._store.frames(), self._zs):
+                spill.append(old, z)
+        self._store = spill
 
     def push(self, downscaled: np.ndarray, z: float) -> None:
-        """Add a frame (history.py:186-226)."""
-        if _cancelled(self.cancel_token):
+        """Record the frame ``downscaled`` taken at ``z``."""
+        if token_is_set(self.cancel_token):
             raise InterruptedError("history recording cancelled")
         if self.max_frames is not None and len(self) >= self.max_frames:
             raise RuntimeError(f"history maximum frame count {self.max_frames} exceeded")
-        frame = self._prepare_frame(downscaled)
-        if self.frame_ndim is None:
-            self.frame_ndim = frame.ndim
-            self.frame_shape = tuple(frame.shape)
-        elif frame.ndim != self.frame_ndim or tuple(frame.shape) != self.frame_shape:
-            raise ValueError(f"history frame shape changed from {self.frame_shape} to {frame.shape}")
-        should_stream = frame.ndim == 2 and self.storage_path is not None and (
-            self.memory_budget_bytes == 0
-            or (self.memory_budget_bytes is not None
-                and sum(item.nbytes for item in self.entries) + frame.nbytes
-                > self.memory_budget_bytes))
-        if should_stream:
-            raise NotImplementedError(
-                "streaming a history to HDF5 needs h5py, which is not installed; "
-                "use history storage 'memory' or a larger memory budget")
-        self.entries.append(np.array(frame, copy=True))
-        self.zs.append(float(z))
+        frame = self.window.apply(downscaled)
+        if self._shape is None:
+            self._shape = tuple(frame.shape)
+        elif tuple(frame.shape) != self._shape:
+            raise ValueError(f"history frame shape changed from {self._shape} to {frame.shape}")
+        if self._spill_needed(frame):
+            if not self.is_streaming:
+                self._start_spill(frame)
+            self._store.append(frame, z)
+        else:
+            self._store.append(frame)
+        self._zs.append(float(z))
         if self.progress_cb is not None:
             self.progress_cb("history_frame", len(self), self.max_frames or 0)
 
+    # -- reading --
     def get_history(self) -> np.ndarray:
-        if not self.entries:
+        """(z, y, x) for 2D frames; 1D frames as (x, z)."""
+        if len(self) == 0:
             return np.empty((0, 0))
-        if self.frame_ndim == 1:
-            return np.stack(self.entries, axis=-1)
-        return np.stack(self.entries, axis=0)
+        stack = self._store.frames()
+        return np.moveaxis(stack, 0, -1) if self.frame_ndim == 1 else stack
 
     def get_z(self) -> list[float]:
-        return self.zs
+        return self._zs
 
     def metadata(self) -> dict[str, Any]:
-        shape = tuple(int(item) for item in self.get_history().shape)
+        two_d = self.frame_ndim == 2
+        shape = [len(self), *self._shape] if self._shape is not None else [0, 0]
+        if self.frame_ndim == 1:
+            shape = [self._shape[0], len(self)]
         return {
-            "format_version": HISTORY_FORMAT_VERSION if self.frame_ndim == 2 else 1,
-            "axis_order": list(HISTORY_AXIS_ORDER_2D if self.frame_ndim == 2 else ("x", "z")),
-            "storage": "npy",
-            "shape": list(shape),
+            "format_version": HISTORY_FORMAT_VERSION if two_d else 1,
+            "axis_order": list(HISTORY_AXIS_ORDER_2D if two_d else HISTORY_AXIS_ORDER_1D),
+            "storage": self._store.storage,
+            "shape": shape,
             "frames": len(self),
             "downsample_yx": list(self.downsample),
             "roi_yxyx": list(self.roi) if self.roi is not None else None,
         }
 
     def finalize(self) -> None:
-        pass
+        if self.is_streaming:
+            self._store._fh.flush()
 
     def close(self) -> None:
-        pass
+        self._store.close()
+
+    def remove_spill(self) -> None:
+        """Delete the spill files once the frames have been saved elsewhere."""
+        if self.is_streaming:
+            self._store.close()
+            self._store.data_path.unlink(missing_ok=True)
+            self._store.index_path.unlink(missing_ok=True)
 
     def __len__(self) -> int:
-        return len(self.entries)
+        return len(self._zs)
 
 
 @dataclass
@@ -162,125 +313,130 @@ class HistoryData:
     metadata: dict | None = None
 
 
-def load_history(path: Path | str) -> HistoryData:
-    """Load a history written by a run (history.py:265-314): canonical
-    (z, y, x) .npy with its metadata, or a legacy (y, x, z) array, which is
-    transposed.  HDF5 histories need h5py."""
+def _read_yaml(path: Path) -> dict:
     import yaml
+    with open(path) as handle:
+        return yaml.safe_load(handle) or {}
+
+
+def load_history(path: Path | str) -> HistoryData:
+    """A saved history (big-wave/history.py:265-314 behaviour): ``history.npy``
+    in canonical (z, y, x) order when its metadata says format 2, otherwise
+    a legacy (y, x, z) array whose last axis matches the z list, returned
+    transposed.  HDF5 files need h5py and are refused."""
     root = Path(path)
-    directory = root if root.is_dir() else root.parent
-    h5_path = root if root.suffix in (".h5", ".hdf5") else directory / "history.h5"
-    if h5_path.exists():
+    folder = root if root.is_dir() else root.parent
+    if root.suffix in (".h5", ".hdf5") or (root.is_dir() and (folder / "history.h5").exists()):
         raise NotImplementedError("reading an HDF5 history needs h5py")
-    x_path, y_path = directory / "history_x.npy", directory / "history_y.npy"
-    z_path = directory / "history_z.npy"
-    x = np.load(x_path, allow_pickle=False) if x_path.exists() else None
-    y = np.load(y_path, allow_pickle=False) if y_path.exists() else None
-    npy_path = root if root.suffix == ".npy" else directory / "history.npy"
-    values = np.load(npy_path, mmap_mode="r", allow_pickle=False)
-    z = np.load(z_path, allow_pickle=False) if z_path.exists() else np.arange(values.shape[-1])
-    metadata_path = (root.with_name(f"{root.stem}_metadata.yaml")
-                     if root.is_file() and root.name != "history.npy"
-                     else directory / "history_metadata.yaml")
-    version, axis_order = 0, None
-    if metadata_path.exists():
-        with open(metadata_path) as f:
-            loaded = yaml.safe_load(f) or {}
-        version, axis_order = int(loaded.get("format_version", 0)), list(loaded.get("axis_order", []))
-    if values.ndim == 3:
-        if version == HISTORY_FORMAT_VERSION and axis_order == list(HISTORY_AXIS_ORDER_2D):
-            canonical = np.asarray(values)
-        elif values.shape[-1] == len(z):
-            canonical = np.transpose(values, (2, 0, 1))
-        else:
+
+    def optional(name):
+        p = folder / name
+        return np.load(p, allow_pickle=False) if p.exists() else None
+
+    x, y, z = optional("history_x.npy"), optional("history_y.npy"), optional("history_z.npy")
+    array_path = root if root.suffix == ".npy" else folder / "history.npy"
+    values = np.load(array_path, mmap_mode="r", allow_pickle=False)
+    if z is None:
+        z = np.arange(values.shape[-1])
+    if root.is_file() and root.name != "history.npy":
+        meta_path = root.with_name(root.stem + "_metadata.yaml")
+    else:
+        meta_path = folder / "history_metadata.yaml"
+    meta = _read_yaml(meta_path) if meta_path.exists() else {}
+    version = int(meta.get("format_version", 0))
+    order = list(meta.get("axis_order", []))
+    if values.ndim != 3:
+        return HistoryData(np.asarray(values), z, x, y,
+                           {"format_version": version, "axis_order": order, "storage": "npy"})
+    canonical = version == HISTORY_FORMAT_VERSION and order == list(HISTORY_AXIS_ORDER_2D)
+    if not canonical:
+        if values.shape[-1] != len(z):
             raise ValueError("cannot infer legacy 2D history axis order")
-        metadata = {"format_version": HISTORY_FORMAT_VERSION,
-                    "axis_order": list(HISTORY_AXIS_ORDER_2D), "storage": "npy"}
-        return HistoryData(canonical, z, x, y, metadata)
+        values = values.transpose(2, 0, 1)
     return HistoryData(np.asarray(values), z, x, y,
-                       {"format_version": version, "axis_order": axis_order, "storage": "npy"})
+                       {"format_version": HISTORY_FORMAT_VERSION,
+                        "axis_order": list(HISTORY_AXIS_ORDER_2D), "storage": "npy"})
 
 
 def migrate_legacy_history(path: Path | str, output_path: Path | str | None = None) -> Path:
-    """Write a legacy 2D history as canonical v2 without overwriting it
-    (history.py:317-333)."""
+    """Save a legacy (y, x, z) history as a canonical one beside it
+    (``history_v2.npy`` by default); the source file is left as it is."""
     from . import config
     root = Path(path)
-    directory = root if root.is_dir() else root.parent
-    loaded = load_history(root)
-    if loaded.values.ndim != 3:
+    data = load_history(root)
+    if data.values.ndim != 3:
         raise ValueError("only legacy 2D histories require migration")
-    target = Path(output_path) if output_path is not None else directory / "history_v2.npy"
-    np.save(target, loaded.values)
-    config.save(target.with_name(f"{target.stem}_metadata.yaml"),
+    folder = root if root.is_dir() else root.parent
+    target = Path(output_path) if output_path is not None else folder / "history_v2.npy"
+    np.save(target, data.values)
+    config.save(target.with_name(target.stem + "_metadata.yaml"),
                 {"format_version": HISTORY_FORMAT_VERSION,
                  "axis_order": list(HISTORY_AXIS_ORDER_2D), "storage": "npy",
-                 "shape": list(loaded.values.shape)})
+                 "shape": list(data.values.shape)})
     return target
 
 
 def generate_analytic_history_2d(history: History, zs: np.ndarray, x_source: float,
                                  y_source: float, params: SimParams, cutoff_freq: float) -> None:
-    """Point-source history frames from the analytic 1/r^2 intensity
-    (history.py:348-367)."""
-    gradient = cutoff_freq * params.wl
-    nx_pix = int(params.get_detector_size_x() // params.detector_pixel_size_x)
-    ny_pix = int(params.get_detector_size_y() // params.detector_pixel_size_y)
-    x = (np.arange(nx_pix) - nx_pix // 2) * params.detector_pixel_size_x - x_source
-    y = (np.arange(ny_pix) - ny_pix // 2) * params.detector_pixel_size_y - y_source
-    x2 = x * x
+    """Frames of the unobstructed point source: pixel area x z / r^3 at each
+    plane z, zero outside the cone |x|, |y| <= cutoff_freq * wl * z that the
+    frequency cutoff lets through (big-wave/history.py:348-367)."""
+    dsx, dsy = params.detector_pixel_size_x, params.detector_pixel_size_y
+    ox = int(params.get_detector_size_x() // dsx)
+    oy = int(params.get_detector_size_y() // dsy)
+    xs = (np.arange(ox) - ox // 2) * dsx - x_source
+    ys = (np.arange(oy) - oy // 2) * dsy - y_source
+    x_sq, y_sq = xs * xs, ys * ys
+    slope = cutoff_freq * params.wl
     for z in zs:
-        intensity = np.empty((ny_pix, nx_pix), dtype=np.float64)
-        for row, y_value in enumerate(y):
-            r2 = x2 + y_value * y_value + z * z
-            intensity[row] = (params.detector_pixel_size_x * params.detector_pixel_size_y * z
-                              / np.power(r2, 1.5))
-            intensity[row, np.abs(x) > gradient * z] = 0
-            if abs(y_value) > gradient * z:
-                intensity[row] = 0
-        history.push(intensity, z)
+        r_sq = x_sq[None, :] + y_sq[:, None] + z * z
+        image = dsx * dsy * z / np.power(r_sq, 1.5)
+        reach = slope * z
+        image[:, np.abs(xs) > reach] = 0.0
+        image[np.abs(ys) > reach, :] = 0.0
+        history.push(image, z)
 
 
 def _history_z_steps(dz: float, hist_dz: float) -> np.ndarray:
-    """history.py:370-378."""
-    zs = np.arange(0, dz, hist_dz)
-    if len(zs) == 0:
-        return np.array([0.0, dz])
-    if dz - zs[-1] < hist_dz / 2.0 and len(zs) > 1:
-        zs[-1] = dz
-    else:
-        zs = np.append(zs, [dz])
-    return zs
+    """Frame planes of a free-space leg of length dz, relative to its start:
+    0, hist_dz, 2 hist_dz, ... below dz, then dz itself; a last regular
+    plane closer than hist_dz / 2 to dz (other than the start) moves to dz
+    (big-wave/history.py:370-378)."""
+    grid = np.arange(0.0, dz, hist_dz)
+    close = grid.size > 1 and dz - grid[-1] < 0.5 * hist_dz
+    head = grid[:-1] if close else (grid if grid.size else np.zeros(1))
+    return np.concatenate([head, [dz]])
 
 
 def propagate_with_history_2d(u: DeviceVector, U: DeviceVector, sim_params: SimParams, dz: float,
                               cutoff_freq: float, current_z: float, skip_fft: bool = False,
                               history: Optional[Tuple[History, float]] = None) -> None:
-    """Free space with a frame every hist_dz, each leg from the start plane
-    (history.py:406-435).  U = FFT2(u) afterwards."""
+    """Free space over dz with a frame at each plane of ``_history_z_steps``.
+    Every plane is reached from the leg's start spectrum, as big-wave does
+    (big-wave/history.py:406-435), so the last frame's field is exactly the
+    field of one propagation over dz.  U = FFT2(u) afterwards."""
     if dz == 0:
         return
-    nx, ny = sim_params.nx, sim_params.ny
+    nx, ny, dx, dy, wl = sim_params.nx, sim_params.ny, sim_params.dx, sim_params.get_dy(), \
+        sim_params.wl
     if history is None:
-        propagate_2d(u, U, sim_params.dx, sim_params.get_dy(), sim_params.wl, dz,
-                     sim_params.chunk_size, cutoff_freq, nx, ny, skip_fft)
+        propagate_2d(u, U, dx, dy, wl, dz, sim_params.chunk_size, cutoff_freq, nx, ny, skip_fft)
         return
-    hist_obj, hist_dz = history
-    zs = _history_z_steps(dz, hist_dz)
+    recorder, step = history
+    planes = _history_z_steps(dz, step)
     if not skip_fft:
         u.fft2(U, nx, ny)
-    U_start = U.copy()
+    start_spectrum = U.copy()
     try:
-        for i in range(1, len(zs)):
-            dz_step = float(zs[i] - zs[0])
-            propagate_2d(u, U, sim_params.dx, sim_params.get_dy(), sim_params.wl, dz_step,
-                         sim_params.chunk_size, cutoff_freq, nx, ny, True)
-            z = float(zs[i] + current_z)
-            hist_obj.push(square_and_downsample_2d(u, sim_params, z), z)
-            if i < len(zs) - 1:
-                U_start.copy_to(U)
+        for k, plane in enumerate(planes[1:], start=1):
+            if k > 1:
+                start_spectrum.copy_to(U)
+            propagate_2d(u, U, dx, dy, wl, float(plane - planes[0]), sim_params.chunk_size,
+                         cutoff_freq, nx, ny, True)
+            z = float(plane + current_z)
+            recorder.push(square_and_downsample_2d(u, sim_params, z), z)
     finally:
-        U_start.drop()
+        start_spectrum.drop()
 
 
 class DeviceFrames:
@@ -375,4 +531,3 @@ def drive_slices(run, element, sim_params: SimParams, history: Optional[History]
         segment(a, b - a)
         callback(b, z_start + b * pz)
         a = b
-

@@ -250,5 +250,7 @@ class Sample:
             else:
                 _lib.call("rsim_sample_apply_2d", *args, first, n, engine, 0, None)
 
+        from ._control import run_scope
         from .history import drive_slices
-        drive_slices(run, self, sim_params, history, start, z_len)
+        with run_scope(u):
+            drive_slices(run, self, sim_params, history, start, z_len)

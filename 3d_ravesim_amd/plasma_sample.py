@@ -128,5 +128,7 @@ class PlasmaSample:
             else:
                 _lib.call("rsim_plasma_apply_2d_history", *args, C.byref(frames), None)
 
+        from ._control import run_scope
         from .history import drive_slices
-        drive_slices(run, self, sim_params, history, start, nz)
+        with run_scope(u):
+            drive_slices(run, self, sim_params, history, start, nz)

@@ -39,8 +39,17 @@ class DeviceVector:
     """A complex64/complex128 vector resident in HBM (one allocation).
 
     ``ptr`` is the device address (an int), usable by the C ABI and by
-    ``torch`` via ``from_blob``-style wrappers.
+    ``torch`` via ``from_blob``-style wrappers.  ``progress_cb(stage,
+    completed, total)`` and ``cancel_token`` (any token form big-wave
+    accepts) serve the slice loops run on this field, as DiskVector's
+    fft2_progress_cb / fft2_cancel_token serve its out-of-core FFT
+    (big-wave/vector.py:115-143): a report every ``progress_every`` slices,
+    and a cancel stops at the next slice boundary with InterruptedError.
     """
+
+    progress_cb = None
+    cancel_token = None
+    progress_every = 1
 
     def __init__(self, n: int, dtype=np.complex128, zero: bool = True):
         self._dtype = np.dtype(dtype)

@@ -1,10 +1,15 @@
-"""Per-source simulation driver for the 2D path (big-wave/wavesim.py:89-409).
+"""One source through the setup on HBM-resident fields.
 
-Same sequencing as the reference: source -> gaps -> elements -> final leg
-(skipping the forward FFT when the last element left U = FFT2(u)) ->
-area_v1 detector, with phase stepping replayed from a snapshot, history
-frames (analytic for the source leg, GPU detector frames in free space and
-at every sample slice) and transactional runtime checkpoints with resume.
+``run_simulation`` keeps the calling contract of big-wave's driver
+(big-wave/wavesim.py:89-409), because the runner, tests and callers of the
+reference use it: the same arguments and return value (one area_v1 image per
+phase step), the same order of side effects (source, gap, snapshot, keypoint,
+element, keypoint, checkpoint, ..., final leg, checkpoint, detector, saved u,
+replayed phase steps) and the same ValueErrors for runs that cannot be
+resumed.  Its structure is this package's own: a ``_Beam`` records where the
+field is (z, the element it has reached, how many of that element's slices
+are done, and whether U holds FFT2(u)), and each stage of the run is one
+method that advances it.
 """
 
 from __future__ import annotations
@@ -21,15 +26,24 @@ from .history import History, propagate_with_history_2d
 from .propagation import SimParams, propagate_2d, square_and_downsample_2d
 from .vector import DeviceVector
 
-# Runtime checkpoint cadence: every N-th sample slice (wavesim.py:17-19).
+# A runtime checkpoint is taken after every CHECKPOINT_SLICE_CADENCE-th slice
+# of a sample (big-wave/wavesim.py:17-19): u and U are GiB-sized at the large
+# grids, so saving at every slice would cost more than the slices.
 CHECKPOINT_SLICE_CADENCE = 5
 
+# the final leg is skipped when the field is already this close to the detector
+Z_TOLERANCE = 1e-8
+
 logger = logging.getLogger("big-wave")
+
+# attributes the driver lends an element for the duration of one apply()
+_LENT = ("_checkpoint_start_slice", "_checkpoint_callback", "_checkpoint_cadence")
 
 
 @dataclass
 class Snapshot:
-    """big-wave/wavesim.py:24-37."""
+    """The field in front of the first phase-stepped element, kept so that
+    later phase steps restart there (big-wave/wavesim.py:24-37)."""
 
     z: float
     element_idx: int
@@ -39,18 +53,162 @@ class Snapshot:
 
 def save_keypoint(save_keypoints_path: Optional[Path], element_idx: int, sub_idx: int,
                   u: DeviceVector) -> None:
-    """keypoint_<element>_<0 before | 1 after>.npy (wavesim.py:40-62)."""
-    if save_keypoints_path is None:
-        return
-    path = Path(save_keypoints_path) / f"keypoint_{element_idx:02}_{sub_idx}.npy"
-    np.save(path, u.to_numpy())
+    """``keypoint_<element:02>_<0 before, 1 after>.npy`` of u
+    (big-wave/wavesim.py:40-62 naming)."""
+    if save_keypoints_path is not None:
+        np.save(Path(save_keypoints_path) / f"keypoint_{element_idx:02}_{sub_idx}.npy",
+                u.to_numpy())
 
 
 def save_u(u: DeviceVector, phase_step: int, sub_dir: Path, vectors_dir: Path,
            save_final_u_vectors: bool) -> None:
-    """u_<phase>.npy in the source's directory (wavesim.py:65-86)."""
+    """``u_<phase:04>.npy`` in the source's directory when asked for
+    (big-wave/wavesim.py:65-86 naming)."""
     if save_final_u_vectors:
         np.save(Path(sub_dir) / f"u_{phase_step:04}.npy", u.to_numpy())
+
+
+@dataclass
+class _Beam:
+    z: float
+    element: int = 0
+    slices_done: int = 0        # of ``element``, for a resumed sample
+    spectrum_valid: bool = False
+
+
+class _SourceRun:
+    def __init__(self, params: SimParams, source, elements: list, cutoff_angles: list,
+                 u, U, history, keypoints: Optional[Path], checkpoint: Optional[RunCheckpoint]):
+        self.p = params
+        self.source = source
+        self.elements = elements
+        self.freqs = [float(np.sin(a)) / params.wl for a in cutoff_angles]
+        self.u, self.U = u, U
+        self.history = history
+        self.keypoints = keypoints
+        self.checkpoint = checkpoint
+        self.snapshot: Optional[Snapshot] = None
+
+    # -- the cutoff in force: index 0 before the first element, i + 1 from
+    # the start of element i on --
+    def freq_before(self, element: int) -> float:
+        return self.freqs[element]
+
+    def freq_inside(self, element: int) -> float:
+        return self.freqs[element + 1]
+
+    def _save(self, beam: _Beam, stage: str) -> None:
+        if self.checkpoint is not None:
+            self.checkpoint.save(self.u, self.U, current_z=beam.z, element_index=beam.element,
+                                 slice_index=beam.slices_done,
+                                 u_fourier_valid=beam.spectrum_valid, stage=stage)
+
+    # -- stages --
+    def from_source(self) -> _Beam:
+        first_z = self.elements[0].z_start if self.elements else self.p.z_detector
+        self.source.propagate_to(first_z, self.p, self.freq_before(0), self.u, self.U,
+                                 self.history)
+        beam = _Beam(z=first_z)
+        self._save(beam, "source_propagated")
+        return beam
+
+    def from_checkpoint(self) -> _Beam:
+        if self.checkpoint is None or not self.checkpoint.exists:
+            raise ValueError("resume_checkpoint requested but no ready checkpoint exists")
+        state = self.checkpoint.load(self.u, self.U)
+        if state.phase_step != 0:
+            raise ValueError("only phase_step=0 runtime checkpoints are supported")
+        logger.info("Resuming checkpoint at z=%sm, element=%s, slice=%s", state.current_z,
+                    state.element_index, state.slice_index)
+        return _Beam(z=state.current_z, element=state.element_index,
+                     slices_done=state.slice_index, spectrum_valid=state.u_fourier_valid)
+
+    def cutoff_after_resume(self, beam: _Beam) -> float:
+        # the field is inside an element once one of its slices is done
+        idx = beam.element + (1 if beam.slices_done > 0 else 0)
+        return self.freqs[min(idx, len(self.freqs) - 1)]
+
+    def through_element(self, beam: _Beam, idx: int, cutoff: float) -> float:
+        """Gap up to element idx, then the element; returns the cutoff in force."""
+        el = self.elements[idx]
+        resuming_inside = beam.slices_done > 0
+        if not resuming_inside:
+            if beam.z < el.z_start:
+                logger.info(f"Propagating from z {beam.z}m to {el.z_start}m")
+                gap = self.p.effective_slice_dz(beam.z, el.z_start - beam.z)
+                propagate_with_history_2d(self.u, self.U, self.p, gap, cutoff, beam.z, False,
+                                          self.history)
+                beam.spectrum_valid = True
+            beam.z = el.z_start
+        if self.snapshot is None and len(el.x_positions) > 1:
+            self.snapshot = Snapshot(z=beam.z, element_idx=idx, u=self.u.copy(),
+                                     nr_phase_steps=len(el.x_positions))
+        cutoff = self.freq_inside(idx)
+        save_keypoint(self.keypoints, idx, 0, self.u)
+        self._lend(el, idx, beam.slices_done)
+        try:
+            el.apply(self.u, self.U, self.p, cutoff, 0,
+                     self.history[0] if self.history is not None else None)
+        finally:
+            for name in _LENT:
+                if hasattr(el, name):
+                    delattr(el, name)
+        save_keypoint(self.keypoints, idx, 1, self.u)
+        beam.z = el.z_start + el.get_thickness()
+        beam.element, beam.slices_done, beam.spectrum_valid = idx + 1, 0, True
+        self._save(beam, "element_complete")
+        return cutoff
+
+    def _lend(self, el, idx: int, first_slice: int) -> None:
+        if first_slice > 0:
+            el._checkpoint_start_slice = first_slice
+        if self.checkpoint is None:
+            return
+        u, U, ck = self.u, self.U, self.checkpoint
+
+        def after_slices(done: int, z: float) -> None:
+            if done % CHECKPOINT_SLICE_CADENCE == 0:
+                ck.save(u, U, current_z=z, element_index=idx, slice_index=done,
+                        u_fourier_valid=True, stage="element_slice")
+
+        el._checkpoint_callback = after_slices
+        el._checkpoint_cadence = CHECKPOINT_SLICE_CADENCE
+
+    def to_detector(self, beam: _Beam, cutoff: float) -> None:
+        if beam.z < self.p.z_detector - Z_TOLERANCE:
+            logger.info(f"Propagating from z {beam.z}m to {self.p.z_detector}m")
+            # a source leaves U undefined; an element leaves U = FFT2(u)
+            propagate_with_history_2d(
+                self.u, self.U, self.p, self.p.effective_final_dz(beam.z), cutoff, beam.z,
+                skip_fft=beam.spectrum_valid,
+                history=None if self.p.use_fresnel_scaling else self.history)
+            beam.z, beam.spectrum_valid = self.p.z_detector, True
+        beam.element, beam.slices_done = len(self.elements), 0
+        self._save(beam, "detector_ready")
+
+    def image(self) -> np.ndarray:
+        return square_and_downsample_2d(self.u, self.p, self.p.z_detector)
+
+    def replay_step(self, step: int) -> np.ndarray:
+        """Phase step ``step`` > 0: from the snapshot to the detector, without
+        history or checkpoints."""
+        snap = self.snapshot
+        snap.u.copy_to(self.u)
+        z, cutoff = snap.z, self.freq_before(snap.element_idx)
+        nx, ny, dx, dy, wl = self.p.nx, self.p.ny, self.p.dx, self.p.get_dy(), self.p.wl
+        for idx in range(snap.element_idx, len(self.elements)):
+            el = self.elements[idx]
+            if z < el.z_start:
+                propagate_2d(self.u, self.U, dx, dy, wl,
+                             self.p.effective_slice_dz(z, el.z_start - z),
+                             self.p.chunk_size, cutoff, nx, ny)
+            cutoff = self.freq_inside(idx)
+            el.apply(self.u, self.U, self.p, cutoff, step if len(el.x_positions) > 1 else 0, None)
+            z = el.z_start + el.get_thickness()
+        if z < self.p.z_detector - Z_TOLERANCE:
+            propagate_2d(self.u, self.U, dx, dy, wl, self.p.effective_final_dz(z),
+                         self.p.chunk_size, cutoff, nx, ny)
+        return self.image()
 
 
 def run_simulation(params: SimParams, source, elements: list, cutoff_angles: list[float],
@@ -59,123 +217,45 @@ def run_simulation(params: SimParams, source, elements: list, cutoff_angles: lis
                    history: Optional[Tuple[History, float]] = None, save_keypoints_path=None,
                    checkpoint: Optional[RunCheckpoint] = None,
                    resume_checkpoint: bool = False) -> list[np.ndarray]:
-    """Run one source through the setup; one detector image per phase step."""
-    z_tolerance = 1e-8
-    if save_keypoints_path is not None and not isinstance(save_keypoints_path, Path):
-        save_keypoints_path = Path(save_keypoints_path)
+    """Run one source through ``elements`` to the detector.
+
+    ``cutoff_angles`` has one entry more than ``elements``: entry 0 holds
+    before the first element, entry i + 1 from the start of element i on.
+    Returns one detector image per phase step of the phase-stepped element
+    (one image without phase stepping).  ``checkpoint`` saves u and U after
+    the source leg, every CHECKPOINT_SLICE_CADENCE slices, after each element
+    and before the detector; ``resume_checkpoint`` continues from the last
+    save instead of starting at the source.
+    """
     if save_final_u_vectors and sub_dir is None:
         raise ValueError("save_final_u_vectors needs sub_dir")
-    cutoff_frequencies = [np.sin(a) / params.wl for a in cutoff_angles]
+    keypoints = Path(save_keypoints_path) if save_keypoints_path is not None else None
     for el in elements:
         el.store_deltabetas(deltabeta_table)
     if checkpoint is not None and any(len(el.x_positions) > 1 for el in elements):
         raise ValueError(
             "runtime checkpointing is not supported together with phase stepping; "
-            "phase Snapshot state is intentionally separate")
+            "the phase-step snapshot is kept in memory only")
 
-    resume_slice = 0
-    u_fourier_valid = False
+    run = _SourceRun(params, source, elements, cutoff_angles, u, U, history, keypoints,
+                     checkpoint)
     if resume_checkpoint:
-        if checkpoint is None or not checkpoint.exists:
-            raise ValueError("resume_checkpoint requested but no ready checkpoint exists")
-        state = checkpoint.load(u, U)
-        if state.phase_step != 0:
-            raise ValueError("only phase_step=0 runtime checkpoints are supported")
-        current_z = state.current_z
-        start_element_idx = state.element_index
-        resume_slice = state.slice_index
-        u_fourier_valid = state.u_fourier_valid
-        cutoff_index = start_element_idx + (1 if resume_slice > 0 else 0)
-        cutoff_freq = cutoff_frequencies[min(cutoff_index, len(cutoff_frequencies) - 1)]
-        logger.info("Resuming checkpoint at z=%sm, element=%s, slice=%s", current_z,
-                    start_element_idx, resume_slice)
+        beam = run.from_checkpoint()
+        cutoff = run.cutoff_after_resume(beam)
     else:
-        cutoff_freq = cutoff_frequencies[0]
-        current_z = elements[0].z_start if len(elements) > 0 else params.z_detector
-        source.propagate_to(current_z, params, cutoff_freq, u, U, history)
-        start_element_idx = 0
-        if checkpoint is not None:
-            checkpoint.save(u, U, current_z=current_z, element_index=0, slice_index=0,
-                            u_fourier_valid=False, stage="source_propagated")
+        beam = run.from_source()
+        cutoff = run.freq_before(0)
+    for idx in range(beam.element, len(elements)):
+        cutoff = run.through_element(beam, idx, cutoff)
+    run.to_detector(beam, cutoff)
 
-    snapshot: Optional[Snapshot] = None
-    for el_idx, el in enumerate(elements[start_element_idx:], start=start_element_idx):
-        element_start_slice = resume_slice if el_idx == start_element_idx else 0
-        if element_start_slice == 0 and current_z < el.z_start:
-            logger.info(f"Propagating from z {current_z}m to {el.z_start}m")
-            gap_dz = params.effective_slice_dz(current_z, el.z_start - current_z)
-            propagate_with_history_2d(u, U, params, gap_dz, cutoff_freq, current_z, False, history)
-            u_fourier_valid = True
-        if element_start_slice == 0:
-            current_z = el.z_start
-        if len(el.x_positions) > 1 and snapshot is None:
-            snapshot = Snapshot(z=current_z, element_idx=el_idx, u=u.copy(),
-                                nr_phase_steps=len(el.x_positions))
-        cutoff_freq = cutoff_frequencies[el_idx + 1]
-        save_keypoint(save_keypoints_path, el_idx, 0, u)
-        if element_start_slice > 0:
-            setattr(el, "_checkpoint_start_slice", element_start_slice)
-        if checkpoint is not None:
-            def checkpoint_slice(next_slice: int, slice_z: float, el_idx=el_idx) -> None:
-                if next_slice % CHECKPOINT_SLICE_CADENCE != 0:
-                    return
-                checkpoint.save(u, U, current_z=slice_z, element_index=el_idx,
-                                slice_index=next_slice, u_fourier_valid=True,
-                                stage="element_slice")
-            setattr(el, "_checkpoint_callback", checkpoint_slice)
-            setattr(el, "_checkpoint_cadence", CHECKPOINT_SLICE_CADENCE)
-        try:
-            el.apply(u, U, params, cutoff_freq, 0, history[0] if history is not None else None)
-        finally:
-            for attr in ("_checkpoint_start_slice", "_checkpoint_callback", "_checkpoint_cadence"):
-                if hasattr(el, attr):
-                    delattr(el, attr)
-        save_keypoint(save_keypoints_path, el_idx, 1, u)
-        current_z = el.z_start + el.get_thickness()
-        u_fourier_valid = True
-        resume_slice = 0
-        if checkpoint is not None:
-            checkpoint.save(u, U, current_z=current_z, element_index=el_idx + 1, slice_index=0,
-                            u_fourier_valid=True, stage="element_complete")
-
-    if current_z < params.z_detector - z_tolerance:
-        logger.info(f"Propagating from z {current_z}m to {params.z_detector}m")
-        propagate_with_history_2d(
-            u, U, params, params.effective_final_dz(current_z), cutoff_freq, current_z,
-            skip_fft=u_fourier_valid,
-            history=None if params.use_fresnel_scaling else history)
-        current_z = params.z_detector
-        u_fourier_valid = True
-    if checkpoint is not None:
-        checkpoint.save(u, U, current_z=current_z, element_index=len(elements), slice_index=0,
-                        u_fourier_valid=u_fourier_valid, stage="detector_ready")
-
-    detector_outputs = [square_and_downsample_2d(u, params, params.z_detector)]
+    images = [run.image()]
     save_u(u, 0, sub_dir, vectors_dir, save_final_u_vectors)
-
-    # remaining phase steps replay from the snapshot, without history
-    if snapshot is not None:
-        el_start_idx = snapshot.element_idx
-        for step in range(1, snapshot.nr_phase_steps):
-            current_z = snapshot.z
-            snapshot.u.copy_to(u)
-            cutoff_freq = cutoff_frequencies[el_start_idx]
-            for el_idx, el in enumerate(elements[el_start_idx:], start=el_start_idx):
-                if current_z < el.z_start:
-                    gap_dz = params.effective_slice_dz(current_z, el.z_start - current_z)
-                    propagate_2d(u, U, params.dx, params.get_dy(), params.wl, gap_dz,
-                                 params.chunk_size, cutoff_freq, params.nx, params.ny)
-                current_z = el.z_start
-                cutoff_freq = cutoff_frequencies[el_idx + 1]
-                el_step = step if len(el.x_positions) > 1 else 0
-                el.apply(u, U, params, cutoff_freq, el_step, None)
-                current_z = el.z_start + el.get_thickness()
-            if current_z < params.z_detector - z_tolerance:
-                propagate_2d(u, U, params.dx, params.get_dy(), params.wl,
-                             params.effective_final_dz(current_z), params.chunk_size,
-                             cutoff_freq, params.nx, params.ny)
-            detector_outputs.append(square_and_downsample_2d(u, params, params.z_detector))
-            save_u(u, step, sub_dir, vectors_dir, save_final_u_vectors)
-        snapshot.u.drop()
-
-    return detector_outputs
+    if run.snapshot is not None:
+        try:
+            for step in range(1, run.snapshot.nr_phase_steps):
+                images.append(run.replay_step(step))
+                save_u(u, step, sub_dir, vectors_dir, save_final_u_vectors)
+        finally:
+            run.snapshot.u.drop()
+    return images

@@ -69,14 +69,41 @@ int rsim_memset(void* dptr, int value, size_t bytes, void* stream);
 int rsim_synchronize(void* stream);
 int rsim_stream_create(void** stream);
 int rsim_stream_destroy(void* stream);
-/* Release cached FFT plans, tables and workspaces of the current device. */
+/* Release cached FFT plans, tables and workspaces of the current device.
+ * Batches created before the call return RSIM_E_INVALID_ARG from
+ * rsim_batch_run afterwards (their graphs point into the freed tables). */
 int rsim_release_caches(void);
+/* Bytes and entries held by the bounded device table caches (axis maps and
+ * chirp tables of every device; at most 256 MiB per device, least recently
+ * used entries that no batch pins are freed first). */
+int rsim_cache_info(size_t* bytes, int* entries);
 
-/* Cooperative cancel + progress (big-wave/propagation.py:669-688,
- * big-fourier/src/lib.rs:132-187): the multislice loop polls the flag between
- * slices and returns RSIM_E_CANCELLED; progress is reported as
- * (stage, completed, total) after each slice. */
+/* Cooperative cancel + progress (big-wave/propagation.py:669-688 detector
+ * tokens, big-fourier/src/lib.rs:132-187 bfpy's progress callback and cancel
+ * token, whose behaviour this follows): a slice loop polls for cancel at every
+ * slice boundary and returns RSIM_E_CANCELLED with the field valid after the
+ * slices it finished (U then does not hold FFT2(u)); progress is reported as
+ * (stage, completed, total) after a slice has finished on the GPU.
+ *
+ * Per run: an rsim_control bound to the calling thread with
+ * rsim_control_bind serves every library call that thread makes until it is
+ * unbound (bind NULL) or destroyed; runs on other threads are not affected.
+ * rsim_control_request_cancel may be called from any thread.  A progress
+ * callback is called on the bound thread after every `every`-th slice and
+ * after the last; each report waits for the GPU (hipStreamSynchronize), so a
+ * large `every` keeps the slice loop asynchronous.  rsim_control_completed
+ * returns the slices the last slice loop under the control finished, and its
+ * length.  Threads with no bound control use the process-wide flag and
+ * callback below (rsim_set_progress reports every slice). */
 typedef void (*rsim_progress_fn)(const char* stage, int64_t completed, int64_t total, void* user);
+typedef struct rsim_control rsim_control;
+int rsim_control_create(rsim_control** ctl);
+int rsim_control_destroy(rsim_control* ctl);
+int rsim_control_set_progress(rsim_control* ctl, rsim_progress_fn fn, void* user, int every);
+int rsim_control_request_cancel(rsim_control* ctl);
+int rsim_control_clear_cancel(rsim_control* ctl);
+int rsim_control_bind(rsim_control* ctl);
+int rsim_control_completed(const rsim_control* ctl, int64_t* completed, int64_t* total);
 int rsim_set_progress(rsim_progress_fn fn, void* user);
 int rsim_request_cancel(void);
 int rsim_clear_cancel(void);
@@ -264,7 +291,9 @@ int rsim_plasma_apply_2d_history(void* u, void* U, int nx, int ny, int dtype, do
  * written to the instances' own u / U buffers, which must stay allocated for
  * the batch's life.  Sizes: the fused engine's (ny <= 8192).  flags: the
  * arithmetic flags of rsim_sample_apply_2d (default: FP32 up to 256 slices).
- * rsim_release_caches invalidates existing batches. */
+ * The cached tables the graph reads are pinned until rsim_batch_destroy;
+ * rsim_release_caches invalidates existing batches (rsim_batch_run then
+ * returns RSIM_E_INVALID_ARG).  Volumes must outlive the batches using them. */
 typedef struct rsim_batch rsim_batch;
 typedef struct {
   void* u;

@@ -1,0 +1,212 @@
+"""Per-run cancel and progress, and the bounds of the device table caches.
+
+Cancel follows big-wave's cooperative contract (big-wave/propagation.py:
+669-688; bfpy's cancel test tests/big_wave_2d/test_p2.py:179-212): a run
+stopped at a slice boundary raises InterruptedError and leaves the field
+valid after the slices it finished, which is checked against the FP64 oracle
+after exactly that many slices (complex128 plain 1e-12, complex64 fused
+3e-6).  Controls are per thread: a run that cancels itself does not stop a
+run on another thread.  A batch captured before rsim_release_caches is
+refused afterwards, and cone-beam runs leave no per-slice tables behind.
+"""
+
+import ctypes as C
+import importlib
+import sys
+import threading
+
+import numpy as np
+import pytest
+
+import oracle as O
+from _rs import rs
+
+pytestmark = pytest.mark.gpu
+lib = rs._lib
+CTL = importlib.import_module("3d_ravesim_amd._control")
+DB3 = np.array([0, 3.5e-7 + 2e-10j, 1.1e-6 + 3e-9j, 2.6e-7 + 1.5e-10j]) * 30
+N, NZ, DX, WL, PZ = 256, 10, 0.5e-6, 0.5e-10, 0.5e-6
+CUT = 0.4 / DX
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def volume_grid(seed=3):
+    rng = np.random.default_rng(seed)
+    g = np.zeros((NZ, N, N), np.uint8)
+    yy, xx = np.mgrid[0:N, 0:N]
+    for k in range(NZ):
+        for _ in range(3):
+            cy, cx, r = rng.uniform(40, 216), rng.uniform(40, 216), rng.uniform(10, 40)
+            g[k][(yy - cy) ** 2 + (xx - cx) ** 2 < r * r] = rng.integers(1, 4)
+    return g
+
+
+def start_field(seed=4):
+    rng = np.random.default_rng(seed)
+    return np.exp(1j * rng.uniform(0, 0.2, (N, N)))
+
+
+def oracle_after(grid, u0, k):
+    spec = O.SampleSpec(grid=grid[:k], db_list=DB3, pixel_size_x=DX, pixel_size_y=DX,
+                        pixel_size_z=PZ)
+    return O.sample_apply_2d(u0, spec, DX, DX, WL, CUT)[0]
+
+
+def run_slices(u, U, vol, engine, stream=None):
+    lib.call("rsim_sample_apply_2d", u.ptr, U.ptr, N, N, u.code, DX, DX, WL, CUT, vol.handle,
+             DX, DX, PZ, 0.0, 0.0, 0, NZ, engine, 0, stream)
+
+
+@pytest.mark.parametrize("engine,dt,tol", [(lib.RSIM_PLAIN, np.complex128, 1e-12),
+                                           (lib.RSIM_FUSED, np.complex64, 3e-6)])
+@pytest.mark.parametrize("k", [1, 4])
+def test_cancel_after_k_slices_leaves_the_field_after_k(engine, dt, tol, k):
+    say("cancel", engine, np.dtype(dt).name, k)
+    grid, u0 = volume_grid(), start_field()
+    vol = rs.DeviceVolume.from_numpy(grid, DB3)
+    u = rs.DeviceVector.from_numpy(u0.astype(dt).reshape(-1))
+    U = rs.DeviceVector(N * N, dt)
+    seen = []
+
+    def progress(stage, done, total):
+        seen.append((stage, done, total))
+        if done == k:
+            ctl.cancel()
+
+    ctl = CTL.RunControl(progress)
+    with ctl:
+        with pytest.raises(InterruptedError, match=f"after {k} of {NZ}"):
+            run_slices(u, U, vol, engine)
+    assert ctl.completed == k
+    assert seen == [("sample_slice", j, NZ) for j in range(1, k + 1)]
+    err = O.relative_l2(u.to_numpy().reshape(N, N), oracle_after(grid, u0, k))
+    ctl.close()
+    say("  field after", k, "slices vs oracle", err)
+    assert err < tol
+
+
+def test_controls_are_per_thread():
+    """Thread A cancels itself after 3 slices; thread B, running at the same
+    time on its own stream with its own control, completes all slices."""
+    say("two threads")
+    grid, u0 = volume_grid(), start_field()
+    vol = rs.DeviceVolume.from_numpy(grid, DB3)
+    fields = {t: (rs.DeviceVector.from_numpy(u0.astype(np.complex64).reshape(-1)),
+                  rs.DeviceVector(N * N, np.complex64)) for t in "AB"}
+    outcome = {}
+    barrier = threading.Barrier(2)
+
+    def worker(tag, cancel_at):
+        st = C.c_void_p()
+        lib.call("rsim_stream_create", C.byref(st))
+        ctl = CTL.RunControl(lambda s, d, t: ctl.cancel() if d == cancel_at else None)
+        try:
+            barrier.wait()
+            with ctl:
+                run_slices(*fields[tag], vol, lib.RSIM_FUSED, st.value)
+            outcome[tag] = ("done", ctl.completed)
+        except InterruptedError:
+            outcome[tag] = ("cancelled", ctl.completed)
+        finally:
+            lib.call("rsim_synchronize", st.value)
+            lib.call("rsim_stream_destroy", st.value)
+            ctl.close()
+
+    threads = [threading.Thread(target=worker, args=("A", 3)),
+               threading.Thread(target=worker, args=("B", -1))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert outcome == {"A": ("cancelled", 3), "B": ("done", NZ)}
+    for tag, k in (("A", 3), ("B", NZ)):
+        err = O.relative_l2(fields[tag][0].to_numpy().reshape(N, N), oracle_after(grid, u0, k))
+        assert err < 3e-6, (tag, err)
+
+
+def test_progress_interval_and_python_token():
+    """progress_every = 4 reports slices 4, 8 and the last; a threading.Event
+    on the field stops Sample.apply with InterruptedError."""
+    say("progress interval + token")
+    grid, u0 = volume_grid(), start_field()
+    p = rs.SimParams(N=N * N, ny=N, dx=DX, z_detector=1.0, detector_size=N * DX,
+                     detector_pixel_size_x=4 * DX, detector_pixel_size_y=4 * DX, wl=WL,
+                     chunk_size=N)
+    table = [(rs.Material(str(i), 1.0), DB3[i + 1]) for i in range(3)]
+    sample = rs.Sample(z_start=0.5, pixel_size_x=DX, pixel_size_z=PZ, grid=grid,
+                       materials=[m for m, _ in table], x_positions=np.array([0.0]),
+                       engine="plain")
+    sample.store_deltabetas(table)
+    u = rs.DeviceVector.from_numpy(u0.reshape(-1))
+    U = rs.DeviceVector(N * N, np.complex128)
+    seen = []
+    u.progress_cb = lambda s, d, t: seen.append(d)
+    u.progress_every = 4
+    sample.apply(u, U, p, CUT, 0, None)
+    assert seen == [4, 8, NZ]
+    err = O.relative_l2(u.to_numpy().reshape(N, N), oracle_after(grid, u0, NZ))
+    assert err < 1e-12
+    token = threading.Event()
+    u.upload(u0.reshape(-1))
+    u.progress_every = 1
+    u.progress_cb = lambda s, d, t: token.set() if d == 6 else None
+    u.cancel_token = token
+    with pytest.raises(InterruptedError, match="after 6 of"):
+        sample.apply(u, U, p, CUT, 0, None)
+    err = O.relative_l2(u.to_numpy().reshape(N, N), oracle_after(grid, u0, 6))
+    assert err < 1e-12
+
+
+def test_release_caches_invalidates_batches():
+    say("batch after release")
+    grid = volume_grid()
+    vol = rs.DeviceVolume.from_numpy(grid, DB3)
+    fields = [(rs.DeviceVector.from_numpy(start_field(s).astype(np.complex64).reshape(-1)),
+               rs.DeviceVector(N * N, np.complex64)) for s in (5, 6)]
+    inst = (lib.BatchInstance * 2)(*[
+        lib.BatchInstance(u.ptr, U.ptr, N, N, DX, DX, WL, CUT, vol.handle, DX, DX, PZ, 0.0, 0.0)
+        for u, U in fields])
+    b = C.c_void_p()
+    lib.call("rsim_batch_create", C.byref(b), inst, 2, 0, 2, 0, None)
+    lib.call("rsim_batch_run", b.value, None)
+    lib.call("rsim_synchronize", None)
+    lib.call("rsim_release_caches")
+    with pytest.raises(ValueError, match="release"):
+        lib.call("rsim_batch_run", b.value, None)
+    lib.call("rsim_batch_destroy", b.value)
+
+
+def test_cone_beam_runs_leave_no_per_slice_tables():
+    """Each cone-beam slice has its own distance and scale; the fused engine
+    keeps those tables for the run only, so repeated runs at new
+    wavelengths add a few cached tables per wavelength, not one set per slice."""
+    say("cone-beam table bound")
+    grid, u0 = volume_grid(), start_field()
+    vol = rs.DeviceVolume.from_numpy(grid, DB3)
+    u = rs.DeviceVector.from_numpy(u0.astype(np.complex64).reshape(-1))
+    U = rs.DeviceVector(N * N, np.complex64)
+    scales = np.linspace(1.001, 1.02, NZ)
+    dzs = PZ / (scales * (scales + 0.001))
+
+    def info():
+        b, e = C.c_size_t(0), C.c_int(0)
+        lib.call("rsim_cache_info", C.byref(b), C.byref(e))
+        return b.value, e.value
+
+    lib.call("rsim_release_caches")
+    counts = []
+    for j, wl in enumerate((0.5e-10, 0.51e-10, 0.52e-10)):
+        lib.call("rsim_sample_apply_2d_scaled", u.ptr, U.ptr, N, N, u.code, DX, DX, wl, CUT,
+                 vol.handle, DX, DX, PZ, 0.0, 0.0, 0, NZ, scales.ctypes.data_as(lib._dp),
+                 dzs.ctypes.data_as(lib._dp), lib.RSIM_FUSED, 0, None)
+        lib.call("rsim_synchronize", None)
+        counts.append(info())
+    say("  cache after each wavelength", counts)
+    growth = counts[2][1] - counts[1][1]
+    assert growth <= 3, counts
+    assert counts[2][0] < 4 * 1024 ** 2
+    assert np.isfinite(u.to_numpy()).all()

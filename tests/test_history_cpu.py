@@ -2,7 +2,7 @@
 
 Mirrors the reference's own P4 tests (tests/big_wave_2d/test_p4.py:102-290):
 canonical (z, y, x) order, ROI and downsampling, frame limit and
-cancellation, the checksummed checkpoint manifest, and a slice-level resume
+cancellation, the disk spill of frames, the checksummed checkpoint file, and a slice-level resume
 that replays neither the source nor the finished slices.  The field vectors
 are host stand-ins with the DeviceVector transfer methods, so no GPU work
 runs here.  Also pins the oracle's history helpers to hand-derived values.
@@ -107,10 +107,38 @@ def test_max_frames_and_cancel_are_hard_gates():
         H.History(cancel_token=True).push(np.ones((2, 2)), 0.1)
 
 
-def test_streaming_needs_h5py(tmp_path):
-    h = H.History(tmp_path / "history.h5", memory_budget_bytes=0)
-    with pytest.raises(NotImplementedError, match="h5py"):
-        h.push(np.ones((2, 2)), 0.1)
+def test_budget_overflow_spills_to_disk_and_resumes(tmp_path):
+    """A zero budget spills from the first frame (big-wave streams at that
+    point); frames read back equal, and a resumed History appends to them."""
+    path = tmp_path / "history"
+    h = H.History(path, memory_budget_bytes=0, downsample=2)
+    frames = [np.arange(36, dtype=np.float64).reshape(6, 6) + 100 * k for k in range(3)]
+    for k, f in enumerate(frames):
+        h.push(f, 0.1 * (k + 1))
+    assert h.is_streaming and h.metadata()["storage"] == "spill"
+    np.testing.assert_array_equal(h.get_history(), np.stack([f[::2, ::2] for f in frames]))
+    h.close()
+    again = H.History(path, memory_budget_bytes=0, downsample=2, resume=True)
+    assert len(again) == 3 and again.get_z() == pytest.approx([0.1, 0.2, 0.3])
+    again.push(frames[0] - 1, 0.4)
+    got = again.get_history()
+    assert got.shape == (4, 3, 3) and got[3, 0, 0] == -1
+    again.remove_spill()
+    assert not any(tmp_path.iterdir())
+
+
+def test_positive_budget_spills_only_when_exceeded(tmp_path):
+    h = H.History(tmp_path / "history", memory_budget_bytes=2 * 4 * 8)
+    h.push(np.ones((2, 2)), 0.1)
+    h.push(np.ones((2, 2)) * 2, 0.2)
+    assert not h.is_streaming
+    h.push(np.ones((2, 2)) * 3, 0.3)
+    assert h.is_streaming
+    np.testing.assert_array_equal(h.get_history()[:, 0, 0], [1, 2, 3])
+    # without a storage path everything stays in memory
+    m = H.History(memory_budget_bytes=0)
+    m.push(np.ones((2, 2)), 0.1)
+    assert not m.is_streaming
 
 
 def test_legacy_axis_migration_does_not_overwrite_source(tmp_path):
@@ -126,28 +154,71 @@ def test_legacy_axis_migration_does_not_overwrite_source(tmp_path):
     np.testing.assert_array_equal(migrated.values, np.transpose(legacy, (2, 0, 1)))
 
 
-def test_checksum_manifest_restores_both_vectors(tmp_path):
+def test_checkpoint_restores_both_vectors_and_rejects_damage(tmp_path):
     ck = CK.RunCheckpoint(tmp_path, {"N": 8, "dtype": np.dtype(np.complex64).str})
     u = HostVec(np.arange(8, dtype=np.complex64))
     U = HostVec((np.arange(8) * 3j).astype(np.complex64))
+    stages = []
+    ck.progress_cb = lambda *a: stages.append(a)
     ck.save(u, U, current_z=2.5, element_index=1, slice_index=4, u_fourier_valid=True,
             stage="element_slice")
+    assert stages[-1] == ("checkpoint_commit", 3, 3)
+    assert sorted(p.name for p in tmp_path.iterdir()) == ["run.ckpt"]
     u.vec[:] = 0
     U.vec[:] = 0
     st = ck.load(u, U)
     np.testing.assert_array_equal(u.vec, np.arange(8, dtype=np.complex64))
     np.testing.assert_array_equal(U.vec, (np.arange(8) * 3j).astype(np.complex64))
-    assert (st.element_index, st.slice_index) == (1, 4)
-    manifest = json.loads((tmp_path / "manifest.json").read_text())
-    assert manifest["vectors"]["u"]["file"].lower() != manifest["vectors"]["U"]["file"].lower()
-    with (tmp_path / manifest["vectors"]["u"]["file"]).open("r+b") as f:
-        f.seek(-1, 2)
-        f.write(b"x")
+    assert (st.current_z, st.element_index, st.slice_index, st.u_fourier_valid,
+            st.stage) == (2.5, 1, 4, True, "element_slice")
+    # a flipped payload byte (inside U) fails its digest
+    raw = bytearray((tmp_path / "run.ckpt").read_bytes())
+    raw[8 * 8 + 5] ^= 0xFF
+    (tmp_path / "run.ckpt").write_bytes(bytes(raw))
     with pytest.raises(ValueError, match="checksum"):
         ck.load(u, U)
     other = CK.RunCheckpoint(tmp_path, {"N": 8, "dtype": np.dtype(np.complex128).str})
     with pytest.raises(ValueError, match="fingerprint"):
         other.load(u, U)
+    # a truncated file is not mistaken for a checkpoint
+    (tmp_path / "run.ckpt").write_bytes(bytes(raw[:40]))
+    with pytest.raises(ValueError):
+        ck.load(u, U)
+
+
+def test_checkpoint_cancel_complete_and_atomic_replace(tmp_path):
+    ck = CK.RunCheckpoint(tmp_path, {"N": 4, "dtype": "<c16"})
+    a = HostVec(np.ones(4, np.complex128))
+    ck.save(a, a, current_z=1.0, element_index=0, slice_index=0, u_fourier_valid=False,
+            stage="source_propagated")
+    before = (tmp_path / "run.ckpt").read_bytes()
+    ck.cancel_token = lambda: True
+    with pytest.raises(InterruptedError):
+        ck.save(a, a, current_z=2.0, element_index=1, slice_index=0, u_fourier_valid=True,
+                stage="element_complete")
+    # the interrupted save left the previous checkpoint untouched, no partial file
+    assert (tmp_path / "run.ckpt").read_bytes() == before
+    assert sorted(p.name for p in tmp_path.iterdir()) == ["run.ckpt"]
+    ck.cancel_token = None
+    ck.mark_complete()
+    assert not ck.exists
+    with pytest.raises(ValueError, match="no ready checkpoint"):
+        ck.load(a, a)
+
+
+def test_cancel_token_forms():
+    from importlib import import_module
+    ctl = import_module("3d_ravesim_amd._control")
+    import threading
+    ev = threading.Event()
+    assert not ctl.token_is_set(None) and ctl.token_is_set(True) and not ctl.token_is_set(False)
+    assert not ctl.token_is_set(ev)
+    ev.set()
+    assert ctl.token_is_set(ev) and ctl.token_is_set(lambda: 1)
+
+    class Flag:
+        cancelled = True
+    assert ctl.token_is_set(Flag())
 
 
 class DummySource:
