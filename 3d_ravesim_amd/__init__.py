@@ -36,6 +36,8 @@ from .source import PointSource
 from .wavesim import run_simulation
 from .multisim import compute_fresnel_cutoff_geometry, fresnel_mode_name, validate_fresnel_mode
 from . import config
+from . import graph
+from . import spectral
 
 __all__ = [
     "Vector", "DeviceVector", "SimParams", "apply_frequency_cutoff_2d", "convert_energy_wavelength",
@@ -43,7 +45,7 @@ __all__ = [
     "propagate_analytically_2d", "require_row_aligned_chunk_size", "square_and_downsample_2d",
     "DeviceVolume", "Material", "OpticalElement", "Sample", "extract_from_deltabeta_table",
     "material_factor", "DevicePlasma", "plasma_delta_beta_grid_multi", "PlasmaSample", "PointSource", "run_simulation", "compute_fresnel_cutoff_geometry",
-    "fresnel_mode_name", "validate_fresnel_mode", "config",
+    "fresnel_mode_name", "validate_fresnel_mode", "config", "graph", "spectral",
 ]
 
 

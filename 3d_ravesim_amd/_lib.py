@@ -73,6 +73,12 @@ SIGNATURES = {
     "rsim_stream_destroy": [_vp],
     "rsim_release_caches": [],
     "rsim_cache_info": [C.POINTER(_sz), _ip],
+    "rsim_graph_begin": [_vp],
+    "rsim_graph_end": [_vp, C.POINTER(_vp)],
+    "rsim_graph_launch": [_vp, _vp],
+    "rsim_graph_info": [_vp, C.POINTER(_sz)],
+    "rsim_graph_destroy": [_vp],
+    "rsim_fill_zero": [_vp, _sz, _vp],
     "rsim_kernel_timing": [_i],
     "rsim_kernel_timing_read": [_dp, _ip],
     "rsim_set_progress": [PROGRESS_FN, _vp],

@@ -55,6 +55,7 @@ int release_detector_plans();
 int release_staging();
 int release_volume_tables(unsigned long long serial);
 int fused_schedule_lookup(int nx, int ny, int* rv, int* cv);
+int fill_zero(void* p, size_t bytes, hipStream_t s);
 
 namespace {
 thread_local std::string t_err;
@@ -109,8 +110,12 @@ int current_device() {
   return d;
 }
 
+hipStream_t capture_stream();
+
 hipStream_t resolve_stream(void* s) {
   if (s) return (hipStream_t)s;
+  // inside a capture, calls without a stream go to the capturing stream
+  if (hipStream_t cs = capture_stream()) return cs;
   std::lock_guard<std::mutex> lk(g_mu);
   const int dev = current_device();
   auto it = g_streams.find(dev);
@@ -221,7 +226,47 @@ int ktime_read(double* ms3, int* n3) {
   return RSIM_OK;
 }
 
+// ---- graph capture ----
+namespace {
+struct CaptureState {
+  hipStream_t s = nullptr;
+  int device = 0;
+  std::vector<const void*> used;   // cached tables the captured work reads
+  std::vector<void*> owned;        // scratch the graph owns
+};
+thread_local CaptureState* t_cap = nullptr;
+}  // namespace
+
+bool capture_active() { return t_cap != nullptr; }
+hipStream_t capture_stream() { return t_cap ? t_cap->s : nullptr; }
+
+int capture_alloc(size_t bytes, void** p) {
+  if (!t_cap) return fail(RSIM_E_INVALID_ARG, "capture_alloc outside a capture");
+  *p = nullptr;
+  RSIM_HIP(hipMalloc(p, bytes ? bytes : 1));
+  t_cap->owned.push_back(*p);
+  return RSIM_OK;
+}
+
+int capture_miss(const char* what) {
+  if (!t_cap) return RSIM_OK;
+  return fail(RSIM_E_UNSUPPORTED,
+              "graph capture: %s are not built yet; run the same calls once before "
+              "rsim_graph_begin so that everything the graph reads exists", what);
+}
+
 }  // namespace rsim
+
+// A captured sequence of library calls (rsim_graph_begin .. rsim_graph_end).
+struct rsim_graph {
+  int device = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  size_t nodes = 0;
+  unsigned long long generation = 0;
+  std::vector<const void*> pinned;
+  std::vector<void*> owned;
+};
 
 // A run control: the cancel flag, progress callback and slice count of the
 // runs started on the thread it is bound to.
@@ -352,6 +397,8 @@ int rsim_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
 int rsim_memset(void* dptr, int value, size_t bytes, void* stream) {
   if (bytes == 0) return RSIM_OK;
   if (!dptr) return fail(RSIM_E_INVALID_ARG, "dptr is NULL");
+  if (capture_active())
+    return fail(RSIM_E_UNSUPPORTED, "rsim_memset inside a graph capture; use rsim_fill_zero");
   RSIM_HIP(hipMemsetAsync(dptr, value, bytes, resolve_stream(stream)));
   return RSIM_OK;
 }
@@ -407,6 +454,97 @@ int rsim_request_cancel(void) {
 int rsim_clear_cancel(void) {
   set_cancel(0);
   return RSIM_OK;
+}
+
+int rsim_graph_begin(void* stream) {
+  if (t_cap) return fail(RSIM_E_INVALID_ARG, "a graph capture is already open on this thread");
+  hipStream_t s = resolve_stream(stream);
+  if (!s) return fail(RSIM_E_HIP, "no stream");
+  // the work queued on the stream before the capture is not part of it
+  RSIM_HIP(hipStreamSynchronize(s));
+  RSIM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+  t_cap = new CaptureState();
+  t_cap->s = s;
+  t_cap->device = current_device();
+  table_use_log(&t_cap->used);
+  ktime_suspend(true);
+  return RSIM_OK;
+}
+
+int rsim_graph_end(void* stream, rsim_graph** out) {
+  if (!out) return fail(RSIM_E_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  if (!t_cap) return fail(RSIM_E_INVALID_ARG, "no graph capture is open on this thread");
+  hipStream_t s = resolve_stream(stream);
+  if (s != t_cap->s)
+    return fail(RSIM_E_INVALID_ARG, "rsim_graph_end on another stream than rsim_graph_begin");
+  CaptureState* cap = t_cap;
+  t_cap = nullptr;
+  table_use_log(nullptr);
+  ktime_suspend(false);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &graph);
+  auto drop_owned = [&] {
+    for (void* p : cap->owned) (void)hipFree(p);
+  };
+  if (e != hipSuccess || !graph) {
+    (void)hipGetLastError();
+    if (graph) hipGraphDestroy(graph);
+    drop_owned();
+    delete cap;
+    return fail(RSIM_E_HIP, "graph capture failed: %s (a call inside it synchronised, "
+                "allocated or failed)", hipGetErrorString(e));
+  }
+  rsim_graph* g = new rsim_graph();
+  g->device = cap->device;
+  g->graph = graph;
+  g->owned = std::move(cap->owned);
+  g->generation = cache_generation();
+  for (const void* p : cap->used)
+    if (table_pin(p, +1)) g->pinned.push_back(p);
+  delete cap;
+  (void)hipGraphGetNodes(graph, nullptr, &g->nodes);
+  const hipError_t ei = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    rsim_graph_destroy(g);
+    return fail(RSIM_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+  }
+  *out = g;
+  return RSIM_OK;
+}
+
+int rsim_graph_launch(rsim_graph* g, void* stream) {
+  if (!g) return fail(RSIM_E_INVALID_ARG, "graph is NULL");
+  if (t_cap) return fail(RSIM_E_INVALID_ARG, "rsim_graph_launch inside a capture");
+  if (current_device() != g->device)
+    return fail(RSIM_E_INVALID_ARG, "graph belongs to device %d", g->device);
+  if (g->generation != cache_generation())
+    return fail(RSIM_E_INVALID_ARG,
+                "graph was captured before rsim_release_caches freed its tables; capture it again");
+  RSIM_HIP(hipGraphLaunch(g->exec, resolve_stream(stream)));
+  return RSIM_OK;
+}
+
+int rsim_graph_info(const rsim_graph* g, size_t* nodes) {
+  if (!g) return fail(RSIM_E_INVALID_ARG, "graph is NULL");
+  if (nodes) *nodes = g->nodes;
+  return RSIM_OK;
+}
+
+int rsim_graph_destroy(rsim_graph* g) {
+  if (!g) return RSIM_OK;
+  if (g->generation == cache_generation())
+    for (const void* p : g->pinned) table_pin(p, -1);
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  for (void* p : g->owned) (void)hipFree(p);
+  delete g;
+  return RSIM_OK;
+}
+
+int rsim_fill_zero(void* dptr, size_t bytes, void* stream) {
+  if (!dptr && bytes) return fail(RSIM_E_INVALID_ARG, "dptr is NULL");
+  return fill_zero(dptr, bytes, resolve_stream(stream));
 }
 
 int rsim_cache_info(size_t* bytes, int* entries) {

@@ -27,8 +27,10 @@ struct Tables {
 
 std::mutex g_mu;
 std::map<std::tuple<int, int, int>, Tables> g_tables;  // (device, dtype, N)
-// scratch field of the out-of-LDS schedule (per device, grown on demand)
-std::map<int, std::pair<void*, size_t>> g_scratch;
+// scratch field of the out-of-LDS schedule, one per (device, stream) so that
+// transforms on different streams never share it (grown on demand, freed by
+// rsim_release_caches)
+std::map<std::pair<int, hipStream_t>, std::pair<void*, size_t>> g_scratch;
 
 // exp(-2*pi*i*m/M) with the angle reduced to the first octant so the FP64
 // value is correct to a few ulp for every m.
@@ -115,6 +117,7 @@ int get_tables(int dtype, int N, Tables** out) {
     *out = &it->second;
     return RSIM_OK;
   }
+  RSIM_CHECK(capture_miss("FFT twiddle tables"));
   Tables t;
   RSIM_CHECK(upload_table(dtype, N, &t.full));
   SchedInfo si;
@@ -187,10 +190,13 @@ int rsim_release_tables() {
       ++it;
     }
   }
-  auto sc = g_scratch.find(dev);
-  if (sc != g_scratch.end()) {
-    hipFree(sc->second.first);
-    g_scratch.erase(sc);
+  for (auto sc = g_scratch.begin(); sc != g_scratch.end();) {
+    if (sc->first.first == dev) {
+      hipFree(sc->second.first);
+      sc = g_scratch.erase(sc);
+    } else {
+      ++sc;
+    }
   }
   return RSIM_OK;
 }
@@ -253,8 +259,9 @@ int launch_lds(int dtype, int dir, const void* in, void* out, const AxisDesc& d,
 
 int axis_scratch(size_t bytes, void** out, hipStream_t s) {
   std::lock_guard<std::mutex> lk(g_mu);
-  auto& e = g_scratch[current_device()];
+  auto& e = g_scratch[std::make_pair(current_device(), s)];
   if (e.second < bytes) {
+    RSIM_CHECK(capture_miss("the FFT scratch field of this stream"));
     if (e.first) {
       RSIM_HIP(hipStreamSynchronize(s));
       RSIM_HIP(hipFree(e.first));

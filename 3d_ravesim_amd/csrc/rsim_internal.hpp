@@ -147,6 +147,16 @@ void ktime_end(int kind, hipStream_t s);
 // suspend kernel timing on this thread (while a stream is being captured)
 void ktime_suspend(bool on);
 
+// ---- graph capture (rsim_graph_begin .. rsim_graph_end on this thread) ----
+bool capture_active();
+// device scratch that a captured graph reads on every replay: allocated now,
+// owned by the graph being captured and freed with it
+int capture_alloc(size_t bytes, void** p);
+// RSIM_OK outside a capture; inside, RSIM_E_UNSUPPORTED naming what would
+// have to be built (tables, tuning and scratch are built by running the same
+// calls once before the capture)
+int capture_miss(const char* what);
+
 // ---- cancel / progress (the run control bound to the calling thread, or
 // the process-wide flag and callback when none is bound) ----
 bool cancel_requested();

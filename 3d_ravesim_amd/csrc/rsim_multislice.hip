@@ -190,6 +190,7 @@ int chirp_tables(int nx, int ny, double dx, double dy, double wl, double dz, dou
     *out = &it->second;
     return RSIM_OK;
   }
+  RSIM_CHECK(capture_miss("chirp tables"));
   const size_t nb = chirp_bytes(nx, ny);
   std::vector<unsigned char> host(nb);
   chirp_build(nx, ny, dx, dy, wl, dz, freq, host.data());
@@ -222,6 +223,7 @@ int material_tables(const rsim_volume* vol, double K, const float4** lut, const 
     *dbf = (const float2*)((char*)it->second.dev + 256 * 16);
     return RSIM_OK;
   }
+  RSIM_CHECK(capture_miss("material tables"));
   MatBuf mb;
   RSIM_HIP(hipMalloc(&mb.dev, 256 * (16 + 8)));
   g_mat[key] = mb;
@@ -330,6 +332,7 @@ int fused_schedule(int nx, int ny, int tm, const StageTw<TW>& tx, const StageTw<
     *cv = it->second.second;
     return RSIM_OK;
   }
+  RSIM_CHECK(capture_miss("launch-shape tuning"));
   void* tmp = nullptr;
   const size_t tbytes = (size_t)nx * ny * 8;
   RSIM_HIP(hipMalloc(&tmp, tbytes));
@@ -536,14 +539,12 @@ static int use_maps(TransArgsF& ta, const AxisMap& xm, const AxisMap& ym, bool i
   return xm.all_integral && ym.all_integral && xm.affine && ym.affine ? TM_LUT : TM_BILINEAR;
 }
 
-// pz: slice thickness of the transmission; prop_dz: propagation distance of
-// the chirp (pz, or its cone-beam effective value); scale: voxel-map scale.
+// Tables of a run whose slices all propagate over their thickness pz with an
+// unscaled voxel map (cone-beam runs add per-slice tables of their own).
 template <typename TW>
 static int fused_prepare(int nx, int ny, double dx, double dy, double wl, double freq,
                          const VoxSrc& src, double px, double py, double pz, double xpos,
-                         double ypos, FusedPrep<TW>* P, double prop_dz = 0.0,
-                         double scale = 1.0) {
-  if (prop_dz == 0.0) prop_dz = pz;
+                         double ypos, FusedPrep<TW>* P) {
   const int tdtype = std::is_same<TW, twc>::value ? kTwComp : sizeof(TW) == 8 ? RSIM_C128 : RSIM_C64;
   const void* twx[4];
   const void* twy[4];
@@ -554,11 +555,11 @@ static int fused_prepare(int nx, int ny, double dx, double dy, double wl, double
     P->tx.t[i] = reinterpret_cast<TP>(twx[i]);
     P->ty.t[i] = reinterpret_cast<TP>(twy[i]);
   }
-  RSIM_CHECK(chirp_tables(nx, ny, dx, dy, wl, prop_dz, freq, &P->ct));
+  RSIM_CHECK(chirp_tables(nx, ny, dx, dy, wl, pz, freq, &P->ct));
   const AxisMap* xm = nullptr;
   const AxisMap* ym = nullptr;
-  RSIM_CHECK(axis_map(nx, dx, scale, xpos, src.nx_g, px, &xm));
-  RSIM_CHECK(axis_map(ny, dy, scale, ypos, src.ny_g, py, &ym));
+  RSIM_CHECK(axis_map(nx, dx, 1.0, xpos, src.nx_g, px, &xm));
+  RSIM_CHECK(axis_map(ny, dy, 1.0, ypos, src.ny_g, py, &ym));
   const double K = ((2.0 * M_PI) * pz) / wl;
   TransArgsF& ta = P->ta;
   ta.nxg = src.nx_g;
@@ -612,6 +613,7 @@ static int build_run_slice_tables(int nx, int ny, double dx, double dy, double w
   const size_t cb = (chirp_bytes(nx, ny) + 255) & ~size_t(255);
   const size_t xb = (axis_map_bytes(nx) + 255) & ~size_t(255);
   const size_t yb = (axis_map_bytes(ny) + 255) & ~size_t(255);
+  RSIM_CHECK(capture_miss("cone-beam per-slice tables"));
   const size_t per = cb + xb + yb;
   const size_t total = per * (size_t)count;
   rt->sets.assign(count, SliceTables{});
@@ -716,6 +718,7 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
     ~XsBuf() { if (p) (void)hipFreeAsync(p, s); }
   } xsb;
   if (hist && count > 1) {
+    RSIM_CHECK(capture_miss("history frames of a slice loop"));
     RSIM_CHECK(detector_xbin_plan(nx, ny, dx, dy, hist->ox, hist->oy, hist->dsx, hist->dsy, &xb,
                                   &xbin));
     if (xbin) {

@@ -78,16 +78,19 @@ int chirp_cutoff(void* U, int nx, int ny, int dtype, double dx, double dy, doubl
 // Point source (big-wave/propagation.py:458-484)
 // ---------------------------------------------------------------------------
 template <typename R>
-__global__ void __launch_bounds__(256) point_source_kernel(cpx<R>* u, int nx, int ny,
-                                                           const double* __restrict__ x2,
-                                                           const double* __restrict__ y2,
+__global__ void __launch_bounds__(256) point_source_kernel(cpx<R>* u, int nx, int ny, double dx,
+                                                           double dy, double xs, double ys,
                                                            double z2, double k) {
   const long long n = (long long)nx * ny;
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < n;
        idx += (long long)gridDim.x * blockDim.x) {
     const int y = (int)(idx / nx);
     const int x = (int)(idx - (long long)y * nx);
-    const double r = sqrt(y2[y] + x2[x] + z2);
+    // x_i = (i - nx/2) dx - xs, squared, in the reference's operation order
+    // (no contraction in this file)
+    const double vx = ((double)x - nx / 2.0) * dx - xs;
+    const double vy = ((double)y - ny / 2.0) * dy - ys;
+    const double r = sqrt(vy * vy + vx * vx + z2);
     double s, c;
     sincos(r * k, &s, &c);
     u[idx] = mk((R)(c / r), (R)(s / r));
@@ -133,32 +136,16 @@ int tilted_plane(void* u, int nx, int ny, int dtype, double dx, double dy, doubl
 
 int point_source(void* u, int nx, int ny, int dtype, double dx, double dy, double z, double xs,
                  double ys, double wl, hipStream_t s) {
-  std::vector<double> hx(nx), hy(ny);
-  for (int i = 0; i < nx; ++i) {
-    const double v = ((double)i - nx / 2.0) * dx - xs;
-    hx[i] = v * v;
-  }
-  for (int j = 0; j < ny; ++j) {
-    const double v = ((double)j - ny / 2.0) * dy - ys;
-    hy[j] = v * v;
-  }
-  double* d = nullptr;
-  RSIM_HIP(hipMallocAsync((void**)&d, sizeof(double) * (nx + ny), s));
-  RSIM_HIP(hipMemcpyAsync(d, hx.data(), sizeof(double) * nx, hipMemcpyHostToDevice, s));
-  RSIM_HIP(hipMemcpyAsync(d + nx, hy.data(), sizeof(double) * ny, hipMemcpyHostToDevice, s));
   const double k = (2.0 * M_PI) / wl;
   const long long n = (long long)nx * ny;
   int blocks = (int)std::min<long long>((n + 255) / 256, 65536);
   if (dtype == RSIM_C64)
     hipLaunchKernelGGL(point_source_kernel<float>, dim3(blocks), dim3(256), 0, s,
-                       (cpx<float>*)u, nx, ny, d, d + nx, z * z, k);
+                       (cpx<float>*)u, nx, ny, dx, dy, xs, ys, z * z, k);
   else
     hipLaunchKernelGGL(point_source_kernel<double>, dim3(blocks), dim3(256), 0, s,
-                       (cpx<double>*)u, nx, ny, d, d + nx, z * z, k);
+                       (cpx<double>*)u, nx, ny, dx, dy, xs, ys, z * z, k);
   RSIM_LAUNCHED();
-  RSIM_HIP(hipFreeAsync(d, s));
-  // host vectors must outlive the async copies
-  RSIM_HIP(hipStreamSynchronize(s));
   return RSIM_OK;
 }
 
@@ -482,6 +469,7 @@ int det_plan_dev(int nx, int ny, double dx, double dy, int ox, int oy, double ds
     *out = &it->second;
     return RSIM_OK;
   }
+  RSIM_CHECK(capture_miss("detector plans"));
   DetPlanHost P;
   det_plan(nx, ny, dx, dy, ox, oy, dsx, dsy, &P);
   Blob b = make_blob({&P.xi0, &P.xi1, &P.jlo, &P.jhi},
@@ -534,8 +522,12 @@ int detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double
   // x-integrals (nj x ox doubles); the one-pass kernel when there is none
   double* xs = nullptr;
   const size_t xs_bytes = (size_t)c->nj * ox * 8;
-  if (c->mono && c->nj > 0 && c->nj <= 65535 && oy <= 65535 &&
-      hipMallocAsync((void**)&xs, xs_bytes, s) != hipSuccess) {
+  const bool two_stage = c->mono && c->nj > 0 && c->nj <= 65535 && oy <= 65535;
+  const bool captured = capture_active();
+  if (two_stage && captured) {
+    // the graph replays this detector: its scratch lives as long as the graph
+    RSIM_CHECK(capture_alloc(xs_bytes, (void**)&xs));
+  } else if (two_stage && hipMallocAsync((void**)&xs, xs_bytes, s) != hipSuccess) {
     (void)hipGetLastError();
     xs = nullptr;
   }
@@ -550,7 +542,7 @@ int detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double
     hipLaunchKernelGGL(area_v1_cols_kernel, g2, dim3(256), 0, s, xs, ox, oy, c->j0, c->pl, dy,
                        c->gyl, z, weight, accumulate ? 1 : 0, out_dev);
     const hipError_t e = hipGetLastError();
-    (void)hipFreeAsync(xs, s);
+    if (!captured) (void)hipFreeAsync(xs, s);
     if (e != hipSuccess) return fail(RSIM_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
     return RSIM_OK;
   }
@@ -727,6 +719,31 @@ int paint_spheres(rsim_volume* vol, const double* sph, const int* mats, int n, h
                        r * r, (uint8_t)mats[k]);
     RSIM_LAUNCHED();
   }
+  return RSIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Zero fill as a kernel (a graph replays it like any other launch; graphs
+// here hold no memset nodes)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) zero_fill_kernel(double2* p, long long n2, double* tail,
+                                                        int n_tail) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n2; i += stride)
+    p[i] = make_double2(0.0, 0.0);
+  if (blockIdx.x == 0 && (int)threadIdx.x < n_tail) tail[threadIdx.x] = 0.0;
+}
+
+int fill_zero(void* p, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return RSIM_OK;
+  if ((reinterpret_cast<uintptr_t>(p) & 15) || (bytes & 7))
+    return fail(RSIM_E_INVALID_ARG, "zero fill needs a 16-byte aligned buffer of 8-byte words");
+  const long long n2 = (long long)(bytes / 16);
+  const int n_tail = (int)((bytes % 16) / 8);
+  const int blocks = (int)std::max<long long>(1, std::min<long long>((n2 + 255) / 256, 8192));
+  hipLaunchKernelGGL(zero_fill_kernel, dim3(blocks), dim3(256), 0, s, (double2*)p, n2,
+                     (double*)((char*)p + 16 * n2), n_tail);
+  RSIM_LAUNCHED();
   return RSIM_OK;
 }
 

@@ -116,6 +116,7 @@ int axis_k2(int n, double spacing, const AxisK2** out) {
     *out = &it->second;
     return RSIM_OK;
   }
+  RSIM_CHECK(capture_miss("frequency tables"));
   std::vector<double> h(n);
   const double val = 1.0 / (double)n;   // numpy fftfreq: results * (1/(n*d))
   const int half = (n - 1) / 2 + 1;
@@ -215,6 +216,7 @@ int axis_map(int n, double spacing, double scale, double pos, int grid_len, doub
     *out = &it->second;
     return RSIM_OK;
   }
+  RSIM_CHECK(capture_miss("voxel axis maps"));
   const size_t bytes = axis_map_bytes(n);
   std::vector<unsigned char> host(bytes, 0);
   AxisMap m;

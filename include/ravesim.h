@@ -108,6 +108,32 @@ int rsim_set_progress(rsim_progress_fn fn, void* user);
 int rsim_request_cancel(void);
 int rsim_clear_cancel(void);
 
+/* ---- Captured pipelines (no reference counterpart: fast-wave launches every
+ * kernel from the host, with a host synchronisation per slice,
+ * fast-wave/fwcuda/kernels.cu:365) ----
+ * rsim_graph_begin starts recording the calls this thread makes on `stream`
+ * (NULL: the library's stream; calls made with a NULL stream during the
+ * capture are recorded on the capturing stream); rsim_graph_end turns them
+ * into one graph, replayed by rsim_graph_launch with a single launch.  The
+ * recorded calls must be stream-ordered device work with device pointers
+ * (no host outputs, no rsim_memset -- use rsim_fill_zero) and must have been
+ * made once before with the same arguments, so that every table, detector
+ * plan and launch-shape choice they use exists; a call that would build one
+ * fails with RSIM_E_UNSUPPORTED and rsim_graph_end then fails too.  Buffers
+ * must stay allocated while the graph lives; scratch the graph needs is owned
+ * by it.  rsim_release_caches invalidates graphs (rsim_graph_launch then
+ * returns RSIM_E_INVALID_ARG).  Progress callbacks and cancel do not apply to
+ * replays. */
+typedef struct rsim_graph rsim_graph;
+int rsim_graph_begin(void* stream);
+int rsim_graph_end(void* stream, rsim_graph** graph);
+int rsim_graph_launch(rsim_graph* graph, void* stream);
+int rsim_graph_info(const rsim_graph* graph, size_t* nodes);
+int rsim_graph_destroy(rsim_graph* graph);
+/* Zero `bytes` (a multiple of 8, 16-byte aligned) of device memory with a
+ * kernel, on `stream` (usable inside a capture). */
+int rsim_fill_zero(void* dptr, size_t bytes, void* stream);
+
 /* Kernel timing (the reference's CudaTimerSection, fast-wave/src/simulation.cpp:14-33):
  * every > 0 brackets one launch in `every` of each kind (row pass, column
  * pass, pointwise launch of the FFT and multislice engines) with hipEvents on

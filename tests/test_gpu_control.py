@@ -210,3 +210,27 @@ def test_cone_beam_runs_leave_no_per_slice_tables():
     assert growth <= 3, counts
     assert counts[2][0] < 4 * 1024 ** 2
     assert np.isfinite(u.to_numpy()).all()
+
+
+def test_zero_slice_distance_is_zero_in_both_engines():
+    """slice_dz[k] = 0 propagates over no distance in the plain and the fused
+    engine alike (it is not a stand-in for the slice thickness)."""
+    say("zero slice distance")
+    grid, u0 = volume_grid(), start_field()
+    vol = rs.DeviceVolume.from_numpy(grid, DB3)
+    scales = np.ones(NZ)
+    dzs = np.full(NZ, PZ)
+    dzs[[3, 7]] = 0.0
+    out = {}
+    for engine, dt in ((lib.RSIM_PLAIN, np.complex128), (lib.RSIM_FUSED, np.complex64)):
+        u = rs.DeviceVector.from_numpy(u0.astype(dt).reshape(-1))
+        U = rs.DeviceVector(N * N, dt)
+        lib.call("rsim_sample_apply_2d_scaled", u.ptr, U.ptr, N, N, u.code, DX, DX, WL, CUT,
+                 vol.handle, DX, DX, PZ, 0.0, 0.0, 0, NZ, scales.ctypes.data_as(lib._dp),
+                 dzs.ctypes.data_as(lib._dp), engine, 0, None)
+        out[engine] = u.to_numpy()
+    err = O.relative_l2(out[lib.RSIM_FUSED], out[lib.RSIM_PLAIN])
+    assert err < 3e-6, err
+    # and differs from propagating over the thickness at those slices
+    ref_full = oracle_after(grid, u0, NZ).reshape(-1)
+    assert O.relative_l2(out[lib.RSIM_PLAIN], ref_full) > 1e-6
