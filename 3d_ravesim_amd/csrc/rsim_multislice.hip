@@ -456,8 +456,36 @@ int release_multislice_tables() {
 // Row lengths with a fused row pass; column lengths with a single-pass
 // fused column kernel, or (kBigCol) the three-pass schedule.
 constexpr int kBigCol = 16384;
-// volumes deeper than this default to compensated FP32 butterflies
-constexpr int kFp32MaxSlices = 256;
+// Default butterfly arithmetic.  FP32 stage twiddles have |W| != 1 by up to
+// ~6e-8, and a spectral component meets the same twiddles in every slice, so
+// FP32's error grows linearly with depth; compensated FP32 grows far slower.
+// Detector error of FP32 per slice (relative L2 versus the complex128 engine,
+// 0.5 m leg, area_v1 4 x 4; seeded sphere phantoms at 128 and 256 slices,
+// MI355X), per transform length:
+static double fp32_error_per_slice(int n) {
+  switch (n) {
+    case 256: return 1.94e-8;
+    case 512: return 4.4e-8;
+    case 1024: return 2.2e-8;
+    case 2048: return 1.72e-8;
+    case 3072: return 2.07e-8;
+    case 4096: return 1.56e-8;
+    case 6144: return 1.84e-8;
+    case 8192: return 2.5e-8;
+    case 16384: return 4.0e-8;   // grows faster than linearly beyond 128 slices
+    default: return 4.4e-8;
+  }
+}
+// FP32 is the default while its predicted detector error stays within half
+// the 1e-5 bar; deeper volumes default to compensated FP32.
+static int fp32_depth_limit(int nx, int ny) {
+  const double c = 0.5 * (fp32_error_per_slice(nx) + fp32_error_per_slice(ny));
+  return (int)(5e-6 / c);
+}
+static int default_math(int nx, int ny, int depth, int flags) {
+  if (flags & (RSIM_FLAG_MATH_FP32 | RSIM_FLAG_MATH_FP64 | RSIM_FLAG_MATH_COMP32)) return flags;
+  return depth > fp32_depth_limit(nx, ny) ? (flags | RSIM_FLAG_MATH_COMP32) : flags;
+}
 static bool fused_row_len(int n) {
   return n == 256 || n == 512 || n == 1024 || n == 2048 || n == 3072 || n == 4096 || n == 6144 ||
          n == 8192 || n == 16384;
@@ -833,14 +861,9 @@ int slices_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double 
     return RSIM_OK;
   }
   // ---- fused complex64 engine ----
-  // Butterfly arithmetic.  FP32 with norm-matched inverse constants keeps the
-  // detector error of a 256-slice run near 4e-6, but its residual norm bias
-  // (FP32 stage twiddles) grows faster than linearly with the slice count
-  // (measured on 4096^2 fields: -3.5e-6 after 256 slices, -8e-6 after 512),
-  // so deeper volumes default to compensated FP32.
-  if (!(flags & (RSIM_FLAG_MATH_FP32 | RSIM_FLAG_MATH_FP64 | RSIM_FLAG_MATH_COMP32)) &&
-      src.nz > kFp32MaxSlices)
-    flags |= RSIM_FLAG_MATH_COMP32;
+  // Butterfly arithmetic: FP32 while its error over the volume's depth is
+  // predicted within half the 1e-5 bar, compensated FP32 beyond.
+  flags = default_math(nx, ny, src.nz, flags);
   if (flags & RSIM_FLAG_MATH_COMP32)
     return run_fused<twc>(u, U, nx, ny, dx, dy, wl, freq, src, px, py, pz, xpos, ypos, first,
                           count, want_spec, scales, dzs, s, hist);
@@ -1213,9 +1236,11 @@ static int batch_capture(const rsim_batch_instance* in, int count, int first, in
 
 int batch_create(rsim_batch** out, const rsim_batch_instance* inst, int count, int first, int n,
                  int flags, hipStream_t s) {
-  if (!(flags & (RSIM_FLAG_MATH_FP32 | RSIM_FLAG_MATH_FP64 | RSIM_FLAG_MATH_COMP32)) &&
-      n > kFp32MaxSlices)
-    flags |= RSIM_FLAG_MATH_COMP32;
+  // one arithmetic for the whole graph: compensated as soon as any instance
+  // would need it
+  const int asked = flags;
+  for (int i = 0; i < count; ++i)
+    flags |= default_math(inst[i].nx, inst[i].ny, n, asked) & RSIM_FLAG_MATH_COMP32;
   const bool want_spec = !(flags & RSIM_FLAG_NO_SPECTRUM);
   // the caller's pending work (uploads) completes before tables and tuning
   RSIM_HIP(hipStreamSynchronize(s));

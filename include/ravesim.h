@@ -46,8 +46,10 @@ enum {
    * inverse rotation constants.  RSIM_FLAG_MATH_COMP32: FP32 data, hi+lo
    * compensated constants and twiddles (16 % slower; its error grows as the
    * square root of the slice count instead of linearly).
-   * RSIM_FLAG_MATH_FP64: FP64 butterflies.  Without any of them: FP32 for
-   * volumes of up to 256 slices, COMP32 for deeper ones. */
+   * RSIM_FLAG_MATH_FP64: FP64 butterflies.  Without any of them: FP32 up
+   * to the depth at which its measured detector error (per slice, per
+   * transform length) reaches half of 1e-5 -- 320 slices at 4096^2, 113 at
+   * 512^2 -- and COMP32 for deeper volumes. */
   RSIM_FLAG_MATH_FP32 = 2,
   RSIM_FLAG_MATH_FP64 = 4,
   RSIM_FLAG_MATH_COMP32 = 8
@@ -316,7 +318,7 @@ int rsim_plasma_apply_2d_history(void* u, void* U, int nx, int ny, int dtype, do
  * RSIM_FLAG_NO_SPECTRUM, U = FFT2(u) on return.  Fields are read from and
  * written to the instances' own u / U buffers, which must stay allocated for
  * the batch's life.  Sizes: the fused engine's (ny <= 8192).  flags: the
- * arithmetic flags of rsim_sample_apply_2d (default: FP32 up to 256 slices).
+ * arithmetic flags of rsim_sample_apply_2d (the same depth-dependent default).
  * The cached tables the graph reads are pinned until rsim_batch_destroy;
  * rsim_release_caches invalidates existing batches (rsim_batch_run then
  * returns RSIM_E_INVALID_ARG).  Volumes must outlive the batches using them. */

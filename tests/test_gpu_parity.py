@@ -378,12 +378,15 @@ def test_area_v1_overhang_and_wide_spans(dt, ny, nx, pix, span):
 
 # ----------------------------------------------------- end to end (P5) ----
 
+@pytest.mark.parametrize("n", [4096, 8192, 16384])
 @pytest.mark.parametrize("mode", ["vacuum", "sample"])
 @pytest.mark.parametrize("dt", [np.complex128, np.complex64])
-def test_end_to_end_reference_p5_golden_4096(mode, dt):
-    """run_simulation vs the reference's own P5 fingerprints at 4096^2."""
-    g = json.load(open(os.path.join(GOLDEN, "p5_convergence_4096.json")))[mode]
-    nx = ny = 4096
+def test_end_to_end_reference_p5_golden(n, mode, dt):
+    """run_simulation vs the reference's own P5 fingerprints at 4096^2,
+    8192^2 and 16384^2 (the same physical field sampled ever finer; at
+    16384^2 complex64 takes the three-pass column schedule)."""
+    g = json.load(open(os.path.join(GOLDEN, f"p5_convergence_{n}.json")))[mode]
+    nx = ny = n
     dx = 3.2768e-3 / nx
     wl = 1e-9
     p = rs.SimParams(N=nx * ny, nx=nx, ny=ny, dx=dx, dy=dx, z_detector=10.0, detector_size=4e-4,
@@ -399,7 +402,7 @@ def test_end_to_end_reference_p5_golden_4096(mode, dt):
         table = [(mat, 1.2e-7 + 2.4e-9j)]
     ang = math.asin(2.5e5 * wl)
     u, U = rs.DeviceVector(nx * ny, dt), rs.DeviceVector(nx * ny, dt)
-    say("p5 e2e", mode, np.dtype(dt).name)
+    say("p5 e2e", n, mode, np.dtype(dt).name)
     det = rs.run_simulation(p, rs.PointSource(x=1e-6, y=-1.4e-6, z=0.0), elements,
                             [ang] * (len(elements) + 1), u, U, table)[0]
     s = g["centerline_start_pixel"]
