@@ -79,6 +79,7 @@ SIGNATURES = {
     "rsim_graph_info": [_vp, C.POINTER(_sz)],
     "rsim_graph_destroy": [_vp],
     "rsim_fill_zero": [_vp, _sz, _vp],
+    "rsim_stream_copy": [_vp, _vp, _sz, _vp],
     "rsim_kernel_timing": [_i],
     "rsim_kernel_timing_read": [_dp, _ip],
     "rsim_set_progress": [PROGRESS_FN, _vp],

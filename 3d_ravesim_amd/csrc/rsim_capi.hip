@@ -56,6 +56,7 @@ int release_staging();
 int release_volume_tables(unsigned long long serial);
 int fused_schedule_lookup(int nx, int ny, int* rv, int* cv);
 int fill_zero(void* p, size_t bytes, hipStream_t s);
+int stream_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 namespace {
 thread_local std::string t_err;
@@ -540,6 +541,12 @@ int rsim_graph_destroy(rsim_graph* g) {
   for (void* p : g->owned) (void)hipFree(p);
   delete g;
   return RSIM_OK;
+}
+
+int rsim_stream_copy(void* dst, const void* src, size_t bytes, void* stream) {
+  if ((!dst || !src) && bytes) return fail(RSIM_E_INVALID_ARG, "NULL pointer in stream copy");
+  if (bytes == 0) return RSIM_OK;
+  return stream_copy(dst, src, bytes, resolve_stream(stream));
 }
 
 int rsim_fill_zero(void* dptr, size_t bytes, void* stream) {

@@ -459,6 +459,63 @@ uint64_t fbits(double v) {
   return b;
 }
 
+// Point c's plan at the blob b of P uploaded to d, and derive the row span
+// and the monotonicity the two-stage form needs.
+void det_plan_bind(const DetPlanHost& P, const Blob& b, unsigned char* d, int ox, int oy, int ny,
+                   DetCached* c) {
+  c->d = d;
+  c->pl.xi0 = (const int*)(d + b.ioff[0]); c->pl.xi1 = (const int*)(d + b.ioff[1]);
+  c->pl.jlo = (const int*)(d + b.ioff[2]); c->pl.jhi = (const int*)(d + b.ioff[3]);
+  c->pl.xf0 = (const double*)(d + b.doff[0]); c->pl.xf1 = (const double*)(d + b.doff[1]);
+  c->pl.xdet = (const double*)(d + b.doff[2]); c->pl.ylo = (const double*)(d + b.doff[3]);
+  c->pl.yhi = (const double*)(d + b.doff[4]); c->pl.ydet = (const double*)(d + b.doff[5]);
+  c->gyl = P.gyl;
+  int jmin = ny, jmax = 0;
+  for (int q = 0; q < oy; ++q)
+    if (P.jhi[q] > P.jlo[q]) {
+      jmin = std::min(jmin, P.jlo[q]);
+      jmax = std::max(jmax, P.jhi[q]);
+    }
+  c->j0 = jmax > jmin ? jmin : 0;
+  c->nj = jmax > jmin ? jmax - jmin : 0;
+  c->mono = true;
+  for (int p = 1; p < ox; ++p)
+    if (P.xi0[p] < P.xi0[p - 1] || P.xi1[p] < P.xi1[p - 1]) c->mono = false;
+}
+
+bool det_two_stage(const DetCached& c, int oy) {
+  return c.mono && c.nj > 0 && c.nj <= 65535 && oy <= 65535;
+}
+
+// The detector's launches: the separable two-stage form through xs (nj x ox
+// doubles) when given, else the one-pass kernel.
+int det_launch(const void* u, int nx, int ny, int dtype, double dx, double dy, int ox, int oy,
+               double z, double weight, bool accumulate, double* out_dev, const DetCached& c,
+               double* xs, hipStream_t s) {
+  if (xs) {
+    const dim3 g1((ox + 255) / 256, c.nj), g2((ox + 255) / 256, oy);
+    if (dtype == RSIM_C64)
+      hipLaunchKernelGGL(area_v1_rows_kernel<float>, g1, dim3(256), 0, s, (const cpx<float>*)u,
+                         nx, ox, c.j0, c.nj, c.pl, dx, xs);
+    else
+      hipLaunchKernelGGL(area_v1_rows_kernel<double>, g1, dim3(256), 0, s,
+                         (const cpx<double>*)u, nx, ox, c.j0, c.nj, c.pl, dx, xs);
+    hipLaunchKernelGGL(area_v1_cols_kernel, g2, dim3(256), 0, s, xs, ox, oy, c.j0, c.pl, dy,
+                       c.gyl, z, weight, accumulate ? 1 : 0, out_dev);
+  } else {
+    dim3 grid((ox + 15) / 16, (oy + 15) / 16);
+    if (dtype == RSIM_C64)
+      hipLaunchKernelGGL(area_v1_kernel<float>, grid, dim3(256), 0, s, (const cpx<float>*)u, nx,
+                         ny, ox, oy, c.pl, dx, dy, c.gyl, z, weight, accumulate ? 1 : 0, out_dev);
+    else
+      hipLaunchKernelGGL(area_v1_kernel<double>, grid, dim3(256), 0, s, (const cpx<double>*)u,
+                         nx, ny, ox, oy, c.pl, dx, dy, c.gyl, z, weight, accumulate ? 1 : 0,
+                         out_dev);
+  }
+  RSIM_LAUNCHED();
+  return RSIM_OK;
+}
+
 int det_plan_dev(int nx, int ny, double dx, double dy, int ox, int oy, double dsx, double dsy,
                  const DetCached** out) {
   std::lock_guard<std::mutex> lk(g_det_mu);
@@ -481,24 +538,7 @@ int det_plan_dev(int nx, int ny, double dx, double dy, int ox, int oy, double ds
     (void)hipFree(c.d);
     return fail(RSIM_E_HIP, "detector plan upload failed: %s", hipGetErrorString(e));
   }
-  unsigned char* d = c.d;
-  c.pl.xi0 = (const int*)(d + b.ioff[0]); c.pl.xi1 = (const int*)(d + b.ioff[1]);
-  c.pl.jlo = (const int*)(d + b.ioff[2]); c.pl.jhi = (const int*)(d + b.ioff[3]);
-  c.pl.xf0 = (const double*)(d + b.doff[0]); c.pl.xf1 = (const double*)(d + b.doff[1]);
-  c.pl.xdet = (const double*)(d + b.doff[2]); c.pl.ylo = (const double*)(d + b.doff[3]);
-  c.pl.yhi = (const double*)(d + b.doff[4]); c.pl.ydet = (const double*)(d + b.doff[5]);
-  c.gyl = P.gyl;
-  int jmin = ny, jmax = 0;
-  for (int q = 0; q < oy; ++q)
-    if (P.jhi[q] > P.jlo[q]) {
-      jmin = std::min(jmin, P.jlo[q]);
-      jmax = std::max(jmax, P.jhi[q]);
-    }
-  c.j0 = jmax > jmin ? jmin : 0;
-  c.nj = jmax > jmin ? jmax - jmin : 0;
-  c.mono = true;
-  for (int p = 1; p < ox; ++p)
-    if (P.xi0[p] < P.xi0[p - 1] || P.xi1[p] < P.xi1[p - 1]) c.mono = false;
+  det_plan_bind(P, b, c.d, ox, oy, ny, &c);
   *out = &(g_det[key] = c);
   return RSIM_OK;
 }
@@ -522,7 +562,7 @@ int detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double
   // x-integrals (nj x ox doubles); the one-pass kernel when there is none
   double* xs = nullptr;
   const size_t xs_bytes = (size_t)c->nj * ox * 8;
-  const bool two_stage = c->mono && c->nj > 0 && c->nj <= 65535 && oy <= 65535;
+  const bool two_stage = det_two_stage(*c, oy);
   const bool captured = capture_active();
   if (two_stage && captured) {
     // the graph replays this detector: its scratch lives as long as the graph
@@ -531,30 +571,10 @@ int detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double
     (void)hipGetLastError();
     xs = nullptr;
   }
-  if (xs) {
-    const dim3 g1((ox + 255) / 256, c->nj), g2((ox + 255) / 256, oy);
-    if (dtype == RSIM_C64)
-      hipLaunchKernelGGL(area_v1_rows_kernel<float>, g1, dim3(256), 0, s, (const cpx<float>*)u,
-                         nx, ox, c->j0, c->nj, c->pl, dx, xs);
-    else
-      hipLaunchKernelGGL(area_v1_rows_kernel<double>, g1, dim3(256), 0, s,
-                         (const cpx<double>*)u, nx, ox, c->j0, c->nj, c->pl, dx, xs);
-    hipLaunchKernelGGL(area_v1_cols_kernel, g2, dim3(256), 0, s, xs, ox, oy, c->j0, c->pl, dy,
-                       c->gyl, z, weight, accumulate ? 1 : 0, out_dev);
-    const hipError_t e = hipGetLastError();
-    if (!captured) (void)hipFreeAsync(xs, s);
-    if (e != hipSuccess) return fail(RSIM_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
-    return RSIM_OK;
-  }
-  dim3 grid((ox + 15) / 16, (oy + 15) / 16);
-  if (dtype == RSIM_C64)
-    hipLaunchKernelGGL(area_v1_kernel<float>, grid, dim3(256), 0, s, (const cpx<float>*)u, nx, ny,
-                       ox, oy, c->pl, dx, dy, c->gyl, z, weight, accumulate ? 1 : 0, out_dev);
-  else
-    hipLaunchKernelGGL(area_v1_kernel<double>, grid, dim3(256), 0, s, (const cpx<double>*)u, nx,
-                       ny, ox, oy, c->pl, dx, dy, c->gyl, z, weight, accumulate ? 1 : 0, out_dev);
-  RSIM_LAUNCHED();
-  return RSIM_OK;
+  const int rc = det_launch(u, nx, ny, dtype, dx, dy, ox, oy, z, weight, accumulate, out_dev, *c,
+                            xs, s);
+  if (xs && !captured) (void)hipFreeAsync(xs, s);
+  return rc;
 }
 
 int detector_xbin_plan(int nx, int ny, double dx, double dy, int ox, int oy, double dsx,
@@ -743,6 +763,35 @@ int fill_zero(void* p, size_t bytes, hipStream_t s) {
   const int blocks = (int)std::max<long long>(1, std::min<long long>((n2 + 255) / 256, 8192));
   hipLaunchKernelGGL(zero_fill_kernel, dim3(blocks), dim3(256), 0, s, (double2*)p, n2,
                      (double*)((char*)p + 16 * n2), n_tail);
+  RSIM_LAUNCHED();
+  return RSIM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// STREAM-style copy: the HBM bandwidth one plain kernel reaches (16-byte
+// vector loads and stores, four per thread per step, grid-stride)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) stream_copy_kernel(const uint4* __restrict__ src,
+                                                          uint4* __restrict__ dst, long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+    dst[i] = a;
+    dst[i + stride] = b;
+    dst[i + 2 * stride] = c;
+    dst[i + 3 * stride] = d;
+  }
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
+int stream_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src) | bytes) & 15)
+    return fail(RSIM_E_INVALID_ARG, "stream copy needs 16-byte aligned buffers and sizes");
+  const long long n = (long long)(bytes / 16);
+  const int blocks = (int)std::max<long long>(1, std::min<long long>((n + 1023) / 1024, 8192));
+  hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, s, (const uint4*)src,
+                     (uint4*)dst, n);
   RSIM_LAUNCHED();
   return RSIM_OK;
 }

@@ -87,7 +87,12 @@ def cpu_info():
                 break
     except OSError:
         pass
-    return model, os.cpu_count()
+    try:
+        ncpu = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        ncpu = os.cpu_count() or 1
+    # one GPU's share of the host: at most 16 threads
+    return model, max(1, min(ncpu, 16))
 
 
 def main():
@@ -98,7 +103,9 @@ def main():
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--slices", type=int, default=256)
     ap.add_argument("--plain-steps", type=int, default=2)
-    ap.add_argument("--cpu-slices", type=int, default=3)
+    ap.add_argument("--cpu-slices", type=int, default=8)
+    ap.add_argument("--no-large-leg", action="store_true",
+                    help="skip the 16384^2 leg of the default run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--workload", choices=["multislice", "adjoint", "large", "batch", "plasma",
@@ -341,14 +348,25 @@ def main():
         lib.call("rsim_memcpy_d2h", slab.ctypes.data, ids_p.value, slab.nbytes, None)
         spec = O.SampleSpec(grid=slab, db_list=DB, pixel_size_x=dx, pixel_size_y=dx, pixel_size_z=pz)
         u0 = np.ones((n, n), dtype=np.complex128)
+        model, ncpu = cpu_info()
+        # the reference's call (scipy.fft default workers=1), then every core
         t0 = time.perf_counter()
         ue, _ = O.sample_apply_2d(u0, spec, dx, dx, wl, 0.4 / dx, workers=1)
         t_cpu = time.perf_counter() - t0
-        model, ncpu = cpu_info()
+        t0 = time.perf_counter()
+        O.sample_apply_2d(u0, spec, dx, dx, wl, 0.4 / dx, workers=ncpu)
+        t_all = time.perf_counter() - t0
         result["cpu_baseline"] = {
             "value": round(k / t_cpu, 4), "unit": "slice-steps/s", "cores": 1, "kind": "port",
-            "sample": f"FP64 numpy/scipy oracle (scipy.fft workers=1), first {k} slice-steps of the "
-                      f"same {n}x{n} workload from a unit plane wave; host {model}, {ncpu} logical CPUs",
+            "sample": f"FP64 numpy/scipy oracle (scipy.fft workers=1, the reference's call), "
+                      f"{k} slice-steps of the same {n}x{n} workload from a unit plane wave "
+                      f"(transmission, FFT2, chirp + cutoff, IFFT2 per slice; no source or "
+                      f"detector); host {model}, {ncpu} logical CPUs",
+        }
+        result["cpu_baseline_all_cores"] = {
+            "value": round(k / t_all, 4), "unit": "slice-steps/s", "cores": ncpu, "kind": "port",
+            "sample": f"the same {k} slice-steps with scipy.fft workers={ncpu} (the transmission "
+                      "gather stays on one core)",
         }
         # the same slices on the GPU (fused) against the oracle
         u.upload(ones64)
@@ -356,11 +374,85 @@ def main():
         lib.call("rsim_synchronize", s)
         got = u.to_numpy().reshape(n, n)
         result["rel_l2_field_fused_vs_oracle_first_slices"] = O.relative_l2(got, ue)
-        log(f"cpu oracle: {k / t_cpu:.3f} slice-steps/s; fused vs oracle ({k} slices): "
-            f"{result['rel_l2_field_fused_vs_oracle_first_slices']:.3e}")
+        log(f"cpu oracle: {k / t_cpu:.3f} slice-steps/s (1 core), {k / t_all:.3f} ({ncpu} cores); "
+            f"fused vs oracle ({k} slices): {result['rel_l2_field_fused_vs_oracle_first_slices']:.3e}")
+
+    # ---- bandwidth ceilings beside the 8 TB/s spec peak ---------------------
+    result["copy_ceiling"] = copy_ceilings(lib, s)
+    log(f"copy ceilings: {result['copy_ceiling']}")
+    del u, U, u2, U2
+    vol.destroy()
+    if not args.no_large_leg:
+        result["large_16384"] = large_leg(rs, lib, s)
 
     lib.call("rsim_stream_destroy", s)
     print(json.dumps(result), flush=True)
+
+
+def copy_ceilings(lib, s, mib=1024, reps=10):
+    """Device-to-device copy of 1 GiB by hipMemcpy and by the library's plain
+    copy kernel; GB/s counts the bytes read plus the bytes written."""
+    nbytes = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    lib.call("rsim_malloc", C.byref(a), nbytes)
+    lib.call("rsim_malloc", C.byref(b), nbytes)
+    lib.call("rsim_fill_zero", a, nbytes, s)
+    out = {"bytes_per_copy": nbytes, "counted": "read + write"}
+    for name, fn in (("hipMemcpy_d2d_GBs", "rsim_memcpy_d2d"), ("copy_kernel_GBs", "rsim_stream_copy")):
+        lib.call(fn, b, a, nbytes, s)
+        lib.call("rsim_synchronize", s)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            lib.call(fn, b, a, nbytes, s)
+        lib.call("rsim_synchronize", s)
+        out[name] = round(2 * nbytes * reps / (time.perf_counter() - t0) / 1e9, 1)
+    out["copy_kernel_frac_of_peak"] = round(out["copy_kernel_GBs"] / HBM_PEAK_GBS, 4)
+    lib.call("rsim_free", a)
+    lib.call("rsim_free", b)
+    return out
+
+
+def large_leg(rs, lib, s, nz=32):
+    """A short leg of the large configuration in the default run: a 16384^2
+    complex64 field through the first 32 slices of 0.25 um of the seeded
+    cylinder-and-crack volume (lambda 0.4 A), default arithmetic (FP32 at
+    this depth) and compensated FP32 (the default at the full 512 slices)."""
+    import torch
+    n, dx, wl, pz = 16384, 0.25e-6, 0.4e-10, 0.25e-6
+    N = n * n
+    vol = rs.DeviceVolume.empty((nz, n, n), DB5)
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        paint_large_volume(torch, lib, vol, n, nz, 5, C.c_void_p(st.cuda_stream), bands=1)
+    torch.cuda.synchronize()
+    u = rs.DeviceVector(N, np.complex64)
+    U = rs.DeviceVector(N, np.complex64)
+    out = {"workload": f"16384x16384 c64, {nz} slices x 0.25um of the seeded cylinder/crack "
+                       "volume, lambda 0.4A (config-5 geometry, first slices)"}
+    for name, fl in (("fp32", 0), ("comp32", lib.RSIM_FLAG_MATH_COMP32)):
+        lib.call("rsim_fill_zero", u.ptr, u.nbytes, s)
+        run_multislice(lib, u, U, n, vol, lib.RSIM_FUSED, dx, wl, pz, 4, flags=fl, stream=s)
+        lib.call("rsim_synchronize", s)
+        lib.call("rsim_kernel_timing_read", None, None)
+        lib.call("rsim_kernel_timing", KT_EVERY)
+        t0 = time.perf_counter()
+        run_multislice(lib, u, U, n, vol, lib.RSIM_FUSED, dx, wl, pz, nz, flags=fl, stream=s)
+        lib.call("rsim_synchronize", s)
+        t = time.perf_counter() - t0
+        lib.call("rsim_kernel_timing", 0)
+        ms3 = (C.c_double * 3)()
+        cnt3 = (C.c_int * 3)()
+        lib.call("rsim_kernel_timing_read", ms3, cnt3)
+        rate = nz / t
+        out[name] = {"slice_steps_per_s": round(rate, 2),
+                     "step_hbm_frac_65B_model": round(65.0 * N * rate / 1e9 / HBM_PEAK_GBS, 4),
+                     "row_us": round(1e3 * ms3[0] / max(cnt3[0], 1), 1),
+                     "three_column_passes_us": round(1e3 * ms3[1] / max(cnt3[1], 1), 1)}
+        log(f"16384^2 leg, {name}: {out[name]}")
+    u.drop()
+    U.drop()
+    vol.destroy()
+    return out
 
 
 def smooth_volume(torch, shape, sigma, seed, device):
@@ -380,16 +472,18 @@ def smooth_volume(torch, shape, sigma, seed, device):
 def adjoint_main(args, rs):
     """Differentiable multislice at 2048^2 x 128 (float32 delta, beta).
 
-    delta in [0, 1e-6], beta in [0, 1e-9] from Gaussian-filtered (sigma = 4
-    voxels) seeded normal noise (seed 4); pz = 0.5 um, lambda = 0.5 A, 0.2 m leg,
-    area_v1 4x4.  Target: the image of a volume perturbed by +5 % of a second
-    smooth field (seed 5).  Loss ||I - I_t||^2.  Times forward and forward +
-    backward (complex64, sqrt(128) checkpointing) on the plain engine and on
-    the fused chain in each butterfly arithmetic, checks the gradient against
-    FP64 (complex128) central differences at --fd-voxels random voxels (seed 6)
-    and against the complex128 adjoint, both from each run's own dL/dI and from
-    the complex128 dL/dI (the accuracy of the adjoint chain on its own: the
-    loss amplifies the image error by |I| / |I - I_t|).
+    delta in [0, 1e-6] and beta in [0, 1e-9] from independent Gaussian-filtered
+    (sigma = 4 voxels) seeded normal noise fields (seeds 4 and 7); pz = 0.5 um,
+    lambda = 0.5 A, 0.2 m leg, area_v1 4x4.  Target: the image of the volume
+    perturbed by +5 % of a third smooth field (seed 5).  Loss ||I - I_t||^2.
+    Times forward and forward + backward in the default mode (image, dL/dI
+    and sqrt(128) checkpoints from the complex128 forward, adjoint chain on
+    the fused complex64 engine) and with the whole run in complex64 (plain
+    engine; fused chain in each butterfly arithmetic).  Checks the complex128
+    gradient against FP64 central differences at --fd-voxels random voxels
+    (seed 6), and every gradient against the complex128 one, from each run's
+    own dL/dI and from the complex128 dL/dI (the adjoint chain on its own; an
+    all-complex64 run's loss amplifies its image error by |I| / |I - I_t|).
     """
     import torch
     AG = importlib.import_module("3d_ravesim_amd.autograd")
@@ -399,7 +493,7 @@ def adjoint_main(args, rs):
     st = torch.cuda.Stream()
     with torch.cuda.stream(st):
         delta = smooth_volume(torch, (nz, n, n), 4.0, 4, dev) * 1e-6
-        beta = (delta * 1e-3).contiguous()
+        beta = (smooth_volume(torch, (nz, n, n), 4.0, 7, dev) * 1e-9).contiguous()
         pert = smooth_volume(torch, (nz, n, n), 4.0, 5, dev)
         G = AG.MultisliceGeometry(dx=dx, wl=wl, pz=pz, cutoff=0.4 / dx, z_leg=z_leg,
                                   out_x=n // 4, out_y=n // 4, ds_x=4 * dx, ds_y=4 * dx,
@@ -444,10 +538,13 @@ def adjoint_main(args, rs):
         log(f"complex128 forward+backward {t_fb128 * 1e3:.1f} ms")
         rel = lambda a, b: float(torch.linalg.vector_norm((a - b).double()) / torch.linalg.vector_norm(b.double()))  # noqa: E731
         runs = {}
-        for name, geom in (("plain", dataclasses.replace(G, engine="plain")),
-                           ("fused_fp32", dataclasses.replace(G, math="fp32")),
-                           ("fused_comp32", dataclasses.replace(G, math="comp32")),
-                           ("fused_fp64", dataclasses.replace(G, math="fp64"))):
+        # default: image, dL/dI and checkpoints from the complex128 forward,
+        # adjoint chain fused complex64; the others run the forward in complex64
+        for name, geom in (("default", G),
+                           ("plain", dataclasses.replace(G, engine="plain", forward="field")),
+                           ("fused_fp32", dataclasses.replace(G, math="fp32", forward="field")),
+                           ("fused_comp32", dataclasses.replace(G, math="comp32", forward="field")),
+                           ("fused_fp64", dataclasses.replace(G, math="fp64", forward="field"))):
             tf, tfb, (gd_, gb_, loss_, I_) = timed(geom)
             # the adjoint chain on its own: backward from the complex128 dL/dI
             d_ = delta.clone().requires_grad_(True)
@@ -468,7 +565,7 @@ def adjoint_main(args, rs):
                 f"adjoint at the c128 dL/dI: delta {r['rel_l2_adjoint_delta_at_c128_dLdI']:.2e}, "
                 f"beta {r['rel_l2_adjoint_beta_at_c128_dLdI']:.2e}")
             del gd_, gb_, I_, d_, b_, ga_d, ga_b
-        main_run = runs["fused_" + G.math]
+        main_run = runs["default"]
 
         def L(dd, bb):
             with torch.no_grad():
@@ -499,13 +596,16 @@ def adjoint_main(args, rs):
         "metric": "differentiable multislice forward+backward ms (2048^2 x 128, c64)",
         "value": main_run["forward_backward_ms"], "unit": "ms", "n_gpus": 1, "steps": reps,
         "higher_is_better": False, "dtype": "c64", "data": "synthetic",
-        "engine": "fused", "math": G.math, "forward_ms": main_run["forward_ms"],
+        "engine": "fused", "math": G.math, "forward": "complex128 image and checkpoints, fused complex64 adjoint chain",
+        "forward_ms": main_run["forward_ms"],
+        "rel_l2_grad_delta_vs_c128": main_run["rel_l2_grad_delta_vs_c128"],
+        "rel_l2_grad_beta_vs_c128": main_run["rel_l2_grad_beta_vs_c128"],
         "c128_forward_backward_ms": round(t_fb128 * 1e3, 2),
         "slice_steps_per_s_forward_backward": round(nz * 1e3 / main_run["forward_backward_ms"], 1),
         "checkpoint_every": every, "checkpoint_MiB": ckb.value >> 20, "work_MiB": wkb.value >> 20,
         "runs": runs,
         "fd_voxels": args.fd_voxels, "fd_max_err_over_max_grad": worst,
-        "config": {"workload": "2048x2048x128 float32 delta/beta (smooth seeded noise, seed 4), "
+        "config": {"workload": "2048x2048x128 float32 delta/beta (smooth seeded noise, seeds 4 and 7), "
                                "lambda 0.5A, pz 0.5um, 0.2 m leg, area_v1 4x4, target +5% (seed 5)"},
     }), flush=True)
 

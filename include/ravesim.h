@@ -135,6 +135,10 @@ int rsim_graph_destroy(rsim_graph* graph);
 /* Zero `bytes` (a multiple of 8, 16-byte aligned) of device memory with a
  * kernel, on `stream` (usable inside a capture). */
 int rsim_fill_zero(void* dptr, size_t bytes, void* stream);
+/* Device-to-device copy by a plain HIP kernel (16-byte vector accesses,
+ * 16-byte aligned buffers and size): the achievable-bandwidth ceiling the
+ * benchmark reports beside the spec peak and hipMemcpy. */
+int rsim_stream_copy(void* dst, const void* src, size_t bytes, void* stream);
 
 /* Kernel timing (the reference's CudaTimerSection, fast-wave/src/simulation.cpp:14-33):
  * every > 0 brackets one launch in `every` of each kind (row pass, column

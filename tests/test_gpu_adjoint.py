@@ -336,3 +336,57 @@ def test_gradient_matches_central_differences_on_gpu():
             fd = (L(dp, bp) - L(dm, bm)) / (vp - vm)
             scale = np.abs(grad).max()
             assert abs(fd - grad[s, y, x]) < 2e-4 * scale, (which, s, y, x, fd, grad[s, y, x])
+
+
+def _smooth_case(n, nz, seed):
+    """Gaussian-filtered (sigma 4 voxels) seeded noise, delta in [0, 1e-6] and
+    beta in [0, 1e-9] from independent fields; target from a +5 % smooth
+    perturbation of delta (the differentiable configuration at small size)."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+
+    def field01(shape):
+        f = gaussian_filter(rng.normal(size=shape), 4.0, mode="wrap")
+        return (f - f.min()) / (f.max() - f.min())
+
+    delta = (1e-6 * field01((nz, n, n))).astype(np.float32)
+    beta = (1e-9 * field01((nz, n, n))).astype(np.float32)
+    pert = field01((nz, n, n))
+    u0 = np.ones((n, n), np.complex128)
+    geo = dict(dx=0.5e-6, dy=0.5e-6, wl=0.5e-10, pz=0.5e-6, cutoff_freq=0.4 / 0.5e-6, z_leg=0.2,
+               size_x=n * 0.5e-6, size_y=n * 0.5e-6, ds_x=2e-6, ds_y=2e-6, z_det=0.2 + nz * 0.5e-6)
+    target = O.multislice_db_intensity(u0, (delta * (1 + 0.05 * pert)).astype(np.float32), beta,
+                                       geo["dx"], geo["dy"], geo["wl"], geo["pz"],
+                                       geo["cutoff_freq"], geo["z_leg"], geo["size_x"],
+                                       geo["size_y"], geo["ds_x"], geo["ds_y"], geo["z_det"])
+    return delta, beta, u0, geo, target
+
+
+def test_complex64_gradient_end_to_end_from_the_gpu_forward():
+    """The gradient a complex64 user gets: the GPU's own forward, loss and
+    backward (no oracle dL/dI), with |I - I_t| / |I| of a few 1e-3 that
+    amplifies image errors.  Default mode (image in complex128, adjoint chain
+    in fused complex64): within 1e-4 of the FP64 oracle gradient."""
+    n, nz = 256, 32
+    delta, beta, u0, geo, target = _smooth_case(n, nz, seed=41)
+    _, gI = loss_and_grad_img(delta, beta, u0, geo, target)
+    I_ref = target + gI / 2
+    ratio = np.linalg.norm(I_ref - target) / np.linalg.norm(I_ref)
+    ref_d, ref_b = oracle_gradient(delta, beta, u0, geo, gI)
+    say(f"end-to-end c64 gradient, |I-I_t|/|I| = {ratio:.2e}")
+    assert 1e-3 < ratio < 2e-2
+    errs = {}
+    oy, ox = O.detector_shape(geo["size_x"], geo["size_y"], geo["ds_x"], geo["ds_y"])
+    for forward in ("auto", "field"):
+        G = AG.MultisliceGeometry(dx=geo["dx"], wl=geo["wl"], pz=geo["pz"],
+                                  cutoff=geo["cutoff_freq"], z_leg=geo["z_leg"], out_x=ox,
+                                  out_y=oy, ds_x=geo["ds_x"], ds_y=geo["ds_y"],
+                                  z_det=geo["z_det"], forward=forward)
+        d = vol(delta).requires_grad_(True)
+        b = vol(beta).requires_grad_(True)
+        I = AG.multislice_intensity(d, b, G, dtype=torch.complex64)
+        ((I - torch.from_numpy(target).cuda()) ** 2).sum().backward()
+        torch.cuda.synchronize()
+        errs[forward] = (rel(d.grad.cpu().numpy(), ref_d), rel(b.grad.cpu().numpy(), ref_b))
+        say(f"  forward={forward}: delta {errs[forward][0]:.2e} beta {errs[forward][1]:.2e}")
+    assert errs["auto"][0] < 1e-4 and errs["auto"][1] < 1e-4
