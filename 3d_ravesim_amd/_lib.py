@@ -109,6 +109,9 @@ SIGNATURES = {
     "rsim_detector_area_v1": [_vp, _i, _i, _i, _d, _d, _i, _i, _d, _d, _d, _dp, _vp],
     "rsim_detector_area_v1_accumulate": [_vp, _i, _i, _i, _d, _d, _i, _i, _d, _d, _d, _d, _vp,
                                          _vp],
+    "rsim_detector_area_v1_accumulate_noalloc": [_vp, _i, _i, _i, _d, _d, _i, _i, _d, _d, _d, _d,
+                                                 _vp, _vp],
+    "rsim_detector_noalloc_capacity": [C.POINTER(_sz)],
     "rsim_transmission_db_2d": [_vp, _i, _i, _i, _d, _d, _vp, _vp, _vp],
     "rsim_multislice_db_2d": [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _vp, _vp, _d, _i, _vp, _i,
                               _i, _i, _vp],

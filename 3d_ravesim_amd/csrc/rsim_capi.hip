@@ -57,6 +57,10 @@ int release_volume_tables(unsigned long long serial);
 int fused_schedule_lookup(int nx, int ny, int* rv, int* cv);
 int fill_zero(void* p, size_t bytes, hipStream_t s);
 int stream_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
+int detector_area_v1_noalloc(const void* u, int nx, int ny, int dtype, double dx, double dy,
+                             int ox, int oy, double dsx, double dsy, double z, double weight,
+                             double* out_dev, hipStream_t s);
+size_t detector_noalloc_capacity();
 
 namespace {
 thread_local std::string t_err;
@@ -926,6 +930,24 @@ int rsim_detector_area_v1_accumulate(const void* u, int nx, int ny, int dtype, d
     return fail(RSIM_E_INVALID_ARG, "2D detector grid and pixel spacings must be positive");
   return detector_area_v1(u, nx, ny, dtype, dx, dy, out_x, out_y, ds_x, ds_y, z, weight, true,
                           out_dev, resolve_stream(stream));
+}
+
+int rsim_detector_area_v1_accumulate_noalloc(const void* u, int nx, int ny, int dtype, double dx,
+                                             double dy, int out_x, int out_y, double ds_x,
+                                             double ds_y, double z, double weight,
+                                             double* out_dev, void* stream) {
+  RSIM_CHECK(check_field(u, nx, ny, dtype, "u"));
+  if (!out_dev) return fail(RSIM_E_INVALID_ARG, "out_dev is NULL");
+  if (!(dx > 0) || !(dy > 0) || !(ds_x > 0) || !(ds_y > 0))
+    return fail(RSIM_E_INVALID_ARG, "2D detector grid and pixel spacings must be positive");
+  return detector_area_v1_noalloc(u, nx, ny, dtype, dx, dy, out_x, out_y, ds_x, ds_y, z, weight,
+                                  out_dev, resolve_stream(stream));
+}
+
+int rsim_detector_noalloc_capacity(size_t* bytes) {
+  if (!bytes) return fail(RSIM_E_INVALID_ARG, "bytes is NULL");
+  *bytes = detector_noalloc_capacity();
+  return RSIM_OK;
 }
 
 int rsim_detector_area_v1_adjoint(const void* v, void* g, int nx, int ny, int dtype, double dx,

@@ -577,6 +577,71 @@ int detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double
   return rc;
 }
 
+// ---------------------------------------------------------------------------
+// The detector without allocations, for callers that cannot allocate device
+// memory.  Its plan and its x-integrals live in one static device array of
+// kNoAllocScratchBytes per GPU (held until the process ends); every call on a
+// GPU reuses that GPU's array, so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch_ptr;   // device -> address of that device's array
+
+// The array's address on the current device, looked up once per device (the
+// array's name in host code is only a stand-in for the symbol).
+int noalloc_scratch(unsigned char** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  const int dev = current_device();
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) {
+    *out = it->second;
+    return RSIM_OK;
+  }
+  void* p = nullptr;
+  RSIM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return fail(RSIM_E_HIP, "no device address for the detector scratch array");
+  g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  *out = static_cast<unsigned char*>(p);
+  return RSIM_OK;
+}
+}  // namespace
+
+size_t detector_noalloc_capacity() { return kNoAllocScratchBytes; }
+
+int detector_area_v1_noalloc(const void* u, int nx, int ny, int dtype, double dx, double dy,
+                             int ox, int oy, double dsx, double dsy, double z, double weight,
+                             double* out_dev, hipStream_t s) {
+  RSIM_CHECK(check_detector(ox, oy, z));
+  if (capture_active())
+    return fail(RSIM_E_UNSUPPORTED, "the no-allocation detector uploads its plan synchronously "
+                "and cannot be recorded into a graph; record rsim_detector_area_v1_accumulate");
+  unsigned char* base = nullptr;
+  RSIM_CHECK(noalloc_scratch(&base));
+  DetPlanHost P;
+  det_plan(nx, ny, dx, dy, ox, oy, dsx, dsy, &P);
+  Blob b = make_blob({&P.xi0, &P.xi1, &P.jlo, &P.jhi},
+                     {&P.xf0, &P.xf1, &P.xdet, &P.ylo, &P.yhi, &P.ydet});
+  DetCached c;
+  det_plan_bind(P, b, base, ox, oy, ny, &c);
+  const size_t plan_bytes = (b.bytes.size() + 255) & ~size_t(255);
+  const bool two = det_two_stage(c, oy);
+  const size_t xs_bytes = two ? (size_t)c.nj * ox * 8 : 0;
+  if (plan_bytes + xs_bytes > kNoAllocScratchBytes)
+    return fail(RSIM_E_INVALID_ARG,
+                "detector of %d x %d pixels over %d field rows needs %zu bytes of scratch; the "
+                "no-allocation detector holds %zu (use rsim_detector_area_v1_accumulate)",
+                ox, oy, c.nj, plan_bytes + xs_bytes, kNoAllocScratchBytes);
+  // the plan replaces the last call's, after the work already on the stream
+  // (which may still read it); the host copy must outlive the transfer
+  RSIM_HIP(hipMemcpyAsync(base, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice, s));
+  RSIM_HIP(hipStreamSynchronize(s));
+  return det_launch(u, nx, ny, dtype, dx, dy, ox, oy, z, weight, true, out_dev, c,
+                    two ? reinterpret_cast<double*>(base + plan_bytes) : nullptr, s);
+}
+
 int detector_xbin_plan(int nx, int ny, double dx, double dy, int ox, int oy, double dsx,
                        double dsy, XBinArgs* xb, bool* ok) {
   *ok = false;

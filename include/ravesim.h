@@ -362,6 +362,21 @@ int rsim_detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, d
 int rsim_detector_area_v1_accumulate(const void* u, int nx, int ny, int dtype, double dx, double dy,
                                      int out_x, int out_y, double ds_x, double ds_y, double z,
                                      double weight, double* out_dev, void* stream);
+/* rsim_detector_area_v1_accumulate without device allocations -- no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid allocation).  The detector
+ * plan and the x-integrals (out_y-covering field rows x out_x doubles) live
+ * in one static device array of rsim_detector_noalloc_capacity() bytes
+ * (64 MiB) per GPU, held until the process ends; larger detectors return
+ * RSIM_E_INVALID_ARG.  Each call uploads its plan and waits for the stream
+ * before launching (the result stays stream-ordered); every call on a GPU
+ * shares that GPU's array, so calls that overlap on one GPU race.  Same
+ * results as rsim_detector_area_v1_accumulate, bit for bit. */
+int rsim_detector_area_v1_accumulate_noalloc(const void* u, int nx, int ny, int dtype, double dx,
+                                             double dy, int out_x, int out_y, double ds_x,
+                                             double ds_y, double z, double weight,
+                                             double* out_dev, void* stream);
+int rsim_detector_noalloc_capacity(size_t* bytes);
 
 /* ---- Differentiable multislice (per-voxel delta/beta volumes) ----
  * The reference has no adjoint; these entry points are the gradient path of
