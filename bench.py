@@ -122,7 +122,8 @@ def main():
     ap.add_argument("--fd-voxels", type=int, default=64)
     ap.add_argument("--large-slices", type=int, default=512)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="only copy MIB MiB device-to-device 3 times (HBM counter calibration)")
+                    help="only copy MIB MiB device-to-device 3 times with the library's copy "
+                         "kernel (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; the multislice engine has no "
@@ -156,9 +157,10 @@ def main():
         lib.call("rsim_malloc", C.byref(b), nbytes)
         lib.call("rsim_memset", a, 1, nbytes, None)
         for _ in range(3):
-            lib.call("rsim_memcpy_d2d", b, a, nbytes, None)
+            lib.call("rsim_stream_copy", b, a, nbytes, None)
         lib.call("rsim_synchronize", None)
-        print(json.dumps({"copy_bytes_per_launch": nbytes, "launches": 3}), flush=True)
+        print(json.dumps({"copy_bytes_per_launch": nbytes, "launches": 3,
+                          "kernel": "stream_copy_kernel"}), flush=True)
         return
     n, nz = args.n, args.slices
     dx = 0.5e-6
@@ -258,6 +260,22 @@ def main():
         lib.call("rsim_synchronize", s)
         result[f"fused_{name}_math_slice_steps_per_s"] = round(k * nz / (time.perf_counter() - t0), 2)
         log(f"fused c64, {name} butterflies: {result[f'fused_{name}_math_slice_steps_per_s']:.1f} slice-steps/s")
+
+    # ---- cost of a progress callback: each report waits for the GPU --------
+    CTL = importlib.import_module("3d_ravesim_amd._control")
+    prog = {}
+    for every in (1, 16):
+        calls = []
+        with CTL.RunControl(lambda st_, d_, t_: calls.append(d_), every=every):
+            run_multislice(lib, u, U, n, vol, lib.RSIM_FUSED, dx, wl, pz, nz, stream=s)
+            lib.call("rsim_synchronize", s)
+            t0 = time.perf_counter()
+            run_multislice(lib, u, U, n, vol, lib.RSIM_FUSED, dx, wl, pz, nz, stream=s)
+            lib.call("rsim_synchronize", s)
+            prog[f"every_{every}_slices"] = round(nz / (time.perf_counter() - t0), 2)
+    prog["none"] = round(value, 2)
+    result["progress_callback_slice_steps_per_s"] = prog
+    log(f"with a progress callback: {prog}")
 
     # ---- the same volume sampled at 1.06 dx: a non-integral voxel map, so
     # every slice interpolates (delta, beta) bilinearly instead of the lookup
@@ -778,19 +796,22 @@ def grating_volume(rs, n, dx, nz, period, duty, db):
 
 
 def batch_main(args, rs):
-    """Batched Talbot-Lau (64 instances of uneven sizes).
+    """Batched Talbot-Lau (64 instances of uneven sizes) as one graph launch.
 
     8 point sources 20 m upstream (x, y offsets ~ U(+-20 um), seed 3) x 8
     energies (linspace 15-40 keV) on grids 2048, 2048, 3072, 3072, 4096, 4096,
-    6144, 6144 over a 1.024 mm field.  A binary phase grating (period 4.2 um,
-    duty 0.5, pi shift at 27.5 keV; delta ~ 1/E^2) in 4 slices, 0.3 m of free
-    space (a one-slice vacuum volume of that thickness), an absorption analyzer
-    (period 2.1 um, delta 5e-6, beta 4e-7 at 27.5 keV, 50 um in 10 slices), then
-    area_v1 on 8 um pixels summed with seeded spectral weights.  Each component
-    is one captured batch over all 64 instances; timed against the same
-    instances run one call at a time, with the detector sums compared.
+    6144, 6144 over a 1.024 mm field.  Per instance: the point source
+    (analytic spherical wave, FFT2, cutoff, IFFT2), a binary phase grating
+    (period 4.2 um, duty 0.5, pi shift at 27.5 keV; delta ~ 1/E^2) in 4 slices,
+    0.3 m of free space, an absorption analyzer (period 2.1 um, delta 5e-6,
+    beta 4e-7 at 27.5 keV, 50 um in 10 slices), a 5 cm leg, and area_v1 on
+    8 um pixels accumulated with seeded spectral weights into one image.  The
+    whole run -- 64 instances, source to weighted detector sum -- is captured
+    once and replayed as a single graph launch; timed against the same calls
+    issued one by one, with the two images compared bit for bit.
     """
     import torch  # noqa: F401  (HIP runtime shared with the library)
+    SP = importlib.import_module("3d_ravesim_amd.spectral")
     B = importlib.import_module("3d_ravesim_amd.batch")
     lib = rs._lib
     fov = 1.024e-3
@@ -800,136 +821,101 @@ def batch_main(args, rs):
     src = rng.uniform(-20e-6, 20e-6, size=(8, 2))
     weights = rng.uniform(0.5, 1.5, 8)
     weights /= weights.sum()
-    e_d, wl_d = 27.5, rs.convert_energy_wavelength(27.5e3)
+    e_d = 27.5
+    wl_d = rs.convert_energy_wavelength(e_d * 1e3)
     delta_g = 6.4e-7
     t_g = wl_d / (2 * delta_g)
+    z_src, gap, leg, ds = 20.0, 0.3, 0.05, 8e-6
     vols, insts = [], []
     t0 = time.perf_counter()
-    for e, n in zip(energies, sizes):
+    for e, n, w in zip(energies, sizes, weights):
         wl = rs.convert_energy_wavelength(e * 1e3)
         dx = fov / n
         s2 = (e_d / e) ** 2
         g = grating_volume(rs, n, dx, 4, 4.2e-6, 0.5, np.array([0, delta_g * s2 + 1e-9j * s2 ** 1.5]))
         a = grating_volume(rs, n, dx, 10, 2.1e-6, 0.5, np.array([0, 5e-6 * s2 + 4e-7j * s2 ** 2]))
-        gap = rs.DeviceVolume.from_numpy(np.zeros((1, 2, 2), np.uint8), np.array([0.0]))
-        vols += [g, a, gap]
+        vols += [g, a]
         for xs, ys in src:
-            u = rs.DeviceVector(n * n, np.complex64)
-            U = rs.DeviceVector(n * n, np.complex64)
-            insts.append(dict(u=u, U=U, n=n, dx=dx, wl=wl, xs=xs, ys=ys, g=g, a=a, gap=gap, e=e))
+            insts.append(SP.Instance(
+                nx=n, ny=n, dx=dx, wl=wl, cutoff_freq=0.4 / dx, source_x=float(xs),
+                source_y=float(ys), source_distance=z_src, weight=float(w) / 8.0,
+                stages=[SP.SampleStage(g, t_g / 4, 4), SP.GapStage(gap),
+                        SP.SampleStage(a, 5e-6, 10)]))
+    det_n = int(fov // ds)
+    z_det = z_src + t_g + gap + 50e-6 + leg
+    run = SP.SpectralRun(insts, SP.Detector(det_n, det_n, ds, ds, z_det, leg))
     log(f"64 instances, volumes built in {time.perf_counter() - t0:.1f}s")
-
-    def init_fields():
-        for q in insts:
-            lib.call("rsim_propagate_analytically_2d", q["u"].ptr, q["n"], q["n"], lib.RSIM_C64,
-                     q["dx"], q["dx"], 20.0, q["xs"], q["ys"], q["wl"], None)
-        lib.call("rsim_synchronize", None)
-
-    def item(q, vol, pz):
-        return B.BatchItem(u=q["u"], U=q["U"], nx=q["n"], ny=q["n"], dx=q["dx"], wl=q["wl"],
-                           cutoff_freq=0.4 / q["dx"], volume=vol, pixel_size_x=q["dx"],
-                           pixel_size_z=pz)
-
-    comps = [("g", 4, t_g / 4), ("gap", 1, 0.3), ("a", 10, 5e-6)]
-    t0 = time.perf_counter()
-    batches = [B.MultisliceBatch([item(q, q[k], pz) for q in insts], 0, ns) for k, ns, pz in comps]
-    log(f"3 batches captured in {time.perf_counter() - t0:.1f}s "
-        f"({sum(b.kernels for b in batches)} kernel nodes, {batches[0].groups} shape groups)")
-    det_n = int(fov // 8e-6)
-    det = C.c_void_p()
-    lib.call("rsim_malloc", C.byref(det), det_n * det_n * 8)
-
-    def detect():
-        lib.call("rsim_memset", det, 0, det_n * det_n * 8, None)
-        for k, q in enumerate(insts):
-            w = float(weights[k // 8]) / 8.0
-            lib.call("rsim_detector_area_v1_accumulate", q["u"].ptr, q["n"], q["n"], lib.RSIM_C64,
-                     q["dx"], q["dx"], det_n, det_n, 8e-6, 8e-6, 20.3, w, det, None)
-        out = np.empty((det_n, det_n))
-        lib.call("rsim_memcpy_d2h", out.ctypes.data, det, out.nbytes, None)
-        return out
-
-    slices = sum(ns for _, ns, _ in comps)
-    points = sum(q["n"] ** 2 for q in insts)
-    # batched: one graph launch per component
-    init_fields()
-    for b in batches:
-        b.run()
+    # call by call (the first run builds tables and tunes the launch shapes)
+    run.run()
+    run.synchronize()
     reps = max(1, args.steps // 5)
     ts = []
     for _ in range(reps):
-        init_fields()
         t0 = time.perf_counter()
-        for b in batches:
-            b.run(sync=False)
-        lib.call("rsim_synchronize", None)
+        run.run()
+        run.synchronize()
         ts.append(time.perf_counter() - t0)
-    t_batch = min(ts)
-    img_b = detect()
-    # the same instances one call at a time
-    def sequential():
-        for k, ns, pz in comps:
-            for q in insts:
-                lib.call("rsim_sample_apply_2d", q["u"].ptr, q["U"].ptr, q["n"], q["n"], lib.RSIM_C64,
-                         q["dx"], q["dx"], q["wl"], 0.4 / q["dx"], q[k].handle, q["dx"], q["dx"], pz,
-                         0.0, 0.0, 0, ns, lib.RSIM_FUSED, 0, None)
-        lib.call("rsim_synchronize", None)
-    init_fields()
-    sequential()
+    t_eager = min(ts)
+    img_eager = run.image()
+    # the same run captured once, replayed as one graph launch
+    t0 = time.perf_counter()
+    graph = run.capture()
+    t_capture = time.perf_counter() - t0
+    run.launch()
+    run.synchronize()
     ts = []
     for _ in range(reps):
-        init_fields()
         t0 = time.perf_counter()
-        sequential()
+        run.launch()
+        run.synchronize()
         ts.append(time.perf_counter() - t0)
-    t_seq = min(ts)
-    img_s = detect()
-    same = bool(np.array_equal(img_b, img_s))
-    log(f"batched {t_batch * 1e3:.1f} ms, sequential {t_seq * 1e3:.1f} ms for 64 instances x {slices} slices; "
-        f"detector sums identical: {same}")
+    t_graph = min(ts)
+    img_graph = run.image()
+    same = bool(np.array_equal(img_graph, img_eager))
+    slices_per_inst = 4 + 1 + 10 + 1     # grating, gap, analyzer, leg (propagations)
+    points = sum(q.nx * q.ny for q in insts)
+    log(f"one graph launch ({graph.nodes} nodes): {t_graph * 1e3:.1f} ms; call by call "
+        f"{t_eager * 1e3:.1f} ms; images identical: {same}")
     # accuracy: two instances (2048^2 and 6144^2) against the plain complex128 path
     errs = {}
     for k in (0, 63):
         q = insts[k]
-        n = q["n"]
-        u2 = rs.DeviceVector(n * n, np.complex128)
-        U2 = rs.DeviceVector(n * n, np.complex128)
-        lib.call("rsim_propagate_analytically_2d", u2.ptr, n, n, lib.RSIM_C128, q["dx"], q["dx"], 20.0,
-                 q["xs"], q["ys"], q["wl"], None)
-        for kk, ns, pz in comps:
-            lib.call("rsim_sample_apply_2d", u2.ptr, U2.ptr, n, n, lib.RSIM_C128, q["dx"], q["dx"],
-                     q["wl"], 0.4 / q["dx"], q[kk].handle, q["dx"], q["dx"], pz, 0.0, 0.0, 0, ns,
-                     lib.RSIM_PLAIN, 0, None)
-        a = q["u"].to_numpy()
-        b2 = u2.to_numpy()
-        errs[f"{n}^2 E={q['e']:.1f}keV"] = float(np.linalg.norm(a - b2) / np.linalg.norm(b2))
-        u2.drop()
-        U2.drop()
-    log(f"fused c64 vs plain c128 field after the setup: {errs}")
-    for b in batches:
-        b.close()
-    # small fields, where one instance cannot fill the GPU: 64 x 512^2, same optics at 27.5 keV
+        one = SP.SpectralRun([q], SP.Detector(det_n, det_n, ds, ds, z_det, leg),
+                             dtype=np.complex128)
+        one.run()
+        one.synchronize()
+        a64 = run.fields[k][0].to_numpy()
+        b128 = one.fields[0][0].to_numpy()
+        errs[f"{q.nx}^2 E={energies[k // 8]:.1f}keV"] = float(np.linalg.norm(a64 - b128) / np.linalg.norm(b128))
+        one.close()
+    log(f"fused c64 vs plain c128 field at the detector plane: {errs}")
+    run.close()
+    # small fields, where one instance cannot fill the GPU: 64 x 512^2, one
+    # batched slice loop (instances grouped 16 per launch) vs one call each
     n_s = 512
     dx_s = fov / 4 / n_s
     wl_s = rs.convert_energy_wavelength(27.5e3)
     g_s = grating_volume(rs, n_s, dx_s, 4, 4.2e-6, 0.5, np.array([0, delta_g + 1e-9j]))
     small = []
     for k in range(64):
-        q = dict(u=rs.DeviceVector(n_s * n_s, np.complex64), U=rs.DeviceVector(n_s * n_s, np.complex64),
-                 n=n_s, dx=dx_s, wl=wl_s, xs=float(src[k % 8, 0]) / 4, ys=float(src[k % 8, 1]) / 4, g=g_s)
-        small.append(q)
-    bs = B.MultisliceBatch([item(q, g_s, t_g / 4) for q in small], 0, 4)
+        small.append(dict(u=rs.DeviceVector(n_s * n_s, np.complex64),
+                          U=rs.DeviceVector(n_s * n_s, np.complex64),
+                          xs=float(src[k % 8, 0]) / 4, ys=float(src[k % 8, 1]) / 4))
+    bs = B.MultisliceBatch([B.BatchItem(u=q["u"], U=q["U"], nx=n_s, ny=n_s, dx=dx_s, wl=wl_s,
+                                        cutoff_freq=0.4 / dx_s, volume=g_s, pixel_size_x=dx_s,
+                                        pixel_size_z=t_g / 4) for q in small], 0, 4)
 
     def init_small():
         for q in small:
-            lib.call("rsim_propagate_analytically_2d", q["u"].ptr, n_s, n_s, lib.RSIM_C64, dx_s, dx_s,
-                     20.0, q["xs"], q["ys"], wl_s, None)
+            lib.call("rsim_propagate_analytically_2d", q["u"].ptr, n_s, n_s, lib.RSIM_C64, dx_s,
+                     dx_s, 20.0, q["xs"], q["ys"], wl_s, None)
         lib.call("rsim_synchronize", None)
 
     def seq_small():
         for q in small:
-            lib.call("rsim_sample_apply_2d", q["u"].ptr, q["U"].ptr, n_s, n_s, lib.RSIM_C64, dx_s, dx_s,
-                     wl_s, 0.4 / dx_s, g_s.handle, dx_s, dx_s, t_g / 4, 0.0, 0.0, 0, 4, lib.RSIM_FUSED, 0, None)
+            lib.call("rsim_sample_apply_2d", q["u"].ptr, q["U"].ptr, n_s, n_s, lib.RSIM_C64, dx_s,
+                     dx_s, wl_s, 0.4 / dx_s, g_s.handle, dx_s, dx_s, t_g / 4, 0.0, 0.0, 0, 4,
+                     lib.RSIM_FUSED, 0, None)
         lib.call("rsim_synchronize", None)
     init_small()
     bs.run()
@@ -948,25 +934,22 @@ def batch_main(args, rs):
     log(f"64 x 512^2 x 4 slices: batched {t_sb * 1e3:.2f} ms, sequential {t_ss * 1e3:.2f} ms "
         f"({t_ss / t_sb:.1f}x)")
     bs.close()
-    lib.call("rsim_free", det)
     print(json.dumps({
-        "metric": "batched Talbot-Lau instance-slice-steps/s (64 instances, 2048^2-6144^2, c64)",
-        "value": round(64 * slices / t_batch, 1), "unit": "instance-slice-steps/s", "n_gpus": 1,
-        "steps": reps, "higher_is_better": True, "dtype": "c64", "data": "synthetic",
-        "ms_per_run": round(t_batch * 1e3, 2), "sequential_ms_per_run": round(t_seq * 1e3, 2),
-        "batched_over_sequential": round(t_seq / t_batch, 3),
-        "gpoint_slices_per_s": round(points * slices / t_batch / 1e9, 2),
-        "graph_kernel_nodes": sum(b.kernels for b in batches), "shape_groups": batches[0].groups,
+        "metric": "batched Talbot-Lau instance-slice-steps/s (64 instances, 2048^2-6144^2, c64, one graph launch)",
+        "value": round(64 * slices_per_inst / t_graph, 1), "unit": "instance-slice-steps/s",
+        "n_gpus": 1, "steps": reps, "higher_is_better": True, "dtype": "c64", "data": "synthetic",
+        "ms_per_run": round(t_graph * 1e3, 2), "call_by_call_ms_per_run": round(t_eager * 1e3, 2),
+        "graph_over_call_by_call": round(t_eager / t_graph, 3), "capture_s": round(t_capture, 2),
+        "graph_nodes": graph.nodes, "gpoint_slices_per_s": round(points * slices_per_inst / t_graph / 1e9, 2),
+        "image_identical_graph_vs_call_by_call": same, "detector_shape": [det_n, det_n],
         "small_512sq_64x4_slices_ms": {"batched": round(t_sb * 1e3, 3), "sequential": round(t_ss * 1e3, 3),
                                        "speedup": round(t_ss / t_sb, 2)},
-        "detector_identical_batched_vs_sequential": same, "detector_shape": [det_n, det_n],
         "rel_l2_field_fused_vs_plain_c128": errs,
-        "config": {"workload": "8 sources (seed 3) x 8 energies 15-40 keV, grids 2048..6144 over 1.024 mm, "
-                               "phase grating 4.2 um (4 slices) + 0.3 m + analyzer 2.1 um 50 um (10 slices), "
-                               "area_v1 8 um with seeded spectral weights"},
+        "config": {"workload": "8 sources (seed 3) x 8 energies 15-40 keV, grids 2048..6144 over 1.024 mm; "
+                               "point source 20 m upstream, phase grating 4.2 um (4 slices) + 0.3 m + "
+                               "analyzer 2.1 um 50 um (10 slices) + 5 cm, area_v1 8 um with seeded "
+                               "spectral weights; the whole run one graph launch"},
     }), flush=True)
-
-
 
 
 def plasma_grids(torch, nz, n, seed):
