@@ -106,6 +106,7 @@ SIGNATURES = {
     "rsim_sample_apply_2d": [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _vp, _d, _d, _d, _d, _d, _i,
                              _i, _i, _i, _vp],
     "rsim_fused_schedule": [_i, _i, _ip, _ip],
+    "rsim_fused_tile": [_i, _i, _ip],
     "rsim_detector_area_v1": [_vp, _i, _i, _i, _d, _d, _i, _i, _d, _d, _d, _dp, _vp],
     "rsim_detector_area_v1_accumulate": [_vp, _i, _i, _i, _d, _d, _i, _i, _d, _d, _d, _d, _vp,
                                          _vp],

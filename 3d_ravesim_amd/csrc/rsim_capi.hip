@@ -54,7 +54,7 @@ int release_tune();
 int release_detector_plans();
 int release_staging();
 int release_volume_tables(unsigned long long serial);
-int fused_schedule_lookup(int nx, int ny, int* rv, int* cv);
+int fused_schedule_lookup(int nx, int ny, int* rv, int* cv, int* tile);
 int fill_zero(void* p, size_t bytes, hipStream_t s);
 int stream_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 int detector_area_v1_noalloc(const void* u, int nx, int ny, int dtype, double dx, double dy,
@@ -898,7 +898,14 @@ int rsim_sample_apply_2d_history(void* u, void* U, int nx, int ny, int dtype, do
 
 int rsim_fused_schedule(int nx, int ny, int* row_variant, int* col_variant) {
   if (!row_variant || !col_variant) return fail(RSIM_E_INVALID_ARG, "NULL output pointer");
-  return fused_schedule_lookup(nx, ny, row_variant, col_variant);
+  int tile = 0;
+  return fused_schedule_lookup(nx, ny, row_variant, col_variant, &tile);
+}
+
+int rsim_fused_tile(int nx, int ny, int* tile_width) {
+  if (!tile_width) return fail(RSIM_E_INVALID_ARG, "NULL output pointer");
+  int rv = 0, cv = 0;
+  return fused_schedule_lookup(nx, ny, &rv, &cv, tile_width);
 }
 
 int rsim_detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double dy,

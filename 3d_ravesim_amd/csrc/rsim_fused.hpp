@@ -170,8 +170,12 @@ struct RowInst {
   cpx<float>* out;
   TransArgsF ta;   // ta.ids: this launch's voxel slice
 };
+// in_tile: layout of the launch's inputs.  0: row-major; w > 0: column tiles
+// of width w as a tiled column pass leaves them (element (y, x) at
+// ((x / w) * ny + y) * w + x % w), w a power of two dividing the shape's TC.
 struct RowBatch {
   RowInst inst[kMaxBatch];
+  int in_tile;
 };
 struct ColInst {
   const cpx<float>* in;
@@ -181,8 +185,12 @@ struct ColInst {
   const double2* Ay;
   const int* Rmax;
 };
+// out_tile: the launch stores its result in column tiles of its own width C
+// (each workgroup writes one contiguous block of C x ny elements), for a row
+// pass that reads them with in_tile = C; 0: row-major.
 struct ColBatch {
   ColInst inst[kMaxBatch];
+  int out_tile;
 };
 
 // Occupancy request of a row launch shape: LD = 2 asks the compiler for three
@@ -209,16 +217,26 @@ __global__ void __launch_bounds__(F::TC * C)
   const int tid = threadIdx.x;
   const int c = tid % C;
   const int t = tid / C;
-  const int y = blockIdx.x * C + c;
+  const int tile = rb.in_tile;   // uniform
+  // tiled input: rows that share 128 B lines of a tile go to one XCD's L2
+  const int y = (tile ? xcd_tile(blockIdx.x, gridDim.x) : (int)blockIdx.x) * C + c;
   const bool live = y < ny;
   const int yy = live ? y : 0;
   const Gio<float> gi(in, nbytes), go(out, nbytes);
   const uint32_t vo = ((uint32_t)yy * N + t) * 8u;
+  // element x = t + m (m a multiple of TC, and so of the tile width) of a
+  // tiled row sits m * ny elements after element t
+  uint32_t vi = vo, es = 8u;
+  if (tile) {
+    const int ts = __builtin_ctz(tile);
+    vi = (((((uint32_t)t >> ts) * (uint32_t)ny + (uint32_t)yy) << ts) + ((uint32_t)t & (tile - 1))) * 8u;
+    es = (uint32_t)ny * 8u;
+  }
   if constexpr (MODE != ROW_LAST && LUT) table_to_lds(lds_lut, ta.lut);
   if constexpr (MODE != ROW_LAST && TM == TM_BILINEAR) table_to_lds(lds_dbf, ta.dbf);
   cpx<float> v[F::E];
 #pragma unroll
-  for (int e = 0; e < F::E; ++e) v[e] = gi.ld(vo, first_off<F>(e, 8u));
+  for (int e = 0; e < F::E; ++e) v[e] = gi.ld(vi, first_off<F>(e, es));
   if constexpr (MODE == ROW_FIRST) {
     const RowT r = row_state<TM>(ta, yy);
     if constexpr (LUT || TM == TM_BILINEAR) __syncthreads();
@@ -335,7 +353,7 @@ __global__ void __launch_bounds__(F::TC * C)
   }
 }
 
-template <class F, int C, int PS, bool SPLIT, int LD, int SPEC, typename TW>
+template <class F, int C, int PS, bool SPLIT, int LD, int SPEC, bool TILE, typename TW>
 __global__ void __launch_bounds__(F::TC * C) ms_col_kernel(ColBatch cb, int nx, uint32_t nbytes,
                                                            StageTw<TW> tw, float spec_scale) {
   constexpr int N = F::N;
@@ -350,7 +368,8 @@ __global__ void __launch_bounds__(F::TC * C) ms_col_kernel(ColBatch cb, int nx, 
   const int tid = threadIdx.x;
   const int c = tid % C;
   const int t = tid / C;
-  const int x = xcd_tile(blockIdx.x, gridDim.x) * C + c;
+  const int g = xcd_tile(blockIdx.x, gridDim.x);
+  const int x = g * C + c;
   const bool live = x < nx;
   const int xx = live ? x : 0;
   const Gio<float> gi(in, nbytes), go(out, nbytes), gs(spec, nbytes);
@@ -383,7 +402,17 @@ __global__ void __launch_bounds__(F::TC * C) ms_col_kernel(ColBatch cb, int nx, 
     if ((e & 3) == 3) compiler_fence();
   }
   reg_fft<F, 1, C, PS, SPLIT>(v, tw, lds, opaque_index<(LD != 0)>(t), opaque_index<(LD != 0)>(c));
-  if (live) {
+  if constexpr (TILE) {
+    // tile g = this workgroup's C columns, rows in order: full 128 B lines.
+    // The row stride of a tile comes from the launch (out_tile == C), so the
+    // per-element offsets stay scalar as with the row-major stride.
+    const uint32_t tb = (uint32_t)cb.out_tile * 8u;
+    const uint32_t vt = ((uint32_t)g * N + (uint32_t)t) * tb + (uint32_t)c * 8u;
+    if (live) {
+#pragma unroll
+      for (int e = 0; e < F::E; ++e) go.st(v[e], vt, last_off<F>(e, tb));
+    }
+  } else if (live) {
 #pragma unroll
     for (int e = 0; e < F::E; ++e) go.st(v[e], vo, last_off<F>(e, row_b));
   }
@@ -420,6 +449,7 @@ struct RowLaunch {
                      hipStream_t s, const AdjArgs& aj) {
     using F = typename S::F;
     constexpr int C = S::CC;
+    if (b.in_tile && (F::TC % b.in_tile != 0 || (b.in_tile & (b.in_tile - 1)) != 0)) return false;
     dim3 grid((ny + C - 1) / C, count), block(S::threads);
     hipLaunchKernelGGL((ms_row_kernel<F, C, S::PS, S::SPLIT, S::LD, MODE, TM, TW>), grid, block, 0, s, b, ny, nb, tw, aj);
     return true;
@@ -457,15 +487,21 @@ struct RowLaunch {
 
 template <class S, typename TW>
 struct ColLaunch {
-  static void go(bool spec_out, const ColBatch& b, int count, int nx, uint32_t nb,
+  static bool go(bool spec_out, const ColBatch& b, int count, int nx, uint32_t nb,
                  const StageTw<TW>& tw, float spec_scale, hipStream_t s) {
     using F = typename S::F;
     constexpr int C = S::CC;
     dim3 grid((nx + C - 1) / C, count), block(S::threads);
-    if (spec_out)
-      hipLaunchKernelGGL((ms_col_kernel<F, C, S::PS, S::SPLIT, S::LD, 1, TW>), grid, block, 0, s, b, nx, nb, tw, spec_scale);
-    else
-      hipLaunchKernelGGL((ms_col_kernel<F, C, S::PS, S::SPLIT, S::LD, 0, TW>), grid, block, 0, s, b, nx, nb, tw, spec_scale);
+    if (b.out_tile) {
+      // whole tiles only; the spectrum output is row-major
+      if (b.out_tile != C || nx % C != 0 || spec_out) return false;
+      hipLaunchKernelGGL((ms_col_kernel<F, C, S::PS, S::SPLIT, S::LD, 0, true, TW>), grid, block, 0, s, b, nx, nb, tw, spec_scale);
+    } else if (spec_out) {
+      hipLaunchKernelGGL((ms_col_kernel<F, C, S::PS, S::SPLIT, S::LD, 1, false, TW>), grid, block, 0, s, b, nx, nb, tw, spec_scale);
+    } else {
+      hipLaunchKernelGGL((ms_col_kernel<F, C, S::PS, S::SPLIT, S::LD, 0, false, TW>), grid, block, 0, s, b, nx, nb, tw, spec_scale);
+    }
+    return true;
   }
 };
 
@@ -533,10 +569,28 @@ bool col_go_ld(int v, bool spec_out, const ColBatch& b, int count, int nx, const
 #define RSIM_ROW_CASE(N, V, E, C, SP, LD) \
   case V:                                \
     return RowLaunch<Shape<N, E, C, SP, LD>, TW>::go(mode, tm, b, count, ny, nb, tw, s, aj);
-#define RSIM_COL_CASE(N, V, E, C, SP, LD)                                              \
-  case V:                                                                             \
-    ColLaunch<Shape<N, E, C, SP, LD>, TW>::go(spec_out, b, count, nx, nb, tw, ss, s); \
-    return true;
+#define RSIM_COL_CASE(N, V, E, C, SP, LD) \
+  case V:                                \
+    return ColLaunch<Shape<N, E, C, SP, LD>, TW>::go(spec_out, b, count, nx, nb, tw, ss, s);
+
+// Columns per workgroup (= the tile width of a tiled output) of column
+// variant v of length N.
+#define RSIM_COL_WIDTH_CASE(N, V, E, C, SP, LD) \
+  case V:                                      \
+    return C;
+template <int N>
+constexpr int col_width(int v) {
+  if (v >= kColBase) {
+    switch (v - kColBase) {
+      RSIM_COL_VARIANTS_LD(N, RSIM_COL_WIDTH_CASE)
+      default: return 0;
+    }
+  }
+  switch (v) {
+    RSIM_COL_VARIANTS(N, RSIM_COL_WIDTH_CASE)
+    default: return 0;
+  }
+}
 
 // Launcher over one variant list (NAME: row_go or row_go_ld); the base one
 // hands the LD ids to row_go_ld (when HAS_LD).

@@ -80,25 +80,42 @@ bool fused_col_b(int N, int v, bool spec_out, const ColBatch& b, int count, int 
   }
 }
 
-// single-field launches
+// Columns per workgroup of column variant v of length N (0: no such variant).
+int fused_col_width(int N, int v) {
+  switch (N) {
+    case 256: return col_width<256>(v);
+    case 512: return col_width<512>(v);
+    case 1024: return col_width<1024>(v);
+    case 2048: return col_width<2048>(v);
+    case 3072: return col_width<3072>(v);
+    case 4096: return col_width<4096>(v);
+    case 6144: return col_width<6144>(v);
+    case 8192: return col_width<8192>(v);
+    default: return 0;
+  }
+}
+
+// single-field launches; in_tile / out_tile: tiled intermediate layout
 template <typename TW>
 bool fused_row(int N, int v, int mode, int tm, const void* in, void* out, int ny,
                const StageTw<TW>& tw, const TransArgsF& ta, hipStream_t s,
-               const AdjArgs& aj = AdjArgs{}) {
+               const AdjArgs& aj = AdjArgs{}, int in_tile = 0) {
   RowBatch b{};
   b.inst[0].in = reinterpret_cast<const cpx<float>*>(in);
   b.inst[0].out = reinterpret_cast<cpx<float>*>(out);
   b.inst[0].ta = ta;
+  b.in_tile = in_tile;
   return fused_row_b<TW>(N, v, mode, tm, b, 1, ny, tw, s, aj);
 }
 
 template <typename TW>
 bool fused_col(int N, int v, bool spec_out, const void* in, void* out, void* spec, int nx,
                const StageTw<TW>& tw, const double2* Bx, const double2* Ay, const int* Rmax,
-               float ss, hipStream_t s) {
+               float ss, hipStream_t s, int out_tile = 0) {
   ColBatch b{};
   b.inst[0] = ColInst{reinterpret_cast<const cpx<float>*>(in), reinterpret_cast<cpx<float>*>(out),
                       reinterpret_cast<cpx<float>*>(spec), Bx, Ay, Rmax};
+  b.out_tile = out_tile;
   return fused_col_b<TW>(N, v, spec_out, b, 1, nx, tw, ss, s);
 }
 
@@ -259,7 +276,16 @@ int material_tables(const rsim_volume* vol, double K, const float4** lut, const 
 // fastest.  RAVESIM_FUSED_VARIANT="row,col" overrides the choice.
 // ---------------------------------------------------------------------------
 namespace {
-std::map<std::tuple<int, int, int, int>, std::pair<int, int>> g_tune;
+// Tuned launch shapes of one (device, nx, ny, transmission, arithmetic).
+// (rv, cv): the fastest slice with both passes in place on one buffer.
+// tile > 0: the fastest slice overall has the column pass store tiles of
+// that width, which the next row pass reads (ping-pong over two buffers),
+// with variants (trv, tcv).
+struct FusedChoice {
+  int rv = 0, cv = 0;
+  int trv = 0, tcv = 0, tile = 0;
+};
+std::map<std::tuple<int, int, int, int>, FusedChoice> g_tune;
 std::mutex g_tune_mu;
 
 // Timing field of the autotuner: hashed values in [-1, 1) (a field of zeros
@@ -309,56 +335,94 @@ int time_launches(hipStream_t s, const std::function<bool()>& launch, float* ms,
   hipEventDestroy(b);
   return RSIM_OK;
 }
+
+// Tile width usable between column variant cv and the next row pass, or 0.
+int tile_width(int nx, int ny, int cv) {
+  const int w = fused_col_width(ny, cv);
+  return (w >= 4 && nx % w == 0) ? w : 0;
+}
+
+void pick(const FusedChoice& fc, int* rv, int* cv, int* tile) {
+  if (tile && fc.tile) {
+    *rv = fc.trv;
+    *cv = fc.tcv;
+    *tile = fc.tile;
+    return;
+  }
+  *rv = fc.rv;
+  *cv = fc.cv;
+  if (tile) *tile = 0;
+}
 }  // namespace
+
+// Launch shapes of a fused slice.  tile == nullptr: the caller runs both
+// passes in place on one buffer; otherwise *tile receives the tile width of
+// the intermediate layout when a tiled slice is faster (0: in place).
+// RAVESIM_FUSED_VARIANT="row,col" or "row,col,tiled" (tiled 1: column tiles
+// where the column width allows): a fixed schedule instead of the tuned one.
+static bool env_schedule(int nx, int ny, int* rv, int* cv, int* tile) {
+  const char* env = std::getenv("RAVESIM_FUSED_VARIANT");
+  if (!env) return false;
+  int r = 0, c = 0, t = 0;
+  const int got = std::sscanf(env, "%d,%d,%d", &r, &c, &t);
+  if (got < 2 || r < 0 || r >= kRowVariants || c < 0 || c >= kColVariants) return false;
+  *rv = r;
+  *cv = c;
+  if (tile) *tile = (got == 3 && t) ? tile_width(nx, ny, c) : 0;
+  return true;
+}
 
 template <typename TW>
 int fused_schedule(int nx, int ny, int tm, const StageTw<TW>& tx, const StageTw<TW>& ty,
-                   const TransArgsF& ta, const ChirpTables* ct, hipStream_t s, int* rv, int* cv) {
-  if (const char* env = std::getenv("RAVESIM_FUSED_VARIANT")) {
-    int r = 0, c = 0;
-    if (std::sscanf(env, "%d,%d", &r, &c) == 2 && r >= 0 && r < kRowVariants && c >= 0 &&
-        c < kColVariants) {
-      *rv = r;
-      *cv = c;
-      return RSIM_OK;
-    }
-  }
+                   const TransArgsF& ta, const ChirpTables* ct, hipStream_t s, int* rv, int* cv,
+                   int* tile = nullptr) {
+  if (env_schedule(nx, ny, rv, cv, tile)) return RSIM_OK;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   const int math = std::is_same<TW, twc>::value ? 4 : sizeof(TW) == 8 ? 2 : 0;
   auto key = std::make_tuple(current_device(), nx, ny, tm + 4 * math);
   auto it = g_tune.find(key);
   if (it != g_tune.end()) {
-    *rv = it->second.first;
-    *cv = it->second.second;
+    pick(it->second, rv, cv, tile);
     return RSIM_OK;
   }
   RSIM_CHECK(capture_miss("launch-shape tuning"));
-  void* tmp = nullptr;
   const size_t tbytes = (size_t)nx * ny * 8;
-  RSIM_HIP(hipMalloc(&tmp, tbytes));
+  struct Bufs {
+    void* a = nullptr;
+    void* b = nullptr;
+    ~Bufs() {
+      if (a) (void)hipFree(a);
+      if (b) (void)hipFree(b);
+    }
+  } buf;
+  RSIM_HIP(hipMalloc(&buf.a, tbytes));
+  RSIM_HIP(hipMalloc(&buf.b, tbytes));
+  void* tmp = buf.a;
+  void* tmp2 = buf.b;
   const bool verbose = std::getenv("RAVESIM_TUNE_VERBOSE") != nullptr;
   const char* mname = math == 4 ? "comp32" : math ? "fp64" : "fp32";
-  auto row = [&](int v) { return fused_row(nx, v, ROW_MID, tm, tmp, tmp, ny, tx, ta, s); };
-  auto col = [&](int v) {
-    return fused_col(ny, v, false, tmp, tmp, tmp, nx, ty, ct->Bx, ct->Ay, ct->Rmax, 1.f, s);
+  auto row = [&](int v, const void* in, void* out, int in_tile) {
+    return fused_row(nx, v, ROW_MID, tm, in, out, ny, tx, ta, s, AdjArgs{}, in_tile);
   };
-  // each shape on its own, two interleaved rounds (least of the two) ...
+  auto col = [&](int v, const void* in, void* out, int out_tile) {
+    return fused_col(ny, v, false, in, out, out, nx, ty, ct->Bx, ct->Ay, ct->Rmax, 1.f, s,
+                     out_tile);
+  };
+  // each shape on its own, in place, two interleaved rounds (least of the two) ...
   float tr[kRowVariants], tc[kColVariants];
   for (float& x : tr) x = 1e30f;
   for (float& x : tc) x = 1e30f;
   for (int round = 0; round < 2; ++round) {
     for (int v = 0; v < kRowVariants; ++v) {
       float ms = -1.f;
-      int rc = tune_fill(tmp, tbytes, s);
-      if (rc == RSIM_OK) rc = time_launches(s, [&] { return row(v); }, &ms, 5);
-      if (rc != RSIM_OK) { hipFree(tmp); return rc; }
+      RSIM_CHECK(tune_fill(tmp, tbytes, s));
+      RSIM_CHECK(time_launches(s, [&] { return row(v, tmp, tmp, 0); }, &ms, 5));
       if (ms > 0.f && ms < tr[v]) tr[v] = ms;
     }
     for (int v = 0; v < kColVariants; ++v) {
       float ms = -1.f;
-      int rc = tune_fill(tmp, tbytes, s);
-      if (rc == RSIM_OK) rc = time_launches(s, [&] { return col(v); }, &ms, 5);
-      if (rc != RSIM_OK) { hipFree(tmp); return rc; }
+      RSIM_CHECK(tune_fill(tmp, tbytes, s));
+      RSIM_CHECK(time_launches(s, [&] { return col(v, tmp, tmp, 0); }, &ms, 5));
       if (ms > 0.f && ms < tc[v]) tc[v] = ms;
     }
   }
@@ -381,45 +445,64 @@ int fused_schedule(int nx, int ny, int tm, const StageTw<TW>& tx, const StageTw<
       std::fprintf(stderr, "[rsim tune] %dx%d math=%s col variant %d: %.1f us\n", nx, ny, mname, v,
                    tc[v] < 1e30f ? tc[v] * 1e3f : -1.f);
   }
-  // ... then the leading pairs as the slice loop runs them (row pass, column
-  // pass, on one buffer): the pair, not the sum of its parts, is what a slice costs
-  int best_r = rb[0] < 0 ? 0 : rb[0], best_c = cb[0] < 0 ? 0 : cb[0];
+  // ... then the leading pairs as the slice loop runs them: row pass and
+  // column pass in place on one buffer, and (where the column width allows)
+  // the tiled pair, row pass from tiles in one buffer into the other and
+  // column pass back as tiles.  The pair, not the sum of its parts, is what
+  // a slice costs.
+  FusedChoice fc;
+  fc.rv = rb[0] < 0 ? 0 : rb[0];
+  fc.cv = cb[0] < 0 ? 0 : cb[0];
   if (rb[0] >= 0 && cb[0] >= 0) {
-    float best = 1e30f;
+    float best = 1e30f, best_any = 1e30f;
     for (int i = 0; i < 2; ++i)
       for (int j = 0; j < 2; ++j) {
         if (rb[i] < 0 || cb[j] < 0) continue;
-        float ms = -1.f;
-        int rc = tune_fill(tmp, tbytes, s);
-        if (rc == RSIM_OK) rc = time_launches(s, [&] { return row(rb[i]) && col(cb[j]); }, &ms, 8);
-        if (rc != RSIM_OK) { hipFree(tmp); return rc; }
-        if (verbose)
-          std::fprintf(stderr, "[rsim tune] %dx%d math=%s pair (%d, %d): %.1f us\n", nx, ny, mname,
-                       rb[i], cb[j], ms * 1e3f);
-        if (ms > 0.f && ms < best) {
-          best = ms;
-          best_r = rb[i];
-          best_c = cb[j];
+        const int w = tile_width(nx, ny, cb[j]);
+        for (int tiled = 0; tiled < 2; ++tiled) {
+          if (tiled && !w) continue;
+          float ms = -1.f;
+          RSIM_CHECK(tune_fill(tmp, tbytes, s));
+          if (tiled) RSIM_CHECK(tune_fill(tmp2, tbytes, s));
+          if (tiled)
+            RSIM_CHECK(time_launches(
+                s, [&] { return row(rb[i], tmp2, tmp, w) && col(cb[j], tmp, tmp2, w); }, &ms, 8));
+          else
+            RSIM_CHECK(time_launches(
+                s, [&] { return row(rb[i], tmp, tmp, 0) && col(cb[j], tmp, tmp, 0); }, &ms, 8));
+          if (verbose)
+            std::fprintf(stderr, "[rsim tune] %dx%d math=%s pair (%d, %d)%s: %.1f us\n", nx, ny,
+                         mname, rb[i], cb[j], tiled ? " tiled" : "", ms * 1e3f);
+          if (ms <= 0.f) continue;
+          if (!tiled && ms < best) {
+            best = ms;
+            fc.rv = rb[i];
+            fc.cv = cb[j];
+          }
+          if (ms < best_any) {
+            best_any = ms;
+            fc.trv = rb[i];
+            fc.tcv = cb[j];
+            fc.tile = tiled ? w : 0;
+          }
         }
       }
   }
   RSIM_HIP(hipStreamSynchronize(s));
-  RSIM_HIP(hipFree(tmp));
-  g_tune[key] = {best_r, best_c};
-  *rv = best_r;
-  *cv = best_c;
+  g_tune[key] = fc;
+  pick(fc, rv, cv, tile);
   return RSIM_OK;
 }
 
-int fused_schedule_lookup(int nx, int ny, int* rv, int* cv) {
+int fused_schedule_lookup(int nx, int ny, int* rv, int* cv, int* tile) {
+  if (env_schedule(nx, ny, rv, cv, tile)) return RSIM_OK;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   *rv = *cv = -1;
+  if (tile) *tile = -1;
   for (auto& kv : g_tune) {
     if (std::get<0>(kv.first) == current_device() && std::get<1>(kv.first) == nx &&
-        std::get<2>(kv.first) == ny) {
-      *rv = kv.second.first;
-      *cv = kv.second.second;
-    }
+        std::get<2>(kv.first) == ny)
+      pick(kv.second, rv, cv, tile);
   }
   return RSIM_OK;
 }
@@ -733,8 +816,6 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
   if (per_slice)
     RSIM_CHECK(build_run_slice_tables(nx, ny, dx, dy, wl, freq, src, px, py, pz, xpos, ypos,
                                       count, scales, dzs, s, &rts));
-  int rv = 0, cv = 0;
-  RSIM_CHECK(fused_schedule<TW>(nx, ny, tm, tx, ty, ta, ct, s, &rv, &cv));
   // History frames after slice 0: the MID pass bins its field in x into xs
   // (nj x ox doubles, stream-ordered for this run); the y stage follows it.
   // Without a separable plan the pass stores the field into u instead.
@@ -755,14 +836,29 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
       xb.xs = xsb.p;
     }
   }
+  // Intermediate layout.  In place: every pass reads and writes U.  Tiled
+  // (where the tuner found it faster): the column pass stores its output as
+  // column tiles into u, which the next row pass reads back into U; the last
+  // column pass stores row-major.  Not with the three-pass columns, nor when
+  // the MID pass stores history fields into u, nor when the launch shapes
+  // may change from slice to slice (cone-beam tables).
+  const bool may_tile = !big && !per_slice && !(hist && !xbin);
+  int rv = 0, cv = 0, tile = 0;
+  RSIM_CHECK(fused_schedule<TW>(nx, ny, tm, tx, ty, ta, ct, s, &rv, &cv, may_tile ? &tile : nullptr));
+  int w_in = 0;   // tile width of the W that the next row pass reads (0: W in U)
   for (int k = 0; k < count; ++k) {
     const int sl = first + k;
     if (cancel_requested()) {
-      // stop at this slice boundary: W of slice k - 1 is in U, so the row
-      // IFFT leaves u = the field after k slices (U then does not hold FFT2(u))
+      // stop at this slice boundary: the row IFFT of W of slice k - 1 leaves
+      // u = the field after k slices (U then does not hold FFT2(u))
       if (k > 0) {
         ktime_begin(KT_ROW, s);
-        fused_row(nx, rv, ROW_LAST, tm, U, u, ny, tx, ta, s);
+        if (w_in) {
+          fused_row(nx, rv, ROW_LAST, tm, u, U, ny, tx, ta, s, AdjArgs{}, w_in);
+          RSIM_HIP(hipMemcpyAsync(u, U, (size_t)nx * ny * 8, hipMemcpyDeviceToDevice, s));
+        } else {
+          fused_row(nx, rv, ROW_LAST, tm, U, u, ny, tx, ta, s);
+        }
         ktime_end(KT_ROW, s);
         RSIM_LAUNCHED();
       }
@@ -774,7 +870,8 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
       const SliceTables& e = rts.sets[k];
       ct = &e.ct;
       tm = use_maps(ta, e.xm, e.ym, src.vol != nullptr);
-      RSIM_CHECK(fused_schedule<TW>(nx, ny, tm, tx, ty, ta, ct, s, &rv, &cv));
+      RSIM_CHECK(fused_schedule<TW>(nx, ny, tm, tx, ty, ta, ct, s, &rv, &cv,
+                                    may_tile ? &tile : nullptr));
     }
     P.at(ta, sl);
     const bool last = k == count - 1;
@@ -789,8 +886,9 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
       else store.fld_out = reinterpret_cast<cpx<float>*>(u);
     }
     ktime_begin(KT_ROW, s);
-    if (!fused_row(nx, rv, k == 0 ? ROW_FIRST : ROW_MID, tm, k == 0 ? u : U, U, ny, tx, ta, s,
-                   store))
+    const void* rin = k == 0 ? u : (w_in ? u : U);
+    if (!fused_row(nx, rv, k == 0 ? ROW_FIRST : ROW_MID, tm, rin, U, ny, tx, ta, s, store,
+                   k == 0 ? 0 : w_in))
       return fail(RSIM_E_HIP, "no fused row kernel for nx=%d", nx);
     ktime_end(KT_ROW, s);
     RSIM_LAUNCHED();
@@ -808,10 +906,13 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
       if (!big_col_go<TW>(U, nx, ny, t128, wN, ct->Bx, ct->Ay, ct->Rmax, s))
         return fail(RSIM_E_HIP, "no three-pass column schedule for ny=%d", ny);
     } else {
-      // last slice: W -> u (and the spectrum -> U); otherwise in place in U
-      if (!fused_col(ny, cv, last && want_spec, U, last ? u : U, U, nx, ty, ct->Bx, ct->Ay,
-                     ct->Rmax, spec_scale, s))
+      // last slice: W -> u (and the spectrum -> U); otherwise in place in U,
+      // or as tiles into u
+      const int w_out = last ? 0 : tile;
+      if (!fused_col(ny, cv, last && want_spec, U, (last || w_out) ? u : U, U, nx, ty, ct->Bx,
+                     ct->Ay, ct->Rmax, spec_scale, s, w_out))
         return fail(RSIM_E_HIP, "no fused column kernel for ny=%d", ny);
+      w_in = w_out;
     }
     ktime_end(KT_COL, s);
     RSIM_LAUNCHED();

@@ -352,9 +352,10 @@ def main():
         result["rel_l2_field_plain_c64_vs_c128"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
         log(f"plain c64 vs c128 field: {result['rel_l2_field_plain_c64_vs_c128']:.3e}")
         del a, b
-    rv, cv = C.c_int(-1), C.c_int(-1)
+    rv, cv, tw = C.c_int(-1), C.c_int(-1), C.c_int(-1)
     lib.call("rsim_fused_schedule", n, n, C.byref(rv), C.byref(cv))
-    result["fused_launch_variants"] = {"row": rv.value, "col": cv.value}
+    lib.call("rsim_fused_tile", n, n, C.byref(tw))
+    result["fused_launch_variants"] = {"row": rv.value, "col": cv.value, "tile": tw.value}
 
     # ---- CPU baseline: the FP64 oracle on a bounded sample ----------------
     if not args.no_cpu:

@@ -344,10 +344,17 @@ int rsim_batch_destroy(rsim_batch* batch);
 
 /* Launch-shape autotuner of the fused engine (no counterpart in the
  * reference): the first RSIM_FUSED call for an (nx, ny) times each row-pass
- * and column-pass variant on a scratch field and keeps the fastest; the
- * environment variable RAVESIM_FUSED_VARIANT="row,col" overrides it.  Returns
- * the variants in use for (nx, ny), or -1 before the first fused call. */
+ * and column-pass variant on a scratch field, then the leading row x column
+ * pairs as a slice runs them, both in place and with the column pass storing
+ * column tiles that the next row pass reads (two buffers, u and U), and
+ * keeps the fastest; the environment variable
+ * RAVESIM_FUSED_VARIANT="row,col[,tiled]" overrides it.  Returns the variants
+ * of the voxel slice loop for (nx, ny), or -1 before the first fused call. */
 int rsim_fused_schedule(int nx, int ny, int* row_variant, int* col_variant);
+/* Width of the column tiles between the passes of the voxel slice loop for
+ * (nx, ny): 0 = in place, row-major; -1 before the first fused call (no
+ * counterpart in the reference). */
+int rsim_fused_tile(int nx, int ny, int* tile_width);
 
 /* ---- Detector ----
  * Replaces square_and_downsample_2d with the area_v1 integrator
