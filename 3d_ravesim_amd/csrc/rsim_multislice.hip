@@ -278,9 +278,10 @@ int material_tables(const rsim_volume* vol, double K, const float4** lut, const 
 namespace {
 // Tuned launch shapes of one (device, nx, ny, transmission, arithmetic).
 // (rv, cv): the fastest slice with both passes in place on one buffer.
-// tile > 0: the fastest slice overall has the column pass store tiles of
-// that width, which the next row pass reads (ping-pong over two buffers),
-// with variants (trv, tcv).
+// tile > 0: slices run with the column pass storing tiles of that width,
+// which the next row pass reads (ping-pong over two buffers), with variants
+// (trv, tcv): the fastest such pair, used unless an in-place pair is more
+// than 1 % faster.
 struct FusedChoice {
   int rv = 0, cv = 0;
   int trv = 0, tcv = 0, tile = 0;
@@ -453,8 +454,8 @@ int fused_schedule(int nx, int ny, int tm, const StageTw<TW>& tx, const StageTw<
   FusedChoice fc;
   fc.rv = rb[0] < 0 ? 0 : rb[0];
   fc.cv = cb[0] < 0 ? 0 : cb[0];
+  float best = 1e30f, best_tiled = 1e30f;
   if (rb[0] >= 0 && cb[0] >= 0) {
-    float best = 1e30f, best_any = 1e30f;
     for (int i = 0; i < 2; ++i)
       for (int j = 0; j < 2; ++j) {
         if (rb[i] < 0 || cb[j] < 0) continue;
@@ -479,15 +480,21 @@ int fused_schedule(int nx, int ny, int tm, const StageTw<TW>& tx, const StageTw<
             fc.rv = rb[i];
             fc.cv = cb[j];
           }
-          if (ms < best_any) {
-            best_any = ms;
+          if (tiled && ms < best_tiled) {
+            best_tiled = ms;
             fc.trv = rb[i];
             fc.tcv = cb[j];
-            fc.tile = tiled ? w : 0;
+            fc.tile = w;
           }
         }
       }
   }
+  // The tiled chain moves fewer HBM bytes (no partial-line column stores);
+  // it is kept unless the in-place chain is more than 1 % faster.
+  if (fc.tile && !(best_tiled <= 1.01f * best)) fc.tile = 0;
+  if (verbose)
+    std::fprintf(stderr, "[rsim tune] %dx%d math=%s chosen (%d, %d)%s\n", nx, ny, mname,
+                 fc.tile ? fc.trv : fc.rv, fc.tile ? fc.tcv : fc.cv, fc.tile ? " tiled" : "");
   RSIM_HIP(hipStreamSynchronize(s));
   g_tune[key] = fc;
   pick(fc, rv, cv, tile);

@@ -347,7 +347,8 @@ int rsim_batch_destroy(rsim_batch* batch);
  * and column-pass variant on a scratch field, then the leading row x column
  * pairs as a slice runs them, both in place and with the column pass storing
  * column tiles that the next row pass reads (two buffers, u and U), and
- * keeps the fastest; the environment variable
+ * keeps the fastest (the tiled form unless an in-place pair is more than 1 %
+ * faster: it moves fewer HBM bytes); the environment variable
  * RAVESIM_FUSED_VARIANT="row,col[,tiled]" overrides it.  Returns the variants
  * of the voxel slice loop for (nx, ny), or -1 before the first fused call. */
 int rsim_fused_schedule(int nx, int ny, int* row_variant, int* col_variant);
