@@ -202,45 +202,93 @@ __device__ __forceinline__ int in_pos(int t, int i, int j) {
   return t + i * F::TC + j * (F::N / RR);
 }
 
+// LDS slot of position pos of sequence c: with PS = pad_code(shift, k), C a
+// power of two and k a multiple of C, slot = lpad(pos * C + c) =
+// (pos + q * (pos >> s)) * C + c with s = shift - log2(C), q = k / C.
+template <int PS, int C>
+struct PadPos {
+  static constexpr int log2c = C >= 16 ? 4 : C >= 8 ? 3 : C >= 4 ? 2 : C >= 2 ? 1 : 0;
+  static constexpr bool ok = (C & (C - 1)) == 0 && C <= 16 &&
+                             (PS == 0 || ((PS & 255) % C == 0 && (PS >> 8) >= log2c));
+  static constexpr int s = PS == 0 ? 0 : (PS >> 8) - log2c;
+  static constexpr int q = PS == 0 ? 0 : (PS & 255) / C;
+  // slot offset of position offset K whose low s bits add to a base without carry
+  static constexpr int off(int K) { return (PS == 0 ? K : K + q * (K >> s)) * C; }
+  __device__ static __forceinline__ int base(int pos, int c) {
+    return (PS == 0 ? pos : pos + q * (pos >> s)) * C + c;
+  }
+};
+// Whether the exchange into stage S has slot addresses of the form
+// base(thread) + constant(i, m) on both sides (every output position of
+// stage S-1 is pb(t) + i*TC*RP + m*LP with pb(t) = (t - t%LP)*RP + t%LP when
+// TC is a multiple of LP; every input position of stage S is t + i*TC +
+// j*N/RN), and the padding adds no carry between the two parts.
+template <class F, int S, int C, int PS>
+struct ExLinear {
+  static constexpr int RP = F::radix(S - 1);
+  static constexpr int LP = F::stride(S - 1);
+  static constexpr int RN = F::radix(S);
+  using P = PadPos<PS, C>;
+  static constexpr int M = 1 << P::s;
+  static constexpr bool wr = P::ok && F::TC % LP == 0 &&
+                             (PS == 0 || ((F::TC * RP) % M == 0 &&
+                                          (LP % M == 0 || (M % LP == 0 && (LP * RP) % M == 0))));
+  static constexpr bool rd = P::ok && (PS == 0 || (F::TC % M == 0 && (F::N / RN) % M == 0));
+};
+
 // Exchange between stage S-1 (outputs in v) and stage S (inputs into v).
 template <class F, int S, int C, int PS, bool SPLIT, typename R>
 __device__ __forceinline__ void reg_exchange(cpx<R> (&v)[F::E], void* lds_raw, int t, int c) {
   constexpr int RP = F::radix(S - 1);
   constexpr int LP = F::stride(S - 1);
   constexpr int RN = F::radix(S);
+  using X = ExLinear<F, S, C, PS>;
+  using P = PadPos<PS, C>;
+  // slot of output (i, m) of stage S-1 and of input (i, j) of stage S
+  int wb = 0, rb = 0;
+  if constexpr (X::wr) wb = P::base((t - t % LP) * RP + t % LP, c);
+  if constexpr (X::rd) rb = P::base(t, c);
+  auto wslot = [&](int i, int m) {
+    if constexpr (X::wr) return wb + P::off(i * F::TC * RP + m * LP);
+    else return lpad<PS>(out_pos<F, RP, LP>(t, i, m) * C + c);
+  };
+  auto rslot = [&](int i, int j) {
+    if constexpr (X::rd) return rb + P::off(i * F::TC + j * (F::N / RN));
+    else return lpad<PS>(in_pos<F, RN>(t, i, j) * C + c);
+  };
   if constexpr (!SPLIT) {
     cpx<R>* lds = reinterpret_cast<cpx<R>*>(lds_raw);
 #pragma unroll
     for (int i = 0; i < F::E / RP; ++i)
 #pragma unroll
-      for (int m = 0; m < RP; ++m) lds[lpad<PS>(out_pos<F, RP, LP>(t, i, m) * C + c)] = v[i * RP + m];
+      for (int m = 0; m < RP; ++m) lds[wslot(i, m)] = v[i * RP + m];
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < F::E / RN; ++i)
 #pragma unroll
-      for (int j = 0; j < RN; ++j) v[i * RN + j] = lds[lpad<PS>(in_pos<F, RN>(t, i, j) * C + c)];
+      for (int j = 0; j < RN; ++j) v[i * RN + j] = lds[rslot(i, j)];
     __syncthreads();
   } else {
     R* lds = reinterpret_cast<R*>(lds_raw);
 #pragma unroll
     for (int i = 0; i < F::E / RP; ++i)
 #pragma unroll
-      for (int m = 0; m < RP; ++m) lds[lpad<PS>(out_pos<F, RP, LP>(t, i, m) * C + c)] = v[i * RP + m].x;
+      for (int m = 0; m < RP; ++m) lds[wslot(i, m)] = v[i * RP + m].x;
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < F::E / RN; ++i)
 #pragma unroll
-      for (int j = 0; j < RN; ++j) v[i * RN + j].x = lds[lpad<PS>(in_pos<F, RN>(t, i, j) * C + c)];
+      for (int j = 0; j < RN; ++j) v[i * RN + j].x = lds[rslot(i, j)];
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < F::E / RP; ++i)
 #pragma unroll
-      for (int m = 0; m < RP; ++m) lds[lpad<PS>(out_pos<F, RP, LP>(t, i, m) * C + c)] = v[i * RP + m].y;
+      for (int m = 0; m < RP; ++m) lds[wslot(i, m)] = v[i * RP + m].y;
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < F::E / RN; ++i)
 #pragma unroll
-      for (int j = 0; j < RN; ++j) v[i * RN + j].y = lds[lpad<PS>(in_pos<F, RN>(t, i, j) * C + c)];
+      for (int j = 0; j < RN; ++j) v[i * RN + j].y = lds[rslot(i, j)];
     __syncthreads();
   }
 }
@@ -344,18 +392,19 @@ __device__ __forceinline__ void reg_stage(cpx<R> (&v)[F::E], const StageTw<TW>& 
       for (int j = 0; j < RR; ++j) v[i * RR + j] = mk((R)x[j].x, (R)x[j].y);
     }
   } else {
-    if constexpr (LS > 1) {
-      const cpx<TW>* T = tw.t[S];
 #pragma unroll
-      for (int i = 0; i < F::E / RR; ++i) {
+    for (int i = 0; i < F::E / RR; ++i) {
+      if constexpr (LS > 1) {
+        const cpx<TW>* T = tw.t[S];
         const int b = t + i * F::TC;
         const int k = b % LS;
 #pragma unroll
         for (int j = 1; j < RR; ++j) v[i * RR + j] = twmul<DIR>(v[i * RR + j], T[k * RR + j]);
       }
+      dft<RR, DIR>(&v[i * RR]);
+      // one butterfly's twiddles in flight at a time when a thread has several
+      if constexpr (F::E > 16 && RR >= 8) compiler_fence();
     }
-#pragma unroll
-    for (int i = 0; i < F::E / RR; ++i) dft<RR, DIR>(&v[i * RR]);
   }
 }
 

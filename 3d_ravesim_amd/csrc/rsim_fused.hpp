@@ -102,10 +102,15 @@ __device__ __forceinline__ cpx<float> transmit(cpx<float> v, const TransArgsF& a
 // the first transform's live through the pointwise step: fewer registers
 // (no spills in the 2048, 3072, 6144, 8192 and 16384 shapes), a few more
 // VALU instructions.  Both forms are launch-shape variants.  The empty asm
-// statement only hides the value from the optimiser; it emits no instruction.
+// statement only hides the value from the optimiser; it emits no
+// instruction.  The mask restores what the optimiser knew of the index, that
+// it lies in [0, 1024), so offsets still fold into instruction immediates.
 template <bool LD>
 __device__ __forceinline__ int opaque_index(int x) {
-  if constexpr (LD) asm volatile("" : "+v"(x));
+  if constexpr (LD) {
+    asm volatile("" : "+v"(x));
+    x &= 1023;
+  }
   return x;
 }
 
@@ -195,10 +200,13 @@ struct ColBatch {
 
 // Occupancy request of a row launch shape: LD = 2 asks the compiler for three
 // workgroups of up to 512 threads per CU (waves per SIMD = 3 x the
-// workgroup's share), i.e. a register budget that lets them in.
+// workgroup's share), LD = 3 for two, i.e. a register budget that lets them
+// in.
 template <class F, int C, int LD>
 constexpr int row_waves_per_eu() {
-  return (LD == 2 && F::TC * C <= 512) ? 3 * ((F::TC * C + 255) / 256) : 1;
+  return (LD == 2 && F::TC * C <= 512)   ? 3 * ((F::TC * C + 255) / 256)
+         : (LD == 3 && F::TC * C <= 512) ? 2 * ((F::TC * C + 255) / 256)
+                                         : 1;
 }
 
 template <class F, int C, int PS, bool SPLIT, int LD, int MODE, int TM, typename TW>
@@ -520,8 +528,8 @@ template <int N> constexpr int vColC(int E, int t) { return vmin(vmin(vmax1(t / 
 // X(N, id, E, C, SPLIT, LD).  The base lists keep the index of the second
 // transform of a kernel as it is (LD 0); the LD lists are copies of some
 // base shapes whose second transform recomputes its addresses
-// (opaque_index; LD 1, or LD 2 with the three-workgroup register budget of
-// row_waves_per_eu).  The two sets are compiled in separate translation units
+// (opaque_index; LD 1, or LD 2 / LD 3 with the three- / two-workgroup
+// register budget of row_waves_per_eu).  The two sets are compiled in separate translation units
 // (rsim_fused_<N>.hip and rsim_fused_<N>_ld.hip): kernels of the other kind
 // in the same module change the code generated for the base kernels.
 // Variant ids: base 0..kRowBase-1, then the LD list.
@@ -534,7 +542,8 @@ template <int N> constexpr int vColC(int E, int t) { return vmin(vmin(vmax1(t / 
   X(N, 0, vE1<N>(), vRowC<N>(vE1<N>(), 256), vsplit(N, vRowC<N>(vE1<N>(), 256), 65536), 1)    \
   X(N, 1, vE1<N>(), vRowC<N>(vE1<N>(), 512), vsplit(N, vRowC<N>(vE1<N>(), 512), 65536), 1)    \
   X(N, 2, vE2<N>(), vRowC<N>(vE2<N>(), 256), vsplit(N, vRowC<N>(vE2<N>(), 256), 65536), 1)    \
-  X(N, 3, vE1<N>(), vRowC<N>(vE1<N>(), 512), true, 2)
+  X(N, 3, vE1<N>(), vRowC<N>(vE1<N>(), 512), true, 2)                                        \
+  X(N, 4, vE2<N>(), vRowC<N>(vE2<N>(), 512), true, 3)
 #define RSIM_COL_VARIANTS(N, X)                                                                 \
   X(N, 0, vE1<N>(), vColC<N>(vE1<N>(), 1024), true, 0)                                      \
   X(N, 1, vE1<N>(), vColC<N>(vE1<N>(), 1024), vsplit(N, vColC<N>(vE1<N>(), 1024), 163840 - 8192), 0) \
@@ -547,7 +556,7 @@ template <int N> constexpr int vColC(int E, int t) { return vmin(vmin(vmax1(t / 
   X(N, 2, vE2<N>(), vColC<N>(vE2<N>(), 512), true, 1)
 constexpr int kRowBase = 4;
 constexpr int kColBase = 5;
-constexpr int kRowVariants = kRowBase + 4;
+constexpr int kRowVariants = kRowBase + 5;
 constexpr int kColVariants = kColBase + 3;
 
 // Per-size launchers (explicitly instantiated in rsim_fused_<N>.hip and

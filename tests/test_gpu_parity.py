@@ -438,7 +438,7 @@ def test_phase_stepping_replays_from_snapshot():
         assert rel(outs[k], exp) < 1e-11
 
 
-N_ROW_VARIANTS, N_COL_VARIANTS = 8, 8
+N_ROW_VARIANTS, N_COL_VARIANTS = 9, 8
 VARIANT_CASES = ([(1024, 1024, (r, 0)) for r in range(N_ROW_VARIANTS)]
                  + [(1024, 1024, (r, c)) for c in range(1, N_COL_VARIANTS)
                     for r in (0, N_ROW_VARIANTS - 1)]
