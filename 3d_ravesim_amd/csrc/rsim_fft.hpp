@@ -207,8 +207,9 @@ __device__ __forceinline__ int in_pos(int t, int i, int j) {
 // (pos + q * (pos >> s)) * C + c with s = shift - log2(C), q = k / C.
 template <int PS, int C>
 struct PadPos {
-  static constexpr int log2c = C >= 16 ? 4 : C >= 8 ? 3 : C >= 4 ? 2 : C >= 2 ? 1 : 0;
-  static constexpr bool ok = (C & (C - 1)) == 0 && C <= 16 &&
+  static constexpr int log2c =
+      C >= 64 ? 6 : C >= 32 ? 5 : C >= 16 ? 4 : C >= 8 ? 3 : C >= 4 ? 2 : C >= 2 ? 1 : 0;
+  static constexpr bool ok = (C & (C - 1)) == 0 && C <= 64 &&
                              (PS == 0 || ((PS & 255) % C == 0 && (PS >> 8) >= log2c));
   static constexpr int s = PS == 0 ? 0 : (PS >> 8) - log2c;
   static constexpr int q = PS == 0 ? 0 : (PS & 255) / C;

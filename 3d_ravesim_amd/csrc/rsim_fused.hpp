@@ -780,10 +780,13 @@ __global__ void __launch_bounds__(F::TC * C) ms_colB_kernel(cpx<float>* buf, int
   }
 }
 
-// Launcher of the three passes on buf (in place); false if ny has no such
-// schedule.
+// Launcher of the three passes on buf (in place) in launch shape v (tiles of
+// 16, 32 or 64 columns: 128-, 256- or 512-byte row segments); false if ny has
+// no such schedule or v no such shape.
+constexpr int kBigColVariants = 3;
 template <typename TW>
-bool big_col_go(void* buf, int nx, int ny, const StageTw<TW>& tw128, const BigTwElem<TW>* wN,
-                const double2* Bx, const double2* Ay, const int* Rmax, hipStream_t s);
+bool big_col_go(int v, void* buf, int nx, int ny, const StageTw<TW>& tw128,
+                const BigTwElem<TW>* wN, const double2* Bx, const double2* Ay, const int* Rmax,
+                hipStream_t s);
 
 }  // namespace rsim

@@ -7,19 +7,24 @@ namespace rsim {
 RSIM_FUSED_INSTANTIATE_ROWS(16384)
 
 namespace {
-// 16 neighbouring columns per tile: every row segment is one 128-byte line.
+// C neighbouring columns per tile: every row segment is C * 8 contiguous
+// bytes (C = 16: one 128-byte line).  Wider tiles read longer runs of each
+// DRAM page per workgroup.
+template <int C_>
 struct BigShape {
   using F = Sched<128>::F;
-  static constexpr int C = 16;
-  static constexpr bool SPLIT = false;   // 128 x 16 x 8 B = 16 KiB
-  static constexpr int PS = pad_code(4 + (C >= 2) + (C >= 4) + (C >= 8) + (C >= 16), C);
+  static constexpr int C = C_;
+  static constexpr bool SPLIT = false;   // 128 x C x 8 B <= 64 KiB
+  static constexpr int PS = pad_code(4 + (C >= 2) + (C >= 4) + (C >= 8) + (C >= 16) + (C >= 32) +
+                                         (C >= 64),
+                                     C);
 };
 
-template <typename TW>
+template <typename TW, int C>
 bool big_col_go_impl(void* buf, int nx, int ny, const StageTw<TW>& tw, const BigTwElem<TW>* wN,
                      const double2* Bx, const double2* Ay, const int* Rmax, hipStream_t s) {
-  using S = BigShape;
-  using F = S::F;
+  using S = BigShape<C>;
+  using F = typename S::F;
   constexpr int N1 = F::N, N2 = F::N;
   if (ny != N1 * N2) return false;
   const unsigned long long bytes = (unsigned long long)nx * ny * 8ull;
@@ -36,24 +41,37 @@ bool big_col_go_impl(void* buf, int nx, int ny, const StageTw<TW>& tw, const Big
                      0, s, b, nx, N2, nb, tw, wN);
   return true;
 }
+
+template <typename TW>
+bool big_col_dispatch(int v, void* buf, int nx, int ny, const StageTw<TW>& tw,
+                      const BigTwElem<TW>* wN, const double2* Bx, const double2* Ay,
+                      const int* Rmax, hipStream_t s) {
+  switch (v) {
+    case 0: return big_col_go_impl<TW, 16>(buf, nx, ny, tw, wN, Bx, Ay, Rmax, s);
+    case 1: return big_col_go_impl<TW, 32>(buf, nx, ny, tw, wN, Bx, Ay, Rmax, s);
+    case 2: return big_col_go_impl<TW, 64>(buf, nx, ny, tw, wN, Bx, Ay, Rmax, s);
+    default: return false;
+  }
+}
 }  // namespace
 
 template <>
-bool big_col_go<float>(void* buf, int nx, int ny, const StageTw<float>& tw,
+bool big_col_go<float>(int v, void* buf, int nx, int ny, const StageTw<float>& tw,
                        const BigTwElem<float>* wN, const double2* Bx, const double2* Ay,
                        const int* Rmax, hipStream_t s) {
-  return big_col_go_impl<float>(buf, nx, ny, tw, wN, Bx, Ay, Rmax, s);
+  return big_col_dispatch<float>(v, buf, nx, ny, tw, wN, Bx, Ay, Rmax, s);
 }
 template <>
-bool big_col_go<double>(void* buf, int nx, int ny, const StageTw<double>& tw,
+bool big_col_go<double>(int v, void* buf, int nx, int ny, const StageTw<double>& tw,
                         const BigTwElem<double>* wN, const double2* Bx, const double2* Ay,
                         const int* Rmax, hipStream_t s) {
-  return big_col_go_impl<double>(buf, nx, ny, tw, wN, Bx, Ay, Rmax, s);
+  return big_col_dispatch<double>(v, buf, nx, ny, tw, wN, Bx, Ay, Rmax, s);
 }
 template <>
-bool big_col_go<twc>(void* buf, int nx, int ny, const StageTw<twc>& tw, const BigTwElem<twc>* wN,
-                     const double2* Bx, const double2* Ay, const int* Rmax, hipStream_t s) {
-  return big_col_go_impl<twc>(buf, nx, ny, tw, wN, Bx, Ay, Rmax, s);
+bool big_col_go<twc>(int v, void* buf, int nx, int ny, const StageTw<twc>& tw,
+                     const BigTwElem<twc>* wN, const double2* Bx, const double2* Ay,
+                     const int* Rmax, hipStream_t s) {
+  return big_col_dispatch<twc>(v, buf, nx, ny, tw, wN, Bx, Ay, Rmax, s);
 }
 
 }  // namespace rsim

@@ -373,10 +373,17 @@ static bool env_schedule(int nx, int ny, int* rv, int* cv, int* tile) {
   return true;
 }
 
+// Three-pass column tables (ny = kBigCol): the 128-point stage tables and W_ny.
+template <typename TW>
+struct BigColTables {
+  StageTw<TW> t128{};
+  const BigTwElem<TW>* wN = nullptr;
+};
+
 template <typename TW>
 int fused_schedule(int nx, int ny, int tm, const StageTw<TW>& tx, const StageTw<TW>& ty,
                    const TransArgsF& ta, const ChirpTables* ct, hipStream_t s, int* rv, int* cv,
-                   int* tile = nullptr) {
+                   int* tile = nullptr, const BigColTables<TW>* big = nullptr) {
   if (env_schedule(nx, ny, rv, cv, tile)) return RSIM_OK;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   const int math = std::is_same<TW, twc>::value ? 4 : sizeof(TW) == 8 ? 2 : 0;
@@ -454,6 +461,25 @@ int fused_schedule(int nx, int ny, int tm, const StageTw<TW>& tx, const StageTw<
   FusedChoice fc;
   fc.rv = rb[0] < 0 ? 0 : rb[0];
   fc.cv = cb[0] < 0 ? 0 : cb[0];
+  if (big) {
+    // three-pass columns: each tile width as the slice runs it, all three
+    // passes in place
+    float bt = 1e30f;
+    for (int v = 0; v < kBigColVariants; ++v) {
+      float ms = -1.f;
+      RSIM_CHECK(tune_fill(tmp, tbytes, s));
+      RSIM_CHECK(time_launches(s, [&] {
+        return big_col_go<TW>(v, tmp, nx, ny, big->t128, big->wN, ct->Bx, ct->Ay, ct->Rmax, s);
+      }, &ms, 3));
+      if (verbose)
+        std::fprintf(stderr, "[rsim tune] %dx%d math=%s three-pass column variant %d: %.1f us\n",
+                     nx, ny, mname, v, ms * 1e3f);
+      if (ms > 0.f && ms < bt) {
+        bt = ms;
+        fc.cv = v;
+      }
+    }
+  }
   float best = 1e30f, best_tiled = 1e30f;
   if (rb[0] >= 0 && cb[0] >= 0) {
     for (int i = 0; i < 2; ++i)
@@ -806,16 +832,15 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
   const bool per_slice = scales || dzs;
   // three-pass columns: 128-point sub-transform tables and the W_ny table
   const bool big = ny == kBigCol;
-  StageTw<TW> t128{};
-  const BigTwElem<TW>* wN = nullptr;
+  BigColTables<TW> bigt;
   if (big) {
     const void* tw128[4];
     if (!reg_stage_tables(tdtype, 128, tw128)) return fail(RSIM_E_HIP, "missing FFT schedule tables");
-    using TP = typename std::remove_reference<decltype(t128.t[0])>::type;
-    for (int i = 0; i < 4; ++i) t128.t[i] = reinterpret_cast<TP>(tw128[i]);
-    wN = reinterpret_cast<const BigTwElem<TW>*>(
+    using TP = typename std::remove_reference<decltype(bigt.t128.t[0])>::type;
+    for (int i = 0; i < 4; ++i) bigt.t128.t[i] = reinterpret_cast<TP>(tw128[i]);
+    bigt.wN = reinterpret_cast<const BigTwElem<TW>*>(
         fft_full_table(std::is_same<TW, double>::value ? RSIM_C128 : kTwComp, ny));
-    if (!wN) return fail(RSIM_E_HIP, "missing FFT twiddle table for %d", ny);
+    if (!bigt.wN) return fail(RSIM_E_HIP, "missing FFT twiddle table for %d", ny);
   }
   const float spec_scale = (float)((double)nx * (double)ny);
   P.at(ta, first);
@@ -851,7 +876,8 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
   // may change from slice to slice (cone-beam tables).
   const bool may_tile = !big && !per_slice && !(hist && !xbin);
   int rv = 0, cv = 0, tile = 0;
-  RSIM_CHECK(fused_schedule<TW>(nx, ny, tm, tx, ty, ta, ct, s, &rv, &cv, may_tile ? &tile : nullptr));
+  RSIM_CHECK(fused_schedule<TW>(nx, ny, tm, tx, ty, ta, ct, s, &rv, &cv, may_tile ? &tile : nullptr,
+                                big ? &bigt : nullptr));
   int w_in = 0;   // tile width of the W that the next row pass reads (0: W in U)
   for (int k = 0; k < count; ++k) {
     const int sl = first + k;
@@ -878,7 +904,7 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
       ct = &e.ct;
       tm = use_maps(ta, e.xm, e.ym, src.vol != nullptr);
       RSIM_CHECK(fused_schedule<TW>(nx, ny, tm, tx, ty, ta, ct, s, &rv, &cv,
-                                    may_tile ? &tile : nullptr));
+                                    may_tile ? &tile : nullptr, big ? &bigt : nullptr));
     }
     P.at(ta, sl);
     const bool last = k == count - 1;
@@ -910,7 +936,7 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
     ktime_begin(KT_COL, s);
     if (big) {
       // in place in U; the spectrum of the final field is formed after the loop
-      if (!big_col_go<TW>(U, nx, ny, t128, wN, ct->Bx, ct->Ay, ct->Rmax, s))
+      if (!big_col_go<TW>(cv, U, nx, ny, bigt.t128, bigt.wN, ct->Bx, ct->Ay, ct->Rmax, s))
         return fail(RSIM_E_HIP, "no three-pass column schedule for ny=%d", ny);
     } else {
       // last slice: W -> u (and the spectrum -> U); otherwise in place in U,
