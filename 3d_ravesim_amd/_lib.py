@@ -50,6 +50,12 @@ class BatchInstance(C.Structure):
                 ("pixel_size_y", _d), ("pixel_size_z", _d), ("x_pos", _d), ("y_pos", _d)]
 
 
+class ChainStep(C.Structure):
+    """rsim_chain_step of include/ravesim.h."""
+    _fields_ = [("vol", _vp), ("pixel_size_x", _d), ("pixel_size_y", _d), ("pixel_size_z", _d),
+                ("x_pos", _d), ("y_pos", _d), ("first_slice", _i), ("n_slices", _i), ("dz", _d)]
+
+
 class HistoryFrames(C.Structure):
     """rsim_history_frames of include/ravesim.h."""
     _fields_ = [("out_x", _i), ("out_y", _i), ("pixel_size_x", _d), ("pixel_size_y", _d),
@@ -105,6 +111,8 @@ SIGNATURES = {
     "rsim_transmission_2d": [_vp, _i, _i, _i, _d, _d, _d, _vp, _i, _d, _d, _d, _d, _d, _vp],
     "rsim_sample_apply_2d": [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _vp, _d, _d, _d, _d, _d, _i,
                              _i, _i, _i, _vp],
+    "rsim_chain_apply_2d": [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _i, C.POINTER(ChainStep), _i,
+                            _i, _i, _vp],
     "rsim_fused_schedule": [_i, _i, _ip, _ip],
     "rsim_fused_tile": [_i, _i, _ip],
     "rsim_detector_area_v1": [_vp, _i, _i, _i, _d, _d, _i, _i, _d, _d, _d, _dp, _vp],

@@ -25,11 +25,6 @@ int detector_area_v1(const void* u, int nx, int ny, int dtype, double dx, double
 int paint_spheres(rsim_volume* vol, const double* sph, const int* mats, int n, hipStream_t s);
 int propagate(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
               double dz, double freq, bool skip_fft, hipStream_t s);
-int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
-                 double freq, const rsim_volume* vol, double px, double py, double pz,
-                 double xpos, double ypos, int first, int count, const double* scales,
-                 const double* dzs, int engine, int flags, hipStream_t s,
-                 const HistSpec* hist = nullptr);
 int detector_area_v1_adjoint(const void* v, void* g, int nx, int ny, int dtype, double dx,
                              double dy, int ox, int oy, double dsx, double dsy, double z,
                              const double* gimg_dev, hipStream_t s);
@@ -819,6 +814,35 @@ int rsim_sample_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx,
   return sample_apply(u, U, nx, ny, dtype, dx, dy, wl, cutoff_freq, vol, pixel_size_x,
                       pixel_size_y, pixel_size_z, x_pos, y_pos, first_slice, n_slices, nullptr,
                       nullptr, engine, flags, resolve_stream(stream));
+}
+
+int rsim_chain_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
+                        double wl, double cutoff_freq, int band_limit,
+                        const rsim_chain_step* steps, int n_steps, int engine, int flags,
+                        void* stream) {
+  RSIM_CHECK(check_field(u, nx, ny, dtype, "u"));
+  RSIM_CHECK(check_field(U, nx, ny, dtype, "U"));
+  RSIM_CHECK(check_fft_sizes(nx, ny));
+  if (engine != RSIM_PLAIN && engine != RSIM_FUSED)
+    return fail(RSIM_E_INVALID_ARG, "engine must be RSIM_PLAIN or RSIM_FUSED");
+  if (!(wl > 0) || !(dx > 0) || !(dy > 0)) return fail(RSIM_E_INVALID_ARG, "dx, dy and wl must be positive");
+  if (n_steps < 0 || (n_steps > 0 && !steps))
+    return fail(RSIM_E_INVALID_ARG, "a chain needs n_steps >= 0 and its steps");
+  for (int j = 0; j < n_steps; ++j) {
+    const rsim_chain_step& st = steps[j];
+    if (st.vol) {
+      const int rc = check_sample(st.vol, st.first_slice, st.n_slices, st.pixel_size_x,
+                                  st.pixel_size_y, st.pixel_size_z);
+      if (rc != RSIM_OK) {
+        const std::string why = rsim_last_error();
+        return fail(rc, "chain step %d: %s", j, why.c_str());
+      }
+    } else if (!std::isfinite(st.dz)) {
+      return fail(RSIM_E_INVALID_ARG, "chain step %d: free-space distance must be finite", j);
+    }
+  }
+  return chain_apply(u, U, nx, ny, dtype, dx, dy, wl, cutoff_freq, band_limit != 0, steps,
+                     n_steps, engine, flags, resolve_stream(stream));
 }
 
 }  // extern "C"

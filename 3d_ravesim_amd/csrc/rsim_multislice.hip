@@ -126,7 +126,9 @@ struct ChirpTables {
   int* Rmax = nullptr;     // largest |ky index| kept for each column, -1: none
 };
 
-std::map<std::tuple<int, int, int, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t>, ChirpTables> g_chirp;
+std::map<std::tuple<int, int, int, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t, uint64_t>,
+         ChirpTables>
+    g_chirp;
 
 uint64_t bits(double v) {
   uint64_t b;
@@ -161,20 +163,31 @@ void chirp_bind(ChirpTables* t, unsigned char* dev, int nx, int ny) {
   t->Rmax = reinterpret_cast<int*>(dev + 16 * (size_t)(nx + ny));
 }
 
-// Host FP64 build of the chirp tables of propagation over dz into img.
+// Global phase exp(2 pi i dz / wl) of propagation over dz
+// (big-wave/propagation.py:428) as a unit complex number: libm's cos and sin
+// reduce the argument (1e10 rad and more at metre distances) exactly, where a
+// reduction by a double-precision 2 pi would be off by up to an ulp of it.
+double2 global_phase(double wl, double dz) {
+  const double g = ((2.0 * M_PI) / wl) * dz;
+  return make_double2(std::cos(g), std::sin(g));
+}
+
+// Host FP64 build of the chirp tables of propagation over dz into img, with
+// global phase gp (global_phase(wl, dz), or the product of several steps'
+// phases when propagations are folded into one).
 void chirp_build(int nx, int ny, double dx, double dy, double wl, double dz, double freq,
-                 unsigned char* img) {
+                 double2 gp, unsigned char* img) {
   const std::vector<double> kx2 = host_k2(nx, dx), ky2 = host_k2(ny, dy);
   const double ctheta = ((-M_PI) * wl) * dz;
-  const double gph = reduce_phase(((2.0 * M_PI) / wl) * dz);
   const double inv_n = 1.0 / ((double)nx * (double)ny);
   double* bx = reinterpret_cast<double*>(img);
   double* ay = reinterpret_cast<double*>(img + 16 * (size_t)nx);
   int* rmax = reinterpret_cast<int*>(img + 16 * (size_t)(nx + ny));
   for (int x = 0; x < nx; ++x) {
-    const double th = reduce_phase(gph + reduce_phase(ctheta * kx2[x]));
-    bx[2 * x] = std::cos(th) * inv_n;
-    bx[2 * x + 1] = std::sin(th) * inv_n;
+    const double th = reduce_phase(ctheta * kx2[x]);
+    const double c = std::cos(th), sn = std::sin(th);
+    bx[2 * x] = (gp.x * c - gp.y * sn) * inv_n;
+    bx[2 * x + 1] = (gp.x * sn + gp.y * c) * inv_n;
   }
   for (int y = 0; y < ny; ++y) {
     const double th = reduce_phase(ctheta * ky2[y]);
@@ -198,9 +211,11 @@ void chirp_build(int nx, int ny, double dx, double dy, double wl, double dz, dou
 }
 
 int chirp_tables(int nx, int ny, double dx, double dy, double wl, double dz, double freq,
-                 const ChirpTables** out) {
+                 const ChirpTables** out, const double2* gphase = nullptr) {
   std::lock_guard<std::recursive_mutex> lk(table_mu());
-  auto key = std::make_tuple(current_device(), nx, ny, bits(dx), bits(dy), bits(wl), bits(dz), bits(freq));
+  const double2 gp = gphase ? *gphase : global_phase(wl, dz);
+  auto key = std::make_tuple(current_device(), nx, ny, bits(dx), bits(dy), bits(wl), bits(dz),
+                             bits(freq), bits(gp.x), bits(gp.y));
   auto it = g_chirp.find(key);
   if (it != g_chirp.end()) {
     table_touch(it->second.Bx);
@@ -210,7 +225,7 @@ int chirp_tables(int nx, int ny, double dx, double dy, double wl, double dz, dou
   RSIM_CHECK(capture_miss("chirp tables"));
   const size_t nb = chirp_bytes(nx, ny);
   std::vector<unsigned char> host(nb);
-  chirp_build(nx, ny, dx, dy, wl, dz, freq, host.data());
+  chirp_build(nx, ny, dx, dy, wl, dz, freq, gp, host.data());
   ChirpTables t;
   unsigned char* dev = nullptr;
   RSIM_HIP(hipMalloc((void**)&dev, nb));
@@ -781,7 +796,7 @@ static int build_run_slice_tables(int nx, int ny, double dx, double dy, double w
       unsigned char* h = st.host + per * (size_t)k;
       const double dz = dzs ? dzs[k] : pz, sc = scales ? scales[k] : 1.0;
       SliceTables& e = rt->sets[k];
-      chirp_build(nx, ny, dx, dy, wl, dz, freq, h);
+      chirp_build(nx, ny, dx, dy, wl, dz, freq, global_phase(wl, dz), h);
       axis_map_build(nx, dx, sc, xpos, src.nx_g, px, &e.xm, h + cb);
       axis_map_build(ny, dy, sc, ypos, src.ny_g, py, &e.ym, h + cb + xb);
       unsigned char* d = rt->dev + per * (size_t)k;
@@ -1006,6 +1021,178 @@ int slices_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double 
                              count, want_spec, scales, dzs, s, hist);
   return run_fused<float>(u, U, nx, ny, dx, dy, wl, freq, src, px, py, pz, xpos, ypos, first,
                           count, want_spec, scales, dzs, s, hist);
+}
+
+// ---------------------------------------------------------------------------
+// Propagation chains (rsim_chain_apply_2d): an optional band limit, then
+// sample steps and free-space steps in order.  On the fused engine the whole
+// chain is one sequence of row and column passes: a propagation with no
+// transmission after it is not completed in real space but carried into the
+// next column pass, whose distance is the sum, P(a) P(b) = P(a + b) (chirps
+// multiply, the cutoff mask is idempotent; big-wave/propagation.py:396-442).
+// ---------------------------------------------------------------------------
+namespace {
+// One pass of a fused chain: a row pass (FIRST, MID or LAST) with the
+// transmission of slice sl of sample step j (j < 0: none), or a column pass
+// over dist.
+struct ChainOp {
+  bool col;
+  int mode, j, sl;
+  double dist;
+  double2 gp;   // product of the folded propagations' global phases
+  const ChirpTables* ct;
+};
+double2 cmul2(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+}  // namespace
+
+template <typename TW>
+static int run_chain(void* u, void* U, int nx, int ny, double dx, double dy, double wl,
+                     double freq, bool band_limit, const rsim_chain_step* steps, int n_steps,
+                     bool want_spec, hipStream_t s) {
+  std::vector<FusedPrep<TW>> prep(n_steps);
+  std::vector<int> rvs(n_steps, 0), cvs(n_steps, 0);
+  int ref = -1;   // first sample step: its launch shapes serve the passes without transmission
+  for (int j = 0; j < n_steps; ++j) {
+    const rsim_chain_step& st = steps[j];
+    if (!st.vol || st.n_slices == 0) continue;
+    FusedPrep<TW>& P = prep[j];
+    RSIM_CHECK(fused_prepare<TW>(nx, ny, dx, dy, wl, freq, src_of(st.vol), st.pixel_size_x,
+                                 st.pixel_size_y, st.pixel_size_z, st.x_pos, st.y_pos, &P));
+    P.at(P.ta, st.first_slice);
+    RSIM_CHECK(fused_schedule<TW>(nx, ny, P.tm, P.tx, P.ty, P.ta, P.ct, s, &rvs[j], &cvs[j]));
+    if (ref < 0) ref = j;
+  }
+  // the pass sequence
+  std::vector<ChainOp> ops;
+  bool pending = false;   // a propagation waits for its column pass
+  double dist = 0.0;
+  double2 gp = make_double2(1.0, 0.0);
+  const double2 one = make_double2(1.0, 0.0);
+  auto row = [&](int mode, int j, int sl) {
+    ops.push_back(ChainOp{false, mode, j, sl, 0.0, one, nullptr});
+  };
+  auto col = [&]() { ops.push_back(ChainOp{true, 0, -1, 0, dist, gp, nullptr}); };
+  if (band_limit) {
+    row(ROW_FIRST, -1, 0);
+    pending = true;
+    dist = 0.0;
+    gp = one;
+  }
+  for (int j = 0; j < n_steps; ++j) {
+    const rsim_chain_step& st = steps[j];
+    if (!st.vol) {
+      if (!pending) row(ROW_FIRST, -1, 0);
+      dist = pending ? dist + st.dz : st.dz;
+      gp = pending ? cmul2(gp, global_phase(wl, st.dz)) : global_phase(wl, st.dz);
+      pending = true;
+      continue;
+    }
+    for (int k = 0; k < st.n_slices; ++k) {
+      if (pending) {
+        col();
+        row(ROW_MID, j, st.first_slice + k);
+      } else {
+        row(ROW_FIRST, j, st.first_slice + k);
+      }
+      pending = true;
+      dist = st.pixel_size_z;
+      gp = global_phase(wl, st.pixel_size_z);
+    }
+  }
+  if (pending) {
+    col();
+    row(ROW_LAST, -1, 0);
+  }
+  // every column pass's tables before any launch (graph captures replay them)
+  for (ChainOp& op : ops)
+    if (op.col) RSIM_CHECK(chirp_tables(nx, ny, dx, dy, wl, op.dist, freq, &op.ct, &op.gp));
+  const int rv0 = ref >= 0 ? rvs[ref] : 0, cv0 = ref >= 0 ? cvs[ref] : 0;
+  FusedPrep<TW> none;
+  if (ref >= 0) none = prep[ref];
+  else {
+    const int tdtype = std::is_same<TW, twc>::value ? kTwComp : sizeof(TW) == 8 ? RSIM_C128 : RSIM_C64;
+    const void* twx[4];
+    const void* twy[4];
+    if (!reg_stage_tables(tdtype, nx, twx) || !reg_stage_tables(tdtype, ny, twy))
+      return fail(RSIM_E_HIP, "missing FFT schedule tables");
+    using TP = typename std::remove_reference<decltype(none.tx.t[0])>::type;
+    for (int i = 0; i < 4; ++i) {
+      none.tx.t[i] = reinterpret_cast<TP>(twx[i]);
+      none.ty.t[i] = reinterpret_cast<TP>(twy[i]);
+    }
+  }
+  const float spec_scale = (float)((double)nx * (double)ny);
+  // field flow: FIRST u -> U; columns and MID in place in U; the last column
+  // pass W -> u (spectrum -> U); LAST u -> u
+  for (size_t i = 0; i < ops.size(); ++i) {
+    const ChainOp& op = ops[i];
+    if (op.col) {
+      const bool last = i + 1 == ops.size() - 1;
+      ktime_begin(KT_COL, s);
+      if (!fused_col(ny, cv0, last && want_spec, U, last ? u : U, U, nx, none.ty, op.ct->Bx,
+                     op.ct->Ay, op.ct->Rmax, spec_scale, s))
+        return fail(RSIM_E_HIP, "no fused column kernel for ny=%d", ny);
+      ktime_end(KT_COL, s);
+    } else {
+      const FusedPrep<TW>& P = op.j >= 0 ? prep[op.j] : none;
+      TransArgsF ta = P.ta;
+      if (op.j >= 0) P.at(ta, op.sl);
+      const int tm = op.j >= 0 ? P.tm : TM_NONE;
+      const int rv = op.j >= 0 ? rvs[op.j] : rv0;
+      const void* in = op.mode == ROW_FIRST ? u : op.mode == ROW_LAST ? u : U;
+      void* out = op.mode == ROW_LAST ? u : U;
+      ktime_begin(KT_ROW, s);
+      if (!fused_row(nx, rv, op.mode, op.mode == ROW_LAST ? TM_LUT : tm, in, out, ny, P.tx, ta, s))
+        return fail(RSIM_E_HIP, "no fused row kernel for nx=%d", nx);
+      ktime_end(KT_ROW, s);
+    }
+    RSIM_LAUNCHED();
+  }
+  if (!pending && want_spec) RSIM_CHECK(fft2(RSIM_C64, RSIM_FORWARD, u, U, nx, ny, s));
+  return RSIM_OK;
+}
+
+int chain_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
+                double freq, bool band_limit, const rsim_chain_step* steps, int n_steps,
+                int engine, int flags, hipStream_t s) {
+  if (cancel_requested()) return fail(RSIM_E_CANCELLED, "propagation chain cancelled before it started");
+  const bool want_spec = !(flags & RSIM_FLAG_NO_SPECTRUM);
+  int depth = 0;
+  for (int j = 0; j < n_steps; ++j)
+    if (steps[j].vol) depth += steps[j].n_slices;
+  if (engine == RSIM_FUSED && fused_supported(nx, ny, dtype) && ny != kBigCol) {
+    flags = default_math(nx, ny, depth, flags);
+    if (flags & RSIM_FLAG_MATH_COMP32)
+      return run_chain<twc>(u, U, nx, ny, dx, dy, wl, freq, band_limit, steps, n_steps, want_spec, s);
+    if (flags & RSIM_FLAG_MATH_FP64)
+      return run_chain<double>(u, U, nx, ny, dx, dy, wl, freq, band_limit, steps, n_steps,
+                               want_spec, s);
+    return run_chain<float>(u, U, nx, ny, dx, dy, wl, freq, band_limit, steps, n_steps, want_spec, s);
+  }
+  // step by step: the source's band limit leaves U undefined
+  // (big-wave/source.py:38-40); every later step leaves U = FFT2(u)
+  bool spectrum_valid = false;
+  if (band_limit) {
+    RSIM_CHECK(fft2(dtype, RSIM_FORWARD, u, U, nx, ny, s));
+    RSIM_CHECK(chirp_cutoff(U, nx, ny, dtype, dx, dy, 1.0, 0.0, freq, false, s));
+    RSIM_CHECK(fft2(dtype, RSIM_INVERSE, U, u, nx, ny, s));
+  }
+  for (int j = 0; j < n_steps; ++j) {
+    const rsim_chain_step& st = steps[j];
+    if (st.vol) {
+      RSIM_CHECK(sample_apply(u, U, nx, ny, dtype, dx, dy, wl, freq, st.vol, st.pixel_size_x,
+                              st.pixel_size_y, st.pixel_size_z, st.x_pos, st.y_pos,
+                              st.first_slice, st.n_slices, nullptr, nullptr, engine,
+                              flags & ~RSIM_FLAG_NO_SPECTRUM, s, nullptr));
+    } else {
+      RSIM_CHECK(propagate(u, U, nx, ny, dtype, dx, dy, wl, st.dz, freq, spectrum_valid, s));
+    }
+    if (!st.vol || st.n_slices > 0) spectrum_valid = true;   // a step that ran leaves FFT2(u)
+  }
+  if (want_spec && !spectrum_valid) RSIM_CHECK(fft2(dtype, RSIM_FORWARD, u, U, nx, ny, s));
+  return RSIM_OK;
 }
 
 // ---------------------------------------------------------------------------

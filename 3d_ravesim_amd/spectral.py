@@ -103,30 +103,32 @@ class SpectralRun:
         self.graph: Optional[CapturedGraph] = None
 
     # -- the run as library calls --
+    def _steps(self, q: Instance):
+        """The instance's stages and last leg as rsim_chain_step records."""
+        rows = []
+        for st in q.stages:
+            if isinstance(st, GapStage):
+                rows.append(_lib.ChainStep(None, 0.0, 0.0, 0.0, 0.0, 0.0, 0, 0, st.dz))
+            else:
+                px = st.pixel_size_x or q.dx
+                rows.append(_lib.ChainStep(st.volume.handle, px, st.pixel_size_y or px,
+                                           st.pixel_size_z, st.x_pos, st.y_pos, 0, st.n_slices,
+                                           0.0))
+        if self.det.final_dz > 0:
+            rows.append(_lib.ChainStep(None, 0.0, 0.0, 0.0, 0.0, 0.0, 0, 0, self.det.final_dz))
+        return (_lib.ChainStep * max(len(rows), 1))(*rows), len(rows)
+
     def _instance(self, q: Instance, u: DeviceVector, U: DeviceVector) -> None:
-        s, code, eng = self.stream, self.code, self.engine
+        s, code = self.stream, self.code
         dy = q.dy or q.dx
         _lib.call("rsim_propagate_analytically_2d", u.ptr, q.nx, q.ny, code, q.dx, dy,
                   q.source_distance, q.source_x, q.source_y, q.wl, s)
-        _lib.call("rsim_fft2", U.ptr, u.ptr, q.nx, q.ny, code, _lib.RSIM_FORWARD, s)
-        _lib.call("rsim_apply_frequency_cutoff_2d", U.ptr, q.nx, q.ny, code, q.cutoff_freq, q.dx,
-                  dy, s)
-        _lib.call("rsim_fft2", u.ptr, U.ptr, q.nx, q.ny, code, _lib.RSIM_INVERSE, s)
-        spectrum_valid = False   # a source leaves U undefined (big-wave/source.py:38-40)
-        for st in q.stages:
-            if isinstance(st, GapStage):
-                _lib.call("rsim_propagate_2d", u.ptr, U.ptr, q.nx, q.ny, code, q.dx, dy, q.wl,
-                          st.dz, q.cutoff_freq, int(spectrum_valid), s)
-            else:
-                px = st.pixel_size_x or q.dx
-                _lib.call("rsim_sample_apply_2d", u.ptr, U.ptr, q.nx, q.ny, code, q.dx, dy, q.wl,
-                          q.cutoff_freq, st.volume.handle, px, st.pixel_size_y or px,
-                          st.pixel_size_z, st.x_pos, st.y_pos, 0, st.n_slices, eng, 0, s)
-            spectrum_valid = True
+        # the source's band limit, the stages and the last leg: one fused
+        # chain on the complex64 engine, call by call otherwise
+        steps, n = self._steps(q)
+        _lib.call("rsim_chain_apply_2d", u.ptr, U.ptr, q.nx, q.ny, code, q.dx, dy, q.wl,
+                  q.cutoff_freq, 1, steps, n, self.engine, _lib.RSIM_FLAG_NO_SPECTRUM, s)
         d = self.det
-        if d.final_dz > 0:
-            _lib.call("rsim_propagate_2d", u.ptr, U.ptr, q.nx, q.ny, code, q.dx, dy, q.wl,
-                      d.final_dz, q.cutoff_freq, int(spectrum_valid), s)
         _lib.call("rsim_detector_area_v1_accumulate", u.ptr, q.nx, q.ny, code, q.dx, dy, d.out_x,
                   d.out_y, d.pixel_size_x, d.pixel_size_y, d.z, float(q.weight),
                   self.image_ptr, s)

@@ -221,6 +221,34 @@ int rsim_sample_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx,
                          double x_pos, double y_pos, int first_slice, int n_slices, int engine,
                          int flags, void* stream);
 
+/* One step of a propagation chain: n_slices slices of vol (each a
+ * transmission, then propagation over pixel_size_z, as rsim_sample_apply_2d),
+ * or, with vol NULL, free space over dz (as rsim_propagate_2d). */
+typedef struct rsim_chain_step {
+  const rsim_volume* vol;
+  double pixel_size_x, pixel_size_y, pixel_size_z;
+  double x_pos, y_pos;
+  int first_slice, n_slices;
+  double dz;
+} rsim_chain_step;
+
+/* Replaces the element loop of big-wave's run_simulation for one source, at
+ * one cutoff frequency for every step (big-wave/wavesim.py:199-300: the gap
+ * to each element, its apply, the last leg, each its own FFT2 ... IFFT2):
+ * with band_limit set, first U = FFT2(u), the circular cutoff, u = IFFT2(U)
+ * (the point source's step, big-wave/source.py:97-107); then the steps in
+ * order.  With engine RSIM_FUSED and RSIM_C64 at a fused size (columns of up
+ * to 8192) the chain is one sequence of fused row and column passes: the
+ * propagations with no transmission between them (band limit, free space,
+ * the slice before it) fold into one column pass over their summed distance,
+ * P(a) P(b) = P(a + b) with one cutoff.  Otherwise every step runs as its own
+ * call.  Unless flags has RSIM_FLAG_NO_SPECTRUM, U holds FFT2(u) on return.
+ * The cancel flag is polled once, before the chain starts. */
+int rsim_chain_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
+                        double wl, double cutoff_freq, int band_limit,
+                        const rsim_chain_step* steps, int n_steps, int engine, int flags,
+                        void* stream);
+
 /* Fresnel-scaled / cone-beam BPM forms (big-wave/propagation.py:114-196,
  * big-wave/optical_element.py:533-592, big-wave/source.py:62-90).
  * rsim_sample_apply_2d_scaled: slice k samples the voxel grid at field
