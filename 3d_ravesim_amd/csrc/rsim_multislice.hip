@@ -547,10 +547,19 @@ int fused_schedule_lookup(int nx, int ny, int* rv, int* cv, int* tile) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   *rv = *cv = -1;
   if (tile) *tile = -1;
+  // the material-id lookup in FP32 butterflies (the default voxel slice loop),
+  // else the first tuned entry of the shape
+  auto it = g_tune.find(std::make_tuple(current_device(), nx, ny, (int)TM_LUT));
+  if (it != g_tune.end()) {
+    pick(it->second, rv, cv, tile);
+    return RSIM_OK;
+  }
   for (auto& kv : g_tune) {
     if (std::get<0>(kv.first) == current_device() && std::get<1>(kv.first) == nx &&
-        std::get<2>(kv.first) == ny)
+        std::get<2>(kv.first) == ny) {
       pick(kv.second, rv, cv, tile);
+      break;
+    }
   }
   return RSIM_OK;
 }
@@ -900,14 +909,12 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
       // stop at this slice boundary: the row IFFT of W of slice k - 1 leaves
       // u = the field after k slices (U then does not hold FFT2(u))
       if (k > 0) {
-        ktime_begin(KT_ROW, s);
         if (w_in) {
           fused_row(nx, rv, ROW_LAST, tm, u, U, ny, tx, ta, s, AdjArgs{}, w_in);
           RSIM_HIP(hipMemcpyAsync(u, U, (size_t)nx * ny * 8, hipMemcpyDeviceToDevice, s));
         } else {
           fused_row(nx, rv, ROW_LAST, tm, U, u, ny, tx, ta, s);
         }
-        ktime_end(KT_ROW, s);
         RSIM_LAUNCHED();
       }
       note_done(k, total);
@@ -933,7 +940,8 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
       if (xbin) store.xb = xb;
       else store.fld_out = reinterpret_cast<cpx<float>*>(u);
     }
-    ktime_begin(KT_ROW, s);
+    // the row-pass samples are MID launches (FIRST and LAST run one transform)
+    if (k > 0) ktime_begin(KT_ROW, s);
     const void* rin = k == 0 ? u : (w_in ? u : U);
     if (!fused_row(nx, rv, k == 0 ? ROW_FIRST : ROW_MID, tm, rin, U, ny, tx, ta, s, store,
                    k == 0 ? 0 : w_in))
@@ -966,9 +974,7 @@ static int run_fused(void* u, void* U, int nx, int ny, double dx, double dy, dou
     RSIM_LAUNCHED();
     if (last) {
       // row IFFT of W
-      ktime_begin(KT_ROW, s);
       fused_row(nx, rv, ROW_LAST, tm, big ? U : u, u, ny, tx, ta, s);
-      ktime_end(KT_ROW, s);
       RSIM_LAUNCHED();
       if (big && want_spec) RSIM_CHECK(fft2(RSIM_C64, RSIM_FORWARD, u, U, nx, ny, s));
     }
@@ -1143,7 +1149,7 @@ static int run_chain(void* u, void* U, int nx, int ny, double dx, double dy, dou
       const int rv = op.j >= 0 ? rvs[op.j] : rv0;
       const void* in = op.mode == ROW_FIRST ? u : op.mode == ROW_LAST ? u : U;
       void* out = op.mode == ROW_LAST ? u : U;
-      ktime_begin(KT_ROW, s);
+      if (op.mode == ROW_MID) ktime_begin(KT_ROW, s);
       if (!fused_row(nx, rv, op.mode, op.mode == ROW_LAST ? TM_LUT : tm, in, out, ny, P.tx, ta, s))
         return fail(RSIM_E_HIP, "no fused row kernel for nx=%d", nx);
       ktime_end(KT_ROW, s);

@@ -833,30 +833,37 @@ int fill_zero(void* p, size_t bytes, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// STREAM-style copy: the HBM bandwidth one plain kernel reaches (16-byte
-// vector loads and stores, four per thread per step, grid-stride)
+// Device copy (the measured copy ceiling beside hipMemcpy): each workgroup
+// copies one contiguous 128 KiB tile, 16 loads of 8 bytes per thread in
+// flight before its 16 stores; every wavefront access is 512 contiguous
+// bytes.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) stream_copy_kernel(const uint4* __restrict__ src,
-                                                          uint4* __restrict__ dst, long long n) {
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  for (; i + 3 * stride < n; i += 4 * stride) {
-    const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
-    dst[i] = a;
-    dst[i + stride] = b;
-    dst[i + 2 * stride] = c;
-    dst[i + 3 * stride] = d;
+constexpr int kCopyThreads = 1024, kCopyPer = 16;
+__global__ void __launch_bounds__(kCopyThreads) stream_copy_kernel(const uint2* __restrict__ src,
+                                                                   uint2* __restrict__ dst,
+                                                                   long long n) {
+  const long long base = (long long)blockIdx.x * (kCopyThreads * kCopyPer) + threadIdx.x;
+  uint2 v[kCopyPer];
+  if (base + (long long)(kCopyPer - 1) * kCopyThreads < n) {
+#pragma unroll
+    for (int e = 0; e < kCopyPer; ++e) v[e] = src[base + e * kCopyThreads];
+#pragma unroll
+    for (int e = 0; e < kCopyPer; ++e) dst[base + e * kCopyThreads] = v[e];
+  } else {
+    for (int e = 0; e < kCopyPer; ++e)
+      if (base + e * kCopyThreads < n) dst[base + e * kCopyThreads] = src[base + e * kCopyThreads];
   }
-  for (; i < n; i += stride) dst[i] = src[i];
 }
 
 int stream_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
-  if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src) | bytes) & 15)
-    return fail(RSIM_E_INVALID_ARG, "stream copy needs 16-byte aligned buffers and sizes");
-  const long long n = (long long)(bytes / 16);
-  const int blocks = (int)std::max<long long>(1, std::min<long long>((n + 1023) / 1024, 8192));
-  hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, s, (const uint4*)src,
-                     (uint4*)dst, n);
+  if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src) | bytes) & 7)
+    return fail(RSIM_E_INVALID_ARG, "stream copy needs 8-byte aligned buffers and sizes");
+  const long long n = (long long)(bytes / 8);
+  const long long tile = (long long)kCopyThreads * kCopyPer;
+  const long long blocks = (n + tile - 1) / tile;
+  if (blocks > 0x7fffffffLL) return fail(RSIM_E_INVALID_ARG, "stream copy of %zu bytes is too large", bytes);
+  hipLaunchKernelGGL(stream_copy_kernel, dim3((unsigned)blocks), dim3(kCopyThreads), 0, s,
+                     (const uint2*)src, (uint2*)dst, n);
   RSIM_LAUNCHED();
   return RSIM_OK;
 }

@@ -43,22 +43,27 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, tag=None):
     """HBM bytes per dispatch of `kernel` (default FP32 engine) from the
     committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary, scaled by the copy
-    calibration recorded there; None when the file holds no figure for it."""
+    calibration recorded there: the line of launch shape `tag` (e.g. "col7",
+    "col7t" when tiled) when the file has one, else its first line for the
+    kernel; None when the file holds no figure for it."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_FILE)
+    first = None
     try:
         with open(path) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) >= 4 and parts[0] == "traffic" and parts[1] == kernel and parts[2] == "fp32":
-                    for tok in parts[3:]:
-                        if tok.isdigit():
-                            return float(tok)
+                if (len(parts) >= 4 and parts[0] == "traffic" and parts[1] == kernel
+                        and parts[2] == "fp32" and parts[3].isdigit()):
+                    if tag is not None and len(parts) >= 5 and parts[4] == tag:
+                        return float(parts[3])
+                    if first is None:
+                        first = float(parts[3])
     except OSError:
         pass
-    return None
+    return first
 
 
 def make_volume(rs, nz, n, seed=2, nsph=200, voxel=0.5e-6):
@@ -356,6 +361,11 @@ def main():
     lib.call("rsim_fused_schedule", n, n, C.byref(rv), C.byref(cv))
     lib.call("rsim_fused_tile", n, n, C.byref(tw))
     result["fused_launch_variants"] = {"row": rv.value, "col": cv.value, "tile": tw.value}
+    rl = result["roofline"]
+    tag = (f"col{cv.value}" if rl["kernel"].startswith("ms_col") else f"row{rv.value}") + \
+        ("t" if tw.value > 0 else "")
+    rl["traffic"] = measured_traffic(rl["kernel"], tag)
+    rl["traffic_shape"] = tag
 
     # ---- CPU baseline: the FP64 oracle on a bounded sample ----------------
     if not args.no_cpu:
@@ -398,6 +408,9 @@ def main():
 
     # ---- bandwidth ceilings beside the 8 TB/s spec peak ---------------------
     result["copy_ceiling"] = copy_ceilings(lib, s)
+    # the 65 B/pt step rate against what a plain copy kernel moves
+    result["step_frac_of_copy_kernel"] = round(
+        result["step_hbm_GBs_65B_model"] / result["copy_ceiling"]["copy_kernel_GBs"], 4)
     log(f"copy ceilings: {result['copy_ceiling']}")
     del u, U, u2, U2
     vol.destroy()

@@ -135,14 +135,16 @@ int rsim_graph_destroy(rsim_graph* graph);
 /* Zero `bytes` (a multiple of 8, 16-byte aligned) of device memory with a
  * kernel, on `stream` (usable inside a capture). */
 int rsim_fill_zero(void* dptr, size_t bytes, void* stream);
-/* Device-to-device copy by a plain HIP kernel (16-byte vector accesses,
- * 16-byte aligned buffers and size): the achievable-bandwidth ceiling the
- * benchmark reports beside the spec peak and hipMemcpy. */
+/* Device-to-device copy by a plain HIP kernel (one contiguous 128 KiB tile
+ * per workgroup, 8-byte accesses; 8-byte aligned buffers and size): the
+ * achievable-bandwidth ceiling the benchmark reports beside the spec peak and
+ * hipMemcpy. */
 int rsim_stream_copy(void* dst, const void* src, size_t bytes, void* stream);
 
 /* Kernel timing (the reference's CudaTimerSection, fast-wave/src/simulation.cpp:14-33):
- * every > 0 brackets one launch in `every` of each kind (row pass, column
- * pass, pointwise launch of the FFT and multislice engines) with hipEvents on
+ * every > 0 brackets one launch in `every` of each kind (row pass -- of the
+ * slice loops, the two-transform MID passes only --, column pass, pointwise
+ * launch of the FFT and multislice engines) with hipEvents on
  * its own stream (every = 1: all launches); 0 disables.  Each event pair adds
  * a few microseconds of dispatch gap, hence the sampling.
  * rsim_kernel_timing_read waits for them and returns, per kind
@@ -378,7 +380,9 @@ int rsim_batch_destroy(rsim_batch* batch);
  * keeps the fastest (the tiled form unless an in-place pair is more than 1 %
  * faster: it moves fewer HBM bytes); the environment variable
  * RAVESIM_FUSED_VARIANT="row,col[,tiled]" overrides it.  Returns the variants
- * of the voxel slice loop for (nx, ny), or -1 before the first fused call. */
+ * of the voxel slice loop for (nx, ny) (material-id lookup, FP32 butterflies,
+ * when it has run; else the first tuned form), or -1 before the first fused
+ * call. */
 int rsim_fused_schedule(int nx, int ny, int* row_variant, int* col_variant);
 /* Width of the column tiles between the passes of the voxel slice loop for
  * (nx, ny): 0 = in place, row-major; -1 before the first fused call (no
