@@ -127,6 +127,10 @@ __device__ __forceinline__ void dft5(cpx<R>* v) {
 constexpr double kC1 = 0.92387953251128675612818318939679;  // cos(pi/8)
 constexpr double kS1 = 0.38268343236508977172845998403040;  // sin(pi/8)
 constexpr double kH = 0.70710678118654752440084436210485;   // sqrt(1/2)
+constexpr double kC16 = 0.98078528040323044912618223613424;  // cos(pi/16)
+constexpr double kS16 = 0.19509032201612826784828486847702;  // sin(pi/16)
+constexpr double kC316 = 0.83146961230254523707878837761791; // cos(3pi/16)
+constexpr double kS316 = 0.55557023301960222474283081394853; // sin(3pi/16)
 #define RSIM_HI(v) ((float)(v))
 #define RSIM_LO(v) ((float)((v) - (double)(float)(v)))
 
@@ -221,6 +225,63 @@ __device__ __forceinline__ void dft16(cpx<R>* v) {
   }
 }
 
+// Rotation constants of the radix-32 butterfly: cos and sin of pi/16 and
+// 3 pi/16.  As for radix 16, the inverse direction takes FP32 values within
+// an ulp that match the forward ones' norm error: |W_inv|^2 |W_fwd|^2 - 1 is
+// 1.6e-9 (pi/16) and 2.9e-9 (3 pi/16), where plain roundings leave -1.2e-7
+// and -3.0e-8 per transform pair.
+template <int DIR, typename R>
+struct RotK32 {
+  static constexpr R a1 = R(kC16), b1 = R(kS16), a3 = R(kC316), b3 = R(kS316);
+};
+template <>
+struct RotK32<1, float> {
+  static constexpr float a1 = 0x1.f6297ep-1f;   // fl32(cos(pi/16)) + 1 ulp
+  static constexpr float b1 = 0x1.8f8b84p-3f;   // fl32(sin(pi/16))
+  static constexpr float a3 = 0x1.a9b664p-1f;   // fl32(cos(3pi/16)) + 1 ulp
+  static constexpr float b3 = 0x1.1c73b2p-1f;   // fl32(sin(3pi/16)) - 1 ulp
+};
+
+// Radix 32 = 2 x 16 (decimation in time): E = DFT16 of the even inputs,
+// O = DFT16 of the odd ones, X[k] = E[k] + W32^k O[k], X[k + 16] =
+// E[k] - W32^k O[k], W32 = exp(DIR 2 pi i / 32).
+template <int DIR, typename R>
+__device__ __forceinline__ void dft32(cpx<R>* v) {
+  cpx<R> e[16], o[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    e[k] = v[2 * k];
+    o[k] = v[2 * k + 1];
+  }
+  dft16<DIR>(e);
+  dft16<DIR>(o);
+  const R c1 = RotK<DIR, R>::c1, s1 = RotK<DIR, R>::s1;
+  const R a1 = RotK32<DIR, R>::a1, b1 = RotK32<DIR, R>::b1;
+  const R a3 = RotK32<DIR, R>::a3, b3 = RotK32<DIR, R>::b3;
+  const R d = R(DIR);
+  // W32^k = (cos(k pi/16), DIR sin(k pi/16))
+  o[1] = cmulcs(o[1], a1, d * b1);
+  o[2] = cmulcs(o[2], c1, d * s1);
+  o[3] = cmulcs(o[3], a3, d * b3);
+  o[4] = rot45<DIR>(o[4]);
+  o[5] = cmulcs(o[5], b3, d * a3);
+  o[6] = cmulcs(o[6], s1, d * c1);
+  o[7] = cmulcs(o[7], b1, d * a1);
+  o[8] = rot<DIR>(o[8]);
+  o[9] = cmulcs(o[9], -b1, d * a1);
+  o[10] = cmulcs(o[10], -s1, d * c1);
+  o[11] = cmulcs(o[11], -b3, d * a3);
+  o[12] = rot<DIR>(rot45<DIR>(o[12]));
+  o[13] = cmulcs(o[13], -a3, d * b3);
+  o[14] = cmulcs(o[14], -c1, d * s1);
+  o[15] = cmulcs(o[15], -a1, d * b1);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    v[k] = e[k] + o[k];
+    v[k + 16] = e[k] - o[k];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Compensated single precision: constants and twiddles carried as FP32
 // hi + lo pairs (value = hi + lo to ~2^-48), products formed with FMA so that
@@ -305,6 +366,44 @@ __device__ __forceinline__ void dft16_comp(cpx<float>* v) {
   }
 }
 
+// Radix 32 in compensated FP32 (dft32 with hi + lo constants).
+template <int DIR>
+__device__ __forceinline__ void dft32_comp(cpx<float>* v) {
+  cpx<float> e[16], o[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    e[k] = v[2 * k];
+    o[k] = v[2 * k + 1];
+  }
+  dft16_comp<DIR>(e);
+  dft16_comp<DIR>(o);
+  const float d = (float)DIR;
+  const float ch = RSIM_HI(kC1), cl = RSIM_LO(kC1), sh = RSIM_HI(kS1), sl = RSIM_LO(kS1);
+  const float hh = RSIM_HI(kH), hl = RSIM_LO(kH);
+  const float ah = RSIM_HI(kC16), al = RSIM_LO(kC16), bh = RSIM_HI(kS16), bl = RSIM_LO(kS16);
+  const float gh = RSIM_HI(kC316), gl = RSIM_LO(kC316), qh = RSIM_HI(kS316), ql = RSIM_LO(kS316);
+  o[1] = cmul_comp(o[1], ah, al, d * bh, d * bl);
+  o[2] = cmul_comp(o[2], ch, cl, d * sh, d * sl);
+  o[3] = cmul_comp(o[3], gh, gl, d * qh, d * ql);
+  o[4] = cmul_comp(o[4], hh, hl, d * hh, d * hl);
+  o[5] = cmul_comp(o[5], qh, ql, d * gh, d * gl);
+  o[6] = cmul_comp(o[6], sh, sl, d * ch, d * cl);
+  o[7] = cmul_comp(o[7], bh, bl, d * ah, d * al);
+  o[8] = rot<DIR>(o[8]);
+  o[9] = cmul_comp(o[9], -bh, -bl, d * ah, d * al);
+  o[10] = cmul_comp(o[10], -sh, -sl, d * ch, d * cl);
+  o[11] = cmul_comp(o[11], -qh, -ql, d * gh, d * gl);
+  o[12] = rot<DIR>(cmul_comp(o[12], hh, hl, d * hh, d * hl));
+  o[13] = cmul_comp(o[13], -gh, -gl, d * qh, d * ql);
+  o[14] = cmul_comp(o[14], -ch, -cl, d * sh, d * sl);
+  o[15] = cmul_comp(o[15], -ah, -al, d * bh, d * bl);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    v[k] = e[k] + o[k];
+    v[k + 16] = e[k] - o[k];
+  }
+}
+
 // Prime-factor (Good-Thomas) radix-6 and radix-12 butterflies: for
 // N = N1 * N2 with coprime factors, x[(N2 n1 + N1 n2) mod N] -> DFT_N2 over n2
 // -> DFT_N1 over n1 -> X[k], k = k1 (mod N1), k = k2 (mod N2), with no
@@ -353,12 +452,14 @@ __device__ __forceinline__ void dft(cpx<R>* v) {
   else if constexpr (RADIX == 8) dft8<DIR>(v);
   else if constexpr (RADIX == 12) dft12<DIR>(v);
   else if constexpr (RADIX == 16) dft16<DIR>(v);
+  else if constexpr (RADIX == 32) dft32<DIR>(v);
 }
 
 template <int RADIX, int DIR>
 __device__ __forceinline__ void dft_comp(cpx<float>* v) {
   if constexpr (RADIX == 8) dft8_comp<DIR>(v);
   else if constexpr (RADIX == 16) dft16_comp<DIR>(v);
+  else if constexpr (RADIX == 32) dft32_comp<DIR>(v);
   else dft<RADIX, DIR>(v);
 }
 

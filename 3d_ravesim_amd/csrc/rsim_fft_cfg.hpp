@@ -22,8 +22,9 @@ template <> struct Sched<8192> { using F = RegFac<8192, 16, 16, 2, 16, 16>; };
 // 2^a * 3 lengths: E = 24 elements per thread (a multiple of 8, 6 and 12)
 template <> struct Sched<3072> { using F = RegFac<3072, 24, 8, 6, 8, 8>; };
 template <> struct Sched<6144> { using F = RegFac<6144, 24, 8, 12, 8, 8>; };
-// row pass of 16384-point axes (1024 threads per row)
-template <> struct Sched<16384> { using F = RegFac<16384, 16, 16, 4, 16, 16>; };
+// row pass of 16384-point axes: 32 elements per thread (512 threads per row),
+// three stages (two LDS exchanges per transform)
+template <> struct Sched<16384> { using F = RegFac<16384, 32, 32, 16, 32>; };
 // sub-transform of the three-pass 16384-point column schedule (128 x 128)
 template <> struct Sched<128> { using F = RegFac<128, 8, 8, 2, 8>; };
 

@@ -514,9 +514,12 @@ struct ColLaunch {
 };
 
 // Variant lists: X(N, id, E, C, SPLIT).  Variant 0 is the default.  E1 is
-// the schedule's own element count (16, or 24 for 2^a * 3 lengths), E2 = 2*E1.
+// the schedule's own element count (16, 24 for 2^a * 3 lengths, 32 for
+// 16384), E2 = 2*E1 (E1 for 16384).
 template <int N> constexpr int vE1() { return Sched<N>::F::E; }
-template <int N> constexpr int vE2() { return 2 * Sched<N>::F::E; }
+// (schedules of 32 elements per thread have no wider variant: 64 complex
+// values per thread leave no registers for the butterflies)
+template <int N> constexpr int vE2() { return Sched<N>::F::E >= 32 ? Sched<N>::F::E : 2 * Sched<N>::F::E; }
 constexpr int vmax1(int v) { return v > 0 ? v : 1; }
 constexpr int vmin(int a, int b) { return a < b ? a : b; }
 constexpr bool vsplit(int n, int c, int cap) { return (long long)n * c * 8 > cap; }
