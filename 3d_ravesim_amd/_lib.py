@@ -56,6 +56,12 @@ class ChainStep(C.Structure):
                 ("x_pos", _d), ("y_pos", _d), ("first_slice", _i), ("n_slices", _i), ("dz", _d)]
 
 
+class ChainInstance(C.Structure):
+    """rsim_chain_instance of include/ravesim.h."""
+    _fields_ = [("u", _vp), ("U", _vp), ("dx", _d), ("dy", _d), ("wl", _d), ("cutoff_freq", _d),
+                ("steps", C.POINTER(ChainStep))]
+
+
 class HistoryFrames(C.Structure):
     """rsim_history_frames of include/ravesim.h."""
     _fields_ = [("out_x", _i), ("out_y", _i), ("pixel_size_x", _d), ("pixel_size_y", _d),
@@ -113,6 +119,7 @@ SIGNATURES = {
                              _i, _i, _i, _vp],
     "rsim_chain_apply_2d": [_vp, _vp, _i, _i, _i, _d, _d, _d, _d, _i, C.POINTER(ChainStep), _i,
                             _i, _i, _vp],
+    "rsim_chain_apply_2d_batch": [C.POINTER(ChainInstance), _i, _i, _i, _i, _i, _i, _i, _i, _vp],
     "rsim_fused_schedule": [_i, _i, _ip, _ip],
     "rsim_fused_tile": [_i, _i, _ip],
     "rsim_detector_area_v1": [_vp, _i, _i, _i, _d, _d, _i, _i, _d, _d, _d, _dp, _vp],

@@ -816,6 +816,31 @@ int rsim_sample_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx,
                       nullptr, engine, flags, resolve_stream(stream));
 }
 
+namespace {
+int check_chain_steps(const rsim_chain_step* steps, int n_steps, int inst) {
+  if (n_steps < 0 || (n_steps > 0 && !steps))
+    return fail(RSIM_E_INVALID_ARG, "a chain needs n_steps >= 0 and its steps");
+  for (int j = 0; j < n_steps; ++j) {
+    const rsim_chain_step& st = steps[j];
+    if (st.vol) {
+      const int rc = check_sample(st.vol, st.first_slice, st.n_slices, st.pixel_size_x,
+                                  st.pixel_size_y, st.pixel_size_z);
+      if (rc != RSIM_OK) {
+        const std::string why = rsim_last_error();
+        if (inst >= 0) return fail(rc, "instance %d, chain step %d: %s", inst, j, why.c_str());
+        return fail(rc, "chain step %d: %s", j, why.c_str());
+      }
+    } else if (!std::isfinite(st.dz)) {
+      if (inst >= 0)
+        return fail(RSIM_E_INVALID_ARG, "instance %d, chain step %d: free-space distance must be finite",
+                    inst, j);
+      return fail(RSIM_E_INVALID_ARG, "chain step %d: free-space distance must be finite", j);
+    }
+  }
+  return RSIM_OK;
+}
+}  // namespace
+
 int rsim_chain_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx, double dy,
                         double wl, double cutoff_freq, int band_limit,
                         const rsim_chain_step* steps, int n_steps, int engine, int flags,
@@ -826,23 +851,43 @@ int rsim_chain_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx, 
   if (engine != RSIM_PLAIN && engine != RSIM_FUSED)
     return fail(RSIM_E_INVALID_ARG, "engine must be RSIM_PLAIN or RSIM_FUSED");
   if (!(wl > 0) || !(dx > 0) || !(dy > 0)) return fail(RSIM_E_INVALID_ARG, "dx, dy and wl must be positive");
-  if (n_steps < 0 || (n_steps > 0 && !steps))
-    return fail(RSIM_E_INVALID_ARG, "a chain needs n_steps >= 0 and its steps");
-  for (int j = 0; j < n_steps; ++j) {
-    const rsim_chain_step& st = steps[j];
-    if (st.vol) {
-      const int rc = check_sample(st.vol, st.first_slice, st.n_slices, st.pixel_size_x,
-                                  st.pixel_size_y, st.pixel_size_z);
-      if (rc != RSIM_OK) {
-        const std::string why = rsim_last_error();
-        return fail(rc, "chain step %d: %s", j, why.c_str());
-      }
-    } else if (!std::isfinite(st.dz)) {
-      return fail(RSIM_E_INVALID_ARG, "chain step %d: free-space distance must be finite", j);
+  RSIM_CHECK(check_chain_steps(steps, n_steps, -1));
+  const rsim_chain_instance one{u, U, dx, dy, wl, cutoff_freq, steps};
+  return chain_apply(&one, 1, nx, ny, dtype, band_limit != 0, n_steps, engine, flags,
+                     resolve_stream(stream));
+}
+
+int rsim_chain_apply_2d_batch(const rsim_chain_instance* instances, int count, int nx, int ny,
+                              int dtype, int band_limit, int n_steps, int engine, int flags,
+                              void* stream) {
+  if (count < 0 || (count > 0 && !instances))
+    return fail(RSIM_E_INVALID_ARG, "a batched chain needs count >= 0 and its instances");
+  RSIM_CHECK(check_fft_sizes(nx, ny));
+  if (engine != RSIM_PLAIN && engine != RSIM_FUSED)
+    return fail(RSIM_E_INVALID_ARG, "engine must be RSIM_PLAIN or RSIM_FUSED");
+  for (int i = 0; i < count; ++i) {
+    const rsim_chain_instance& q = instances[i];
+    RSIM_CHECK(check_field(q.u, nx, ny, dtype, "u"));
+    RSIM_CHECK(check_field(q.U, nx, ny, dtype, "U"));
+    if (!(q.wl > 0) || !(q.dx > 0) || !(q.dy > 0))
+      return fail(RSIM_E_INVALID_ARG, "instance %d: dx, dy and wl must be positive", i);
+    RSIM_CHECK(check_chain_steps(q.steps, n_steps, i));
+    for (int j = 0; j < n_steps; ++j) {
+      const rsim_chain_step& a = q.steps[j];
+      const rsim_chain_step& b = instances[0].steps[j];
+      if ((a.vol == nullptr) != (b.vol == nullptr) || (a.vol && a.n_slices != b.n_slices))
+        return fail(RSIM_E_INVALID_ARG,
+                    "instance %d, chain step %d: every instance needs the same step kinds and "
+                    "slice counts as instance 0",
+                    i, j);
     }
+    for (int k = 0; k < i; ++k)
+      if (instances[k].u == q.u || instances[k].U == q.U || instances[k].u == q.U ||
+          instances[k].U == q.u)
+        return fail(RSIM_E_INVALID_ARG, "instances %d and %d share a field", k, i);
   }
-  return chain_apply(u, U, nx, ny, dtype, dx, dy, wl, cutoff_freq, band_limit != 0, steps,
-                     n_steps, engine, flags, resolve_stream(stream));
+  return chain_apply(instances, count, nx, ny, dtype, band_limit != 0, n_steps, engine, flags,
+                     resolve_stream(stream));
 }
 
 }  // extern "C"

@@ -143,10 +143,9 @@ int sample_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double 
                  double xpos, double ypos, int first, int count, const double* scales,
                  const double* dzs, int engine, int flags, hipStream_t s,
                  const HistSpec* hist = nullptr);
-// rsim_chain_apply_2d after its argument checks
-int chain_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
-                double freq, bool band_limit, const rsim_chain_step* steps, int n_steps,
-                int engine, int flags, hipStream_t s);
+// rsim_chain_apply_2d[_batch] after their argument checks
+int chain_apply(const rsim_chain_instance* in, int count, int nx, int ny, int dtype,
+                bool band_limit, int n_steps, int engine, int flags, hipStream_t s);
 
 // ---- kernel timing (kinds: 0 row pass, 1 column pass, 2 pointwise) ----
 enum { KT_ROW = 0, KT_COL = 1, KT_PW = 2 };

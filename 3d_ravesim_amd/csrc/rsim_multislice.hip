@@ -1039,165 +1039,226 @@ int slices_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double 
 // ---------------------------------------------------------------------------
 namespace {
 // One pass of a fused chain: a row pass (FIRST, MID or LAST) with the
-// transmission of slice sl of sample step j (j < 0: none), or a column pass
-// over dist.
+// transmission of slice k of sample step j (j < 0: none), or a column pass
+// whose chirp tables (one per instance: each has its own wavelength,
+// distances and cutoff) cover the propagations folded into it.
 struct ChainOp {
   bool col;
-  int mode, j, sl;
-  double dist;
-  double2 gp;   // product of the folded propagations' global phases
-  const ChirpTables* ct;
+  int mode, j, k;
+  std::vector<const ChirpTables*> ct;
 };
 double2 cmul2(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 }  // namespace
 
+// The chain of every instance (all of one grid, with the same step kinds and
+// slice counts) as one pass sequence; each pass launches the instances
+// together, up to kMaxBatch per launch (and one launch per transmission mode
+// where their voxel maps differ in it).
 template <typename TW>
-static int run_chain(void* u, void* U, int nx, int ny, double dx, double dy, double wl,
-                     double freq, bool band_limit, const rsim_chain_step* steps, int n_steps,
-                     bool want_spec, hipStream_t s) {
-  std::vector<FusedPrep<TW>> prep(n_steps);
-  std::vector<int> rvs(n_steps, 0), cvs(n_steps, 0);
-  int ref = -1;   // first sample step: its launch shapes serve the passes without transmission
-  for (int j = 0; j < n_steps; ++j) {
-    const rsim_chain_step& st = steps[j];
-    if (!st.vol || st.n_slices == 0) continue;
-    FusedPrep<TW>& P = prep[j];
-    RSIM_CHECK(fused_prepare<TW>(nx, ny, dx, dy, wl, freq, src_of(st.vol), st.pixel_size_x,
-                                 st.pixel_size_y, st.pixel_size_z, st.x_pos, st.y_pos, &P));
-    P.at(P.ta, st.first_slice);
-    RSIM_CHECK(fused_schedule<TW>(nx, ny, P.tm, P.tx, P.ty, P.ta, P.ct, s, &rvs[j], &cvs[j]));
-    if (ref < 0) ref = j;
-  }
-  // the pass sequence
+static int run_chain(const rsim_chain_instance* in, int count, int nx, int ny, bool band_limit,
+                     int n_steps, bool want_spec, hipStream_t s) {
+  const rsim_chain_step* st0 = in[0].steps;
+  // per instance and sample step: tables, transmission arguments, launch shapes
+  std::vector<std::vector<FusedPrep<TW>>> prep(count, std::vector<FusedPrep<TW>>(n_steps));
+  std::vector<std::vector<int>> rvs(count, std::vector<int>(n_steps, 0));
+  int rv0 = 0, cv0 = 0;
+  bool have_ref = false;   // the first sample step's shapes serve the passes without transmission
+  for (int i = 0; i < count; ++i)
+    for (int j = 0; j < n_steps; ++j) {
+      const rsim_chain_step& st = in[i].steps[j];
+      if (!st.vol || st.n_slices == 0) continue;
+      FusedPrep<TW>& P = prep[i][j];
+      RSIM_CHECK(fused_prepare<TW>(nx, ny, in[i].dx, in[i].dy, in[i].wl, in[i].cutoff_freq,
+                                   src_of(st.vol), st.pixel_size_x, st.pixel_size_y,
+                                   st.pixel_size_z, st.x_pos, st.y_pos, &P));
+      P.at(P.ta, st.first_slice);
+      int cv = 0;
+      RSIM_CHECK(fused_schedule<TW>(nx, ny, P.tm, P.tx, P.ty, P.ta, P.ct, s, &rvs[i][j], &cv));
+      if (!have_ref) {
+        rv0 = rvs[i][j];
+        cv0 = cv;
+        have_ref = true;
+      }
+    }
+  // the pass sequence (the same for every instance) and each instance's
+  // folded distance and global phase per column pass
   std::vector<ChainOp> ops;
+  std::vector<double> dist(count, 0.0);
+  std::vector<double2> gp(count, make_double2(1.0, 0.0));
   bool pending = false;   // a propagation waits for its column pass
-  double dist = 0.0;
-  double2 gp = make_double2(1.0, 0.0);
-  const double2 one = make_double2(1.0, 0.0);
-  auto row = [&](int mode, int j, int sl) {
-    ops.push_back(ChainOp{false, mode, j, sl, 0.0, one, nullptr});
+  auto row = [&](int mode, int j, int k) { ops.push_back(ChainOp{false, mode, j, k, {}}); };
+  auto col = [&]() -> int {
+    ChainOp op{true, 0, -1, 0, std::vector<const ChirpTables*>(count, nullptr)};
+    for (int i = 0; i < count; ++i)
+      RSIM_CHECK(chirp_tables(nx, ny, in[i].dx, in[i].dy, in[i].wl, dist[i], in[i].cutoff_freq,
+                              &op.ct[i], &gp[i]));
+    ops.push_back(std::move(op));
+    return RSIM_OK;
   };
-  auto col = [&]() { ops.push_back(ChainOp{true, 0, -1, 0, dist, gp, nullptr}); };
   if (band_limit) {
     row(ROW_FIRST, -1, 0);
     pending = true;
-    dist = 0.0;
-    gp = one;
+    for (int i = 0; i < count; ++i) {
+      dist[i] = 0.0;
+      gp[i] = make_double2(1.0, 0.0);
+    }
   }
   for (int j = 0; j < n_steps; ++j) {
-    const rsim_chain_step& st = steps[j];
-    if (!st.vol) {
+    if (!st0[j].vol) {
       if (!pending) row(ROW_FIRST, -1, 0);
-      dist = pending ? dist + st.dz : st.dz;
-      gp = pending ? cmul2(gp, global_phase(wl, st.dz)) : global_phase(wl, st.dz);
+      for (int i = 0; i < count; ++i) {
+        const double dz = in[i].steps[j].dz;
+        const double2 g = global_phase(in[i].wl, dz);
+        dist[i] = pending ? dist[i] + dz : dz;
+        gp[i] = pending ? cmul2(gp[i], g) : g;
+      }
       pending = true;
       continue;
     }
-    for (int k = 0; k < st.n_slices; ++k) {
+    for (int k = 0; k < st0[j].n_slices; ++k) {
       if (pending) {
-        col();
-        row(ROW_MID, j, st.first_slice + k);
+        RSIM_CHECK(col());
+        row(ROW_MID, j, k);
       } else {
-        row(ROW_FIRST, j, st.first_slice + k);
+        row(ROW_FIRST, j, k);
       }
       pending = true;
-      dist = st.pixel_size_z;
-      gp = global_phase(wl, st.pixel_size_z);
+      for (int i = 0; i < count; ++i) {
+        dist[i] = in[i].steps[j].pixel_size_z;
+        gp[i] = global_phase(in[i].wl, dist[i]);
+      }
     }
   }
   if (pending) {
-    col();
+    RSIM_CHECK(col());
     row(ROW_LAST, -1, 0);
   }
-  // every column pass's tables before any launch (graph captures replay them)
-  for (ChainOp& op : ops)
-    if (op.col) RSIM_CHECK(chirp_tables(nx, ny, dx, dy, wl, op.dist, freq, &op.ct, &op.gp));
-  const int rv0 = ref >= 0 ? rvs[ref] : 0, cv0 = ref >= 0 ? cvs[ref] : 0;
-  FusedPrep<TW> none;
-  if (ref >= 0) none = prep[ref];
-  else {
+  // twiddle tables (shared: one grid, one arithmetic)
+  StageTw<TW> tx{}, ty{};
+  {
     const int tdtype = std::is_same<TW, twc>::value ? kTwComp : sizeof(TW) == 8 ? RSIM_C128 : RSIM_C64;
     const void* twx[4];
     const void* twy[4];
     if (!reg_stage_tables(tdtype, nx, twx) || !reg_stage_tables(tdtype, ny, twy))
       return fail(RSIM_E_HIP, "missing FFT schedule tables");
-    using TP = typename std::remove_reference<decltype(none.tx.t[0])>::type;
-    for (int i = 0; i < 4; ++i) {
-      none.tx.t[i] = reinterpret_cast<TP>(twx[i]);
-      none.ty.t[i] = reinterpret_cast<TP>(twy[i]);
+    using TP = typename std::remove_reference<decltype(tx.t[0])>::type;
+    for (int q = 0; q < 4; ++q) {
+      tx.t[q] = reinterpret_cast<TP>(twx[q]);
+      ty.t[q] = reinterpret_cast<TP>(twy[q]);
     }
   }
   const float spec_scale = (float)((double)nx * (double)ny);
-  // field flow: FIRST u -> U; columns and MID in place in U; the last column
-  // pass W -> u (spectrum -> U); LAST u -> u
-  for (size_t i = 0; i < ops.size(); ++i) {
-    const ChainOp& op = ops[i];
+  // field flow per instance: FIRST u -> U; columns and MID in place in U; the
+  // last column pass W -> u (spectrum -> U); LAST u -> u
+  for (size_t o = 0; o < ops.size(); ++o) {
+    const ChainOp& op = ops[o];
     if (op.col) {
-      const bool last = i + 1 == ops.size() - 1;
-      ktime_begin(KT_COL, s);
-      if (!fused_col(ny, cv0, last && want_spec, U, last ? u : U, U, nx, none.ty, op.ct->Bx,
-                     op.ct->Ay, op.ct->Rmax, spec_scale, s))
-        return fail(RSIM_E_HIP, "no fused column kernel for ny=%d", ny);
-      ktime_end(KT_COL, s);
-    } else {
-      const FusedPrep<TW>& P = op.j >= 0 ? prep[op.j] : none;
-      TransArgsF ta = P.ta;
-      if (op.j >= 0) P.at(ta, op.sl);
-      const int tm = op.j >= 0 ? P.tm : TM_NONE;
-      const int rv = op.j >= 0 ? rvs[op.j] : rv0;
-      const void* in = op.mode == ROW_FIRST ? u : op.mode == ROW_LAST ? u : U;
-      void* out = op.mode == ROW_LAST ? u : U;
-      if (op.mode == ROW_MID) ktime_begin(KT_ROW, s);
-      if (!fused_row(nx, rv, op.mode, op.mode == ROW_LAST ? TM_LUT : tm, in, out, ny, P.tx, ta, s))
-        return fail(RSIM_E_HIP, "no fused row kernel for nx=%d", nx);
-      ktime_end(KT_ROW, s);
+      const bool last = o + 1 == ops.size() - 1;
+      for (int c0 = 0; c0 < count; c0 += kMaxBatch) {
+        const int cnt = std::min(kMaxBatch, count - c0);
+        ColBatch b{};
+        for (int q = 0; q < cnt; ++q) {
+          const int i = c0 + q;
+          const ChirpTables* ct = op.ct[i];
+          b.inst[q] = ColInst{reinterpret_cast<const cpx<float>*>(in[i].U),
+                              reinterpret_cast<cpx<float>*>(last ? in[i].u : in[i].U),
+                              reinterpret_cast<cpx<float>*>(in[i].U), ct->Bx, ct->Ay, ct->Rmax};
+        }
+        ktime_begin(KT_COL, s);
+        const bool ok = fused_col_b<TW>(ny, cv0, last && want_spec, b, cnt, nx, ty, spec_scale, s);
+        ktime_end(KT_COL, s);
+        if (!ok) return fail(RSIM_E_HIP, "no fused column kernel for ny=%d", ny);
+        RSIM_LAUNCHED();
+      }
+      continue;
     }
-    RSIM_LAUNCHED();
+    // row pass: the instances of each transmission mode together
+    int modes[8];
+    int nmodes = 0;
+    auto tm_of = [&](int i) { return op.j >= 0 ? prep[i][op.j].tm : op.mode == ROW_LAST ? TM_LUT : TM_NONE; };
+    for (int i = 0; i < count; ++i) {
+      const int tm = tm_of(i);
+      bool seen = false;
+      for (int q = 0; q < nmodes; ++q) seen = seen || modes[q] == tm;
+      if (!seen) modes[nmodes++] = tm;
+    }
+    for (int q = 0; q < nmodes; ++q) {
+      const int tm = modes[q];
+      std::vector<int> idx;
+      for (int i = 0; i < count; ++i)
+        if (tm_of(i) == tm) idx.push_back(i);
+      const int rv = op.j >= 0 ? rvs[idx[0]][op.j] : rv0;
+      for (size_t c0 = 0; c0 < idx.size(); c0 += kMaxBatch) {
+        const int cnt = (int)std::min<size_t>(kMaxBatch, idx.size() - c0);
+        RowBatch b{};
+        for (int r = 0; r < cnt; ++r) {
+          const int i = idx[c0 + r];
+          const void* src = op.mode == ROW_MID ? in[i].U : in[i].u;
+          void* dst = op.mode == ROW_LAST ? in[i].u : in[i].U;
+          b.inst[r].in = reinterpret_cast<const cpx<float>*>(src);
+          b.inst[r].out = reinterpret_cast<cpx<float>*>(dst);
+          if (op.j >= 0) {
+            const FusedPrep<TW>& P = prep[i][op.j];
+            b.inst[r].ta = P.ta;
+            P.at(b.inst[r].ta, in[i].steps[op.j].first_slice + op.k);
+          }
+        }
+        if (op.mode == ROW_MID) ktime_begin(KT_ROW, s);
+        const bool ok = fused_row_b<TW>(nx, rv, op.mode, tm, b, cnt, ny, tx, s);
+        ktime_end(KT_ROW, s);
+        if (!ok) return fail(RSIM_E_HIP, "no fused row kernel for nx=%d", nx);
+        RSIM_LAUNCHED();
+      }
+    }
   }
-  if (!pending && want_spec) RSIM_CHECK(fft2(RSIM_C64, RSIM_FORWARD, u, U, nx, ny, s));
+  if (!pending && want_spec)
+    for (int i = 0; i < count; ++i) RSIM_CHECK(fft2(RSIM_C64, RSIM_FORWARD, in[i].u, in[i].U, nx, ny, s));
   return RSIM_OK;
 }
 
-int chain_apply(void* u, void* U, int nx, int ny, int dtype, double dx, double dy, double wl,
-                double freq, bool band_limit, const rsim_chain_step* steps, int n_steps,
-                int engine, int flags, hipStream_t s) {
+int chain_apply(const rsim_chain_instance* in, int count, int nx, int ny, int dtype,
+                bool band_limit, int n_steps, int engine, int flags, hipStream_t s) {
   if (cancel_requested()) return fail(RSIM_E_CANCELLED, "propagation chain cancelled before it started");
+  if (count == 0) return RSIM_OK;
   const bool want_spec = !(flags & RSIM_FLAG_NO_SPECTRUM);
   int depth = 0;
   for (int j = 0; j < n_steps; ++j)
-    if (steps[j].vol) depth += steps[j].n_slices;
+    if (in[0].steps[j].vol) depth += in[0].steps[j].n_slices;
   if (engine == RSIM_FUSED && fused_supported(nx, ny, dtype) && ny != kBigCol) {
     flags = default_math(nx, ny, depth, flags);
     if (flags & RSIM_FLAG_MATH_COMP32)
-      return run_chain<twc>(u, U, nx, ny, dx, dy, wl, freq, band_limit, steps, n_steps, want_spec, s);
+      return run_chain<twc>(in, count, nx, ny, band_limit, n_steps, want_spec, s);
     if (flags & RSIM_FLAG_MATH_FP64)
-      return run_chain<double>(u, U, nx, ny, dx, dy, wl, freq, band_limit, steps, n_steps,
-                               want_spec, s);
-    return run_chain<float>(u, U, nx, ny, dx, dy, wl, freq, band_limit, steps, n_steps, want_spec, s);
+      return run_chain<double>(in, count, nx, ny, band_limit, n_steps, want_spec, s);
+    return run_chain<float>(in, count, nx, ny, band_limit, n_steps, want_spec, s);
   }
   // step by step: the source's band limit leaves U undefined
   // (big-wave/source.py:38-40); every later step leaves U = FFT2(u)
-  bool spectrum_valid = false;
-  if (band_limit) {
-    RSIM_CHECK(fft2(dtype, RSIM_FORWARD, u, U, nx, ny, s));
-    RSIM_CHECK(chirp_cutoff(U, nx, ny, dtype, dx, dy, 1.0, 0.0, freq, false, s));
-    RSIM_CHECK(fft2(dtype, RSIM_INVERSE, U, u, nx, ny, s));
-  }
-  for (int j = 0; j < n_steps; ++j) {
-    const rsim_chain_step& st = steps[j];
-    if (st.vol) {
-      RSIM_CHECK(sample_apply(u, U, nx, ny, dtype, dx, dy, wl, freq, st.vol, st.pixel_size_x,
-                              st.pixel_size_y, st.pixel_size_z, st.x_pos, st.y_pos,
-                              st.first_slice, st.n_slices, nullptr, nullptr, engine,
-                              flags & ~RSIM_FLAG_NO_SPECTRUM, s, nullptr));
-    } else {
-      RSIM_CHECK(propagate(u, U, nx, ny, dtype, dx, dy, wl, st.dz, freq, spectrum_valid, s));
+  for (int i = 0; i < count; ++i) {
+    const rsim_chain_instance& q = in[i];
+    bool spectrum_valid = false;
+    if (band_limit) {
+      RSIM_CHECK(fft2(dtype, RSIM_FORWARD, q.u, q.U, nx, ny, s));
+      RSIM_CHECK(chirp_cutoff(q.U, nx, ny, dtype, q.dx, q.dy, 1.0, 0.0, q.cutoff_freq, false, s));
+      RSIM_CHECK(fft2(dtype, RSIM_INVERSE, q.U, q.u, nx, ny, s));
     }
-    if (!st.vol || st.n_slices > 0) spectrum_valid = true;   // a step that ran leaves FFT2(u)
+    for (int j = 0; j < n_steps; ++j) {
+      const rsim_chain_step& st = q.steps[j];
+      if (st.vol) {
+        RSIM_CHECK(sample_apply(q.u, q.U, nx, ny, dtype, q.dx, q.dy, q.wl, q.cutoff_freq, st.vol,
+                                st.pixel_size_x, st.pixel_size_y, st.pixel_size_z, st.x_pos,
+                                st.y_pos, st.first_slice, st.n_slices, nullptr, nullptr, engine,
+                                flags & ~RSIM_FLAG_NO_SPECTRUM, s, nullptr));
+      } else {
+        RSIM_CHECK(propagate(q.u, q.U, nx, ny, dtype, q.dx, q.dy, q.wl, st.dz, q.cutoff_freq,
+                             spectrum_valid, s));
+      }
+      if (!st.vol || st.n_slices > 0) spectrum_valid = true;   // a step that ran leaves FFT2(u)
+    }
+    if (want_spec && !spectrum_valid) RSIM_CHECK(fft2(dtype, RSIM_FORWARD, q.u, q.U, nx, ny, s));
   }
-  if (want_spec && !spectrum_valid) RSIM_CHECK(fft2(dtype, RSIM_FORWARD, u, U, nx, ny, s));
   return RSIM_OK;
 }
 

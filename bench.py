@@ -931,6 +931,20 @@ def batch_main(args, rs):
     t_graph = min(ts)
     img_graph = run.image()
     same = bool(np.array_equal(img_graph, img_eager))
+    # the same run with every grid's instances sharing launches (the default
+    # shares them only below 2048^2)
+    run.batched = True
+    run.run()
+    run.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        run.run()
+        run.synchronize()
+        ts.append(time.perf_counter() - t0)
+    t_shared = min(ts)
+    run.batched = None
+    log(f"all grids' instances sharing launches: {t_shared * 1e3:.1f} ms")
     slices_per_inst = 4 + 1 + 10 + 1     # grating, gap, analyzer, leg (propagations)
     points = sum(q.nx * q.ny for q in insts)
     log(f"one graph launch ({graph.nodes} nodes): {t_graph * 1e3:.1f} ms; call by call "
@@ -993,6 +1007,46 @@ def batch_main(args, rs):
     log(f"64 x 512^2 x 4 slices: batched {t_sb * 1e3:.2f} ms, sequential {t_ss * 1e3:.2f} ms "
         f"({t_ss / t_sb:.1f}x)")
     bs.close()
+    # the whole Talbot-Lau run on small grids: 64 instances (8 sources x 8
+    # energies) on 512^2 over 0.256 mm, batched chains vs one chain each
+    a_s = grating_volume(rs, n_s, dx_s, 10, 2.1e-6, 0.5, np.array([0, 5e-6 + 4e-7j]))
+    sm_inst = []
+    for e, w in zip(energies, weights):
+        wl = rs.convert_energy_wavelength(e * 1e3)
+        for xs, ys in src:
+            sm_inst.append(SP.Instance(
+                nx=n_s, ny=n_s, dx=dx_s, wl=wl, cutoff_freq=0.4 / dx_s, source_x=float(xs) / 4,
+                source_y=float(ys) / 4, source_distance=z_src, weight=float(w) / 8.0,
+                stages=[SP.SampleStage(g_s, t_g / 4, 4), SP.GapStage(gap),
+                        SP.SampleStage(a_s, 5e-6, 10)]))
+    det_s = int((fov / 4) // ds)
+    srun = SP.SpectralRun(sm_inst, SP.Detector(det_s, det_s, ds, ds, z_det, leg))
+    small_ms = {}
+    for batched in (True, False):
+        srun.batched = batched
+        srun.run()
+        srun.synchronize()
+        ts = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            srun.run()
+            srun.synchronize()
+            ts.append(time.perf_counter() - t0)
+        small_ms["batched_chains" if batched else "per_instance_chains"] = round(min(ts) * 1e3, 3)
+    srun.batched = None
+    g_small = srun.capture()
+    srun.launch()
+    srun.synchronize()
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        srun.launch()
+        srun.synchronize()
+        ts.append(time.perf_counter() - t0)
+    small_ms["one_graph_launch"] = round(min(ts) * 1e3, 3)
+    small_ms["graph_nodes"] = g_small.nodes
+    log(f"Talbot-Lau 64 x 512^2 (4 + 10 slices, gap, leg): {small_ms}")
+    srun.close()
     print(json.dumps({
         "metric": "batched Talbot-Lau instance-slice-steps/s (64 instances, 2048^2-6144^2, c64, one graph launch)",
         "value": round(64 * slices_per_inst / t_graph, 1), "unit": "instance-slice-steps/s",
@@ -1001,8 +1055,10 @@ def batch_main(args, rs):
         "graph_over_call_by_call": round(t_eager / t_graph, 3), "capture_s": round(t_capture, 2),
         "graph_nodes": graph.nodes, "gpoint_slices_per_s": round(points * slices_per_inst / t_graph / 1e9, 2),
         "image_identical_graph_vs_call_by_call": same, "detector_shape": [det_n, det_n],
+        "shared_launch_chains_ms_per_run": round(t_shared * 1e3, 2),
         "small_512sq_64x4_slices_ms": {"batched": round(t_sb * 1e3, 3), "sequential": round(t_ss * 1e3, 3),
                                        "speedup": round(t_ss / t_sb, 2)},
+        "small_512sq_talbot_lau_64_instances_ms": small_ms,
         "rel_l2_field_fused_vs_plain_c128": errs,
         "config": {"workload": "8 sources (seed 3) x 8 energies 15-40 keV, grids 2048..6144 over 1.024 mm; "
                                "point source 20 m upstream, phase grating 4.2 um (4 slices) + 0.3 m + "

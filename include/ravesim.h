@@ -251,6 +251,27 @@ int rsim_chain_apply_2d(void* u, void* U, int nx, int ny, int dtype, double dx, 
                         const rsim_chain_step* steps, int n_steps, int engine, int flags,
                         void* stream);
 
+/* One instance of a batched chain: its fields, pixel pitch, wavelength,
+ * cutoff and its own n_steps steps. */
+typedef struct rsim_chain_instance {
+  void* u;
+  void* U;
+  double dx, dy, wl, cutoff_freq;
+  const rsim_chain_step* steps;
+} rsim_chain_instance;
+
+/* rsim_chain_apply_2d for count independent instances of one nx x ny grid
+ * (the multi-source runner's instances, big-wave/multisim.py:258-519, each the
+ * element loop above): every instance has n_steps steps of the same kinds
+ * with the same slice counts, and its own volumes, placements, distances,
+ * wavelength and cutoff.  On the fused engine each pass of the chain is one
+ * launch for up to 16 instances (per transmission mode), and the results
+ * equal count calls of rsim_chain_apply_2d bit for bit; otherwise the
+ * instances run one after another. */
+int rsim_chain_apply_2d_batch(const rsim_chain_instance* instances, int count, int nx, int ny,
+                              int dtype, int band_limit, int n_steps, int engine, int flags,
+                              void* stream);
+
 /* Fresnel-scaled / cone-beam BPM forms (big-wave/propagation.py:114-196,
  * big-wave/optical_element.py:533-592, big-wave/source.py:62-90).
  * rsim_sample_apply_2d_scaled: slice k samples the voxel grid at field

@@ -163,3 +163,96 @@ def test_chain_argument_checks():
     with pytest.raises(ValueError):
         lib.call("rsim_chain_apply_2d", u.ptr, U.ptr, n, n, u.code, DX, DX, WL, 0.4 / DX, 1, None,
                  2, lib.RSIM_FUSED, 0, None)
+
+
+def _instances(n, nz, dt, band_rng=5):
+    """Five instances of one grid with their own wavelengths, cutoffs,
+    distances and volumes; one volume is sampled at 1.06 dx (bilinear
+    transmission), the others on the field grid (material lookup)."""
+    rng = np.random.default_rng(band_rng)
+    out = []
+    for i in range(5):
+        grid = volume(n, nz, 20 + i)
+        vol = rs.DeviceVolume.from_numpy(grid, DB)
+        wl = WL * (0.8 + 0.1 * i)
+        px = DX * (1.06 if i == 3 else 1.0)
+        steps = (lib.ChainStep * 4)(
+            lib.ChainStep(vol.handle, px, px, PZ, 0.0, 0.0, 0, 2, 0.0),
+            lib.ChainStep(None, 0, 0, 0, 0, 0, 0, 0, 0.01 * (i + 1)),
+            lib.ChainStep(vol.handle, px, px, PZ, 0.0, 0.0, 2, 3, 0.0),
+            lib.ChainStep(None, 0, 0, 0, 0, 0, 0, 0, 0.02))
+        u0 = np.exp(1j * rng.uniform(0, 0.3, size=(n, n))).astype(dt)
+        out.append(dict(vol=vol, steps=steps, wl=wl, cut=(0.35 + 0.02 * i) / DX, u0=u0))
+    return out
+
+
+@pytest.mark.parametrize("dt,engine", [(np.complex64, lib.RSIM_FUSED), (np.complex128, lib.RSIM_PLAIN)])
+def test_batched_chain_equals_single_chains(dt, engine):
+    """rsim_chain_apply_2d_batch over five instances (different wavelengths,
+    cutoffs, distances, volumes and transmission modes) equals five calls of
+    rsim_chain_apply_2d, bit for bit, field and spectrum."""
+    n = 256
+    inst = _instances(n, 5, dt)
+    say("batched chain", np.dtype(dt).name)
+    single = []
+    for q in inst:
+        u = rs.DeviceVector.from_numpy(q["u0"].reshape(-1))
+        U = rs.DeviceVector(n * n, dt)
+        lib.call("rsim_chain_apply_2d", u.ptr, U.ptr, n, n, u.code, DX, DX, q["wl"], q["cut"], 1,
+                 q["steps"], 4, engine, 0, None)
+        single.append((u.to_numpy(), U.to_numpy()))
+    fields = [(rs.DeviceVector.from_numpy(q["u0"].reshape(-1)), rs.DeviceVector(n * n, dt))
+              for q in inst]
+    rows = [lib.ChainInstance(u.ptr, U.ptr, DX, DX, q["wl"], q["cut"], q["steps"])
+            for (u, U), q in zip(fields, inst)]
+    arr = (lib.ChainInstance * len(rows))(*rows)
+    lib.call("rsim_chain_apply_2d_batch", arr, len(rows), n, n, fields[0][0].code, 1, 4, engine, 0,
+             None)
+    for k, ((u, U), (su, sU)) in enumerate(zip(fields, single)):
+        assert np.array_equal(u.to_numpy(), su), k
+        assert np.array_equal(U.to_numpy(), sU), k
+
+
+def test_batched_chain_instance_against_the_oracle():
+    """One instance of a fused batch (bilinear map, 1.06 dx) against the FP64
+    oracle run step by step."""
+    n = 256
+    inst = _instances(n, 5, np.complex64)
+    fields = [(rs.DeviceVector.from_numpy(q["u0"].reshape(-1)), rs.DeviceVector(n * n, np.complex64))
+              for q in inst]
+    rows = [lib.ChainInstance(u.ptr, U.ptr, DX, DX, q["wl"], q["cut"], q["steps"])
+            for (u, U), q in zip(fields, inst)]
+    arr = (lib.ChainInstance * len(rows))(*rows)
+    lib.call("rsim_chain_apply_2d_batch", arr, len(rows), n, n, lib.RSIM_C64, 1, 4, lib.RSIM_FUSED,
+             lib.RSIM_FLAG_NO_SPECTRUM, None)
+    for k in (0, 3):
+        q = inst[k]
+        grid = volume(n, 5, 20 + k)
+        px = DX * (1.06 if k == 3 else 1.0)
+        u = q["u0"].astype(np.complex128)
+        u = O.ifft2(O.apply_frequency_cutoff_2d(O.fft2(u), q["cut"], DX, DX))
+        spec = O.SampleSpec(grid=grid, db_list=DB, pixel_size_x=px, pixel_size_y=px, pixel_size_z=PZ)
+        u, _ = O.sample_apply_2d(u, spec, DX, DX, q["wl"], q["cut"], first_slice=0, n_slices=2)
+        u, _ = O.propagate_2d(u, DX, DX, q["wl"], 0.01 * (k + 1), q["cut"])
+        u, _ = O.sample_apply_2d(u, spec, DX, DX, q["wl"], q["cut"], first_slice=2, n_slices=3)
+        u, _ = O.propagate_2d(u, DX, DX, q["wl"], 0.02, q["cut"])
+        err = O.relative_l2(fields[k][0].to_numpy().reshape(n, n), u)
+        say(f"  batched instance {k} vs oracle {err:.2e}")
+        assert err < 3e-6
+
+
+def test_batched_chain_argument_checks():
+    n = 256
+    inst = _instances(n, 5, np.complex64)
+    fields = [(rs.DeviceVector(n * n, np.complex64), rs.DeviceVector(n * n, np.complex64)) for _ in range(2)]
+    other = (lib.ChainStep * 4)(*[inst[1]["steps"][j] for j in (1, 0, 2, 3)])   # gap first
+    rows = [lib.ChainInstance(fields[0][0].ptr, fields[0][1].ptr, DX, DX, WL, 0.4 / DX, inst[0]["steps"]),
+            lib.ChainInstance(fields[1][0].ptr, fields[1][1].ptr, DX, DX, WL, 0.4 / DX, other)]
+    with pytest.raises(ValueError, match="same step kinds"):
+        lib.call("rsim_chain_apply_2d_batch", (lib.ChainInstance * 2)(*rows), 2, n, n, lib.RSIM_C64, 1,
+                 4, lib.RSIM_FUSED, 0, None)
+    rows[1] = lib.ChainInstance(fields[0][0].ptr, fields[1][1].ptr, DX, DX, WL, 0.4 / DX,
+                                inst[1]["steps"])
+    with pytest.raises(ValueError, match="share a field"):
+        lib.call("rsim_chain_apply_2d_batch", (lib.ChainInstance * 2)(*rows), 2, n, n, lib.RSIM_C64, 1,
+                 4, lib.RSIM_FUSED, 0, None)
