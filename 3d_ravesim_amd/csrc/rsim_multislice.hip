@@ -584,10 +584,12 @@ int release_volume_tables(unsigned long long serial) {
   return RSIM_OK;
 }
 
+static void release_run_tables();
 int release_multislice_tables() {
   std::lock_guard<std::recursive_mutex> lk(table_mu());
   for (auto& kv : g_chirp) hipFree(kv.second.Bx);   // one allocation per table set
   g_chirp.clear();
+  release_run_tables();
   for (auto& kv : g_mat) hipFree(kv.second.dev);
   g_mat.clear();
   return RSIM_OK;
@@ -748,8 +750,11 @@ static int fused_prepare(int nx, int ny, double dx, double dy, double wl, double
 // Cone-beam runs: each slice has its own propagation distance and voxel-map
 // scale, so its own chirp tables and axis maps.  They are built for the whole
 // run on the host (in parallel over slices, straight into a pinned staging
-// buffer), uploaded with one copy into one stream-ordered allocation, and
-// freed on the stream when the run ends: nothing per slice is cached.
+// buffer) and uploaded with one copy into one allocation.  A run's set is one
+// entry of the bounded table cache, keyed by everything that shapes it, so a
+// repeated geometry reuses it and the least recently used sets go first; a
+// set above half the cache bound is the run's own, freed on its stream when
+// the run ends.  Nothing is cached per slice.
 struct SliceTables {
   ChirpTables ct;
   AxisMap xm, ym;
@@ -758,12 +763,31 @@ struct RunSliceTables {
   std::vector<SliceTables> sets;
   unsigned char* dev = nullptr;
   hipStream_t s = nullptr;
+  const void* pinned = nullptr;   // a cached set, held for this run
   ~RunSliceTables() {
     if (dev) (void)hipFreeAsync(dev, s);
+    if (pinned) table_pin(pinned, -1);
   }
 };
 
 namespace {
+// Cached cone-beam table sets: (device, parameter bits..., scales, distances)
+// -> the slices' table descriptions pointing into the set's allocation.
+struct RunTabEntry {
+  std::vector<SliceTables> sets;
+  unsigned char* dev = nullptr;
+};
+std::map<std::vector<uint64_t>, RunTabEntry> g_runtab;
+}  // namespace
+
+// Under table_mu (rsim_release_caches): free every cached cone-beam set.
+static void release_run_tables() {
+  for (auto& kv : g_runtab) (void)hipFree(kv.second.dev);
+  g_runtab.clear();
+}
+
+namespace {
+
 // Pinned host staging buffer per device, reused once its last upload is done.
 struct Staging {
   unsigned char* host = nullptr;
@@ -781,12 +805,34 @@ static int build_run_slice_tables(int nx, int ny, double dx, double dy, double w
   const size_t cb = (chirp_bytes(nx, ny) + 255) & ~size_t(255);
   const size_t xb = (axis_map_bytes(nx) + 255) & ~size_t(255);
   const size_t yb = (axis_map_bytes(ny) + 255) & ~size_t(255);
-  RSIM_CHECK(capture_miss("cone-beam per-slice tables"));
   const size_t per = cb + xb + yb;
   const size_t total = per * (size_t)count;
+  std::lock_guard<std::recursive_mutex> tl(table_mu());
+  std::vector<uint64_t> key{(uint64_t)current_device(), (uint64_t)nx, (uint64_t)ny, bits(dx), bits(dy),
+                            bits(wl), bits(freq), (uint64_t)src.nx_g, (uint64_t)src.ny_g, bits(px),
+                            bits(py), bits(pz), bits(xpos), bits(ypos), (uint64_t)count};
+  for (int k = 0; k < count; ++k) {
+    key.push_back(bits(scales ? scales[k] : 1.0));
+    key.push_back(bits(dzs ? dzs[k] : pz));
+  }
+  auto hit = g_runtab.find(key);
+  if (hit != g_runtab.end()) {
+    table_touch(hit->second.dev);
+    rt->sets = hit->second.sets;
+    if (table_pin(hit->second.dev, +1)) rt->pinned = hit->second.dev;
+    return RSIM_OK;
+  }
+  RSIM_CHECK(capture_miss("cone-beam per-slice tables"));
+  const bool cache = total <= kTableCacheBytes / 2;
   rt->sets.assign(count, SliceTables{});
-  RSIM_HIP(hipMallocAsync((void**)&rt->dev, total, s));
-  rt->s = s;
+  unsigned char* dev = nullptr;
+  if (cache) {
+    RSIM_HIP(hipMalloc((void**)&dev, total));
+  } else {
+    RSIM_HIP(hipMallocAsync((void**)&dev, total, s));
+    rt->dev = dev;   // the run's own: freed on its stream when the run ends
+    rt->s = s;
+  }
   std::lock_guard<std::mutex> lk(g_staging_mu);
   Staging& st = g_staging[current_device()];
   if (st.done) RSIM_HIP(hipEventSynchronize(st.done));
@@ -808,7 +854,7 @@ static int build_run_slice_tables(int nx, int ny, double dx, double dy, double w
       chirp_build(nx, ny, dx, dy, wl, dz, freq, global_phase(wl, dz), h);
       axis_map_build(nx, dx, sc, xpos, src.nx_g, px, &e.xm, h + cb);
       axis_map_build(ny, dy, sc, ypos, src.ny_g, py, &e.ym, h + cb + xb);
-      unsigned char* d = rt->dev + per * (size_t)k;
+      unsigned char* d = dev + per * (size_t)k;
       chirp_bind(&e.ct, d, nx, ny);
       axis_map_bind(&e.xm, d + cb, nx);
       axis_map_bind(&e.ym, d + cb + xb, ny);
@@ -818,8 +864,13 @@ static int build_run_slice_tables(int nx, int ny, double dx, double dy, double w
   for (int t = 1; t < nthreads; ++t) pool.emplace_back(work, t);
   work(0);
   for (auto& th : pool) th.join();
-  RSIM_HIP(hipMemcpyAsync(rt->dev, st.host, total, hipMemcpyHostToDevice, s));
+  RSIM_HIP(hipMemcpyAsync(dev, st.host, total, hipMemcpyHostToDevice, s));
   RSIM_HIP(hipEventRecord(st.done, s));
+  if (cache) {
+    RSIM_CHECK(table_insert(dev, total, [key] { g_runtab.erase(key); }));
+    g_runtab[key] = RunTabEntry{rt->sets, dev};
+    if (table_pin(dev, +1)) rt->pinned = dev;
+  }
   return RSIM_OK;
 }
 

@@ -7,7 +7,8 @@ valid after the slices it finished, which is checked against the FP64 oracle
 after exactly that many slices (complex128 plain 1e-12, complex64 fused
 3e-6).  Controls are per thread: a run that cancels itself does not stop a
 run on another thread.  A batch captured before rsim_release_caches is
-refused afterwards, and cone-beam runs leave no per-slice tables behind.
+refused afterwards, and cone-beam runs cache one table set per run geometry
+(a repeated geometry reuses it), within the cache's byte bound.
 """
 
 import ctypes as C
@@ -180,11 +181,12 @@ def test_release_caches_invalidates_batches():
     lib.call("rsim_batch_destroy", b.value)
 
 
-def test_cone_beam_runs_leave_no_per_slice_tables():
+def test_cone_beam_run_tables_are_cached_and_bounded():
     """Each cone-beam slice has its own distance and scale; the fused engine
-    keeps those tables for the run only, so repeated runs at new
-    wavelengths add a few cached tables per wavelength, not one set per slice."""
-    say("cone-beam table bound")
+    caches one table set per run geometry (not one entry per slice), so a new
+    wavelength adds a few entries, a repeated geometry adds none and gives the
+    same field bit for bit, and the cache stays within its byte bound."""
+    say("cone-beam table cache")
     grid, u0 = volume_grid(), start_field()
     vol = rs.DeviceVolume.from_numpy(grid, DB3)
     u = rs.DeviceVector.from_numpy(u0.astype(np.complex64).reshape(-1))
@@ -197,19 +199,25 @@ def test_cone_beam_runs_leave_no_per_slice_tables():
         lib.call("rsim_cache_info", C.byref(b), C.byref(e))
         return b.value, e.value
 
-    lib.call("rsim_release_caches")
-    counts = []
-    for j, wl in enumerate((0.5e-10, 0.51e-10, 0.52e-10)):
+    def run(wl):
+        u.upload(u0.astype(np.complex64).reshape(-1))
         lib.call("rsim_sample_apply_2d_scaled", u.ptr, U.ptr, N, N, u.code, DX, DX, wl, CUT,
                  vol.handle, DX, DX, PZ, 0.0, 0.0, 0, NZ, scales.ctypes.data_as(lib._dp),
                  dzs.ctypes.data_as(lib._dp), lib.RSIM_FUSED, 0, None)
         lib.call("rsim_synchronize", None)
+        return u.to_numpy()
+
+    lib.call("rsim_release_caches")
+    counts, fields = [], []
+    for wl in (0.5e-10, 0.51e-10, 0.52e-10, 0.5e-10):
+        fields.append(run(wl))
         counts.append(info())
-    say("  cache after each wavelength", counts)
-    growth = counts[2][1] - counts[1][1]
-    assert growth <= 3, counts
-    assert counts[2][0] < 4 * 1024 ** 2
-    assert np.isfinite(u.to_numpy()).all()
+    say("  cache after each run", counts)
+    assert counts[2][1] - counts[1][1] <= 3, counts
+    assert counts[3] == counts[2], counts          # the repeated geometry reuses its set
+    assert np.array_equal(fields[3], fields[0])
+    assert counts[3][0] < 4 * 1024 ** 2
+    assert np.isfinite(fields[0]).all()
 
 
 def test_zero_slice_distance_is_zero_in_both_engines():
