@@ -47,6 +47,13 @@ __device__ __forceinline__ void table_to_lds(T* lds_tab, const T* tab) {
   for (int i = threadIdx.x; i < 256; i += blockDim.x) lds_tab[i] = tab[i];
 }
 
+// v times a material factor held as FP32 hi + lo of the FP64 value.
+__device__ __forceinline__ cpx<float> lut_mul(cpx<float> v, float4 t) {
+  const float rr = v.x * t.x - v.y * t.y + (v.x * t.z - v.y * t.w);
+  const float ii = v.x * t.y + v.y * t.x + (v.x * t.w + v.y * t.z);
+  return mk(rr, ii);
+}
+
 // Transmission of one element.  TM_LUT: the voxel map is integral, hence
 // affine (x0 = ax*x + cx); the factor is a per-material lookup (LDS) of the
 // FP64 value split hi+lo.  Otherwise bilinear delta-beta interpolation from
@@ -64,10 +71,7 @@ __device__ __forceinline__ cpx<float> transmit(cpx<float> v, const TransArgsF& a
     return mk(v.x * t.x - v.y * t.y, v.x * t.y + v.y * t.x);
   } else if constexpr (TM == TM_LUT) {
     if (!(iny && x >= a.xlo && x < a.xhi)) return v;
-    const float4 t = lds_lut[row0[a.ax * x + a.cx]];
-    const float rr = v.x * t.x - v.y * t.y + (v.x * t.z - v.y * t.w);
-    const float ii = v.x * t.y + v.y * t.x + (v.x * t.w + v.y * t.z);
-    return mk(rr, ii);
+    return lut_mul(v, lds_lut[row0[a.ax * x + a.cx]]);
   } else {
     if (!(iny && a.xm.inside[x])) return v;
     const int x0 = a.xm.i0[x];
@@ -240,11 +244,38 @@ __global__ void __launch_bounds__(F::TC * C)
     vi = (((((uint32_t)t >> ts) * (uint32_t)ny + (uint32_t)yy) << ts) + ((uint32_t)t & (tile - 1))) * 8u;
     es = (uint32_t)ny * 8u;
   }
-  if constexpr (MODE != ROW_LAST && LUT) table_to_lds(lds_lut, ta.lut);
-  if constexpr (MODE != ROW_LAST && TM == TM_BILINEAR) table_to_lds(lds_dbf, ta.dbf);
+  // The field's loads go out first: the material tables' copy into LDS
+  // waits on its own loads, and would otherwise hold them back.
   cpx<float> v[F::E];
 #pragma unroll
   for (int e = 0; e < F::E; ++e) v[e] = gi.ld(vi, first_off<F>(e, es));
+  // MID with material lookups: the row's voxel ids are loaded with the field
+  // (four per register) rather than between the two transforms, where their
+  // latency would stall the row
+  constexpr bool PRE = (MODE == ROW_MID || MODE == ROW_MID_OUT) && LUT;
+  uint32_t idw[PRE ? (F::E + 3) / 4 : 1];
+  if constexpr (PRE) {
+    // byte loads through a buffer resource over the row's nxg ids: a column
+    // outside the volume (its factor is not applied) may give an offset
+    // outside the row, which reads 0 instead of faulting
+    const RowT r0 = row_state<TM>(ta, yy);
+    const __amdgpu_buffer_rsrc_t rid =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(r0.row0), (short)0, ta.nxg, 0x00020000);
+    const uint32_t ib = (uint32_t)(ta.ax * t + ta.cx);
+    uint32_t b[F::E];
+#pragma unroll
+    for (int e = 0; e < F::E; ++e)
+      b[e] = __builtin_amdgcn_raw_buffer_load_b8(rid, ib + (uint32_t)(ta.ax * (last_pos<F>(t, e) - t)), 0, 0);
+#pragma unroll
+    for (int k = 0; k < (F::E + 3) / 4; ++k) {
+      uint32_t w = 0;
+#pragma unroll
+      for (int q = 0; q < 4 && 4 * k + q < F::E; ++q) w |= b[4 * k + q] << (8 * q);
+      idw[k] = w;
+    }
+  }
+  if constexpr (MODE != ROW_LAST && LUT) table_to_lds(lds_lut, ta.lut);
+  if constexpr (MODE != ROW_LAST && TM == TM_BILINEAR) table_to_lds(lds_dbf, ta.dbf);
   if constexpr (MODE == ROW_FIRST) {
     const RowT r = row_state<TM>(ta, yy);
     if constexpr (LUT || TM == TM_BILINEAR) __syncthreads();
@@ -321,7 +352,13 @@ __global__ void __launch_bounds__(F::TC * C)
     const RowT r = row_state<TM>(ta, yy);
 #pragma unroll
     for (int e = 0; e < F::E; ++e) {
-      v[e] = transmit<TM>(v[e], ta, last_pos<F>(t, e), r.y0, r.fy, r.iny, r.row0, lds_lut, lds_dbf);
+      if constexpr (PRE) {
+        const int x = last_pos<F>(t, e);
+        if (r.iny && x >= ta.xlo && x < ta.xhi)
+          v[e] = lut_mul(v[e], lds_lut[(idw[e / 4] >> (8 * (e % 4))) & 255u]);
+      } else {
+        v[e] = transmit<TM>(v[e], ta, last_pos<F>(t, e), r.y0, r.fy, r.iny, r.row0, lds_lut, lds_dbf);
+      }
       if ((e & 3) == 3) compiler_fence();
     }
     reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, opaque_index<(LD != 0)>(t), opaque_index<(LD != 0)>(c));
@@ -437,16 +474,35 @@ struct FacE {
   using F = RegFac<N, E, B::R0, B::R1, B::R2, B::R3>;
 };
 
+// Exchange padding of a launch shape: C pad slots after every 2^shift slots
+// (C sequences interleaved).  Lanes of the first exchange's writes sit
+// R0 * C slots apart (R0 = the first radix), so 2^shift = 16 * C spreads them
+// over the 32 banks of 4 bytes for radix-16 schedules.  The radix-32 16384
+// schedule needs 32 * C (its split exchanges took 2 LDS cycles per access,
+// row MID 2471 -> 2427 us).  The radix-8 first stage of the 2^a * 3 lengths
+// is conflict-free with 8 * C for two sequences in 8-byte slots (3072 row
+// 67.8 -> 63.7 us) and for four or more in 4-byte slots; the latter only
+// where one workgroup per CU is all the LDS holds either way (6144 columns
+// 305 -> 298 us): 3072's four-column buffer grows from 52 to 55 KiB, which
+// leaves room for two workgroups instead of three (62 -> 75 us).  Chosen with
+// a model of every exchange instruction's bank accesses; each choice keeps
+// the exchange addresses linear in the element index (the per-element part
+// folds into the instruction offset) wherever 16 * C did.
+constexpr int exchange_shift(int n, int r0, int C, bool split) {
+  const int lc = (C >= 2) + (C >= 4) + (C >= 8) + (C >= 16);
+  if (r0 >= 32) return 5 + lc;
+  if (r0 == 8 && C == 2 && !split) return 3 + lc;
+  if (r0 == 8 && C >= 4 && split && (long long)n * C * 4 > 81920) return 3 + lc;
+  return 4 + lc;
+}
+
 template <int N, int E, int C, bool SPLIT_ = ((size_t)N * C * 8 > 65536), int LD_ = 0>
 struct Shape {
   static constexpr int LD = LD_;
   using F = typename FacE<N, E>::F;
   static constexpr int CC = C;
   static constexpr bool SPLIT = SPLIT_;
-  // C pad slots per 16*C slots: spreads the strided stage writes over the
-  // banks and keeps every exchange address linear in the element index (the
-  // per-element part folds into the instruction offset)
-  static constexpr int PS = pad_code(4 + (C >= 2) + (C >= 4) + (C >= 8) + (C >= 16), C);
+  static constexpr int PS = pad_code(exchange_shift(N, F::R0, C, SPLIT_), C);
   static constexpr int threads = F::TC * C;
 };
 
