@@ -213,15 +213,35 @@ constexpr int row_waves_per_eu() {
                                          : 1;
 }
 
+// Where a MID row keeps the voxel ids it loads with the field until the
+// transmission between its transforms: 1 = packed four per register, or
+// 2 = packed four per word in LDS for the three-workgroup register budget
+// (LD = 2) where three workgroups still fit beside them: no registers held
+// through the first transform (the 4096 row's spills go away, MID 93.2 ->
+// 90.8 us).  Shapes with registers to spare run faster with 1 (2048 row
+// 24.7 us against 26.7 with 2).
+template <class F, int C, int PS, bool SPLIT, int LD>
+constexpr int row_id_mode() {
+  const long long lds = (long long)LdsSize<F, C, PS>::slots * (SPLIT ? 4 : 8) + 256 * 16 +
+                        (long long)((F::E + 3) / 4) * 4 * F::TC * C;
+  return LD == 2 && 3 * lds <= 163840 ? 2 : 1;
+}
+
 template <class F, int C, int PS, bool SPLIT, int LD, int MODE, int TM, typename TW>
 __global__ void __launch_bounds__(F::TC * C)
     __attribute__((amdgpu_waves_per_eu(row_waves_per_eu<F, C, LD>())))
     ms_row_kernel(RowBatch rb, int ny, uint32_t nbytes, StageTw<TW> tw, AdjArgs aj) {
   constexpr bool LUT = TM == TM_LUT;
   constexpr int N = F::N;
-  __shared__ unsigned char lds[LdsSize<F, C, PS>::slots * (SPLIT ? 4 : 8)];
+  constexpr bool PRE = (MODE == ROW_MID || MODE == ROW_MID_OUT) && LUT;
+  constexpr int IDM = PRE ? row_id_mode<F, C, PS, SPLIT, LD>() : 0;
+  constexpr int NW = (F::E + 3) / 4;
+  constexpr int XB = LdsSize<F, C, PS>::slots * (SPLIT ? 4 : 8);
+  // the exchange buffer, then (IDM = 2) the packed ids
+  __shared__ unsigned char lds[XB + (IDM == 2 ? NW * 4 * F::TC * C : 0)];
   __shared__ float4 lds_lut[(MODE != ROW_LAST && LUT) ? 256 : 1];
   __shared__ float2 lds_dbf[(MODE != ROW_LAST && TM == TM_BILINEAR) ? 256 : 1];
+  uint32_t* const lds_ids = reinterpret_cast<uint32_t*>(lds + XB);
   const RowInst& I = rb.inst[blockIdx.y];
   const cpx<float>* in = I.in;
   cpx<float>* out = I.out;
@@ -250,11 +270,10 @@ __global__ void __launch_bounds__(F::TC * C)
 #pragma unroll
   for (int e = 0; e < F::E; ++e) v[e] = gi.ld(vi, first_off<F>(e, es));
   // MID with material lookups: the row's voxel ids are loaded with the field
-  // (four per register) rather than between the two transforms, where their
-  // latency would stall the row
-  constexpr bool PRE = (MODE == ROW_MID || MODE == ROW_MID_OUT) && LUT;
-  uint32_t idw[PRE ? (F::E + 3) / 4 : 1];
-  if constexpr (PRE) {
+  // rather than between the two transforms, where their latency would stall
+  // the row (each element's load waited on its inside-the-volume test)
+  uint32_t idw[IDM ? NW : 1];
+  if constexpr (IDM != 0) {
     // byte loads through a buffer resource over the row's nxg ids: a column
     // outside the volume (its factor is not applied) may give an offset
     // outside the row, which reads 0 instead of faulting
@@ -267,11 +286,15 @@ __global__ void __launch_bounds__(F::TC * C)
     for (int e = 0; e < F::E; ++e)
       b[e] = __builtin_amdgcn_raw_buffer_load_b8(rid, ib + (uint32_t)(ta.ax * (last_pos<F>(t, e) - t)), 0, 0);
 #pragma unroll
-    for (int k = 0; k < (F::E + 3) / 4; ++k) {
+    for (int k = 0; k < NW; ++k) {
       uint32_t w = 0;
 #pragma unroll
       for (int q = 0; q < 4 && 4 * k + q < F::E; ++q) w |= b[4 * k + q] << (8 * q);
       idw[k] = w;
+    }
+    if constexpr (IDM == 2) {
+#pragma unroll
+      for (int k = 0; k < NW; ++k) lds_ids[k * F::TC * C + tid] = idw[k];
     }
   }
   if constexpr (MODE != ROW_LAST && LUT) table_to_lds(lds_lut, ta.lut);
@@ -350,9 +373,13 @@ __global__ void __launch_bounds__(F::TC * C)
       }
     }
     const RowT r = row_state<TM>(ta, yy);
+    if constexpr (IDM == 2) {
+#pragma unroll
+      for (int k = 0; k < NW; ++k) idw[k] = lds_ids[k * F::TC * C + tid];
+    }
 #pragma unroll
     for (int e = 0; e < F::E; ++e) {
-      if constexpr (PRE) {
+      if constexpr (IDM != 0) {
         const int x = last_pos<F>(t, e);
         if (r.iny && x >= ta.xlo && x < ta.xhi)
           v[e] = lut_mul(v[e], lds_lut[(idw[e / 4] >> (8 * (e % 4))) & 255u]);
@@ -435,7 +462,7 @@ __global__ void __launch_bounds__(F::TC * C) ms_col_kernel(ColBatch cb, int nx, 
     const int y = last_pos<F>(t, e);
     const int sy = y < half ? y : y - N;
     const int ay = sy < 0 ? -sy : sy;
-    const double2 a = Ay[y];
+    const double2 a = Ay[ay];   // Ay[y] = Ay[ny - y]: a table half the size in the caches
     const double mr = bx.x * a.x - bx.y * a.y;
     const double mi = bx.x * a.y + bx.y * a.x;
     const double vr = (double)v[e].x, vi = (double)v[e].y;
