@@ -450,10 +450,11 @@ __global__ void __launch_bounds__(F::TC * C) ms_col_kernel(ColBatch cb, int nx, 
   cpx<float> v[F::E];
 #pragma unroll
   for (int e = 0; e < F::E; ++e) v[e] = gi.ld(vo, first_off<F>(e, row_b));
-  reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
-  compiler_fence();
+  // the column's chirp factor and cutoff go out with the field
   const double2 bx = Bx[xx];
   const int rmax = Rmax[xx];
+  reg_fft<F, -1, C, PS, SPLIT>(v, tw, lds, t, c);
+  compiler_fence();
   constexpr int half = (N - 1) / 2 + 1;
   // groups of 4 with a compiler fence in between bound the number of FP64
   // table values in flight (register pressure)

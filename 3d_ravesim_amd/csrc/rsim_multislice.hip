@@ -1111,6 +1111,17 @@ template <typename TW>
 static int run_chain(const rsim_chain_instance* in, int count, int nx, int ny, bool band_limit,
                      int n_steps, bool want_spec, hipStream_t s) {
   const rsim_chain_step* st0 = in[0].steps;
+  // Every cached table the passes will read is pinned as soon as it is made,
+  // so that making the later ones cannot evict it before the launches.
+  struct Pins {
+    std::vector<const void*> p;
+    void add(const void* x) {
+      if (x && table_pin(x, +1)) p.push_back(x);
+    }
+    ~Pins() {
+      for (const void* x : p) table_pin(x, -1);
+    }
+  } pins;
   // per instance and sample step: tables, transmission arguments, launch shapes
   std::vector<std::vector<FusedPrep<TW>>> prep(count, std::vector<FusedPrep<TW>>(n_steps));
   std::vector<std::vector<int>> rvs(count, std::vector<int>(n_steps, 0));
@@ -1121,9 +1132,15 @@ static int run_chain(const rsim_chain_instance* in, int count, int nx, int ny, b
       const rsim_chain_step& st = in[i].steps[j];
       if (!st.vol || st.n_slices == 0) continue;
       FusedPrep<TW>& P = prep[i][j];
-      RSIM_CHECK(fused_prepare<TW>(nx, ny, in[i].dx, in[i].dy, in[i].wl, in[i].cutoff_freq,
-                                   src_of(st.vol), st.pixel_size_x, st.pixel_size_y,
-                                   st.pixel_size_z, st.x_pos, st.y_pos, &P));
+      {
+        std::lock_guard<std::recursive_mutex> tl(table_mu());
+        RSIM_CHECK(fused_prepare<TW>(nx, ny, in[i].dx, in[i].dy, in[i].wl, in[i].cutoff_freq,
+                                     src_of(st.vol), st.pixel_size_x, st.pixel_size_y,
+                                     st.pixel_size_z, st.x_pos, st.y_pos, &P));
+        pins.add(P.ct->Bx);
+        pins.add(P.ta.xm.frac);
+        pins.add(P.ta.ym.frac);
+      }
       P.at(P.ta, st.first_slice);
       int cv = 0;
       RSIM_CHECK(fused_schedule<TW>(nx, ny, P.tm, P.tx, P.ty, P.ta, P.ct, s, &rvs[i][j], &cv));
@@ -1142,9 +1159,12 @@ static int run_chain(const rsim_chain_instance* in, int count, int nx, int ny, b
   auto row = [&](int mode, int j, int k) { ops.push_back(ChainOp{false, mode, j, k, {}}); };
   auto col = [&]() -> int {
     ChainOp op{true, 0, -1, 0, std::vector<const ChirpTables*>(count, nullptr)};
-    for (int i = 0; i < count; ++i)
+    std::lock_guard<std::recursive_mutex> tl(table_mu());
+    for (int i = 0; i < count; ++i) {
       RSIM_CHECK(chirp_tables(nx, ny, in[i].dx, in[i].dy, in[i].wl, dist[i], in[i].cutoff_freq,
                               &op.ct[i], &gp[i]));
+      pins.add(op.ct[i]->Bx);
+    }
     ops.push_back(std::move(op));
     return RSIM_OK;
   };
