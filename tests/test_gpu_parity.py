@@ -271,6 +271,39 @@ def test_sample_apply_matches_oracle(engine, aligned):
         assert rel(host(U, n, n), Ue) < tol, (engine, dt)
 
 
+@pytest.mark.parametrize("case", ["smaller", "larger_half"])
+def test_fused_lookup_rows_at_volume_edges(case):
+    """Integral voxel maps whose volume does not cover the field row for row:
+    a volume smaller than the field, shifted (columns and rows outside it get
+    no factor), and a half-pitch volume wider than the field, shifted so far
+    that the field's last columns fall beyond it (affine map x0 = 2x + c).  The MID rows load
+    their voxel ids before testing which columns lie inside, so offsets off
+    either end of a volume row are part of this case; fused c64 vs the FP64
+    oracle."""
+    n, nz = 256, 4
+    rng = np.random.default_rng(17)
+    dx = 0.5e-6
+    if case == "smaller":
+        grid = _spheres_grid(rng, nz, 180, 200, 12, 3)
+        px, xpos, ypos = dx, 9 * dx, -5 * dx
+    else:
+        grid = _spheres_grid(rng, nz, 2 * n + 10, 2 * n + 40, 20, 3)
+        px, xpos, ypos = dx / 2, 70 * dx, 4 * dx
+    u0 = np.exp(1j * rng.uniform(0, 0.1, size=(n, n))) * (1 + 0.01 * rng.normal(size=(n, n)))
+    spec = O.SampleSpec(grid=grid, db_list=DB3 * 30, pixel_size_x=px, pixel_size_y=px,
+                        pixel_size_z=0.5e-6, x_pos=xpos, y_pos=ypos)
+    wl, cut = 0.5e-10, 0.4 / dx
+    ue, _ = O.sample_apply_2d(u0, spec, dx, dx, wl, cut)
+    vol = rs.DeviceVolume.from_numpy(grid, DB3 * 30)
+    u, U = dev(u0.astype(np.complex64)), rs.DeviceVector(n * n, np.complex64)
+    say("fused lookup rows at volume edges", case)
+    lib.call("rsim_sample_apply_2d", u.ptr, U.ptr, n, n, u.code, dx, dx, wl, cut, vol.handle,
+             px, px, 0.5e-6, xpos, ypos, 0, nz, lib.RSIM_FUSED, 0, None)
+    err = rel(host(u, n, n), ue)
+    say("  vs oracle", err)
+    assert err < 2e-6
+
+
 @pytest.mark.parametrize("n", [512, 1024, 2048, 3072, 6144])
 def test_fused_sizes_match_plain_c128(n):
     """Fused complex64 against the plain complex128 GPU path (itself pinned to the oracle)."""
