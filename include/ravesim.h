@@ -75,9 +75,10 @@ int rsim_stream_destroy(void* stream);
  * Batches created before the call return RSIM_E_INVALID_ARG from
  * rsim_batch_run afterwards (their graphs point into the freed tables). */
 int rsim_release_caches(void);
-/* Bytes and entries held by the bounded device table caches (axis maps and
- * chirp tables of every device; at most 256 MiB per device, least recently
- * used entries that no batch pins are freed first). */
+/* Bytes and entries held by the bounded device table caches (axis maps,
+ * chirp tables and cone-beam runs' per-slice table sets of every device; at
+ * most 256 MiB per device, least recently used entries that no batch, graph
+ * or running call pins are freed first). */
 int rsim_cache_info(size_t* bytes, int* entries);
 
 /* Cooperative cancel + progress (big-wave/propagation.py:669-688 detector
