@@ -309,3 +309,14 @@ def test_drive_slices_chunks_on_the_cadence():
     del el._checkpoint_cadence
     H.drive_slices(lambda a, n, f: calls.append(("run", a, n)), el, None, None, 0, 2)
     assert calls == [("run", 0, 1), ("cb", 1, 2.5), ("run", 1, 1), ("cb", 2, 3.0)]
+
+
+def test_history_axis_coordinates_follow_roi_and_stride():
+    """history_x/y.npy: pixel i of a detector axis of ``count`` pixels sits at
+    (i - floor(count/2)) * pitch, then the ROI span and the stride select
+    (big-wave/multisim.py:475-484).  Hand-derived values."""
+    MS = importlib.import_module("3d_ravesim_amd.multisim")
+    np.testing.assert_array_equal(MS._frame_axis(5, 1.0, None, 1), [-2, -1, 0, 1, 2])
+    np.testing.assert_array_equal(MS._frame_axis(6, 2.0, None, 1), [-6, -4, -2, 0, 2, 4])
+    np.testing.assert_array_equal(MS._frame_axis(5, 1.0, (1, 4), 2), [-1, 1])
+    np.testing.assert_array_equal(MS._frame_axis(8, 0.5, (2, 8), 3), [-1.0, 0.5])
