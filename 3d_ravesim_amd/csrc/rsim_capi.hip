@@ -651,7 +651,8 @@ int rsim_propagate_analytically_2d(void* u, int nx, int ny, int dtype, double dx
   return point_source(u, nx, ny, dtype, dx, dy, z, x_source, y_source, wl, resolve_stream(stream));
 }
 
-static int volume_alloc(rsim_volume** vol, int nz, int ny_g, int nx_g, const double* db, int n_db) {
+// Shape and material-table checks of a new volume (nothing allocated).
+static int volume_args(rsim_volume** vol, int nz, int ny_g, int nx_g, const double* db, int n_db) {
   if (!vol) return fail(RSIM_E_INVALID_ARG, "vol is NULL");
   *vol = nullptr;
   if (nz <= 0 || ny_g < 2 || nx_g < 2)
@@ -660,6 +661,11 @@ static int volume_alloc(rsim_volume** vol, int nz, int ny_g, int nx_g, const dou
     return fail(RSIM_E_INVALID_ARG, "deltabeta table must hold 1..256 entries (got %d)", n_db);
   if (db[0] != 0.0 || db[1] != 0.0)
     return fail(RSIM_E_INVALID_ARG, "deltabeta entry 0 (vacuum) must be 0");
+  return RSIM_OK;
+}
+
+static int volume_alloc(rsim_volume** vol, int nz, int ny_g, int nx_g, const double* db, int n_db) {
+  RSIM_CHECK(volume_args(vol, nz, ny_g, nx_g, db, n_db));
   static std::atomic<unsigned long long> serial{0};
   rsim_volume* v = new rsim_volume();
   v->serial = ++serial;
@@ -689,29 +695,32 @@ static int volume_alloc(rsim_volume** vol, int nz, int ny_g, int nx_g, const dou
   return RSIM_OK;
 }
 
+static inline uint32_t host_id(const void* ids, int id_bytes, size_t i) {
+  if (id_bytes == 1) return ((const uint8_t*)ids)[i];
+  if (id_bytes == 2) return ((const uint16_t*)ids)[i];
+  return ((const uint32_t*)ids)[i];
+}
+
 int rsim_volume_create(rsim_volume** vol, const void* ids, int id_bytes, int nz, int ny_g, int nx_g,
                        const double* deltabeta, int n_db) {
   if (!ids) return fail(RSIM_E_INVALID_ARG, "ids is NULL");
   if (id_bytes != 1 && id_bytes != 2 && id_bytes != 4)
     return fail(RSIM_E_INVALID_ARG, "id_bytes must be 1, 2 or 4 (got %d)", id_bytes);
+  RSIM_CHECK(volume_args(vol, nz, ny_g, nx_g, deltabeta, n_db));
+  // every id is checked on the host before any device memory is taken
+  const size_t plane = (size_t)ny_g * nx_g;
+  const size_t total = (size_t)nz * plane;
+  for (size_t i = 0; i < total; ++i) {
+    const uint32_t id = host_id(ids, id_bytes, i);
+    if (id >= (uint32_t)n_db)
+      return fail(RSIM_E_INVALID_ARG, "material id %u at voxel %zu is outside the %d-entry deltabeta table", id, i, n_db);
+  }
   RSIM_CHECK(volume_alloc(vol, nz, ny_g, nx_g, deltabeta, n_db));
   rsim_volume* v = *vol;
-  const size_t plane = (size_t)ny_g * nx_g;
   std::vector<uint8_t> buf(plane);
   for (int z = 0; z < nz; ++z) {
     const size_t off = (size_t)z * plane;
-    for (size_t i = 0; i < plane; ++i) {
-      uint32_t id;
-      if (id_bytes == 1) id = ((const uint8_t*)ids)[off + i];
-      else if (id_bytes == 2) id = ((const uint16_t*)ids)[off + i];
-      else id = ((const uint32_t*)ids)[off + i];
-      if (id >= (uint32_t)n_db) {
-        rsim_volume_destroy(v);
-        *vol = nullptr;
-        return fail(RSIM_E_INVALID_ARG, "material id %u at voxel %zu is outside the %d-entry deltabeta table", id, off + i, n_db);
-      }
-      buf[i] = (uint8_t)id;
-    }
+    for (size_t i = 0; i < plane; ++i) buf[i] = (uint8_t)host_id(ids, id_bytes, off + i);
     hipError_t e = hipMemcpy(v->ids + off, buf.data(), plane, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       rsim_volume_destroy(v);

@@ -38,6 +38,12 @@ def test_invalid_arguments_map_to_value_error_without_a_device():
     assert rc == lib.RSIM_E_INVALID_ARG and b"vacuum" in L.rsim_last_error()
     rc = L.rsim_volume_create(C.byref(vol), ids.ctypes.data, 3, 1, 2, 2, db.ctypes.data_as(lib._dp), 1)
     assert rc == lib.RSIM_E_INVALID_ARG
+    # material ids are checked on the host before any device allocation
+    db2 = np.zeros(4, dtype=np.float64)
+    ids2 = np.array([[[0, 1], [2, 3]]], dtype=np.uint16)
+    rc = L.rsim_volume_create(C.byref(vol), ids2.ctypes.data, 2, 1, 2, 2, db2.ctypes.data_as(lib._dp), 2)
+    assert rc == lib.RSIM_E_INVALID_ARG and b"material id 2 at voxel 2" in L.rsim_last_error()
+    assert not vol.value
     with pytest.raises(ValueError):
         lib.call("rsim_kernel_timing", -1)
     b = C.c_void_p()
