@@ -11,8 +11,11 @@
 //                   and plain slice loops, the cone-beam tables built by host
 //                   threads, chains, batches, captured graphs, per-run cancel
 //                   and progress, the detectors (allocating and static
-//                   scratch), the adjoint with its checkpoint workspace, and
-//                   cache release invalidating batches and graphs
+//                   scratch), the adjoint with its checkpoint workspace,
+//                   cache release invalidating batches and graphs; then FFTs,
+//                   the point source, per-voxel and plasma transmissions,
+//                   history frames in both engines, the detector adjoint and
+//                   batched chains
 //
 // Prints "abi_check <mode>: OK" and exits 0 when every expectation holds.
 
@@ -344,6 +347,140 @@ void check_device() {
   EXPECT_RC(rsim_release_caches(), RSIM_OK);
 }
 
+// The remaining entry points with host-side work of their own: FFTs (1D rows,
+// a four-step length), the point source and the cutoff, per-voxel and plasma
+// transmissions (the plasma volume's tables cached per energy), history frames
+// in both engines, the detector adjoint, and batched chains.
+void check_device_more() {
+  const int n = 256, nz = 4;
+  const double dx = 0.5e-6, wl = 0.5e-10, pz = 0.5e-6, cut = 0.4 / dx;
+  const size_t pts = (size_t)n * n;
+  Field u(pts), U(pts);
+
+  STEP("transforms, point source, cutoff");
+  EXPECT_RC(rsim_propagate_analytically_2d(u.d, n, n, RSIM_C64, dx, dx, 0.5, 1e-6, -2e-6, wl,
+                                           nullptr),
+            RSIM_OK);
+  EXPECT_RC(rsim_fft2(U.d, u.d, n, n, RSIM_C64, RSIM_FORWARD, nullptr), RSIM_OK);
+  EXPECT_RC(rsim_apply_frequency_cutoff_2d(U.d, n, n, RSIM_C64, cut, dx, dx, nullptr), RSIM_OK);
+  EXPECT_RC(rsim_fft2(u.d, U.d, n, n, RSIM_C64, RSIM_INVERSE, nullptr), RSIM_OK);
+  EXPECT_RC(rsim_fft1d_rows(U.d, u.d, n, n, RSIM_C64, RSIM_FORWARD, nullptr), RSIM_OK);
+  {
+    const int nl = 16384, rows = 4;   // a four-step length: per-stream scratch
+    Field a((size_t)nl * rows), b((size_t)nl * rows);
+    EXPECT_RC(rsim_fill_zero(a.d, a.bytes, nullptr), RSIM_OK);
+    EXPECT_RC(rsim_fft1d_rows(b.d, a.d, nl, rows, RSIM_C64, RSIM_FORWARD, nullptr), RSIM_OK);
+    Field c(4096 * 8), d(4096 * 8);
+    EXPECT_RC(rsim_stream_copy(d.d, c.d, c.bytes, nullptr), RSIM_OK);
+  }
+  EXPECT_RC(rsim_tilted_plane_2d(u.d, n, n, RSIM_C64, dx, dx, wl, 0.02, 1e-6, 0.0, nullptr), RSIM_OK);
+
+  STEP("per-voxel transmission and detector adjoint");
+  void *dl = nullptr, *bt = nullptr, *gimg = nullptr, *g = nullptr;
+  EXPECT_RC(rsim_malloc(&dl, pts * 4), RSIM_OK);
+  EXPECT_RC(rsim_malloc(&bt, pts * 4), RSIM_OK);
+  EXPECT_RC(rsim_fill_zero(dl, pts * 4, nullptr), RSIM_OK);
+  EXPECT_RC(rsim_fill_zero(bt, pts * 4, nullptr), RSIM_OK);
+  EXPECT_RC(rsim_transmission_db_2d(u.d, n, n, RSIM_C64, wl, pz, static_cast<const float*>(dl),
+                                    static_cast<const float*>(bt), nullptr),
+            RSIM_OK);
+  const int ox = n / 4, oy = n / 4;
+  EXPECT_RC(rsim_malloc(&gimg, (size_t)ox * oy * 8), RSIM_OK);
+  EXPECT_RC(rsim_malloc(&g, pts * 8), RSIM_OK);
+  EXPECT_RC(rsim_fill_zero(gimg, (size_t)ox * oy * 8, nullptr), RSIM_OK);
+  EXPECT_RC(rsim_detector_area_v1_adjoint(u.d, g, n, n, RSIM_C64, dx, dx, ox, oy, 4 * dx, 4 * dx,
+                                          0.5, static_cast<const double*>(gimg), nullptr),
+            RSIM_OK);
+  for (void* q : {dl, bt, gimg, g}) rsim_free(q);
+
+  STEP("plasma volume");
+  const size_t nvox = (size_t)nz * pts;
+  std::vector<double> ne(nvox, 0.0), ni(nvox, 0.0), te(nvox, 0.0), zs(nvox, 0.0);
+  for (int k = 0; k < nz; ++k)
+    for (int y = 0; y < n; ++y)
+      for (int x = 0; x < n; ++x) {
+        const int dy = y - 128, dxp = x - 120 - 4 * k;
+        if (dy * dy + dxp * dxp < 60 * 60) {
+          const size_t i = (size_t)k * pts + (size_t)y * n + x;
+          ne[i] = 1e22 * (1 + 0.01 * (x % 7));
+          zs[i] = (x % 3) ? 8.0 : 0.0;
+          ni[i] = ne[i] / 8.0;
+          te[i] = (y % 2) ? 300.0 : 0.0;
+        }
+      }
+  const double optics[4] = {13.31, 0.187, 13.0, 169.0};
+  rsim_plasma* pl = nullptr;
+  EXPECT_RC(rsim_plasma_create(&pl, ne.data(), ni.data(), te.data(), zs.data(), 8, nz, n, n),
+            RSIM_OK);
+  void* db = nullptr;
+  EXPECT_RC(rsim_malloc(&db, nvox * 16), RSIM_OK);
+  EXPECT_RC(rsim_plasma_deltabeta(pl, 0, nz, 8000.0, optics, db, nullptr), RSIM_OK);
+  rsim_free(db);
+  EXPECT_RC(rsim_plasma_transmission_2d(u.d, n, n, RSIM_C64, dx, dx, wl, pl, 1, 8000.0, optics, dx,
+                                        dx, pz, 0.0, 0.0, 1.0, nullptr),
+            RSIM_OK);
+  for (int engine : {RSIM_PLAIN, RSIM_FUSED})
+    for (double energy : {8000.0, 8500.0}) {   // a second energy builds a second cached volume
+      EXPECT_RC(rsim_plasma_apply_2d(u.d, U.d, n, n, RSIM_C64, dx, dx, wl, cut, pl, energy,
+                                     optics, dx, dx, pz, 0.0, 0.0, 0, nz, nullptr, nullptr, engine,
+                                     0, nullptr),
+                RSIM_OK);
+    }
+
+  STEP("history frames");
+  rsim_volume* vol = nullptr;
+  std::vector<uint8_t> grid(nvox, 0);
+  for (size_t i = 0; i < nvox; ++i) grid[i] = (uint8_t)((i / 97) % 4 == 0 ? 1 + (i % 3) : 0);
+  EXPECT_RC(rsim_volume_create(&vol, grid.data(), 1, nz, n, n, kDb, 4), RSIM_OK);
+  rsim_history_frames hf = {};
+  hf.out_x = ox;
+  hf.out_y = oy;
+  hf.pixel_size_x = hf.pixel_size_y = 4 * dx;
+  hf.z_start = 0.1;
+  EXPECT_RC(rsim_malloc(reinterpret_cast<void**>(&hf.frames), (size_t)nz * ox * oy * 8), RSIM_OK);
+  for (int engine : {RSIM_PLAIN, RSIM_FUSED}) {
+    EXPECT_RC(rsim_sample_apply_2d_history(u.d, U.d, n, n, RSIM_C64, dx, dx, wl, cut, vol, dx, dx,
+                                           pz, 0.0, 0.0, 0, nz, nullptr, nullptr, engine, 0, &hf,
+                                           nullptr),
+              RSIM_OK);
+    EXPECT_RC(rsim_plasma_apply_2d_history(u.d, U.d, n, n, RSIM_C64, dx, dx, wl, cut, pl, 8000.0,
+                                           optics, dx, dx, pz, 0.0, 0.0, 0, nz, nullptr, nullptr,
+                                           engine, 0, &hf, nullptr),
+              RSIM_OK);
+  }
+  EXPECT_RC(rsim_synchronize(nullptr), RSIM_OK);
+  rsim_free(hf.frames);
+
+  STEP("batched chains");
+  Field ub(3 * pts), Ub(3 * pts);
+  std::vector<rsim_chain_step> steps(3 * 2);
+  std::vector<rsim_chain_instance> inst(3);
+  for (int i = 0; i < 3; ++i) {
+    rsim_chain_step* s = &steps[2 * i];
+    s[0] = rsim_chain_step{};
+    s[0].vol = vol;
+    s[0].pixel_size_x = s[0].pixel_size_y = dx;
+    s[0].pixel_size_z = pz;
+    s[0].x_pos = 1e-6 * i;
+    s[0].n_slices = nz;
+    s[1] = rsim_chain_step{};
+    s[1].dz = 0.05 + 0.01 * i;
+    inst[i].u = static_cast<char*>(ub.d) + i * pts * 8;
+    inst[i].U = static_cast<char*>(Ub.d) + i * pts * 8;
+    inst[i].dx = inst[i].dy = dx;
+    inst[i].wl = wl * (1 + 0.05 * i);
+    inst[i].cutoff_freq = cut;
+    inst[i].steps = s;
+  }
+  EXPECT_RC(rsim_fill_zero(ub.d, ub.bytes, nullptr), RSIM_OK);
+  EXPECT_RC(rsim_chain_apply_2d_batch(inst.data(), 3, n, n, RSIM_C64, 1, 2, RSIM_FUSED, 0, nullptr),
+            RSIM_OK);
+  EXPECT_RC(rsim_synchronize(nullptr), RSIM_OK);
+  EXPECT_RC(rsim_plasma_destroy(pl), RSIM_OK);
+  EXPECT_RC(rsim_volume_destroy(vol), RSIM_OK);
+  EXPECT_RC(rsim_release_caches(), RSIM_OK);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -361,6 +498,7 @@ int main(int argc, char** argv) {
       return 3;
     }
     check_device();
+    check_device_more();
   }
   if (g_fail) {
     std::fprintf(stderr, "abi_check %s: %d expectation(s) failed\n", mode.c_str(), g_fail);
