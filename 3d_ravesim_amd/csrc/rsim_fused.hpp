@@ -205,11 +205,13 @@ struct ColBatch {
 // Occupancy request of a row launch shape: LD = 2 asks the compiler for three
 // workgroups of up to 512 threads per CU (waves per SIMD = 3 x the
 // workgroup's share), LD = 3 for two, i.e. a register budget that lets them
-// in.
+// in; LD = 4 for four waves per SIMD whatever the workgroup size (128
+// registers: four 256-thread or two 512-thread workgroups per CU).
 template <class F, int C, int LD>
 constexpr int row_waves_per_eu() {
   return (LD == 2 && F::TC * C <= 512)   ? 3 * ((F::TC * C + 255) / 256)
          : (LD == 3 && F::TC * C <= 512) ? 2 * ((F::TC * C + 255) / 256)
+         : (LD == 4 && F::TC * C <= 512) ? 4
                                          : 1;
 }
 
@@ -219,12 +221,14 @@ constexpr int row_waves_per_eu() {
 // (LD = 2) where three workgroups still fit beside them: no registers held
 // through the first transform (the 4096 row's spills go away, MID 93.2 ->
 // 90.8 us).  Shapes with registers to spare run faster with 1 (2048 row
-// 24.7 us against 26.7 with 2).
+// 24.7 us against 26.7 with 2).  The four-waves budget (LD = 4) stashes them
+// in LDS too where its workgroups (16 waves per CU) still fit.
 template <class F, int C, int PS, bool SPLIT, int LD>
 constexpr int row_id_mode() {
   const long long lds = (long long)LdsSize<F, C, PS>::slots * (SPLIT ? 4 : 8) + 256 * 16 +
                         (long long)((F::E + 3) / 4) * 4 * F::TC * C;
-  return LD == 2 && 3 * lds <= 163840 ? 2 : 1;
+  const long long wgs4 = 16 / ((F::TC * C + 63) / 64);
+  return (LD == 2 && 3 * lds <= 163840) || (LD == 4 && wgs4 >= 1 && wgs4 * lds <= 163840) ? 2 : 1;
 }
 
 template <class F, int C, int PS, bool SPLIT, int LD, int MODE, int TM, typename TW>
@@ -615,8 +619,8 @@ template <int N> constexpr int vColC(int E, int t) { return vmin(vmin(vmax1(t / 
 // X(N, id, E, C, SPLIT, LD).  The base lists keep the index of the second
 // transform of a kernel as it is (LD 0); the LD lists are copies of some
 // base shapes whose second transform recomputes its addresses
-// (opaque_index; LD 1, or LD 2 / LD 3 with the three- / two-workgroup
-// register budget of row_waves_per_eu).  The two sets are compiled in separate translation units
+// (opaque_index; LD 1, or LD 2 / LD 3 / LD 4 with the three- / two-workgroup
+// or four-waves register budget of row_waves_per_eu).  The two sets are compiled in separate translation units
 // (rsim_fused_<N>.hip and rsim_fused_<N>_ld.hip): kernels of the other kind
 // in the same module change the code generated for the base kernels.
 // Variant ids: base 0..kRowBase-1, then the LD list.
@@ -630,7 +634,8 @@ template <int N> constexpr int vColC(int E, int t) { return vmin(vmin(vmax1(t / 
   X(N, 1, vE1<N>(), vRowC<N>(vE1<N>(), 512), vsplit(N, vRowC<N>(vE1<N>(), 512), 65536), 1)    \
   X(N, 2, vE2<N>(), vRowC<N>(vE2<N>(), 256), vsplit(N, vRowC<N>(vE2<N>(), 256), 65536), 1)    \
   X(N, 3, vE1<N>(), vRowC<N>(vE1<N>(), 512), true, 2)                                        \
-  X(N, 4, vE2<N>(), vRowC<N>(vE2<N>(), 512), true, 3)
+  X(N, 4, vE2<N>(), vRowC<N>(vE2<N>(), 512), true, 3)                                        \
+  X(N, 5, vE1<N>(), vRowC<N>(vE1<N>(), 256), true, 4)
 #define RSIM_COL_VARIANTS(N, X)                                                                 \
   X(N, 0, vE1<N>(), vColC<N>(vE1<N>(), 1024), true, 0)                                      \
   X(N, 1, vE1<N>(), vColC<N>(vE1<N>(), 1024), vsplit(N, vColC<N>(vE1<N>(), 1024), 163840 - 8192), 0) \
@@ -643,7 +648,7 @@ template <int N> constexpr int vColC(int E, int t) { return vmin(vmin(vmax1(t / 
   X(N, 2, vE2<N>(), vColC<N>(vE2<N>(), 512), true, 1)
 constexpr int kRowBase = 4;
 constexpr int kColBase = 5;
-constexpr int kRowVariants = kRowBase + 5;
+constexpr int kRowVariants = kRowBase + 6;
 constexpr int kColVariants = kColBase + 3;
 
 // Per-size launchers (explicitly instantiated in rsim_fused_<N>.hip and

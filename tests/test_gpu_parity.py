@@ -471,8 +471,10 @@ def test_phase_stepping_replays_from_snapshot():
         assert rel(outs[k], exp) < 1e-11
 
 
-N_ROW_VARIANTS, N_COL_VARIANTS = 9, 8
+N_ROW_VARIANTS, N_COL_VARIANTS = 10, 8
 VARIANT_CASES = ([(1024, 1024, (r, 0)) for r in range(N_ROW_VARIANTS)]
+                 # the four-waves row budget at every fused length
+                 + [(256, n, (9, 0)) for n in (256, 512, 2048, 3072, 4096, 6144, 8192)]
                  + [(1024, 1024, (r, c)) for c in range(1, N_COL_VARIANTS)
                     for r in (0, N_ROW_VARIANTS - 1)]
                  + [(4096, 4096, (i % N_ROW_VARIANTS, i)) for i in range(N_COL_VARIANTS)]
