@@ -645,11 +645,12 @@ template <int N> constexpr int vColC(int E, int t) { return vmin(vmin(vmax1(t / 
 #define RSIM_COL_VARIANTS_LD(N, X)                                                              \
   X(N, 0, vE1<N>(), vColC<N>(vE1<N>(), 1024), true, 1)                                       \
   X(N, 1, vE1<N>(), vColC<N>(vE1<N>(), 1024), vsplit(N, vColC<N>(vE1<N>(), 1024), 163840 - 8192), 1) \
-  X(N, 2, vE2<N>(), vColC<N>(vE2<N>(), 512), true, 1)
+  X(N, 2, vE2<N>(), vColC<N>(vE2<N>(), 512), true, 1)                                       \
+  X(N, 3, vE2<N>(), vColC<N>(vE2<N>(), 1024), true, 1)
 constexpr int kRowBase = 4;
 constexpr int kColBase = 5;
 constexpr int kRowVariants = kRowBase + 6;
-constexpr int kColVariants = kColBase + 3;
+constexpr int kColVariants = kColBase + 4;
 
 // Per-size launchers (explicitly instantiated in rsim_fused_<N>.hip and
 // rsim_fused_<N>_ld.hip); count instances (1..kMaxBatch) of shape (N, ny) or

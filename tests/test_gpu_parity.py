@@ -471,7 +471,7 @@ def test_phase_stepping_replays_from_snapshot():
         assert rel(outs[k], exp) < 1e-11
 
 
-N_ROW_VARIANTS, N_COL_VARIANTS = 10, 8
+N_ROW_VARIANTS, N_COL_VARIANTS = 10, 9
 VARIANT_CASES = ([(1024, 1024, (r, 0)) for r in range(N_ROW_VARIANTS)]
                  # the four-waves row budget at every fused length
                  + [(256, n, (9, 0)) for n in (256, 512, 2048, 3072, 4096, 6144, 8192)]
